@@ -1,0 +1,983 @@
+// api.hip -- the C-ABI (include/ace_hip.h): context, model tables, structure upload, neighbour-list
+// build, batched energy/force pass, the plain FP64 port, and per-kernel HIP-event timing.
+// One translation unit: the kernel sources are included below.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ace_common.hpp"
+#include "neighbors.hip"
+#include "basis.hip"
+#include "mlp.hip"
+#include "reduce.hip"
+#include "plain.hip"
+#include "train.hip"
+
+namespace {
+
+struct HSpecies {
+    int K = 0, nR = 0, L = 0, act = 0;
+    double c = 1, outw = 1, outb = 0, E0 = 0;
+    std::vector<int> r_form, r_cut, r_n;
+    std::vector<double> r_rc;
+    std::vector<std::vector<double>> r_rs, r_eta;
+    std::vector<int> a_form, a_cut, a_n;
+    std::vector<double> a_rc;
+    std::vector<std::vector<double>> a_eta, a_zeta;
+    std::vector<std::vector<int>> a_lambda;
+    std::vector<int> width;
+    std::vector<double> inpw, inpb, params;
+    int nparams() const {
+        int s = 0;
+        for (int l = 0; l < L; ++l) s += width[l + 1] + width[l + 1] * width[l];
+        return s;
+    }
+};
+
+struct ProfRec {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct ace_ctx {
+    std::string err;
+    hipStream_t stream = nullptr;
+    // model
+    int nt = 0;
+    double rc = 0;
+    std::vector<HSpecies> sp;
+    int kmax = 0, ldg = 0, maxA = 0, maxR = 0, ngmax = 1;
+    DevModel hdm;
+    DBuf<DevModel> d_model;
+    DBuf<float> d_r_rs, d_r_eta, d_a_lamf, d_a_zeta, d_params, d_inpw, d_inpb;
+    DBuf<int> d_a_zint, d_a_grp;
+    bool plain_ok = false;
+    std::string plain_why;
+    PlainModel hpm;
+    DBuf<PlainModel> d_pmodel;
+    DBuf<double> d_r_rs64, d_r_eta64, d_a_eta64, d_a_zeta64, d_a_lamf64, d_nn64, d_inpw64, d_inpb64;
+    bool params_dirty = true;
+    // structures
+    int S = 0, N = 0;
+    std::vector<int> natoms, atom_off, type_h, row_of_h, struc_of_h;
+    std::vector<double> cell;
+    DBuf<int> d_type, d_struc_of, d_atom_off, d_natoms, d_row_of, d_atom_of_row;
+    DBuf<double> d_pos, d_cell;
+    std::vector<int4> tiles_h;
+    DBuf<int4> d_tiles;
+    int tm = 32;
+    DBuf<float> d_G, d_dEdG;
+    DBuf<double> d_e_atom, d_vir_atom, d_F, d_E, d_vir;
+    // neighbours
+    bool nb_valid = false;
+    bool disp_dirty = false;
+    double nb_rc = 0;
+    std::vector<StrucInfo> sinfo_h;
+    DBuf<StrucInfo> d_sinfo;
+    int ncells = 0;
+    DBuf<int> d_cell_of, d_wrap_of, d_cell_count, d_cell_start, d_cell_cursor, d_cell_atoms, d_nb_count, d_nb_off,
+        d_scan_blk, d_scal;
+    long long npairs = 0;
+    int maxnb = 0;
+    DBuf<int> d_nb_j, d_nb_img, d_nb_src, d_nb_rev;
+    DBuf<float4> d_nb_d, d_pair_g;
+    // plain port scratch
+    DBuf<double> d_G64;
+    // training
+    TrainState tr;
+    // profiling
+    bool prof = false;
+    std::vector<ProfRec> pending;
+    std::map<std::string, std::pair<double, long long>> acc;
+};
+
+namespace {
+
+int fail(ace_ctx* c, const std::string& msg) {
+    if (c) c->err = msg;
+    return -1;
+}
+int fail_hip(ace_ctx* c, const char* what, hipError_t e) {
+    return fail(c, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HC(x)                                                         \
+    do {                                                              \
+        hipError_t e_ = (x);                                          \
+        if (e_ != hipSuccess) return fail_hip(ctx, #x, e_);          \
+    } while (0)
+
+template <class T>
+int ensure(ace_ctx* ctx, DBuf<T>& b, size_t n) {
+    if (n == 0) n = 1;
+    if (b.cap >= n) return 0;
+    if (b.p) HC(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+    HC(hipMalloc((void**)&b.p, n * sizeof(T)));
+    b.cap = n;
+    return 0;
+}
+template <class T>
+void release(DBuf<T>& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+template <class T>
+int upload(ace_ctx* ctx, DBuf<T>& b, const T* h, size_t n) {
+    if (ensure(ctx, b, n)) return -1;
+    if (n) HC(hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return 0;
+}
+
+// kernel launch with optional HIP-event timing on the context's stream
+template <class F>
+int timed(ace_ctx* ctx, const char* name, F&& launch) {
+    ProfRec r;
+    if (ctx->prof) {
+        r.name = name;
+        HC(hipEventCreate(&r.a));
+        HC(hipEventCreate(&r.b));
+        HC(hipEventRecord(r.a, ctx->stream));
+    }
+    launch();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail_hip(ctx, name, e);
+    if (ctx->prof) {
+        HC(hipEventRecord(r.b, ctx->stream));
+        ctx->pending.push_back(r);
+    }
+    return 0;
+}
+
+int flush_prof(ace_ctx* ctx) {
+    if (ctx->pending.empty()) return 0;
+    HC(hipStreamSynchronize(ctx->stream));
+    for (auto& r : ctx->pending) {
+        float ms = 0;
+        HC(hipEventElapsedTime(&ms, r.a, r.b));
+        auto& a = ctx->acc[r.name];
+        a.first += ms;
+        a.second += 1;
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    ctx->pending.clear();
+    return 0;
+}
+
+inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+int exclusive_scan(ace_ctx* ctx, const int* in, int* out, int n) {
+    const int nb = cdiv(n > 0 ? n : 1, SCAN_BLOCK);
+    if (ensure(ctx, ctx->d_scan_blk, nb + 1)) return -1;
+    int* blk = ctx->d_scan_blk.p;
+    int* total = ctx->d_scal.p + 1;
+    if (timed(ctx, "scan_local", [&] { k_scan_local<<<nb, SCAN_BLOCK, 0, ctx->stream>>>(n, in, out, blk); })) return -1;
+    if (timed(ctx, "scan_sums", [&] { k_scan_sums<<<1, SCAN_BLOCK, 0, ctx->stream>>>(nb, blk, total, n, out); })) return -1;
+    if (timed(ctx, "scan_add", [&] { k_scan_add<<<nb, SCAN_BLOCK, 0, ctx->stream>>>(n, out, blk); })) return -1;
+    return 0;
+}
+
+// ---- model ------------------------------------------------------------------------------------
+
+int parse_model(ace_ctx* ctx, const ace_model_desc* m) {
+    if (!m) return fail(ctx, "ace_create: null model");
+    if (m->ntypes < 1 || m->ntypes > ACE_MAX_TYPES) return fail(ctx, "ace_create: ntypes out of range [1,8]");
+    if (!(m->rc > 0)) return fail(ctx, "ace_create: invalid cutoff radius");
+    const int nt = m->ntypes, nab = nt * (nt + 1) / 2;
+    ctx->nt = nt;
+    ctx->rc = m->rc;
+    ctx->sp.assign(nt, HSpecies());
+    for (int I = 0; I < nt; ++I) {
+        HSpecies& s = ctx->sp[I];
+        for (int J = 0; J < nt; ++J) {
+            const int b = I * nt + J;
+            s.r_form.push_back(m->r_form[b]);
+            s.r_cut.push_back(m->r_cut[b]);
+            s.r_rc.push_back(m->r_rc[b]);
+            s.r_n.push_back(m->r_n[b]);
+            const double* rs = m->r_rs + m->r_off[b];
+            const double* et = m->r_eta + m->r_off[b];
+            s.r_rs.emplace_back(rs, rs + m->r_n[b]);
+            s.r_eta.emplace_back(et, et + m->r_n[b]);
+            if (m->r_n[b] < 0) return fail(ctx, "ace_create: negative radial block size");
+            if (m->r_n[b] > 0 && (m->r_form[b] < ACE_RAD_GAUSSIAN || m->r_form[b] > ACE_RAD_LOGCOSH2))
+                return fail(ctx, "ace_create: invalid radial function name");
+            if (m->r_n[b] > 0 && (m->r_cut[b] < ACE_CUT_STEP || m->r_cut[b] > ACE_CUT_POLY3))
+                return fail(ctx, "ace_create: invalid cutoff name");
+            if (m->r_n[b] > 0 && !(m->r_rc[b] > 0)) return fail(ctx, "ace_create: invalid basis cutoff");
+        }
+        for (int b = 0; b < nab; ++b) {
+            const int q = I * nab + b;
+            s.a_form.push_back(m->a_form[q]);
+            s.a_cut.push_back(m->a_cut[q]);
+            s.a_rc.push_back(m->a_rc[q]);
+            s.a_n.push_back(m->a_n[q]);
+            const int o = m->a_off[q], n = m->a_n[q];
+            s.a_eta.emplace_back(m->a_eta + o, m->a_eta + o + n);
+            s.a_zeta.emplace_back(m->a_zeta + o, m->a_zeta + o + n);
+            s.a_lambda.emplace_back(m->a_lambda + o, m->a_lambda + o + n);
+            if (n < 0) return fail(ctx, "ace_create: negative angular block size");
+            if (n > 0 && m->a_form[q] != ACE_ANG_GAUSS)
+                return fail(ctx, "ace_create: angular form not supported by the GPU path (GAUSS only)");
+            if (n > 0 && (m->a_cut[q] < ACE_CUT_STEP || m->a_cut[q] > ACE_CUT_POLY3))
+                return fail(ctx, "ace_create: invalid cutoff name");
+            if (n > 0 && !(m->a_rc[q] > 0)) return fail(ctx, "ace_create: invalid basis cutoff");
+        }
+        s.L = m->nn_nlayer[I];
+        if (s.L < 1 || s.L > ACE_MAX_LAYERS) return fail(ctx, "ace_create: nlayer out of range");
+        const int* w = m->nn_width + I * (ACE_MAX_LAYERS + 1);
+        s.width.assign(w, w + s.L + 1);
+        for (int l = 0; l <= s.L; ++l)
+            if (s.width[l] < 1) return fail(ctx, "ace_create: invalid layer width");
+        if (s.width[s.L] != 1) return fail(ctx, "ace_create: the output layer must have width 1");
+        s.act = m->nn_act[I];
+        if (s.act != ACE_ACT_LINEAR && s.act != ACE_ACT_TANH && s.act != ACE_ACT_SIGMOID && s.act != ACE_ACT_SWISH &&
+            s.act != ACE_ACT_SOFTPLUS)
+            return fail(ctx, "ace_create: activation not supported");
+        s.c = m->nn_c[I];
+        s.outw = m->outw[I];
+        s.outb = m->outb[I];
+        s.E0 = m->E0[I];
+        int K = 0;
+        for (int J = 0; J < nt; ++J) K += s.r_n[J];
+        s.nR = K;
+        for (int b = 0; b < nab; ++b) K += s.a_n[b];
+        s.K = K;
+        if (s.width[0] != K) return fail(ctx, "ace_create: network input width differs from the basis size");
+        s.inpw.assign(m->inpw + m->in_off[I], m->inpw + m->in_off[I] + K);
+        s.inpb.assign(m->inpb + m->in_off[I], m->inpb + m->in_off[I] + K);
+        s.params.assign(m->params + m->par_off[I], m->params + m->par_off[I] + s.nparams());
+    }
+    return 0;
+}
+
+// BasisA::init run grouping (basis_angular.cpp:178-205), merged over the blocks of one species
+int build_tables(ace_ctx* ctx) {
+    const int nt = ctx->nt, nab = nt * (nt + 1) / 2;
+    DevModel& dm = ctx->hdm;
+    memset(&dm, 0, sizeof(dm));
+    dm.ntypes = nt;
+    std::vector<float> r_rs, r_eta, a_lamf, a_zeta, prm, iw, ib;
+    std::vector<int> a_zint, a_grp;
+    int kmax = 0, maxA = 1, maxR = 1, ngmax = 1;
+    for (int I = 0; I < nt; ++I) {
+        const HSpecies& s = ctx->sp[I];
+        DevSpecies& d = dm.sp[I];
+        d.K = s.K;
+        d.nR = s.nR;
+        kmax = std::max(kmax, s.K);
+        int goff = 0;
+        double rcm = 0;
+        for (int J = 0; J < nt; ++J) {
+            d.r_form[J] = s.r_form[J];
+            d.r_cut[J] = s.r_cut[J];
+            d.r_n[J] = s.r_n[J];
+            d.r_rc[J] = (float)s.r_rc[J];
+            d.r_goff[J] = goff;
+            d.r_poff[J] = (int)r_rs.size();
+            goff += s.r_n[J];
+            for (int q = 0; q < s.r_n[J]; ++q) {
+                r_rs.push_back((float)s.r_rs[J][q]);
+                r_eta.push_back((float)s.r_eta[J][q]);
+            }
+            if (s.r_n[J] > 0) rcm = std::max(rcm, s.r_rc[J]);
+            maxR = std::max(maxR, s.r_n[J]);
+        }
+        // angular: one cutoff per species, distinct etas merged into groups
+        int acut = -1;
+        double arc = 0;
+        std::vector<double> groups;
+        for (int b = 0; b < nab; ++b) {
+            if (s.a_n[b] == 0) continue;
+            if (acut < 0) { acut = s.a_cut[b]; arc = s.a_rc[b]; }
+            else if (acut != s.a_cut[b] || arc != s.a_rc[b])
+                return fail(ctx, "ace_create: the GPU path needs one angular cutoff per species");
+        }
+        d.a_cut = acut < 0 ? ACE_CUT_COS : acut;
+        d.a_rc = (float)(acut < 0 ? ctx->rc : arc);
+        if (acut >= 0) rcm = std::max(rcm, arc);
+        d.rc_max = (float)std::min(rcm, ctx->rc);
+        for (int b = 0; b < nab; ++b) {
+            d.a_form[b] = s.a_form[b];
+            d.a_n[b] = s.a_n[b];
+            d.a_goff[b] = goff;
+            d.a_poff[b] = (int)a_lamf.size();
+            goff += s.a_n[b];
+            maxA = std::max(maxA, s.a_n[b]);
+            // run grouping within the block: consecutive etas closer than 1e-6 share the first one
+            double run_eta = 0;
+            for (int q = 0; q < s.a_n[b]; ++q) {
+                const double e = s.a_eta[b][q];
+                if (q == 0 || std::fabs(e - s.a_eta[b][q - 1]) >= 1.0e-6) run_eta = e;
+                int g = -1;
+                for (size_t t = 0; t < groups.size(); ++t)
+                    if (groups[t] == run_eta) g = (int)t;
+                if (g < 0) {
+                    if ((int)groups.size() >= ACE_MAXG)
+                        return fail(ctx, "ace_create: more than 4 distinct angular eta values in one species");
+                    groups.push_back(run_eta);
+                    g = (int)groups.size() - 1;
+                }
+                a_lamf.push_back((float)(0.5 * s.a_lambda[b][q]));
+                a_zeta.push_back((float)s.a_zeta[b][q]);
+                const double z = s.a_zeta[b][q];
+                a_zint.push_back((z == std::floor(z) && z >= 1 && z <= 64) ? (int)z : -1);
+                a_grp.push_back(g);
+                if (!(z > 0)) return fail(ctx, "ace_create: zeta must be positive");
+            }
+        }
+        d.ng = (int)groups.size();
+        for (int g = 0; g < d.ng; ++g) d.ietar2[g] = (float)(1.0 / (groups[g] * groups[g]));
+        ngmax = std::max(ngmax, d.ng);
+        // network
+        d.L = s.L;
+        for (int l = 0; l <= s.L; ++l) d.width[l] = s.width[l];
+        d.act = s.act;
+        d.c = (float)s.c;
+        int cnt = 0;
+        for (int l = 0; l < s.L; ++l) { d.boff[l] = cnt; cnt += s.width[l + 1]; }
+        for (int l = 0; l < s.L; ++l) { d.woff[l] = cnt; cnt += s.width[l + 1] * s.width[l]; }
+        d.ldp = cnt;
+        d.poff = (int)prm.size();
+        for (double v : s.params) prm.push_back((float)v);
+        d.in_off = (int)iw.size();
+        for (int k = 0; k < s.K; ++k) {
+            iw.push_back((float)s.inpw[k]);
+            ib.push_back((float)s.inpb[k]);
+        }
+        d.outw = (float)s.outw;
+        d.outb = (float)s.outb;
+        d.E0 = s.E0;
+    }
+    dm.kmax = kmax;
+    dm.ng_max = ngmax;
+    ctx->kmax = kmax;
+    ctx->ldg = (kmax + 3) & ~3;
+    ctx->maxA = maxA;
+    ctx->maxR = maxR;
+    ctx->ngmax = ngmax;
+    if (upload(ctx, ctx->d_model, &dm, 1)) return -1;
+    if (upload(ctx, ctx->d_r_rs, r_rs.data(), r_rs.size())) return -1;
+    if (upload(ctx, ctx->d_r_eta, r_eta.data(), r_eta.size())) return -1;
+    if (upload(ctx, ctx->d_a_lamf, a_lamf.data(), a_lamf.size())) return -1;
+    if (upload(ctx, ctx->d_a_zeta, a_zeta.data(), a_zeta.size())) return -1;
+    if (upload(ctx, ctx->d_a_zint, a_zint.data(), a_zint.size())) return -1;
+    if (upload(ctx, ctx->d_a_grp, a_grp.data(), a_grp.size())) return -1;
+    if (upload(ctx, ctx->d_params, prm.data(), prm.size())) return -1;
+    if (upload(ctx, ctx->d_inpw, iw.data(), iw.size())) return -1;
+    if (upload(ctx, ctx->d_inpb, ib.data(), ib.size())) return -1;
+
+    // FP64 tables of the plain port
+    ctx->plain_ok = true;
+    PlainModel& pm = ctx->hpm;
+    memset(&pm, 0, sizeof(pm));
+    pm.ntypes = nt;
+    std::vector<double> rrs, reta, aeta, azeta, alamf, nn, iw64, ib64;
+    for (int I = 0; I < nt; ++I) {
+        const HSpecies& s = ctx->sp[I];
+        PlainSpecies& p = pm.sp[I];
+        p.K = s.K; p.nR = s.nR; p.L = s.L; p.act = s.act;
+        p.c = s.c; p.outw = s.outw; p.outb = s.outb; p.E0 = s.E0;
+        if (s.K > PL_WMAX || s.L > PL_LMAX) { ctx->plain_ok = false; ctx->plain_why = "network too large"; }
+        if (s.act != ACE_ACT_TANH && s.act != ACE_ACT_LINEAR) { ctx->plain_ok = false; ctx->plain_why = "activation"; }
+        int goff = 0;
+        for (int J = 0; J < nt; ++J) {
+            p.r_n[J] = s.r_n[J]; p.r_goff[J] = goff; p.r_poff[J] = (int)rrs.size(); p.r_rc[J] = s.r_rc[J];
+            goff += s.r_n[J];
+            if (s.r_n[J] && (s.r_form[J] != ACE_RAD_GAUSSIAN || s.r_cut[J] != ACE_CUT_COS)) {
+                ctx->plain_ok = false; ctx->plain_why = "radial form";
+            }
+            for (int q = 0; q < s.r_n[J]; ++q) { rrs.push_back(s.r_rs[J][q]); reta.push_back(s.r_eta[J][q]); }
+        }
+        for (int b = 0; b < nab; ++b) {
+            p.a_n[b] = s.a_n[b]; p.a_goff[b] = goff; p.a_poff[b] = (int)aeta.size(); p.a_rc[b] = s.a_rc[b];
+            goff += s.a_n[b];
+            if (s.a_n[b] && s.a_cut[b] != ACE_CUT_COS) { ctx->plain_ok = false; ctx->plain_why = "angular cutoff"; }
+            for (int q = 0; q < s.a_n[b]; ++q) {
+                aeta.push_back(s.a_eta[b][q]); azeta.push_back(s.a_zeta[b][q]); alamf.push_back(0.5 * s.a_lambda[b][q]);
+            }
+        }
+        for (int l = 0; l <= s.L && l <= ACE_MAXL; ++l) p.width[l] = s.width[l];
+        int cnt = 0;
+        for (int l = 0; l < s.L; ++l) { p.boff[l] = cnt; cnt += s.width[l + 1]; }
+        for (int l = 0; l < s.L; ++l) { p.woff[l] = cnt; cnt += s.width[l + 1] * s.width[l]; }
+        p.poff = (int)nn.size();
+        nn.insert(nn.end(), s.params.begin(), s.params.end());
+        p.in_off = (int)iw64.size();
+        iw64.insert(iw64.end(), s.inpw.begin(), s.inpw.end());
+        ib64.insert(ib64.end(), s.inpb.begin(), s.inpb.end());
+    }
+    if (upload(ctx, ctx->d_pmodel, &pm, 1)) return -1;
+    if (upload(ctx, ctx->d_r_rs64, rrs.data(), rrs.size())) return -1;
+    if (upload(ctx, ctx->d_r_eta64, reta.data(), reta.size())) return -1;
+    if (upload(ctx, ctx->d_a_eta64, aeta.data(), aeta.size())) return -1;
+    if (upload(ctx, ctx->d_a_zeta64, azeta.data(), azeta.size())) return -1;
+    if (upload(ctx, ctx->d_a_lamf64, alamf.data(), alamf.size())) return -1;
+    if (upload(ctx, ctx->d_nn64, nn.data(), nn.size())) return -1;
+    if (upload(ctx, ctx->d_inpw64, iw64.data(), iw64.size())) return -1;
+    if (upload(ctx, ctx->d_inpb64, ib64.data(), ib64.size())) return -1;
+    HC(hipStreamSynchronize(ctx->stream));   // host vectors above go out of scope
+    ctx->params_dirty = false;
+    return 0;
+}
+
+int refresh_params(ace_ctx* ctx) {
+    if (!ctx->params_dirty) return 0;
+    std::vector<float> prm, iw, ib;
+    std::vector<double> nn, iw64, ib64;
+    for (int I = 0; I < ctx->nt; ++I) {
+        const HSpecies& s = ctx->sp[I];
+        for (double v : s.params) { prm.push_back((float)v); nn.push_back(v); }
+        for (int k = 0; k < s.K; ++k) {
+            iw.push_back((float)s.inpw[k]); ib.push_back((float)s.inpb[k]);
+            iw64.push_back(s.inpw[k]); ib64.push_back(s.inpb[k]);
+        }
+    }
+    if (upload(ctx, ctx->d_params, prm.data(), prm.size())) return -1;
+    if (upload(ctx, ctx->d_inpw, iw.data(), iw.size())) return -1;
+    if (upload(ctx, ctx->d_inpb, ib.data(), ib.size())) return -1;
+    if (upload(ctx, ctx->d_nn64, nn.data(), nn.size())) return -1;
+    if (upload(ctx, ctx->d_inpw64, iw64.data(), iw64.size())) return -1;
+    if (upload(ctx, ctx->d_inpb64, ib64.data(), ib64.size())) return -1;
+    HC(hipStreamSynchronize(ctx->stream));
+    ctx->params_dirty = false;
+    return 0;
+}
+
+DevBasisParams basis_params(ace_ctx* ctx) {
+    DevBasisParams P;
+    P.r_rs = ctx->d_r_rs.p;
+    P.r_eta = ctx->d_r_eta.p;
+    P.a_lamf = ctx->d_a_lamf.p;
+    P.a_zeta = ctx->d_a_zeta.p;
+    P.a_zint = ctx->d_a_zint.p;
+    P.a_grp = ctx->d_a_grp.p;
+    return P;
+}
+
+// ---- basis kernel dispatch ------------------------------------------------------------------------
+
+typedef void (*BasisFn)(int, const DevModel*, DevBasisParams, const int*, const int*, const int*, const float4*,
+                        const int*, int, float*, const float*, float4*, int*, int, const double*);
+
+template <int A, int R>
+void pick_basis(BasisFn* f) {
+    f[0] = k_basis<A, R, 0>;
+    f[1] = k_basis<A, R, 1>;
+    f[2] = k_basis<A, R, 2>;
+}
+
+// register-array sizes: the largest angular / radial block, rounded up to 8, 16 or 32
+void basis_fns(ace_ctx* ctx, BasisFn* f) {
+    const int A = ctx->maxA <= 8 ? 8 : (ctx->maxA <= 16 ? 16 : 32);
+    const int R = ctx->maxR <= 8 ? 8 : (ctx->maxR <= 16 ? 16 : 32);
+    if (A == 8 && R == 8) pick_basis<8, 8>(f);
+    else if (A == 8 && R == 16) pick_basis<8, 16>(f);
+    else if (A == 16 && R == 8) pick_basis<16, 8>(f);
+    else if (A == 16 && R == 16) pick_basis<16, 16>(f);
+    else if (A == 32 && R == 16) pick_basis<32, 16>(f);
+    else if (A == 16 && R == 32) pick_basis<16, 32>(f);
+    else pick_basis<32, 32>(f);
+}
+
+int nb_slots(ace_ctx* ctx) { return std::max(32, (ctx->maxnb + 31) & ~31); }
+
+// mode 0: G -> out; mode 1: per-pair gradients from dE/dG; mode 2: tangent rows along eps -> out
+int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = nullptr) {
+    BasisFn fns[3];
+    basis_fns(ctx, fns);
+    const int nbm = nb_slots(ctx);
+    const int bytes = BW * basis_lds_wave_bytes(nbm, ctx->ngmax, ctx->kmax, mode);
+    if (bytes > 160 * 1024) return fail(ctx, "basis: neighbour list too long for LDS");
+    BasisFn f = fns[mode];
+    if (bytes > 64 * 1024) HC(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    DevBasisParams P = basis_params(ctx);
+    int* err = ctx->d_scal.p + 3;
+    float* rows = out ? out : ctx->d_G.p;
+    static const char* names[3] = {"basis_symm", "basis_adjoint", "basis_tangent"};
+    return timed(ctx, names[mode], [&] {
+        f<<<cdiv(ctx->N, BW), BW * 64, bytes, ctx->stream>>>(ctx->N, ctx->d_model.p, P, ctx->d_type.p, ctx->d_nb_off.p,
+                                                             ctx->d_nb_j.p, ctx->d_nb_d.p, ctx->d_row_of.p, ctx->ldg,
+                                                             rows, ctx->d_dEdG.p, ctx->d_pair_g.p, err, nbm, eps);
+    });
+}
+
+int run_mlp(ace_ctx* ctx) {
+    int lds = 0;
+    for (int I = 0; I < ctx->nt; ++I) lds = std::max(lds, mlp_lds_floats(ctx->hdm.sp[I], ctx->tm) * 4);
+    if (lds > 160 * 1024) return fail(ctx, "mlp: network too wide for LDS");
+    if (lds > 64 * 1024) HC(hipFuncSetAttribute((const void*)k_mlp_energy_grad, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    return timed(ctx, "mlp_energy_grad", [&] {
+        k_mlp_energy_grad<<<(int)ctx->tiles_h.size(), MLP_THREADS, lds, ctx->stream>>>(
+            ctx->d_model.p, ctx->d_params.p, ctx->d_inpw.p, ctx->d_inpb.p, ctx->d_tiles.p, ctx->ldg, ctx->d_G.p,
+            ctx->d_dEdG.p, ctx->d_atom_of_row.p, ctx->d_e_atom.p, ctx->tm);
+    });
+}
+
+int check_err_flag(ace_ctx* ctx, const char* what) {
+    int e = 0;
+    HC(hipMemcpyAsync(&e, ctx->d_scal.p + 3, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    if (e) return fail(ctx, std::string(what) + ": an atom has more neighbours than the LDS slots");
+    return 0;
+}
+
+int require_nb(ace_ctx* ctx) {
+    if (ctx->S == 0) return fail(ctx, "no structures uploaded");
+    if (!ctx->nb_valid) return fail(ctx, "neighbour list not built");
+    if (ctx->disp_dirty) {
+        if (timed(ctx, "nb_disp", [&] {
+                k_nb_disp<<<cdiv(ctx->npairs, 256), 256, 0, ctx->stream>>>(
+                    (int)ctx->npairs, ctx->d_nb_src.p, ctx->d_nb_j.p, ctx->d_nb_img.p, ctx->d_struc_of.p,
+                    ctx->d_sinfo.p, ctx->d_pos.p, ctx->d_nb_d.p);
+            }))
+            return -1;
+        ctx->disp_dirty = false;
+    }
+    return refresh_params(ctx);
+}
+
+}  // namespace
+
+#include "train_api.hip"
+
+// =================================================================================================
+// C-ABI
+// =================================================================================================
+extern "C" {
+
+int ace_create(ace_ctx** out, const ace_model_desc* model) {
+    if (!out) return -1;
+    *out = nullptr;
+    ace_ctx* ctx = new ace_ctx();
+    *out = ctx;
+    // host-side validation first: an invalid model is reported without touching the GPU
+    if (parse_model(ctx, model)) return -4;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n < 1) {
+        ctx->err = "ace_create: no HIP device";
+        return -2;
+    }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        ctx->err = "ace_create: stream creation failed";
+        return -3;
+    }
+    if (ensure(ctx, ctx->d_scal, 16)) return -5;
+    if (build_tables(ctx)) return -6;
+    return 0;
+}
+
+void ace_destroy(ace_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& r : ctx->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    release(ctx->d_model); release(ctx->d_r_rs); release(ctx->d_r_eta); release(ctx->d_a_lamf);
+    release(ctx->d_a_zeta); release(ctx->d_params); release(ctx->d_inpw); release(ctx->d_inpb);
+    release(ctx->d_a_zint); release(ctx->d_a_grp); release(ctx->d_pmodel); release(ctx->d_r_rs64);
+    release(ctx->d_r_eta64); release(ctx->d_a_eta64); release(ctx->d_a_zeta64); release(ctx->d_a_lamf64);
+    release(ctx->d_nn64); release(ctx->d_inpw64); release(ctx->d_inpb64);
+    release(ctx->d_type); release(ctx->d_struc_of); release(ctx->d_atom_off); release(ctx->d_natoms);
+    release(ctx->d_row_of); release(ctx->d_atom_of_row); release(ctx->d_pos); release(ctx->d_cell);
+    release(ctx->d_tiles); release(ctx->d_G); release(ctx->d_dEdG); release(ctx->d_e_atom);
+    release(ctx->d_vir_atom); release(ctx->d_F); release(ctx->d_E); release(ctx->d_vir);
+    release(ctx->d_sinfo); release(ctx->d_cell_of); release(ctx->d_wrap_of); release(ctx->d_cell_count);
+    release(ctx->d_cell_start); release(ctx->d_cell_cursor); release(ctx->d_cell_atoms); release(ctx->d_nb_count);
+    release(ctx->d_nb_off); release(ctx->d_scan_blk); release(ctx->d_scal); release(ctx->d_nb_j);
+    release(ctx->d_nb_img); release(ctx->d_nb_src); release(ctx->d_nb_rev); release(ctx->d_nb_d);
+    release(ctx->d_pair_g); release(ctx->d_G64);
+    train_release(ctx);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* ace_last_error(const ace_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ace_input_width(const ace_ctx* ctx, int type) {
+    if (!ctx || type < 0 || type >= ctx->nt) return -1;
+    return ctx->sp[type].K;
+}
+
+int ace_param_count(const ace_ctx* ctx, int type) {
+    if (!ctx || type < 0 || type >= ctx->nt) return -1;
+    return ctx->sp[type].nparams();
+}
+
+int ace_set_params(ace_ctx* ctx, int type, const double* params, int n) {
+    if (!ctx) return -1;
+    if (type < 0 || type >= ctx->nt) return fail(ctx, "ace_set_params: invalid type");
+    if (n != ctx->sp[type].nparams()) return fail(ctx, "ace_set_params: invalid size: vector and network mismatch");
+    ctx->sp[type].params.assign(params, params + n);
+    ctx->params_dirty = true;
+    return 0;
+}
+
+int ace_get_params(const ace_ctx* ctx, int type, double* params, int n) {
+    if (!ctx || type < 0 || type >= ctx->nt) return -1;
+    if (n != ctx->sp[type].nparams()) return -1;
+    std::copy(ctx->sp[type].params.begin(), ctx->sp[type].params.end(), params);
+    return 0;
+}
+
+int ace_set_input_scaling(ace_ctx* ctx, int type, const double* inpw, const double* inpb, int n) {
+    if (!ctx) return -1;
+    if (type < 0 || type >= ctx->nt) return fail(ctx, "ace_set_input_scaling: invalid type");
+    if (n != ctx->sp[type].K) return fail(ctx, "ace_set_input_scaling: size differs from the input width");
+    ctx->sp[type].inpw.assign(inpw, inpw + n);
+    ctx->sp[type].inpb.assign(inpb, inpb + n);
+    ctx->params_dirty = true;
+    return 0;
+}
+
+int ace_upload(ace_ctx* ctx, int n_struc, const int* natoms, const double* pos, const int* type,
+               const double* cell) {
+    if (!ctx) return -1;
+    if (n_struc < 1) return fail(ctx, "ace_upload: no structures");
+    long long N = 0;
+    for (int s = 0; s < n_struc; ++s) {
+        if (natoms[s] < 1) return fail(ctx, "ace_upload: a structure has no atoms");
+        N += natoms[s];
+    }
+    if (N > (1LL << 28)) return fail(ctx, "ace_upload: too many atoms");
+    for (long long i = 0; i < N; ++i)
+        if (type[i] < 0 || type[i] >= ctx->nt) return fail(ctx, "ace_upload: atom type out of range");
+    ctx->S = n_struc;
+    ctx->N = (int)N;
+    ctx->natoms.assign(natoms, natoms + n_struc);
+    ctx->atom_off.assign(n_struc + 1, 0);
+    for (int s = 0; s < n_struc; ++s) ctx->atom_off[s + 1] = ctx->atom_off[s] + natoms[s];
+    ctx->type_h.assign(type, type + N);
+    ctx->cell.assign(cell, cell + 9 * n_struc);
+    ctx->struc_of_h.resize(N);
+    for (int s = 0; s < n_struc; ++s)
+        for (int i = ctx->atom_off[s]; i < ctx->atom_off[s + 1]; ++i) ctx->struc_of_h[i] = s;
+    // species-sorted rows (stable), tiles of tm rows of one species
+    std::vector<int> cnt(ctx->nt, 0), start(ctx->nt + 1, 0);
+    for (long long i = 0; i < N; ++i) cnt[type[i]]++;
+    for (int t = 0; t < ctx->nt; ++t) start[t + 1] = start[t] + cnt[t];
+    ctx->row_of_h.resize(N);
+    std::vector<int> atom_of_row(N), fillp(start.begin(), start.end() - 1);
+    for (long long i = 0; i < N; ++i) {
+        const int r = fillp[type[i]]++;
+        ctx->row_of_h[i] = r;
+        atom_of_row[r] = (int)i;
+    }
+    int lds32 = 0;
+    for (int I = 0; I < ctx->nt; ++I) lds32 = std::max(lds32, mlp_lds_floats(ctx->hdm.sp[I], 32) * 4);
+    ctx->tm = (lds32 <= 64 * 1024) ? 32 : 16;
+    ctx->tiles_h.clear();
+    for (int t = 0; t < ctx->nt; ++t)
+        for (int r = start[t]; r < start[t + 1]; r += ctx->tm)
+            ctx->tiles_h.push_back(make_int4(t, r, std::min(ctx->tm, start[t + 1] - r), 0));
+    if (upload(ctx, ctx->d_type, type, N)) return -1;
+    if (upload(ctx, ctx->d_pos, pos, 3 * N)) return -1;
+    if (upload(ctx, ctx->d_struc_of, ctx->struc_of_h.data(), N)) return -1;
+    if (upload(ctx, ctx->d_atom_off, ctx->atom_off.data(), n_struc + 1)) return -1;
+    if (upload(ctx, ctx->d_natoms, natoms, n_struc)) return -1;
+    if (upload(ctx, ctx->d_row_of, ctx->row_of_h.data(), N)) return -1;
+    if (upload(ctx, ctx->d_atom_of_row, atom_of_row.data(), N)) return -1;
+    if (upload(ctx, ctx->d_cell, cell, 9 * n_struc)) return -1;
+    if (upload(ctx, ctx->d_tiles, ctx->tiles_h.data(), ctx->tiles_h.size())) return -1;
+    if (ensure(ctx, ctx->d_G, (size_t)N * ctx->ldg)) return -1;
+    if (ensure(ctx, ctx->d_dEdG, (size_t)N * ctx->ldg)) return -1;
+    HC(hipMemsetAsync(ctx->d_G.p, 0, (size_t)N * ctx->ldg * sizeof(float), ctx->stream));
+    HC(hipMemsetAsync(ctx->d_dEdG.p, 0, (size_t)N * ctx->ldg * sizeof(float), ctx->stream));
+    if (ensure(ctx, ctx->d_e_atom, N)) return -1;
+    if (ensure(ctx, ctx->d_vir_atom, 9 * N)) return -1;
+    if (ensure(ctx, ctx->d_F, 3 * N)) return -1;
+    if (ensure(ctx, ctx->d_E, n_struc)) return -1;
+    if (ensure(ctx, ctx->d_vir, 9 * n_struc)) return -1;
+    HC(hipStreamSynchronize(ctx->stream));
+    ctx->nb_valid = false;
+    train_on_upload(ctx);
+    return 0;
+}
+
+int ace_update_positions(ace_ctx* ctx, const double* pos) {
+    if (!ctx) return -1;
+    if (ctx->S == 0) return fail(ctx, "ace_update_positions: no structures uploaded");
+    HC(hipMemcpyAsync(ctx->d_pos.p, pos, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    ctx->disp_dirty = true;
+    ctx->tr.g_valid = false;
+    return 0;
+}
+
+int ace_build_neighbors(ace_ctx* ctx, double skin) {
+    if (!ctx) return -1;
+    if (ctx->S == 0) return fail(ctx, "ace_build_neighbors: no structures uploaded");
+    if (skin < 0) return fail(ctx, "ace_build_neighbors: negative skin");
+    const double rcl = ctx->rc + skin;
+    const double rsearch = rcl * (1.0 + 1e-9) + 1e-9;
+    ctx->nb_rc = rcl;
+    ctx->nb_valid = false;
+    ctx->tr.g_valid = false;
+    ctx->sinfo_h.resize(ctx->S);
+    int cell_off = 0;
+    for (int s = 0; s < ctx->S; ++s) {
+        StrucInfo& I = ctx->sinfo_h[s];
+        memset(&I, 0, sizeof(I));
+        const double* R = &ctx->cell[9 * s];
+        for (int q = 0; q < 9; ++q) I.R[q] = R[q];
+        const double det = R[0] * (R[4] * R[8] - R[7] * R[5]) - R[3] * (R[1] * R[8] - R[7] * R[2]) +
+                           R[6] * (R[1] * R[5] - R[4] * R[2]);
+        double fro = 0;
+        for (int q = 0; q < 9; ++q) fro += R[q] * R[q];
+        if (!(std::fabs(det) > 1e-9) || !(std::sqrt(fro) > 1e-12))
+            return fail(ctx, "ace_build_neighbors: singular cell (non-periodic structures are not supported)");
+        // inverse, column-major: Rinv(r,c)
+        const double id = 1.0 / det;
+        double inv[3][3];
+        const double m[3][3] = {{R[0], R[3], R[6]}, {R[1], R[4], R[7]}, {R[2], R[5], R[8]}};
+        inv[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) * id;
+        inv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
+        inv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
+        inv[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * id;
+        inv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id;
+        inv[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id;
+        inv[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * id;
+        inv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * id;
+        inv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) I.Rinv[r + 3 * c] = inv[r][c];
+        long long ncell = 1;
+        for (int d = 0; d < 3; ++d) {
+            const double coln = std::sqrt(R[3 * d] * R[3 * d] + R[3 * d + 1] * R[3 * d + 1] + R[3 * d + 2] * R[3 * d + 2]);
+            I.cf[d] = (int)std::ceil(3.0 * rcl / coln);                       // neighbor.cpp:49-53
+            const double rown = std::sqrt(inv[d][0] * inv[d][0] + inv[d][1] * inv[d][1] + inv[d][2] * inv[d][2]);
+            const double h = 1.0 / rown;                                      // perpendicular height
+            int nc = (int)std::floor(h / rsearch);
+            nc = std::max(1, std::min(nc, 1024));
+            I.nc[d] = nc;
+            I.st[d] = std::max(1, (int)std::ceil(rsearch * nc / h));
+            if (I.cf[d] > 511) return fail(ctx, "ace_build_neighbors: image range too large for this cell");
+            ncell *= nc;
+        }
+        I.atom_off = ctx->atom_off[s];
+        I.natoms = ctx->natoms[s];
+        I.cell_off = cell_off;
+        if (cell_off + ncell > (1LL << 30)) return fail(ctx, "ace_build_neighbors: too many cells");
+        cell_off += (int)ncell;
+    }
+    ctx->ncells = cell_off;
+    const int N = ctx->N, nc = ctx->ncells;
+    if (upload(ctx, ctx->d_sinfo, ctx->sinfo_h.data(), ctx->S)) return -1;
+    if (ensure(ctx, ctx->d_cell_of, N) || ensure(ctx, ctx->d_wrap_of, N) || ensure(ctx, ctx->d_cell_count, nc + 1) ||
+        ensure(ctx, ctx->d_cell_start, nc + 1) || ensure(ctx, ctx->d_cell_cursor, nc + 1) ||
+        ensure(ctx, ctx->d_cell_atoms, N) || ensure(ctx, ctx->d_nb_count, N + 1) || ensure(ctx, ctx->d_nb_off, N + 1))
+        return -1;
+    HC(hipMemsetAsync(ctx->d_cell_count.p, 0, (nc + 1) * sizeof(int), ctx->stream));
+    HC(hipMemsetAsync(ctx->d_cell_cursor.p, 0, (nc + 1) * sizeof(int), ctx->stream));
+    HC(hipMemsetAsync(ctx->d_scal.p, 0, 16 * sizeof(int), ctx->stream));
+    if (timed(ctx, "nb_assign", [&] {
+            k_nb_assign<<<cdiv(N, 256), 256, 0, ctx->stream>>>(N, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p,
+                                                                ctx->d_cell_of.p, ctx->d_wrap_of.p, ctx->d_cell_count.p);
+        }))
+        return -1;
+    if (exclusive_scan(ctx, ctx->d_cell_count.p, ctx->d_cell_start.p, nc)) return -1;
+    if (timed(ctx, "nb_scatter", [&] {
+            k_nb_scatter<<<cdiv(N, 256), 256, 0, ctx->stream>>>(N, ctx->d_cell_of.p, ctx->d_cell_start.p,
+                                                                 ctx->d_cell_cursor.p, ctx->d_cell_atoms.p);
+        }))
+        return -1;
+    const double rc2 = rcl * rcl;
+    if (timed(ctx, "nb_count", [&] {
+            k_nb_search<0><<<cdiv(N, NB_WAVES), NB_WAVES * 64, 0, ctx->stream>>>(
+                N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
+                ctx->d_cell_start.p, ctx->d_cell_atoms.p, ctx->d_nb_count.p, ctx->d_scal.p, nullptr, nullptr, nullptr,
+                nullptr);
+        }))
+        return -1;
+    if (exclusive_scan(ctx, ctx->d_nb_count.p, ctx->d_nb_off.p, N)) return -1;
+    int h[2] = {0, 0};
+    HC(hipMemcpyAsync(&h[0], ctx->d_scal.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipMemcpyAsync(&h[1], ctx->d_nb_off.p + N, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    if (h[0] > NB_LMAX) return fail(ctx, "ace_build_neighbors: an atom has more than 1024 neighbours");
+    ctx->maxnb = h[0];
+    ctx->npairs = h[1];
+    const long long P = ctx->npairs;
+    if (ensure(ctx, ctx->d_nb_j, P) || ensure(ctx, ctx->d_nb_img, P) || ensure(ctx, ctx->d_nb_src, P) ||
+        ensure(ctx, ctx->d_nb_rev, P) || ensure(ctx, ctx->d_nb_d, P) || ensure(ctx, ctx->d_pair_g, P))
+        return -1;
+    if (timed(ctx, "nb_fill", [&] {
+            k_nb_search<1><<<cdiv(N, NB_WAVES), NB_WAVES * 64, 0, ctx->stream>>>(
+                N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
+                ctx->d_cell_start.p, ctx->d_cell_atoms.p, nullptr, nullptr, ctx->d_nb_off.p, ctx->d_nb_j.p,
+                ctx->d_nb_img.p, ctx->d_nb_d.p);
+        }))
+        return -1;
+    if (timed(ctx, "nb_src", [&] {
+            k_nb_src<<<cdiv(N, 256), 256, 0, ctx->stream>>>(N, ctx->d_nb_off.p, ctx->d_nb_src.p);
+        }))
+        return -1;
+    if (P > 0 && timed(ctx, "nb_reverse", [&] {
+            k_nb_reverse<<<cdiv(P, 256), 256, 0, ctx->stream>>>((int)P, ctx->d_nb_src.p, ctx->d_nb_j.p, ctx->d_nb_img.p,
+                                                                 ctx->d_nb_off.p, ctx->d_struc_of.p, ctx->d_sinfo.p,
+                                                                 ctx->d_nb_rev.p, ctx->d_scal.p + 2);
+        }))
+        return -1;
+    int rev_err = 0;
+    HC(hipMemcpyAsync(&rev_err, ctx->d_scal.p + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    if (rev_err) return fail(ctx, "ace_build_neighbors: neighbour list is not symmetric");
+    ctx->nb_valid = true;
+    ctx->disp_dirty = false;
+    return flush_prof(ctx);
+}
+
+int ace_neighbor_count(const ace_ctx* ctx, long long* npairs) {
+    if (!ctx || !npairs || !ctx->nb_valid) return -1;
+    *npairs = ctx->npairs;
+    return 0;
+}
+
+int ace_export_neighbors(ace_ctx* ctx, long long* offsets, int* j, int* img) {
+    if (!ctx) return -1;
+    if (!ctx->nb_valid) return fail(ctx, "ace_export_neighbors: neighbour list not built");
+    const int N = ctx->N;
+    const long long P = ctx->npairs;
+    std::vector<int> off(N + 1), jj(P), im(P);
+    HC(hipMemcpyAsync(off.data(), ctx->d_nb_off.p, (N + 1) * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (P) {
+        HC(hipMemcpyAsync(jj.data(), ctx->d_nb_j.p, P * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HC(hipMemcpyAsync(im.data(), ctx->d_nb_img.p, P * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HC(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i <= N; ++i) offsets[i] = off[i];
+    for (int i = 0; i < N; ++i) {
+        const int a0 = ctx->atom_off[ctx->struc_of_h[i]];
+        for (int p = off[i]; p < off[i + 1]; ++p) {
+            j[p] = jj[p] - a0;
+            img_unpack(im[p], img[3 * p], img[3 * p + 1], img[3 * p + 2]);
+        }
+    }
+    return 0;
+}
+
+int ace_symm(ace_ctx* ctx, double* G, int ldg) {
+    if (!ctx) return -1;
+    if (require_nb(ctx)) return -1;
+    HC(hipMemsetAsync(ctx->d_scal.p + 3, 0, sizeof(int), ctx->stream));
+    if (run_basis(ctx, 0)) return -1;
+    if (check_err_flag(ctx, "ace_symm")) return -1;
+    if (G) {
+        if (ldg < ctx->kmax) return fail(ctx, "ace_symm: ldg smaller than the widest input vector");
+        DBuf<double> tmp;
+        if (ensure(ctx, tmp, (size_t)ctx->N * ldg)) return -1;
+        dim3 grid(cdiv(ldg, 128), ctx->N);
+        k_export_rows<<<grid, 128, 0, ctx->stream>>>(ctx->N, ctx->kmax, ctx->d_row_of.p, ctx->ldg, ctx->d_G.p, ldg,
+                                                    tmp.p);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(G, tmp.p, (size_t)ctx->N * ldg * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        release(tmp);
+        if (e != hipSuccess) return fail_hip(ctx, "ace_symm export", e);
+        // rows of species with fewer inputs than kmax: zero the tail
+        for (int i = 0; i < ctx->N; ++i)
+            for (int k = ctx->sp[ctx->type_h[i]].K; k < ldg; ++k) G[(long long)i * ldg + k] = 0.0;
+    }
+    return flush_prof(ctx);
+}
+
+int ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, double* virial) {
+    if (!ctx) return -1;
+    if (precision != ACE_PREC_FP32) return fail(ctx, "ace_energy_forces: precision mode not available");
+    if (require_nb(ctx)) return -1;
+    HC(hipMemsetAsync(ctx->d_scal.p + 3, 0, sizeof(int), ctx->stream));
+    if (run_basis(ctx, 0)) return -1;
+    if (run_mlp(ctx)) return -1;
+    if (run_basis(ctx, 1)) return -1;
+    const int N = ctx->N;
+    if (timed(ctx, "force_gather", [&] {
+            k_force_gather<<<cdiv(N, 256), 256, 0, ctx->stream>>>(N, ctx->d_nb_off.p, ctx->d_nb_rev.p, ctx->d_nb_d.p,
+                                                                   ctx->d_pair_g.p, ctx->d_F.p, ctx->d_vir_atom.p);
+        }))
+        return -1;
+    if (timed(ctx, "struct_reduce", [&] {
+            k_struct_reduce<<<ctx->S, RED_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p, ctx->d_e_atom.p,
+                                                                     ctx->d_vir_atom.p, ctx->d_E.p, ctx->d_vir.p);
+        }))
+        return -1;
+    if (E_struc) HC(hipMemcpyAsync(E_struc, ctx->d_E.p, ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (F) HC(hipMemcpyAsync(F, ctx->d_F.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (virial) HC(hipMemcpyAsync(virial, ctx->d_vir.p, 9 * ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (E_struc || F || virial || ctx->prof) {
+        if (check_err_flag(ctx, "ace_energy_forces")) return -1;
+    }
+    return flush_prof(ctx);
+}
+
+int ace_plain_energy_forces(ace_ctx* ctx, double* E_struc, double* F) {
+    if (!ctx) return -1;
+    if (!ctx->plain_ok) return fail(ctx, "ace_plain_energy_forces: model outside the port's forms (" + ctx->plain_why + ")");
+    if (require_nb(ctx)) return -1;
+    int nmax = 0;
+    for (int s = 0; s < ctx->S; ++s) nmax = std::max(nmax, ctx->natoms[s]);
+    if (ensure(ctx, ctx->d_G64, (size_t)nmax * PL_KMAX)) return -1;
+    PlainParams P;
+    P.r_rs = ctx->d_r_rs64.p; P.r_eta = ctx->d_r_eta64.p; P.a_eta = ctx->d_a_eta64.p; P.a_zeta = ctx->d_a_zeta64.p;
+    P.a_lamf = ctx->d_a_lamf64.p; P.nn = ctx->d_nn64.p; P.inpw = ctx->d_inpw64.p; P.inpb = ctx->d_inpb64.p;
+    HC(hipMemsetAsync(ctx->d_F.p, 0, 3 * (size_t)ctx->N * sizeof(double), ctx->stream));
+    HC(hipMemsetAsync(ctx->d_E.p, 0, ctx->S * sizeof(double), ctx->stream));
+    for (int s = 0; s < ctx->S; ++s) {
+        const int n = ctx->natoms[s], a0 = ctx->atom_off[s];
+        const double* R = ctx->d_cell.p + 9 * s;
+        if (timed(ctx, "plain_symm", [&] {
+                k_plain_symm<<<cdiv(n, 64), 64, 0, ctx->stream>>>(n, a0, ctx->d_pmodel.p, P, ctx->d_pos.p, ctx->d_type.p, R,
+                                                                   ctx->d_nb_off.p, ctx->d_nb_j.p, ctx->d_nb_img.p,
+                                                                   ctx->d_G64.p);
+            }))
+            return -1;
+        if (timed(ctx, "plain_force", [&] {
+                k_plain_force<<<cdiv(n, 64), 64, 0, ctx->stream>>>(n, a0, ctx->d_pmodel.p, P, ctx->d_pos.p, ctx->d_type.p, R,
+                                                                    ctx->d_nb_off.p, ctx->d_nb_j.p, ctx->d_nb_img.p,
+                                                                    ctx->d_G64.p, ctx->d_F.p, ctx->d_E.p + s);
+            }))
+            return -1;
+    }
+    if (E_struc) HC(hipMemcpyAsync(E_struc, ctx->d_E.p, ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (F) HC(hipMemcpyAsync(F, ctx->d_F.p, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    return flush_prof(ctx);
+}
+
+int ace_profile_enable(ace_ctx* ctx, int on) {
+    if (!ctx) return -1;
+    if (flush_prof(ctx)) return -1;
+    ctx->prof = on != 0;
+    if (on) ctx->acc.clear();
+    return 0;
+}
+
+int ace_profile_read(const ace_ctx* cctx, const char* kernel, double* total_ms, long long* launches) {
+    ace_ctx* ctx = const_cast<ace_ctx*>(cctx);
+    if (!ctx) return -1;
+    if (flush_prof(ctx)) return -1;
+    auto it = ctx->acc.find(kernel ? kernel : "");
+    if (it == ctx->acc.end()) {
+        if (total_ms) *total_ms = 0;
+        if (launches) *launches = 0;
+        return 0;
+    }
+    if (total_ms) *total_ms = it->second.first;
+    if (launches) *launches = it->second.second;
+    return 0;
+}
+
+int ace_sync(ace_ctx* ctx) {
+    if (!ctx) return -1;
+    HC(hipStreamSynchronize(ctx->stream));
+    return flush_prof(ctx);
+}
+
+}  // extern "C"

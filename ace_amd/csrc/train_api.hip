@@ -1,0 +1,162 @@
+// train_api.hip -- host side of the batched training step (included by api.hip after the helpers).
+namespace {
+
+void train_release(ace_ctx* ctx) {
+    TrainState& T = ctx->tr;
+    release(T.d_tiles); release(T.d_Eref); release(T.d_Fref); release(T.d_gfac); release(T.d_eps);
+    release(T.d_out); release(T.d_grad); release(T.d_V); release(T.d_part);
+}
+
+void train_on_upload(ace_ctx* ctx) {
+    ctx->tr.have_targets = false;
+    ctx->tr.have_forces = false;
+    ctx->tr.g_valid = false;
+    ctx->tr.tiles.clear();
+}
+
+// training tiles: the largest row count whose LDS fits 64 KB (at least 8 rows)
+int train_tiles(ace_ctx* ctx) {
+    TrainState& T = ctx->tr;
+    if (!T.tiles.empty()) return 0;
+    int tm = 32;
+    for (; tm > 8; tm >>= 1) {
+        int lds = 0;
+        for (int I = 0; I < ctx->nt; ++I) lds = std::max(lds, train_lds_floats(ctx->hdm.sp[I], tm) * 4);
+        if (lds <= 64 * 1024) break;
+    }
+    T.tm = tm;
+    std::vector<int> cnt(ctx->nt, 0);
+    for (int t : ctx->type_h) cnt[t]++;
+    T.tile_first.assign(ctx->nt + 1, 0);
+    int r = 0;
+    for (int I = 0; I < ctx->nt; ++I) {
+        T.tile_first[I] = (int)T.tiles.size();
+        for (int q = 0; q < cnt[I]; q += tm) T.tiles.push_back(make_int4(I, r + q, std::min(tm, cnt[I] - q), 0));
+        r += cnt[I];
+    }
+    T.tile_first[ctx->nt] = (int)T.tiles.size();
+    int ldp = 0;
+    for (int I = 0; I < ctx->nt; ++I) ldp = std::max(ldp, ctx->sp[I].nparams());
+    T.ldpart = ldp;
+    if (upload(ctx, T.d_tiles, T.tiles.data(), T.tiles.size())) return -1;
+    if (ensure(ctx, T.d_part, (size_t)T.tiles.size() * ldp)) return -1;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ace_set_targets(ace_ctx* ctx, const double* E_ref, const double* F_ref) {
+    if (!ctx) return -1;
+    if (ctx->S == 0) return fail(ctx, "ace_set_targets: no structures uploaded");
+    if (!E_ref) return fail(ctx, "ace_set_targets: E_ref is required");
+    TrainState& T = ctx->tr;
+    if (upload(ctx, T.d_Eref, E_ref, ctx->S)) return -1;
+    T.have_forces = false;
+    if (F_ref) {
+        if (upload(ctx, T.d_Fref, F_ref, 3 * (size_t)ctx->N)) return -1;
+        T.have_forces = true;
+    }
+    HC(hipStreamSynchronize(ctx->stream));
+    T.have_targets = true;
+    return 0;
+}
+
+int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* grad) {
+    if (!ctx) return -1;
+    if (!Ld) return fail(ctx, "ace_train_step: null loss description");
+    TrainState& T = ctx->tr;
+    if (!T.have_targets) return fail(ctx, "ace_train_step: targets not set");
+    const bool wf = Ld->w_force != 0.0;
+    if (wf && !T.have_forces) return fail(ctx, "ace_train_step: force term requested without reference forces");
+    if (Ld->loss < ACE_LOSS_MSE || Ld->loss > ACE_LOSS_ASINH) return fail(ctx, "ace_train_step: invalid loss");
+    if (Ld->norm < ACE_NORM_IDENTITY || Ld->norm > ACE_NORM_LOG) return fail(ctx, "Invalid normalization method.");
+    if ((Ld->loss == ACE_LOSS_HUBER || Ld->loss == ACE_LOSS_ASINH) && !(Ld->delta > 0))
+        return fail(ctx, "ace_train_step: delta must be positive");
+    if (require_nb(ctx)) return -1;
+    if (train_tiles(ctx)) return -1;
+    const int N = ctx->N, S = ctx->S;
+    if (ensure(ctx, T.d_gfac, S) || ensure(ctx, T.d_eps, 3 * (size_t)N) || ensure(ctx, T.d_out, 3 * (size_t)S))
+        return -1;
+    HC(hipMemsetAsync(ctx->d_scal.p + 3, 0, sizeof(int), ctx->stream));
+    // G depends on positions only: computed once per geometry (the reference precomputes it too)
+    if (!T.g_valid) {
+        if (run_basis(ctx, 0)) return -1;
+        T.g_valid = true;
+    }
+    if (run_mlp(ctx)) return -1;
+    if (wf) {
+        if (run_basis(ctx, 1)) return -1;
+        if (timed(ctx, "force_gather", [&] {
+                k_force_gather<<<cdiv(N, 256), 256, 0, ctx->stream>>>(N, ctx->d_nb_off.p, ctx->d_nb_rev.p, ctx->d_nb_d.p,
+                                                                       ctx->d_pair_g.p, ctx->d_F.p, nullptr);
+            }))
+            return -1;
+    }
+    if (timed(ctx, "struct_reduce", [&] {
+            k_struct_reduce<<<S, RED_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p, ctx->d_e_atom.p,
+                                                                nullptr, ctx->d_E.p, nullptr);
+        }))
+        return -1;
+    const ace_loss_desc L = *Ld;
+    if (timed(ctx, "loss", [&] {
+            k_loss<<<S, LOSS_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p, ctx->d_E.p, T.d_Eref.p,
+                                                        ctx->d_F.p, wf ? T.d_Fref.p : nullptr, L, T.d_gfac.p,
+                                                        T.d_eps.p, T.d_out.p);
+        }))
+        return -1;
+    if (wf) {
+        if (ensure(ctx, T.d_V, (size_t)N * ctx->ldg)) return -1;
+        if (run_basis(ctx, 2, T.d_V.p, T.d_eps.p)) return -1;
+    }
+    if (grad) {
+        int lds = 0;
+        for (int I = 0; I < ctx->nt; ++I) lds = std::max(lds, train_lds_floats(ctx->hdm.sp[I], T.tm) * 4);
+        if (timed(ctx, "mlp_train", [&] {
+                k_mlp_train<<<(int)T.tiles.size(), 256, lds, ctx->stream>>>(
+                    ctx->d_model.p, ctx->d_params.p, ctx->d_inpw.p, ctx->d_inpb.p, T.d_tiles.p, ctx->ldg, ctx->d_G.p,
+                    wf ? T.d_V.p : nullptr, ctx->d_atom_of_row.p, ctx->d_struc_of.p, T.d_gfac.p, T.tm, T.ldpart,
+                    T.d_part.p);
+            }))
+            return -1;
+        size_t ntot = 0;
+        for (int I = 0; I < ctx->nt; ++I) ntot += ctx->sp[I].nparams();
+        if (ensure(ctx, T.d_grad, ntot)) return -1;
+        size_t off = 0;
+        for (int I = 0; I < ctx->nt; ++I) {
+            const int np = ctx->sp[I].nparams();
+            const int t0 = T.tile_first[I], t1 = T.tile_first[I + 1];
+            double* gout = T.d_grad.p + off;
+            if (t1 > t0) {
+                if (timed(ctx, "grad_reduce", [&] {
+                        k_grad_reduce<<<cdiv(np, 256), 256, 0, ctx->stream>>>(np, t0, t1, T.ldpart, T.d_part.p, gout);
+                    }))
+                    return -1;
+            } else {
+                HC(hipMemsetAsync(gout, 0, np * sizeof(double), ctx->stream));
+            }
+            off += np;
+        }
+        HC(hipMemcpyAsync(grad, T.d_grad.p, ntot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    std::vector<double> out(3 * (size_t)S);
+    HC(hipMemcpyAsync(out.data(), T.d_out.p, 3 * (size_t)S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    if (check_err_flag(ctx, "ace_train_step")) return -1;
+    double le = 0, lf = 0, rm = 0;
+    for (int s = 0; s < S; ++s) {
+        le += out[3 * s];
+        lf += out[3 * s + 1];
+        rm += out[3 * s + 2];
+    }
+    if (loss) {
+        loss[0] = le + lf;
+        loss[1] = le;
+        loss[2] = lf;
+        loss[3] = rm;
+    }
+    return flush_prof(ctx);
+}
+
+}  // extern "C"

@@ -1,0 +1,191 @@
+"""Benchmark: batched E+F throughput (atoms/s) of the Behler-Parrinello hot path on one MI355X.
+
+Workload (one step = NNP::compute of the whole batch: neighbour-list build, symmetry functions,
+networks, adjoint forces, FP64 energy/virial reductions): 2 elements (50/50), 1000 atoms on a
+jittered (0.15 A) bcc lattice at 0.085 atoms/A^3, periodic cell, cutoff 6.0 A, 16 G2 per neighbour
+species + 16 G4 per species pair = 80 inputs, 80-32-32-1 tanh networks, FP32 arithmetic with FP64
+reductions.  Inputs are resident in HBM before the timed region.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atoms/s E+F eval and train structs/s: batched adjoint vs plain FP64 GPU"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def work_counts(model, strucs, off, j, typ):
+    """Pairs P, triplets T, radial evaluations E_R and angular evaluations E_A of one E+F pass,
+    counted from the exported neighbour list (skin 0: every listed pair is inside rc)."""
+    from ace_amd.model import tri_index
+    nt = model.ntypes
+    base = np.repeat(np.concatenate([[0], np.cumsum([s.natoms for s in strucs])[:-1]]),
+                     [s.natoms for s in strucs])
+    P = T = ER = EA = 0
+    for i in range(len(typ)):
+        I = typ[i]
+        n = int(off[i + 1] - off[i])
+        cnt = np.bincount(typ[base[i] + j[off[i]:off[i + 1]]], minlength=nt)
+        P += n
+        T += n * (n - 1) // 2
+        ER += sum(int(cnt[J]) * len(model.radial[I][J].rs) for J in range(nt))
+        for J in range(nt):
+            for K in range(J, nt):
+                tr = cnt[J] * (cnt[J] - 1) // 2 if J == K else cnt[J] * cnt[K]
+                EA += int(tr) * len(model.angular[I][tri_index(J, K)].eta)
+    return P, T, ER, EA
+
+
+def mlp_flops(model, typ):
+    f = 0
+    for I in range(model.ntypes):
+        w = model.nets[I].widths
+        f += 4 * np.count_nonzero(typ == I) * sum(w[l] * w[l + 1] for l in range(len(w) - 1))
+    return f
+
+
+def cpu_baseline(model, s, samples):
+    """The FP64 oracle (reference loop structure, brute-force neighbour list) on one host thread."""
+    from oracle import oracle as O
+    orc = O.Oracle(model)
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(samples):
+        out = orc.compute(s)
+    dt = (time.perf_counter() - t0) / samples
+    return s.natoms / dt, out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-samples", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
+              "'How the path would shard')", file=sys.stderr)
+        sys.exit(2)
+
+    from ace_amd import NNPDevice, configs
+
+    model, [s] = configs.config2()
+    dev = NNPDevice(model)
+    dev.upload([s])
+    dev.build_neighbors()
+    G = dev.compute_symm()
+    configs.set_input_scaling(model, [s], G)       # PreBias MEAN / PreScale DEV from the GPU's G
+    dev.sync_model_scaling()
+
+    def step():
+        dev.build_neighbors()
+        dev.compute_device()
+
+    for _ in range(args.warmup):
+        step()
+    dev.sync()
+    dev.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dev.sync()
+    t1 = time.perf_counter()
+    kern = {}
+    for k in ("nb_assign", "nb_count", "nb_fill", "nb_reverse", "basis_symm", "mlp_energy_grad",
+              "basis_adjoint", "force_gather", "struct_reduce"):
+        kern[k] = dev.profile_read(k)
+    dev.profile(False)
+    ms = (t1 - t0) / args.steps * 1e3
+    value = s.natoms / (ms * 1e-3)
+
+    # same K steps without per-kernel events (event overhead)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dev.sync()
+    ms_noev = (time.perf_counter() - t0) / args.steps * 1e3
+
+    # algorithmic work of the dominant kernel
+    off, j, img = dev.neighbors()
+    P, T, ER, EA = work_counts(model, [s], off, j, s.type)
+    flops = {
+        "basis_symm": 10 * P + 6 * ER + 15 * T + 7 * EA,
+        "basis_adjoint": 22 * P + 6 * ER + 45 * T + 7 * EA,
+        "mlp_energy_grad": mlp_flops(model, s.type),
+    }
+    avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kern.items()}
+    dom = max(("basis_symm", "basis_adjoint"), key=lambda k: avg[k])
+    achieved = flops[dom] / (avg[dom] * 1e-3) / 1e12
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"{dom}_traffic.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+
+    # plain unbatched FP64 GPU port (the 5x denominator), same step shape
+    for _ in range(3):
+        dev.build_neighbors()
+        dev.plain_compute()
+    nplain = max(3, args.steps // 20)
+    t0 = time.perf_counter()
+    for _ in range(nplain):
+        dev.build_neighbors()
+        dev.plain_compute()
+    plain_ms = (time.perf_counter() - t0) / nplain * 1e3
+    plain_value = s.natoms / (plain_ms * 1e-3)
+
+    E, F, _ = dev.compute()
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atoms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded jittered bcc, random-init networks)",
+        "config": {"workload": "bcc 1000 atoms, 2 elements 50/50, rc 6.0 A, 80 inputs (16 G2 + 16 G4 per "
+                               "species pair), 80-32-32-1 tanh; step = neighbour build + E + F + virial",
+                   "global_batch": 1, "atoms": int(s.natoms), "pairs": int(P), "triplets": int(T)},
+        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "flops_per_launch": int(flops[dom]), "avg_launch_ms": avg[dom],
+                     "note": "FP32 vector-ALU kernel priced against the FP32 dense peak"},
+        "kernels_avg_us": {k: round(v * 1e3, 2) for k, v in avg.items()},
+        "ms_per_step_without_events": ms_noev,
+        "plain_fp64_port": {"value": plain_value, "unit": "atoms/s", "ms_per_step": plain_ms,
+                            "speedup_batched_vs_plain": value / plain_value},
+    }
+    if not args.no_cpu:
+        cpu_value, (E_ref, F_ref, _) = cpu_baseline(model, s, args.cpu_samples)
+        res["cpu_baseline"] = {"value": cpu_value, "unit": "atoms/s", "cores": 1, "kind": "port",
+                               "sample": f"{args.cpu_samples} full E+F evaluations of the same 1000-atom structure "
+                                         "(brute-force neighbour list, compute_symm, compute_force), FP64, 1 thread"}
+        res["speedup_vs_cpu"] = value / cpu_value
+        res["error_vs_fp64_oracle"] = {"energy_per_atom": abs(E[0] - E_ref) / s.natoms,
+                                       "force_max_abs": float(np.max(np.abs(F - F_ref)))}
+    dev.close()
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

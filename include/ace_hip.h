@@ -1,0 +1,179 @@
+/*
+ * ace_hip.h -- C-ABI of the MI355X Behler-Parrinello energy/force/training path.
+ *
+ * Every entry point is plain C: borrowed host pointers, sizes, an int status
+ * (0 = success, < 0 = failure; the message is in ace_last_error()).  No C++
+ * exception, torch type or HIP type crosses this boundary.
+ *
+ * Reference interfaces replaced (paths relative to the ace source tree):
+ *   ace_model_desc / ace_create ... NNP + NNH + BasisR/BasisA + NN::ANN set-up
+ *                                   (src/nnp/nnp.hpp:64-121, src/nnp/nnh.hpp:46-155,
+ *                                    src/nnp/basis_radial.hpp:23-89,
+ *                                    src/nnp/basis_angular.hpp:23-103, src/ml/nn.hpp:243-369)
+ *   ace_set_params / ace_get_params NN::ANN operator<< / operator>> (src/ml/nn.cpp:881-917)
+ *   ace_upload ..................... Structure positions/types/cell (src/struc/structure.hpp:147-164)
+ *   ace_build_neighbors ............ NeighborList::build (src/struc/neighbor.cpp:46-87)
+ *   ace_export_neighbors ........... NeighborList::index/img accessors (src/struc/neighbor.hpp:34-40)
+ *   ace_symm ....................... NNP::compute_symm (src/nnp/nnp.cpp:350-392)
+ *   ace_energy_forces .............. NNP::compute / compute_force / compute_energy
+ *                                   (src/nnp/nnp.cpp:329-338, 412-494)
+ *   ace_train_step ................. NNPTEFR::error_cost_o1 (src/nnp/nnptefr.cpp:1292-1425)
+ *                                   extended with the force-matching term d(F)/dW
+ *                                   (network factor NN::D2ODZDI, src/ml/nn.cpp:2047-2140)
+ *   ace_plain_energy_forces ........ same as ace_energy_forces, unbatched FP64 one-thread-per-atom
+ *                                   GPU port (the baseline the batched path is measured against)
+ */
+#ifndef ACE_HIP_H
+#define ACE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ACE_MAX_TYPES 8
+#define ACE_MAX_LAYERS 6   /* weight layers per network, output layer included */
+
+/* enum ids follow the reference's enum orders */
+enum { ACE_CUT_NONE = 0, ACE_CUT_STEP = 1, ACE_CUT_COS = 2, ACE_CUT_TANH = 3, ACE_CUT_POLY3 = 4 };
+enum { ACE_RAD_NONE = 0, ACE_RAD_GAUSSIAN = 1, ACE_RAD_SECH = 2, ACE_RAD_LOGISTIC = 3,
+       ACE_RAD_TANH = 4, ACE_RAD_LOGCOSH = 5, ACE_RAD_LOGCOSH2 = 6 };
+enum { ACE_ANG_NONE = 0, ACE_ANG_GAUSS = 1, ACE_ANG_GAUSS2 = 2, ACE_ANG_SECH = 3,
+       ACE_ANG_STUDENT3 = 4, ACE_ANG_STUDENT4 = 5, ACE_ANG_STUDENT5 = 6 };
+enum { ACE_ACT_LINEAR = 1, ACE_ACT_SIGMOID = 2, ACE_ACT_TANH = 3, ACE_ACT_SWISH = 10,
+       ACE_ACT_SOFTPLUS = 15 };
+enum { ACE_PREC_FP32 = 0, ACE_PREC_BF16L1 = 1 };
+enum { ACE_LOSS_MSE = 0, ACE_LOSS_MAE = 1, ACE_LOSS_HUBER = 2, ACE_LOSS_ASINH = 3 };
+enum { ACE_NORM_IDENTITY = 0, ACE_NORM_LINEAR = 1, ACE_NORM_SQRT = 2, ACE_NORM_CBRT = 3,
+       ACE_NORM_LOG = 4 };
+
+/*
+ * Model description: the content of an NNP (one NNH per species).
+ *   radial block   (I,J)   -> index I*ntypes + J             (NNH::basisR(J))
+ *   angular block  (I,J,K) -> index I*nab + b, nab = ntypes*(ntypes+1)/2,
+ *                             b = max(J,K)*(max(J,K)+1)/2 + min(J,K)
+ *                             (TensorTri<2> order, src/nnp/tensortri.hpp:99-111)
+ *   input vector of species I = [radial blocks J=0..][angular blocks b=0..]
+ *                             (NNH::init_input, src/nnp/nnh.cpp:183-201)
+ *   network of species I: nn_nlayer[I] weight layers, widths
+ *     nn_width[I*(ACE_MAX_LAYERS+1) + l], l = 0..nlayer (width 0 = inputs, last = 1);
+ *     hidden activation nn_act[I] with sharpness nn_c[I], output layer linear
+ *     (src/ml/nn.cpp:1349-1352); ins = inpw*(G+inpb), out = outb + outw*a_last.
+ *   params of species I start at par_off[I]: all biases b_0..b_{L-1}, then all
+ *     weights W_0..W_{L-1} column-major (src/ml/nn.cpp:881-896).
+ */
+typedef struct ace_model_desc {
+    int ntypes;
+    double rc;                 /* neighbour-list cutoff (NNP::rc_) */
+    /* radial */
+    const int*    r_form;
+    const int*    r_cut;
+    const double* r_rc;
+    const int*    r_n;
+    const int*    r_off;       /* offset of the block in r_rs / r_eta */
+    const double* r_rs;
+    const double* r_eta;
+    /* angular */
+    const int*    a_form;
+    const int*    a_cut;
+    const double* a_rc;
+    const int*    a_n;
+    const int*    a_off;       /* offset of the block in a_eta / a_zeta / a_lambda */
+    const double* a_eta;
+    const double* a_zeta;
+    const int*    a_lambda;
+    /* networks */
+    const int*    nn_nlayer;
+    const int*    nn_width;
+    const int*    nn_act;
+    const double* nn_c;
+    const int*    in_off;      /* offset of species I in inpw / inpb */
+    const double* inpw;
+    const double* inpb;
+    const double* outw;        /* per species */
+    const double* outb;        /* per species */
+    const double* E0;          /* per species (NNH::type().energy()) */
+    const int*    par_off;     /* offset of species I in params */
+    const double* params;
+} ace_model_desc;
+
+/* Training-loss description (energy term as NNPTEFR::error_cost_o1; force term new). */
+typedef struct ace_loss_desc {
+    int loss;          /* ACE_LOSS_* applied to the normalised energy error dU */
+    int norm;          /* ACE_NORM_* : dU = (E - E_ref) * norm(nAtoms) */
+    double delta;      /* HUBER / ASINH scale */
+    double w_energy;   /* multiplies the energy term; the reference divides by the batch size */
+    double w_force;    /* multiplies sum over atoms,xyz of 0.5*(F - F_ref)^2 */
+} ace_loss_desc;
+
+typedef struct ace_ctx ace_ctx;
+
+/* Context: device buffers, one HIP stream, model copy.  One context per host thread. */
+int  ace_create(ace_ctx** ctx, const ace_model_desc* model);
+void ace_destroy(ace_ctx* ctx);
+const char* ace_last_error(const ace_ctx* ctx);
+int  ace_input_width(const ace_ctx* ctx, int type);          /* NNH::nInput */
+int  ace_param_count(const ace_ctx* ctx, int type);          /* NN::ANN::size */
+
+/* NN::ANN operator<< / operator>> for species `type` (parameter order of nn.cpp:881-896). */
+int  ace_set_params(ace_ctx* ctx, int type, const double* params, int n);
+int  ace_get_params(const ace_ctx* ctx, int type, double* params, int n);
+/* input preconditioning (ANN::inpw / inpb), as NNPTEFR::train sets it (nnptefr.cpp:786-880) */
+int  ace_set_input_scaling(ace_ctx* ctx, int type, const double* inpw, const double* inpb, int n);
+
+/*
+ * Structures: n_struc structures, atom counts natoms[s], positions pos (sum(natoms) x 3, Angstrom,
+ * structure after structure), species type (sum(natoms)), cells cell (n_struc x 9, column-major,
+ * columns = lattice vectors, as Structure::R()).  Copied to HBM; kept until the next upload.
+ */
+int  ace_upload(ace_ctx* ctx, int n_struc, const int* natoms, const double* pos, const int* type,
+                const double* cell);
+/* Update positions only (same structures, same atom order); for MD and finite differences. */
+int  ace_update_positions(ace_ctx* ctx, const double* pos);
+
+/*
+ * Neighbour list for every uploaded structure: all (j, n) with |r_i - r_j - R n|^2 < (rc+skin)^2,
+ * j = i allowed only for n != 0, n within the reference's image range ceil(3 rc / |R e_d|)
+ * (NeighborList::build, neighbor.cpp:46-87).  Decided in FP64 with the reference formula.
+ */
+int  ace_build_neighbors(ace_ctx* ctx, double skin);
+int  ace_neighbor_count(const ace_ctx* ctx, long long* npairs);
+/* offsets: sum(natoms)+1 (global atom index, per-atom list start); j: local atom index within the
+ * structure; img: 3 ints per pair.  Lists are in the reference order: j ascending, then n
+ * lexicographic. */
+int  ace_export_neighbors(ace_ctx* ctx, long long* offsets, int* j, int* img);
+
+/* Symmetry functions G (FP32 on the GPU); G exported row per atom with stride ldg (>= max width),
+ * zero-padded, as FP64.  G may be NULL (computed and kept on the device). */
+int  ace_symm(ace_ctx* ctx, double* G, int ldg);
+
+/*
+ * Energy and forces of every uploaded structure in one batched pass (neighbour list must be built):
+ *   E_struc[s] = sum_i (NN_i + E0_i)            (NNP::compute_energy convention, FP64 reduction)
+ *   F          = -dE/dr, sum(natoms) x 3         (NNP::compute_force)
+ *   virial     = n_struc x 9, W_ab = -sum_pairs d_a dE/dd_b (FP64 reduction); may be NULL
+ * precision: ACE_PREC_FP32 or ACE_PREC_BF16L1.  Output pointers may be NULL: results stay in HBM.
+ */
+int  ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, double* virial);
+
+/* Same quantities from the unbatched FP64 port: one structure per launch, one thread per atom. */
+int  ace_plain_energy_forces(ace_ctx* ctx, double* E_struc, double* F);
+
+/*
+ * One training step over all uploaded structures (the batch): loss and d(loss)/d(params) for every
+ * species, params concatenated over species (NNPTEFR packs gElement_ the same way,
+ * nnptefr.cpp:983-987).  E_ref per structure (total energy incl. E0), F_ref sum(natoms) x 3
+ * (may be NULL when w_force == 0).  loss[0] = total, loss[1] = energy part, loss[2] = force part,
+ * loss[3] = sum_s (dE/nAtoms)^2.  grad may be NULL.
+ */
+int  ace_set_targets(ace_ctx* ctx, const double* E_ref, const double* F_ref);
+int  ace_train_step(ace_ctx* ctx, const ace_loss_desc* loss_desc, double* loss, double* grad);
+
+/* Timing of the last calls: per-kernel HIP-event durations accumulated while enabled. */
+int  ace_profile_enable(ace_ctx* ctx, int on);
+int  ace_profile_read(const ace_ctx* ctx, const char* kernel, double* total_ms, long long* launches);
+int  ace_sync(ace_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,189 @@
+"""HIP path vs the FP64 oracle, through the C-ABI.  Tolerances (FP32 basis and network, FP64
+reductions): neighbour lists bit-exact; G relative 2e-5 (floor 1); energy 1e-6 eV/atom; forces 1e-4 eV/A
+absolute; the FP64 plain port 1e-9."""
+import os
+
+import numpy as np
+import pytest
+
+from ace_amd import NNPDevice, configs
+from oracle import oracle as O
+
+from conftest import log
+
+pytestmark = pytest.mark.gpu
+
+TOL_G = 2e-5
+TOL_E_ATOM = 1e-6
+TOL_F = 1e-4
+
+
+def structure(cell, pos, typ):
+    return configs.Structure(np.asarray(cell, float), np.asarray(pos, float), np.asarray(typ, dtype=np.int32))
+
+
+def check_against_oracle(model, strucs, name, check_plain=True):
+    orc = O.Oracle(model)
+    dev = NNPDevice(model)
+    dev.upload(strucs)
+    dev.build_neighbors()
+    off, j, img = dev.neighbors()
+    Gd = dev.compute_symm()
+    E, F, W = dev.compute()
+    if check_plain:
+        Ep, Fp = dev.plain_compute()
+    a0 = 0
+    p0 = 0
+    for k, s in enumerate(strucs):
+        nl = orc.neighbors(s)
+        n = s.natoms
+        # neighbour list: same pairs, same order
+        o = off[a0:a0 + n + 1] - off[a0]
+        assert np.array_equal(o, nl.off), f"{name}[{k}] neighbour offsets"
+        assert np.array_equal(j[p0:p0 + len(nl.j)], nl.j), f"{name}[{k}] neighbour indices"
+        assert np.array_equal(img[p0:p0 + len(nl.j)], nl.img), f"{name}[{k}] image vectors"
+        G = orc.symm(s, nl)
+        kk = G.shape[1]
+        gerr = np.max(np.abs(Gd[a0:a0 + n, :kk] - G) / np.maximum(np.abs(G), 1.0))
+        assert gerr < TOL_G, f"{name}[{k}] G rel err {gerr}"
+        pe, Fr, Wr = orc.force(s, G, nl)
+        Er = pe + np.asarray(model.E0)[s.type].sum()
+        de = abs(E[k] - Er) / n
+        df = np.max(np.abs(F[a0:a0 + n] - Fr))
+        dw = np.max(np.abs(W[k] - Wr)) / max(1.0, np.max(np.abs(Wr)))
+        log(f"{name}[{k}] n={n} pairs={len(nl.j)} G {gerr:.1e} dE/atom {de:.1e} dF {df:.1e} dW {dw:.1e}")
+        assert de < TOL_E_ATOM and df < TOL_F and dw < 1e-5
+        if check_plain:
+            assert abs(Ep[k] - Er) < 1e-9 * max(1, abs(Er)) and np.max(np.abs(Fp[a0:a0 + n] - Fr)) < 1e-9
+        a0 += n
+        p0 += len(nl.j)
+    dev.close()
+
+
+def scaled_model(model, strucs):
+    orc = O.Oracle(model)
+    G = np.concatenate([orc.symm(s) for s in strucs])
+    configs.set_input_scaling(model, strucs, G)
+    return model
+
+
+def test_c1_golden(golden_dir):
+    g = np.load(os.path.join(golden_dir, "c1_oracle.npz"))
+    model, [s] = configs.config1()
+    model.nets[0].inpw, model.nets[0].inpb = g["inpw"], g["inpb"]
+    dev = NNPDevice(model)
+    dev.upload([s])
+    dev.build_neighbors()
+    off, j, img = dev.neighbors()
+    assert np.array_equal(off, g["off"]) and np.array_equal(j, g["j"]) and np.array_equal(img, g["img"])
+    G = dev.compute_symm()
+    assert np.max(np.abs(G - g["G"]) / np.maximum(np.abs(g["G"]), 1.0)) < TOL_G
+    E, F, W = dev.compute()
+    assert abs(E[0] - float(g["E"])) / s.natoms < TOL_E_ATOM
+    assert np.max(np.abs(F - g["F"])) < TOL_F
+    Ep, Fp = dev.plain_compute()
+    assert abs(Ep[0] - float(g["E"])) < 1e-9 and np.max(np.abs(Fp - g["F"])) < 1e-9
+    dev.close()
+
+
+def test_small_and_triclinic_cells():
+    """Cells shorter than 2 rc (several images of one neighbour), skewed cells, unwrapped atoms."""
+    rng = np.random.default_rng(5)
+    m = configs.c3_model(rng)
+    strucs = []
+    for L_, skew, n in [(7.4, 0.0, 32), (8.2, 0.2, 40), (12.0, 0.35, 60), (6.1, 0.1, 18)]:
+        cell = np.array([[L_, skew * L_, 0.2 * skew * L_], [0, L_, 0.5 * skew * L_], [0, 0, L_]])
+        f = rng.uniform(0, 1, (n, 3))
+        f[:3] += np.array([[1, 0, 0], [-1, 2, 0], [0, 0, -3]])     # atoms outside the cell
+        strucs.append(structure(cell, (cell @ f.T).T, rng.integers(0, 3, n)))
+    scaled_model(m, strucs)
+    check_against_oracle(m, strucs, "small")
+
+
+def test_isolated_and_single_atom():
+    rng = np.random.default_rng(6)
+    m = configs.c3_model(rng)
+    strucs = [
+        structure(np.eye(3) * 40.0, [[1.0, 1.0, 1.0], [20.0, 20.0, 20.0]], [0, 1]),      # no neighbours
+        structure(np.eye(3) * 4.0, [[0.5, 0.5, 0.5]], [2]),                                # self images only
+        structure(np.eye(3) * 9.0, rng.uniform(0, 9, (20, 3)), rng.integers(0, 3, 20)),
+    ]
+    for I in range(3):
+        m.nets[I].inpw = np.full(m.input_width(I), 0.05)
+    check_against_oracle(m, strucs, "edge")
+
+
+def test_c2_full_size():
+    m, strucs = configs.config2()
+    scaled_model(m, strucs)
+    check_against_oracle(m, strucs, "c2")
+
+
+def test_c2_forces_central_fd():
+    """FP32 adjoint forces vs central finite differences (h = 1e-4 A) of the FP64 energy."""
+    m, [s] = configs.config2()
+    scaled_model(m, [s])
+    dev = NNPDevice(m)
+    dev.upload([s])
+    dev.build_neighbors()
+    _, F, _ = dev.compute()
+    h = 1e-4
+    rng = np.random.default_rng(1)
+    worst = 0.0
+    for a in rng.choice(s.natoms, 6, replace=False):
+        for d in range(3):
+            e = []
+            for sg in (1, -1):
+                p = s.pos.copy()
+                p[a, d] += sg * h
+                dev.update_positions(p)
+                e.append(dev.plain_compute()[0][0])
+            fd = -(e[0] - e[1]) / (2 * h)
+            worst = max(worst, abs(fd - F[a, d]))
+    log(f"c2 FD force max err {worst:.2e}")
+    assert worst < TOL_F
+    dev.close()
+
+
+def test_batching_is_invariant_and_deterministic():
+    rng = np.random.default_rng(9)
+    m = configs.c3_model(rng)
+    strucs = []
+    for n in (37, 120, 64):
+        cell, pos = configs.random_packed(n, 0.08, 1.6, rng)
+        strucs.append(structure(cell, pos, rng.integers(0, 3, n)))
+    scaled_model(m, strucs)
+    dev = NNPDevice(m)
+    dev.upload(strucs)
+    dev.build_neighbors()
+    E, F, W = dev.compute()
+    E2, F2, W2 = dev.compute()
+    assert np.array_equal(E, E2) and np.array_equal(F, F2) and np.array_equal(W, W2)
+    a0 = 0
+    for k, s in enumerate(strucs):
+        one = NNPDevice(m)
+        one.upload([s])
+        one.build_neighbors()
+        e, f, w = one.compute()
+        assert e[0] == E[k] and np.array_equal(f, F[a0:a0 + s.natoms]) and np.array_equal(w[0], W[k])
+        a0 += s.natoms
+        one.close()
+    dev.close()
+
+
+def test_positions_update_matches_rebuild():
+    m, [s] = configs.config1()
+    scaled_model(m, [s])
+    dev = NNPDevice(m)
+    dev.upload([s])
+    dev.build_neighbors(skin=1.0)
+    p = s.pos + np.random.default_rng(2).normal(0, 0.05, s.pos.shape)
+    dev.update_positions(p)
+    E, F, _ = dev.compute()
+    ref = NNPDevice(m)
+    ref.upload([structure(s.cell, p, s.type)])
+    ref.build_neighbors()
+    Er, Fr, _ = ref.compute()
+    assert abs(E[0] - Er[0]) / s.natoms < 1e-9 and np.max(np.abs(F - Fr)) < 1e-6
+    dev.close()
+    ref.close()

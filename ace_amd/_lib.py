@@ -32,6 +32,7 @@ SIGNATURES = {
     "ace_build_neighbors": (C.c_int, [P, C.c_double]),
     "ace_neighbor_count": (C.c_int, [P, lp]),
     "ace_export_neighbors": (C.c_int, [P, lp, ip, ip]),
+    "ace_set_precision": (C.c_int, [P, C.c_int]),
     "ace_symm": (C.c_int, [P, dp, C.c_int]),
     "ace_energy_forces": (C.c_int, [P, C.c_int, dp, dp, dp]),
     "ace_plain_energy_forces": (C.c_int, [P, dp, dp]),

@@ -102,9 +102,11 @@ def zeta_lambda(zetas):
     return [(z, l) for z in zetas for l in (1, -1)]
 
 
-def set_input_scaling(model: Model, structures, G):
-    """inpb = -mean, inpw = 1/dev over the atoms of each species (NNPTEFR::train preconditioning with
-    PreBias::MEAN and PreScale::DEV, src/nnp/nnptefr.cpp:786-880)."""
+def set_input_scaling(model: Model, structures, G, dev_floor=1e-3):
+    """inpb = -mean, inpw = 1/max(dev, dev_floor) over the atoms of each species (NNPTEFR::train
+    preconditioning with PreBias::MEAN and PreScale::DEV, src/nnp/nnptefr.cpp:786-880).  The floor keeps a
+    nearly constant input (e.g. a G2 with eta_B = 2 A^-2 that is ~1e-6 for every atom) from being
+    multiplied by 1e5 or more, which would amplify rounding noise into the energies."""
     types = np.concatenate([s.type for s in structures])
     for I in range(model.ntypes):
         K = model.input_width(I)
@@ -112,8 +114,8 @@ def set_input_scaling(model: Model, structures, G):
         if len(g) == 0:
             continue
         mu = g.mean(axis=0)
-        dev = g.std(axis=0)
-        w = np.where(dev > 0, 1.0 / np.where(dev > 0, dev, 1.0), 1.0)
+        dev = np.maximum(g.std(axis=0), dev_floor)
+        w = 1.0 / dev
         model.nets[I].inpb = -mu
         model.nets[I].inpw = w
 

@@ -53,17 +53,19 @@ struct ace_ctx {
     int nt = 0;
     double rc = 0;
     std::vector<HSpecies> sp;
-    int kmax = 0, ldg = 0, maxA = 0, maxR = 0, ngmax = 1;
+    int kmax = 0, ldg = 0, maxA = 0, maxR = 0, ngmax = 1, ngk = 1;
     DevModel hdm;
     DBuf<DevModel> d_model;
     DBuf<float> d_r_rs, d_r_eta, d_a_lamf, d_a_zeta, d_params, d_inpw, d_inpb;
-    DBuf<int> d_a_zint, d_a_grp;
+    DBuf<int> d_a_zint, d_a_grp, d_a_sel, d_a_cidx;
     bool plain_ok = false;
     std::string plain_why;
     PlainModel hpm;
     DBuf<PlainModel> d_pmodel;
     DBuf<double> d_r_rs64, d_r_eta64, d_a_eta64, d_a_zeta64, d_a_lamf64, d_nn64, d_inpw64, d_inpb64;
     bool params_dirty = true;
+    int precision = ACE_PREC_FP32;   // default of ace_symm / ace_train_step
+    bool basis_f64 = false;          // precision of the basis kernels of the current call
     // structures
     int S = 0, N = 0;
     std::vector<int> natoms, atom_off, type_h, row_of_h, struc_of_h;
@@ -84,8 +86,13 @@ struct ace_ctx {
     int ncells = 0;
     DBuf<int> d_cell_of, d_wrap_of, d_cell_count, d_cell_start, d_cell_cursor, d_cell_atoms, d_nb_count, d_nb_off,
         d_scan_blk, d_scal;
-    long long npairs = 0;
-    int maxnb = 0;
+    long long npairs = 0;       // pairs of the current list (exact after a synchronising call)
+    long long pcap = 0;         // pair capacity of the device arrays
+    int slots_cap = 32;         // neighbour slots per atom held in LDS by the basis kernels
+    bool nb_async_ok = false;   // capacities known: rebuilds need no host round trip
+    bool nb_same_layout = false;
+    bool nb_pending = false;    // a rebuild's overflow / symmetry flags are not checked yet
+    double nb_skin = 0;
     DBuf<int> d_nb_j, d_nb_img, d_nb_src, d_nb_rev;
     DBuf<float4> d_nb_d, d_pair_g;
     // plain port scratch
@@ -267,7 +274,7 @@ int build_tables(ace_ctx* ctx) {
     memset(&dm, 0, sizeof(dm));
     dm.ntypes = nt;
     std::vector<float> r_rs, r_eta, a_lamf, a_zeta, prm, iw, ib;
-    std::vector<int> a_zint, a_grp;
+    std::vector<int> a_zint, a_grp, a_sel;
     int kmax = 0, maxA = 1, maxR = 1, ngmax = 1;
     for (int I = 0; I < nt; ++I) {
         const HSpecies& s = ctx->sp[I];
@@ -282,6 +289,7 @@ int build_tables(ace_ctx* ctx) {
             d.r_cut[J] = s.r_cut[J];
             d.r_n[J] = s.r_n[J];
             d.r_rc[J] = (float)s.r_rc[J];
+            d.r_rc64[J] = s.r_rc[J];
             d.r_goff[J] = goff;
             d.r_poff[J] = (int)r_rs.size();
             goff += s.r_n[J];
@@ -304,6 +312,7 @@ int build_tables(ace_ctx* ctx) {
         }
         d.a_cut = acut < 0 ? ACE_CUT_COS : acut;
         d.a_rc = (float)(acut < 0 ? ctx->rc : arc);
+        d.a_rc64 = acut < 0 ? ctx->rc : arc;
         if (acut >= 0) rcm = std::max(rcm, arc);
         d.rc_max = (float)std::min(rcm, ctx->rc);
         for (int b = 0; b < nab; ++b) {
@@ -333,10 +342,22 @@ int build_tables(ace_ctx* ctx) {
                 a_zint.push_back((z == std::floor(z) && z >= 1 && z <= 64) ? (int)z : -1);
                 a_grp.push_back(g);
                 if (!(z > 0)) return fail(ctx, "ace_create: zeta must be positive");
+                // power-of-two zeta <= 64 with lambda = +-1: one pick from the per-triplet power cache
+                int sel = -1;
+                const int zi = (z == std::floor(z) && z >= 1 && z <= 64) ? (int)z : 0;
+                if (zi && (zi & (zi - 1)) == 0 && std::abs(s.a_lambda[b][q]) == 1) {
+                    int k = 0;
+                    while ((1 << k) < zi) ++k;
+                    sel = k + (s.a_lambda[b][q] < 0 ? 7 : 0);
+                }
+                a_sel.push_back(sel);
             }
         }
         d.ng = (int)groups.size();
-        for (int g = 0; g < d.ng; ++g) d.ietar2[g] = (float)(1.0 / (groups[g] * groups[g]));
+        for (int g = 0; g < d.ng; ++g) {
+            d.ietar2[g] = (float)(1.0 / (groups[g] * groups[g]));
+            d.ietar2_64[g] = 1.0 / (groups[g] * groups[g]);
+        }
         ngmax = std::max(ngmax, d.ng);
         // network
         d.L = s.L;
@@ -358,6 +379,31 @@ int build_tables(ace_ctx* ctx) {
         d.outb = (float)s.outb;
         d.E0 = s.E0;
     }
+    // canonical angular blocks: every function lambda = +-1, zeta = 2^k (k <= 6) and one function per
+    // (sign, level, eta group); their sums become a register outer product in the basis kernel
+    const int ngk = ngmax <= 1 ? 1 : (ngmax <= 2 ? 2 : 4);
+    std::vector<int> a_cidx(a_sel.size(), 0);
+    for (int I = 0; I < nt; ++I) {
+        DevSpecies& d = dm.sp[I];
+        int nz = 1;
+        for (int b = 0; b < nab; ++b) {
+            bool canon = d.a_n[b] > 0;
+            std::vector<char> used(14 * ngk, 0);
+            for (int q = 0; q < d.a_n[b] && canon; ++q) {
+                const int fi = d.a_poff[b] + q, sel = a_sel[fi];
+                if (sel < 0) { canon = false; break; }
+                const int c = sel * ngk + a_grp[fi];
+                if (used[c]) { canon = false; break; }
+                used[c] = 1;
+                a_cidx[fi] = c;
+            }
+            d.a_canon[b] = canon ? 1 : 0;
+            if (canon)
+                for (int q = 0; q < d.a_n[b]; ++q) nz = std::max(nz, a_sel[d.a_poff[b] + q] % 7 + 1);
+        }
+        d.nz = nz;
+    }
+    ctx->ngk = ngk;
     dm.kmax = kmax;
     dm.ng_max = ngmax;
     ctx->kmax = kmax;
@@ -372,6 +418,8 @@ int build_tables(ace_ctx* ctx) {
     if (upload(ctx, ctx->d_a_zeta, a_zeta.data(), a_zeta.size())) return -1;
     if (upload(ctx, ctx->d_a_zint, a_zint.data(), a_zint.size())) return -1;
     if (upload(ctx, ctx->d_a_grp, a_grp.data(), a_grp.size())) return -1;
+    if (upload(ctx, ctx->d_a_sel, a_sel.data(), a_sel.size())) return -1;
+    if (upload(ctx, ctx->d_a_cidx, a_cidx.data(), a_cidx.size())) return -1;
     if (upload(ctx, ctx->d_params, prm.data(), prm.size())) return -1;
     if (upload(ctx, ctx->d_inpw, iw.data(), iw.size())) return -1;
     if (upload(ctx, ctx->d_inpb, ib.data(), ib.size())) return -1;
@@ -459,55 +507,92 @@ DevBasisParams basis_params(ace_ctx* ctx) {
     P.r_eta = ctx->d_r_eta.p;
     P.a_lamf = ctx->d_a_lamf.p;
     P.a_zeta = ctx->d_a_zeta.p;
+    P.r_rs64 = ctx->d_r_rs64.p;
+    P.r_eta64 = ctx->d_r_eta64.p;
+    P.a_lamf64 = ctx->d_a_lamf64.p;
+    P.a_zeta64 = ctx->d_a_zeta64.p;
     P.a_zint = ctx->d_a_zint.p;
     P.a_grp = ctx->d_a_grp.p;
+    P.a_sel = ctx->d_a_sel.p;
+    P.a_cidx = ctx->d_a_cidx.p;
+    P.inpw = ctx->d_inpw.p;
+    P.inpb = ctx->d_inpb.p;
+    P.inpw64 = ctx->d_inpw64.p;
+    P.inpb64 = ctx->d_inpb64.p;
     return P;
 }
 
 // ---- basis kernel dispatch ------------------------------------------------------------------------
 
 typedef void (*BasisFn)(int, const DevModel*, DevBasisParams, const int*, const int*, const int*, const float4*,
-                        const int*, int, float*, const float*, float4*, int*, int, const double*);
+                        const int*, const double*, const int*, const StrucInfo*, const int*, int, float*,
+                        const float*, float4*, int*, int, const double*);
 
-template <int A, int R>
+template <class T, int NG>
 void pick_basis(BasisFn* f) {
-    f[0] = k_basis<A, R, 0>;
-    f[1] = k_basis<A, R, 1>;
-    f[2] = k_basis<A, R, 2>;
+    f[0] = k_basis<T, NG, 0>;
+    f[1] = k_basis<T, NG, 1>;
+    f[2] = k_basis<T, NG, 2>;
 }
 
-// register-array sizes: the largest angular / radial block, rounded up to 8, 16 or 32
-void basis_fns(ace_ctx* ctx, BasisFn* f) {
-    const int A = ctx->maxA <= 8 ? 8 : (ctx->maxA <= 16 ? 16 : 32);
-    const int R = ctx->maxR <= 8 ? 8 : (ctx->maxR <= 16 ? 16 : 32);
-    if (A == 8 && R == 8) pick_basis<8, 8>(f);
-    else if (A == 8 && R == 16) pick_basis<8, 16>(f);
-    else if (A == 16 && R == 8) pick_basis<16, 8>(f);
-    else if (A == 16 && R == 16) pick_basis<16, 16>(f);
-    else if (A == 32 && R == 16) pick_basis<32, 16>(f);
-    else if (A == 16 && R == 32) pick_basis<16, 32>(f);
-    else pick_basis<32, 32>(f);
+// the canonical angular accumulators are sized by the largest number of eta groups (1, 2 or 4)
+void basis_fns(ace_ctx* ctx, bool f64, BasisFn* f) {
+    if (f64) {
+        if (ctx->ngk == 1) pick_basis<double, 1>(f);
+        else if (ctx->ngk == 2) pick_basis<double, 2>(f);
+        else pick_basis<double, 4>(f);
+    } else {
+        if (ctx->ngk == 1) pick_basis<float, 1>(f);
+        else if (ctx->ngk == 2) pick_basis<float, 2>(f);
+        else pick_basis<float, 4>(f);
+    }
 }
 
-int nb_slots(ace_ctx* ctx) { return std::max(32, (ctx->maxnb + 31) & ~31); }
+int nb_slots(ace_ctx* ctx) { return ctx->slots_cap; }
+
+// reads the neighbour-build flags (one host round trip); on overflow the next build is synchronous
+int nb_verify(ace_ctx* ctx) {
+    if (!ctx->nb_pending) return 0;
+    int h[5] = {0, 0, 0, 0, 0};
+    HC(hipMemcpyAsync(h, ctx->d_scal.p, 5 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    int tot = 0;
+    HC(hipMemcpyAsync(&tot, ctx->d_nb_off.p + ctx->N, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    ctx->nb_pending = false;
+    if (h[4]) {
+        ctx->nb_valid = false;
+        ctx->nb_async_ok = false;
+        return fail(ctx, "neighbour list outgrew its capacity in an asynchronous rebuild; build it again");
+    }
+    if (h[2]) {
+        ctx->nb_valid = false;
+        return fail(ctx, "ace_build_neighbors: neighbour list is not symmetric");
+    }
+    ctx->npairs = tot;
+    return 0;
+}
 
 // mode 0: G -> out; mode 1: per-pair gradients from dE/dG; mode 2: tangent rows along eps -> out
 int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = nullptr) {
     BasisFn fns[3];
-    basis_fns(ctx, fns);
+    const bool f64 = ctx->basis_f64;
+    basis_fns(ctx, f64, fns);
     const int nbm = nb_slots(ctx);
-    const int bytes = BW * basis_lds_wave_bytes(nbm, ctx->ngmax, ctx->kmax, mode);
+    const int bytes = f64 ? basis_lds_bytes<double>(nbm, ctx->ngmax, ctx->kmax, mode)
+                          : basis_lds_bytes<float>(nbm, ctx->ngmax, ctx->kmax, mode);
     if (bytes > 160 * 1024) return fail(ctx, "basis: neighbour list too long for LDS");
     BasisFn f = fns[mode];
     if (bytes > 64 * 1024) HC(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     DevBasisParams P = basis_params(ctx);
     int* err = ctx->d_scal.p + 3;
     float* rows = out ? out : ctx->d_G.p;
-    static const char* names[3] = {"basis_symm", "basis_adjoint", "basis_tangent"};
-    return timed(ctx, names[mode], [&] {
-        f<<<cdiv(ctx->N, BW), BW * 64, bytes, ctx->stream>>>(ctx->N, ctx->d_model.p, P, ctx->d_type.p, ctx->d_nb_off.p,
-                                                             ctx->d_nb_j.p, ctx->d_nb_d.p, ctx->d_row_of.p, ctx->ldg,
-                                                             rows, ctx->d_dEdG.p, ctx->d_pair_g.p, err, nbm, eps);
+    static const char* names[2][3] = {{"basis_symm", "basis_adjoint", "basis_tangent"},
+                                      {"basis_symm_f64", "basis_adjoint_f64", "basis_tangent_f64"}};
+    return timed(ctx, names[f64][mode], [&] {
+        f<<<ctx->N, BT, bytes, ctx->stream>>>(ctx->N, ctx->d_model.p, P, ctx->d_type.p, ctx->d_nb_off.p, ctx->d_nb_j.p,
+                                              ctx->d_nb_d.p, ctx->d_nb_img.p, ctx->d_pos.p, ctx->d_struc_of.p,
+                                              ctx->d_sinfo.p, ctx->d_row_of.p, ctx->ldg, rows, ctx->d_dEdG.p,
+                                              ctx->d_pair_g.p, err, nbm, eps);
     });
 }
 
@@ -536,8 +621,8 @@ int require_nb(ace_ctx* ctx) {
     if (!ctx->nb_valid) return fail(ctx, "neighbour list not built");
     if (ctx->disp_dirty) {
         if (timed(ctx, "nb_disp", [&] {
-                k_nb_disp<<<cdiv(ctx->npairs, 256), 256, 0, ctx->stream>>>(
-                    (int)ctx->npairs, ctx->d_nb_src.p, ctx->d_nb_j.p, ctx->d_nb_img.p, ctx->d_struc_of.p,
+                k_nb_disp<<<cdiv(ctx->pcap, 256), 256, 0, ctx->stream>>>(
+                    ctx->d_nb_off.p + ctx->N, ctx->d_nb_src.p, ctx->d_nb_j.p, ctx->d_nb_img.p, ctx->d_struc_of.p,
                     ctx->d_sinfo.p, ctx->d_pos.p, ctx->d_nb_d.p);
             }))
             return -1;
@@ -583,7 +668,7 @@ void ace_destroy(ace_ctx* ctx) {
     for (auto& r : ctx->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     release(ctx->d_model); release(ctx->d_r_rs); release(ctx->d_r_eta); release(ctx->d_a_lamf);
     release(ctx->d_a_zeta); release(ctx->d_params); release(ctx->d_inpw); release(ctx->d_inpb);
-    release(ctx->d_a_zint); release(ctx->d_a_grp); release(ctx->d_pmodel); release(ctx->d_r_rs64);
+    release(ctx->d_a_zint); release(ctx->d_a_grp); release(ctx->d_a_sel); release(ctx->d_a_cidx); release(ctx->d_pmodel); release(ctx->d_r_rs64);
     release(ctx->d_r_eta64); release(ctx->d_a_eta64); release(ctx->d_a_zeta64); release(ctx->d_a_lamf64);
     release(ctx->d_nn64); release(ctx->d_inpw64); release(ctx->d_inpb64);
     release(ctx->d_type); release(ctx->d_struc_of); release(ctx->d_atom_off); release(ctx->d_natoms);
@@ -635,6 +720,7 @@ int ace_set_input_scaling(ace_ctx* ctx, int type, const double* inpw, const doub
     ctx->sp[type].inpw.assign(inpw, inpw + n);
     ctx->sp[type].inpb.assign(inpb, inpb + n);
     ctx->params_dirty = true;
+    ctx->tr.g_valid = false;     // cached network-input rows include the scaling
     return 0;
 }
 
@@ -671,9 +757,17 @@ int ace_upload(ace_ctx* ctx, int n_struc, const int* natoms, const double* pos, 
         ctx->row_of_h[i] = r;
         atom_of_row[r] = (int)i;
     }
-    int lds32 = 0;
-    for (int I = 0; I < ctx->nt; ++I) lds32 = std::max(lds32, mlp_lds_floats(ctx->hdm.sp[I], 32) * 4);
-    ctx->tm = (lds32 <= 64 * 1024) ? 32 : 16;
+    // rows per network tile: 32 when there are enough atoms to give every CU several tiles, fewer
+    // (down to 8) for small batches, and never more than 64 KB of LDS
+    int tm = 32;
+    while (tm > 8 && N / tm < 1024) tm >>= 1;
+    for (;;) {
+        int lds = 0;
+        for (int I = 0; I < ctx->nt; ++I) lds = std::max(lds, mlp_lds_floats(ctx->hdm.sp[I], tm) * 4);
+        if (lds <= 64 * 1024 || tm <= 4) break;
+        tm >>= 1;
+    }
+    ctx->tm = tm;
     ctx->tiles_h.clear();
     for (int t = 0; t < ctx->nt; ++t)
         for (int r = start[t]; r < start[t + 1]; r += ctx->tm)
@@ -698,6 +792,8 @@ int ace_upload(ace_ctx* ctx, int n_struc, const int* natoms, const double* pos, 
     if (ensure(ctx, ctx->d_vir, 9 * n_struc)) return -1;
     HC(hipStreamSynchronize(ctx->stream));
     ctx->nb_valid = false;
+    ctx->nb_same_layout = false;
+    ctx->nb_pending = false;
     train_on_upload(ctx);
     return 0;
 }
@@ -718,7 +814,9 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
     if (skin < 0) return fail(ctx, "ace_build_neighbors: negative skin");
     const double rcl = ctx->rc + skin;
     const double rsearch = rcl * (1.0 + 1e-9) + 1e-9;
+    if (rcl != ctx->nb_rc) ctx->nb_same_layout = false;
     ctx->nb_rc = rcl;
+    ctx->nb_skin = skin;
     ctx->nb_valid = false;
     ctx->tr.g_valid = false;
     ctx->sinfo_h.resize(ctx->S);
@@ -794,49 +892,62 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
             k_nb_search<0><<<cdiv(N, NB_WAVES), NB_WAVES * 64, 0, ctx->stream>>>(
                 N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
                 ctx->d_cell_start.p, ctx->d_cell_atoms.p, ctx->d_nb_count.p, ctx->d_scal.p, nullptr, nullptr, nullptr,
-                nullptr);
+                nullptr, nullptr);
         }))
         return -1;
     if (exclusive_scan(ctx, ctx->d_nb_count.p, ctx->d_nb_off.p, N)) return -1;
-    int h[2] = {0, 0};
-    HC(hipMemcpyAsync(&h[0], ctx->d_scal.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HC(hipMemcpyAsync(&h[1], ctx->d_nb_off.p + N, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HC(hipStreamSynchronize(ctx->stream));
-    if (h[0] > NB_LMAX) return fail(ctx, "ace_build_neighbors: an atom has more than 1024 neighbours");
-    ctx->maxnb = h[0];
-    ctx->npairs = h[1];
-    const long long P = ctx->npairs;
+    // Capacities: the first build (or one after an overflow) reads the exact sizes back; later builds
+    // reuse the capacities with a margin and run without a host round trip.  An overflow empties every
+    // list on the device (k_nb_check) and is reported at the next synchronising call.
+    const bool async = ctx->nb_async_ok && ctx->nb_same_layout;
+    if (!async) {
+        int h[2] = {0, 0};
+        HC(hipMemcpyAsync(&h[0], ctx->d_scal.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HC(hipMemcpyAsync(&h[1], ctx->d_nb_off.p + N, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HC(hipStreamSynchronize(ctx->stream));
+        if (h[0] > NB_LMAX) return fail(ctx, "ace_build_neighbors: an atom has more than 1024 neighbours");
+        ctx->pcap = (long long)(1.1 * h[1]) + 1024;
+        ctx->slots_cap = std::min(NB_LMAX, ((h[0] + 16 + 31) & ~31));
+        if (ctx->pcap > (1LL << 31) - 1) return fail(ctx, "ace_build_neighbors: too many pairs");
+        ctx->npairs = h[1];
+    }
+    const long long P = ctx->pcap;
     if (ensure(ctx, ctx->d_nb_j, P) || ensure(ctx, ctx->d_nb_img, P) || ensure(ctx, ctx->d_nb_src, P) ||
         ensure(ctx, ctx->d_nb_rev, P) || ensure(ctx, ctx->d_nb_d, P) || ensure(ctx, ctx->d_pair_g, P))
+        return -1;
+    if (timed(ctx, "nb_check", [&] {
+            k_nb_check<<<cdiv(N + 1, 256), 256, 0, ctx->stream>>>(N, ctx->d_nb_off.p, ctx->d_scal.p, ctx->d_scal.p + 1,
+                                                                   (int)P, ctx->slots_cap, ctx->d_scal.p + 4);
+        }))
         return -1;
     if (timed(ctx, "nb_fill", [&] {
             k_nb_search<1><<<cdiv(N, NB_WAVES), NB_WAVES * 64, 0, ctx->stream>>>(
                 N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
                 ctx->d_cell_start.p, ctx->d_cell_atoms.p, nullptr, nullptr, ctx->d_nb_off.p, ctx->d_nb_j.p,
-                ctx->d_nb_img.p, ctx->d_nb_d.p);
+                ctx->d_nb_img.p, ctx->d_nb_d.p, ctx->d_nb_src.p);
         }))
         return -1;
-    if (timed(ctx, "nb_src", [&] {
-            k_nb_src<<<cdiv(N, 256), 256, 0, ctx->stream>>>(N, ctx->d_nb_off.p, ctx->d_nb_src.p);
+    if (timed(ctx, "nb_reverse", [&] {
+            k_nb_reverse<<<cdiv(P, 256), 256, 0, ctx->stream>>>(ctx->d_nb_off.p + N, ctx->d_nb_src.p, ctx->d_nb_j.p,
+                                                                 ctx->d_nb_img.p, ctx->d_nb_off.p, ctx->d_struc_of.p,
+                                                                 ctx->d_sinfo.p, ctx->d_nb_rev.p, ctx->d_scal.p + 2);
         }))
         return -1;
-    if (P > 0 && timed(ctx, "nb_reverse", [&] {
-            k_nb_reverse<<<cdiv(P, 256), 256, 0, ctx->stream>>>((int)P, ctx->d_nb_src.p, ctx->d_nb_j.p, ctx->d_nb_img.p,
-                                                                 ctx->d_nb_off.p, ctx->d_struc_of.p, ctx->d_sinfo.p,
-                                                                 ctx->d_nb_rev.p, ctx->d_scal.p + 2);
-        }))
-        return -1;
-    int rev_err = 0;
-    HC(hipMemcpyAsync(&rev_err, ctx->d_scal.p + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HC(hipStreamSynchronize(ctx->stream));
-    if (rev_err) return fail(ctx, "ace_build_neighbors: neighbour list is not symmetric");
     ctx->nb_valid = true;
+    ctx->nb_pending = true;
     ctx->disp_dirty = false;
+    if (!async) {
+        if (nb_verify(ctx)) return -1;
+        ctx->nb_async_ok = true;
+    }
+    ctx->nb_same_layout = true;
     return flush_prof(ctx);
 }
 
-int ace_neighbor_count(const ace_ctx* ctx, long long* npairs) {
+int ace_neighbor_count(const ace_ctx* cctx, long long* npairs) {
+    ace_ctx* ctx = const_cast<ace_ctx*>(cctx);
     if (!ctx || !npairs || !ctx->nb_valid) return -1;
+    if (nb_verify(ctx)) return -1;
     *npairs = ctx->npairs;
     return 0;
 }
@@ -844,6 +955,7 @@ int ace_neighbor_count(const ace_ctx* ctx, long long* npairs) {
 int ace_export_neighbors(ace_ctx* ctx, long long* offsets, int* j, int* img) {
     if (!ctx) return -1;
     if (!ctx->nb_valid) return fail(ctx, "ace_export_neighbors: neighbour list not built");
+    if (nb_verify(ctx)) return -1;
     const int N = ctx->N;
     const long long P = ctx->npairs;
     std::vector<int> off(N + 1), jj(P), im(P);
@@ -864,34 +976,45 @@ int ace_export_neighbors(ace_ctx* ctx, long long* offsets, int* j, int* img) {
     return 0;
 }
 
+int ace_set_precision(ace_ctx* ctx, int precision) {
+    if (!ctx) return -1;
+    if (precision != ACE_PREC_FP32 && precision != ACE_PREC_FP64_BASIS)
+        return fail(ctx, "ace_set_precision: precision mode not available");
+    if (precision != ctx->precision) ctx->tr.g_valid = false;
+    ctx->precision = precision;
+    return 0;
+}
+
 int ace_symm(ace_ctx* ctx, double* G, int ldg) {
     if (!ctx) return -1;
+    ctx->basis_f64 = ctx->precision == ACE_PREC_FP64_BASIS;
     if (require_nb(ctx)) return -1;
     HC(hipMemsetAsync(ctx->d_scal.p + 3, 0, sizeof(int), ctx->stream));
     if (run_basis(ctx, 0)) return -1;
     if (check_err_flag(ctx, "ace_symm")) return -1;
+    if (ctx->nb_pending && nb_verify(ctx)) return -1;
     if (G) {
         if (ldg < ctx->kmax) return fail(ctx, "ace_symm: ldg smaller than the widest input vector");
         DBuf<double> tmp;
         if (ensure(ctx, tmp, (size_t)ctx->N * ldg)) return -1;
         dim3 grid(cdiv(ldg, 128), ctx->N);
-        k_export_rows<<<grid, 128, 0, ctx->stream>>>(ctx->N, ctx->kmax, ctx->d_row_of.p, ctx->ldg, ctx->d_G.p, ldg,
+        k_export_rows<<<grid, 128, 0, ctx->stream>>>(ctx->N, ctx->d_row_of.p, ctx->d_type.p, ctx->d_model.p,
+                                                    ctx->d_inpw64.p, ctx->d_inpb64.p, ctx->ldg, ctx->d_G.p, ldg,
                                                     tmp.p);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(G, tmp.p, (size_t)ctx->N * ldg * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         release(tmp);
         if (e != hipSuccess) return fail_hip(ctx, "ace_symm export", e);
-        // rows of species with fewer inputs than kmax: zero the tail
-        for (int i = 0; i < ctx->N; ++i)
-            for (int k = ctx->sp[ctx->type_h[i]].K; k < ldg; ++k) G[(long long)i * ldg + k] = 0.0;
     }
     return flush_prof(ctx);
 }
 
 int ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, double* virial) {
     if (!ctx) return -1;
-    if (precision != ACE_PREC_FP32) return fail(ctx, "ace_energy_forces: precision mode not available");
+    if (precision != ACE_PREC_FP32 && precision != ACE_PREC_FP64_BASIS)
+        return fail(ctx, "ace_energy_forces: precision mode not available");
+    ctx->basis_f64 = precision == ACE_PREC_FP64_BASIS;
     if (require_nb(ctx)) return -1;
     HC(hipMemsetAsync(ctx->d_scal.p + 3, 0, sizeof(int), ctx->stream));
     if (run_basis(ctx, 0)) return -1;
@@ -899,7 +1022,7 @@ int ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, d
     if (run_basis(ctx, 1)) return -1;
     const int N = ctx->N;
     if (timed(ctx, "force_gather", [&] {
-            k_force_gather<<<cdiv(N, 256), 256, 0, ctx->stream>>>(N, ctx->d_nb_off.p, ctx->d_nb_rev.p, ctx->d_nb_d.p,
+            k_force_gather<<<cdiv((long long)N * GL, 256), 256, 0, ctx->stream>>>(N, ctx->d_nb_off.p, ctx->d_nb_rev.p, ctx->d_nb_d.p,
                                                                    ctx->d_pair_g.p, ctx->d_F.p, ctx->d_vir_atom.p);
         }))
         return -1;
@@ -913,6 +1036,12 @@ int ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, d
     if (virial) HC(hipMemcpyAsync(virial, ctx->d_vir.p, 9 * ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (E_struc || F || virial || ctx->prof) {
         if (check_err_flag(ctx, "ace_energy_forces")) return -1;
+        if (ctx->nb_pending && nb_verify(ctx)) {
+            // an asynchronous rebuild outgrew its capacities: rebuild with exact sizes and recompute
+            if (ctx->nb_async_ok) return -1;
+            if (ace_build_neighbors(ctx, ctx->nb_skin)) return -1;
+            return ace_energy_forces(ctx, precision, E_struc, F, virial);
+        }
     }
     return flush_prof(ctx);
 }
@@ -948,6 +1077,7 @@ int ace_plain_energy_forces(ace_ctx* ctx, double* E_struc, double* F) {
     if (E_struc) HC(hipMemcpyAsync(E_struc, ctx->d_E.p, ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (F) HC(hipMemcpyAsync(F, ctx->d_F.p, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HC(hipStreamSynchronize(ctx->stream));
+    if (ctx->nb_pending && nb_verify(ctx)) return -1;
     return flush_prof(ctx);
 }
 
@@ -977,6 +1107,7 @@ int ace_profile_read(const ace_ctx* cctx, const char* kernel, double* total_ms, 
 int ace_sync(ace_ctx* ctx) {
     if (!ctx) return -1;
     HC(hipStreamSynchronize(ctx->stream));
+    if (ctx->nb_valid && nb_verify(ctx)) return -1;
     return flush_prof(ctx);
 }
 

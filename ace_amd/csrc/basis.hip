@@ -1,63 +1,84 @@
-// basis.hip -- symmetry functions G (NNP::compute_symm, src/nnp/nnp.cpp:350-392) and their adjoint
-// dE/dd for every neighbour pair (the geometric half of NNP::compute_force, src/nnp/nnp.cpp:412-476).
+// basis.hip -- symmetry functions G (NNP::compute_symm, src/nnp/nnp.cpp:350-392), their adjoint dE/dd
+// for every neighbour pair (the geometric half of NNP::compute_force, src/nnp/nnp.cpp:412-476), and
+// their tangent along per-atom displacements (dG/dr recomputed for the force-matching gradient).
 //
-// One wavefront per centre atom.  The atom's neighbours inside the basis cutoff are bucketed by
-// species into LDS (unit vector, distance, and per distinct angular eta the separable factor
-// E_s = fc(r_s) exp(-r_s^2/eta^2) with its r-derivative), so a GAUSS triplet term
+// One workgroup (4 wavefronts) per centre atom; the real type T (float or double) is a template
+// parameter: on gfx950 without packed-FP32 code the vector ALU runs FP64 FMAs at the FP32 rate, and the
+// triplet loop below has no transcendental, so T = double costs registers and LDS, not ALU rate.
+//
+// Wave 0 buckets the atom's neighbours inside the basis cutoff by species (ballot order = list order,
+// so the slot assignment is deterministic); all threads then fill LDS with each slot's unit vector and
+// distance, its radial cutoff, and, per distinct angular eta, the separable factor
+// E_s = fc(r_s) exp(-r_s^2/eta^2) with its r-derivative, so a GAUSS triplet term
 //   fc(rij) fc(rik) |1/2 + lambda/2 cos|^zeta exp(-(rij^2+rik^2)/eta^2)      (basis_angular.cpp:371-382)
-// costs one product E_j E_k instead of an exp.  Triplets of an angular block (J,K) -- the rectangle
-// J x K, or the triangle j<k of J x J -- are flattened over the 64 lanes; each lane keeps the block's
-// functions in registers and the wave reduces them at the end of the block.
-#include "ace_common.hpp"
+// costs one product E_j E_k.  Window powers come from a per-triplet cache of (1 +- cos)/2 raised to
+// 2^k and 2^k - 1, so a power-of-two zeta is one register pick.  Triplets of an angular block (J,K) --
+// the rectangle J x K, or the triangle j<k of J x J -- are spread over the 256 threads; functions are
+// kept in registers and reduced wave by wave, the four wave partials added in a fixed order.  Radial
+// functions map one function to one lane and neighbour groups to lane groups.  The adjoint accumulates
+// dE/dd per slot in one LDS array per wave and sums the four in a fixed order: results are
+// bit-reproducible.
+#include "basis_math.hpp"
 
-#define BW 4            // waves per workgroup
+#define BW 4            // waves per workgroup (one atom)
+#define BT (BW * 64)
 
-// Modes: 0 = G rows; 1 = adjoint (per-pair dE/dd from dE/dG rows); 2 = tangent (directional
-// derivative of G along per-atom displacements eps: v_i = sum_a eps_a . dG_i/dr_a).
-// Dynamic LDS, per wave: u[nbm] (float4: unit vector, r), E[ng][nbm], row[kmax]; modes 1 and 2 add
-// Ed[ng][nbm]; mode 1 adds g[3 nbm], p[nbm]; mode 2 adds te[nbm] (float4: eps_i - eps_j, projection).
-// nbm = neighbour slots per atom (largest list, rounded up).
-__host__ __device__ inline int basis_lds_wave_bytes(int nbm, int ng, int kmax, int mode) {
-    int b = 16 * nbm + 4 * ng * nbm + 4 * ((kmax + 3) & ~3);
-    if (mode >= 1) b += 4 * ng * nbm;
-    if (mode == 1) b += 12 * nbm + 4 * nbm;
-    if (mode == 2) b += 16 * nbm;
-    return (b + 15) & ~15;
+// Modes: 0 = network input rows inpw*(G + inpb); 1 = adjoint (per-pair dE/dd from dE/dG rows);
+// 2 = tangent rows v = sum_a eps_a . dG/dr_a.
+// Dynamic LDS per workgroup (nbm = neighbour slots, kp = kmax rounded up to 4), in units of T:
+//   u[4 nbm], te[4 nbm] (mode 2), E[ng nbm], Ed[ng nbm] (modes 1,2), fR[nbm], gR[nbm],
+//   g[BW 3 nbm] (mode 1), row[kp], red[BW 64]; then p[nbm] (int).
+template <class T>
+__host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int mode) {
+    const int kp = (kmax + 3) & ~3;
+    int n = 4 * nbm + ng * nbm + 2 * nbm + kp + BW * 64;
+    if (mode == 2) n += 4 * nbm;
+    if (mode >= 1) n += ng * nbm;
+    if (mode == 1) n += BW * 3 * nbm;
+    return ((int)sizeof(T) * n + 4 * nbm + 15) & ~15;
 }
 
+template <class T>
 struct BasisLds {
-    float4* u;
-    float4* te;
-    float* E;
-    float* Ed;
-    float* g;
+    T* u;      // [nbm][4]
+    T* te;     // [nbm][4]
+    T* E;      // [ng][nbm]
+    T* Ed;     // [ng][nbm]
+    T* fR;
+    T* gR;
+    T* g;      // [BW][3 nbm]
+    T* row;
+    T* red;    // [BW][64]
     int* p;
-    float* row;
-    int nbm;
 };
 
-__device__ inline BasisLds basis_lds(void* base, int wv, int nbm, int ng, int kmax, int mode) {
-    char* c = (char*)base + (size_t)wv * basis_lds_wave_bytes(nbm, ng, kmax, mode);
-    BasisLds L;
-    L.nbm = nbm;
-    L.u = (float4*)c; c += 16 * nbm;
-    L.te = (float4*)c; if (mode == 2) c += 16 * nbm;
-    L.E = (float*)c; c += 4 * ng * nbm;
-    L.row = (float*)c; c += 4 * ((kmax + 3) & ~3);
-    L.Ed = (float*)c; if (mode >= 1) c += 4 * ng * nbm;
-    L.g = (float*)c; if (mode == 1) c += 12 * nbm;
+template <class T>
+__device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, int mode) {
+    T* c = (T*)base;
+    BasisLds<T> L;
+    L.u = c; c += 4 * nbm;
+    L.te = c; if (mode == 2) c += 4 * nbm;
+    L.E = c; c += ng * nbm;
+    L.Ed = c; if (mode >= 1) c += ng * nbm;
+    L.fR = c; c += nbm;
+    L.gR = c; c += nbm;
+    L.g = c; if (mode == 1) c += BW * 3 * nbm;
+    L.row = c; c += (kmax + 3) & ~3;
+    L.red = c; c += BW * 64;
     L.p = (int*)c;
     return L;
 }
 
 // (a, c) of flattened triplet t in block (J,K): rectangle a in J, c in K; triangle a<c in J.
-__device__ inline void tri_decode(int t, bool same, int nJ, int nK, int& a, int& c) {
+__device__ inline void tri_decode(int t, bool same, int nJ, int nK, float rnK, int& a, int& c) {
     if (!same) {
-        a = t / nK;
+        a = (int)(((float)t + 0.5f) * rnK);
         c = t - a * nK;
+        if (c < 0) { --a; c += nK; }
+        else if (c >= nK) { ++a; c -= nK; }
     } else {
         const float n2 = 2.f * nJ - 1.f;
-        int r = (int)floorf(0.5f * (n2 - sqrtf(n2 * n2 - 8.f * (float)t)));
+        int r = (int)floorf(0.5f * (n2 - sqrtf(fmaxf(n2 * n2 - 8.f * (float)t, 0.f))));
         if (r < 0) r = 0;
         // S(r) = r (2 nJ - r - 1) / 2 pairs precede row r
         while (r > 0 && (r * (2 * nJ - r - 1)) / 2 > t) --r;
@@ -67,298 +88,544 @@ __device__ inline void tri_decode(int t, bool same, int nJ, int nK, int& a, int&
     }
 }
 
-// angular damping product and its partials for group g (GAUSS: separable)
-template <int ADJ>
-__device__ inline void load_groups(const BasisLds& L, int ng, int sa, int sc, float* Da, float* Dc,
-                                   float* Dda, float* Ddc) {
-#pragma unroll
-    for (int g = 0; g < ACE_MAXG; ++g) {
-        if (g < ng) {
-            Da[g] = L.E[g * L.nbm + sa];
-            Dc[g] = L.E[g * L.nbm + sc];
-            if (ADJ) {
-                Dda[g] = L.Ed[g * L.nbm + sa];
-                Ddc[g] = L.Ed[g * L.nbm + sc];
-            }
-        } else {
-            Da[g] = 0.f; Dc[g] = 0.f;
-            if (ADJ) { Dda[g] = 0.f; Ddc[g] = 0.f; }
-        }
-    }
-}
-
-__device__ inline float pick4(const float* v, int g) {
-    float r = v[0];
+// pick v[g] for a workgroup-uniform g < ACE_MAXG
+template <class T>
+__device__ inline T pick4(const T* v, int g) {
+    T r = v[0];
     r = (g == 1) ? v[1] : r;
     r = (g == 2) ? v[2] : r;
     r = (g == 3) ? v[3] : r;
     return r;
 }
 
-template <int MAXA, int MAXR, int MODE>
-__global__ __launch_bounds__(BW * 64) void k_basis(
-    int natoms, const DevModel* __restrict__ M, DevBasisParams P, const int* __restrict__ type,
-    const int* __restrict__ nb_off, const int* __restrict__ nb_j, const float4* __restrict__ nb_d,
-    const int* __restrict__ row_of, int ldg, float* __restrict__ G, const float* __restrict__ dEdG,
-    float4* __restrict__ pair_g, int* __restrict__ err, int nbm, const double* __restrict__ eps) {
+// c[s] for a workgroup-uniform s < 14 (a scalar branch, no indexed register access)
+template <class T>
+__device__ inline T sel14(const T* c, int s) {
+    switch (s) {
+        case 0: return c[0];   case 1: return c[1];   case 2: return c[2];   case 3: return c[3];
+        case 4: return c[4];   case 5: return c[5];   case 6: return c[6];   case 7: return c[7];
+        case 8: return c[8];   case 9: return c[9];   case 10: return c[10]; case 11: return c[11];
+        case 12: return c[12]; default: return c[13];
+    }
+}
+
+template <class T>
+__device__ inline T prm(const float* f, const double* d, int i) {
+    if constexpr (sizeof(T) == 8) return d[i];
+    else return f[i];
+}
+
+// One angular block whose functions are all canonical: lambda = +-1 and zeta = 2^k, k < nz <= 7, one
+// function per (sign s, level k, eta group g).  Per triplet the windows b_s = |1/2 + s cos/2| are raised
+// to 2^k (cp) and 2^k - 1 (cq) by squaring, and the block's sums are the outer products
+//   G[s][k][g] += b_s^(2^k) E_j[g] E_k[g]
+// held in registers with static indices; cidx maps each function of the block to (s*7 + k)*NG + g.
+template <class T, int NG, int MODE>
+__device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int ng, int bJ, int bK, int nJ, int nK,
+                                     bool same, int T3, const int* __restrict__ cidx, int nA, int goff, T* gw,
+                                     int tid, int wv, int lane) {
+    constexpr int NZ = 7;
     constexpr int ADJ = (MODE == 1);
     constexpr int TAN = (MODE == 2);
-    extern __shared__ float4 basis_smem[];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = blockIdx.x * BW + wv;
+    T acc[2][NZ][NG], wgt[2][NZ][NG];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int k = 0; k < NZ; ++k)
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                acc[s][k][g] = T(0);
+                wgt[s][k][g] = T(0);
+            }
+    if (ADJ) {
+        // dE/dG of the block scattered to the (s, k, g) table, then read back into registers
+        for (int q = tid; q < 2 * NZ * NG; q += BT) L.red[q] = T(0);
+        __syncthreads();
+        for (int m = tid; m < nA; m += BT) L.red[cidx[m]] = L.row[goff + m];
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < NZ; ++k)
+#pragma unroll
+                for (int g = 0; g < NG; ++g) wgt[s][k][g] = L.red[(s * NZ + k) * NG + g];
+        __syncthreads();
+    }
+    const float rnK = nK > 0 ? 1.f / (float)nK : 0.f;
+    for (int t = tid; t < T3; t += BT) {
+        int a, c;
+        tri_decode(t, same, nJ, nK, rnK, a, c);
+        const int sa = bJ + a, sc = bK + c;
+        const T uax = L.u[4 * sa], uay = L.u[4 * sa + 1], uaz = L.u[4 * sa + 2], ra = L.u[4 * sa + 3];
+        const T ucx = L.u[4 * sc], ucy = L.u[4 * sc + 1], ucz = L.u[4 * sc + 2], rc = L.u[4 * sc + 3];
+        const T cs = uax * ucx + uay * ucy + uaz * ucz;
+        T Da[NG], Dc[NG], Dda[NG], Ddc[NG], D[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const bool on = g < ng;
+            Da[g] = on ? L.E[g * nbm + sa] : T(0);
+            Dc[g] = on ? L.E[g * nbm + sc] : T(0);
+            Dda[g] = (on && (ADJ || TAN)) ? L.Ed[g * nbm + sa] : T(0);
+            Ddc[g] = (on && (ADJ || TAN)) ? L.Ed[g * nbm + sc] : T(0);
+            D[g] = Da[g] * Dc[g];
+        }
+        T cp[2][NZ], cq[2][NZ];
+        cp[0][0] = fabs(T(0.5) + T(0.5) * cs);
+        cp[1][0] = fabs(T(0.5) - T(0.5) * cs);
+        cq[0][0] = T(1);
+        cq[1][0] = T(1);
+#pragma unroll
+        for (int k = 1; k < NZ; ++k) {
+            if (k < nz) {
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    cq[s][k] = cq[s][k - 1] * cp[s][k - 1];
+                    cp[s][k] = cp[s][k - 1] * cp[s][k - 1];
+                }
+            }
+        }
+        if (MODE == 0) {
+#pragma unroll
+            for (int k = 0; k < NZ; ++k)
+                if (k < nz)
+#pragma unroll
+                    for (int s = 0; s < 2; ++s)
+#pragma unroll
+                        for (int g = 0; g < NG; ++g) acc[s][k][g] += cp[s][k] * D[g];
+        } else if (TAN) {
+            const T tax = L.te[4 * sa], tay = L.te[4 * sa + 1], taz = L.te[4 * sa + 2], tap = L.te[4 * sa + 3];
+            const T tcx = L.te[4 * sc], tcy = L.te[4 * sc + 1], tcz = L.te[4 * sc + 2], tcp = L.te[4 * sc + 3];
+            const T dcs = (ucx * tax + ucy * tay + ucz * taz - cs * tap) / ra +
+                          (uax * tcx + uay * tcy + uaz * tcz - cs * tcp) / rc;
+            T X[NG], Y[NG];
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                X[g] = D[g] * dcs;
+                Y[g] = Dda[g] * Dc[g] * tap + Da[g] * Ddc[g] * tcp;
+            }
+#pragma unroll
+            for (int k = 0; k < NZ; ++k)
+                if (k < nz)
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        const T dp = T(1 << k) * (s ? T(-0.5) : T(0.5)) * cq[s][k];   // d b^(2^k) / d cos
+#pragma unroll
+                        for (int g = 0; g < NG; ++g) acc[s][k][g] += dp * X[g] + cp[s][k] * Y[g];
+                    }
+        } else {
+            T S[NG], Sp[NG];
+#pragma unroll
+            for (int g = 0; g < NG; ++g) { S[g] = T(0); Sp[g] = T(0); }
+#pragma unroll
+            for (int k = 0; k < NZ; ++k)
+                if (k < nz)
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        const T dp = T(1 << k) * (s ? T(-0.5) : T(0.5)) * cq[s][k];
+#pragma unroll
+                        for (int g = 0; g < NG; ++g) {
+                            S[g] += wgt[s][k][g] * cp[s][k];
+                            Sp[g] += wgt[s][k][g] * dp;
+                        }
+                    }
+            T dEdc = T(0), dEda = T(0), dEdcc = T(0);
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                dEdc += Sp[g] * D[g];
+                dEda += S[g] * Dda[g] * Dc[g];
+                dEdcc += S[g] * Da[g] * Ddc[g];
+            }
+            const T ia = T(1) / ra, ic = T(1) / rc;
+            const T ka = dEda - dEdc * cs * ia, kc = dEdcc - dEdc * cs * ic;
+            const T xa = dEdc * ia, xc = dEdc * ic;
+            atomicAdd(&gw[3 * sa], ka * uax + xa * ucx);
+            atomicAdd(&gw[3 * sa + 1], ka * uay + xa * ucy);
+            atomicAdd(&gw[3 * sa + 2], ka * uaz + xa * ucz);
+            atomicAdd(&gw[3 * sc], kc * ucx + xc * uax);
+            atomicAdd(&gw[3 * sc + 1], kc * ucy + xc * uay);
+            atomicAdd(&gw[3 * sc + 2], kc * ucz + xc * uaz);
+        }
+    }
+    if (!ADJ) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < NZ; ++k)
+                if (k < nz)
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) {
+                        const T v = t_wave_sum<T>(acc[s][k][g]);
+                        if (lane == 0) L.red[wv * 64 + (s * NZ + k) * NG + g] = v;
+                    }
+        __syncthreads();
+        for (int m = tid; m < nA; m += BT) {
+            T v = T(0);
+            for (int w = 0; w < BW; ++w) v += L.red[w * 64 + cidx[m]];
+            L.row[goff + m] = v;
+        }
+        __syncthreads();
+    }
+}
+
+template <class T, int NG, int MODE>
+__global__ __launch_bounds__(BT) void k_basis(
+    int natoms, const DevModel* __restrict__ M, DevBasisParams P, const int* __restrict__ type,
+    const int* __restrict__ nb_off, const int* __restrict__ nb_j, const float4* __restrict__ nb_d,
+    const int* __restrict__ nb_img, const double* __restrict__ pos, const int* __restrict__ struc_of,
+    const StrucInfo* __restrict__ sinfo, const int* __restrict__ row_of, int ldg, float* __restrict__ G,
+    const float* __restrict__ dEdG, float4* __restrict__ pair_g, int* __restrict__ err, int nbm,
+    const double* __restrict__ eps) {
+    constexpr int ADJ = (MODE == 1);
+    constexpr int TAN = (MODE == 2);
+    constexpr bool F64 = sizeof(T) == 8;
+    constexpr int MAXA = 16;   // functions per register chunk of the generic path
+    extern __shared__ double4 basis_smem[];
+    __shared__ int s_cnt[ACE_MAXT + 1];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int i = blockIdx.x;
     if (i >= natoms) return;
     const int I = type[i];
     const DevSpecies& sp = M->sp[I];
-    const BasisLds L = basis_lds(basis_smem, wv, nbm, M->ng_max, M->kmax, MODE);
+    const BasisLds<T> L = basis_lds<T>(basis_smem, nbm, M->ng_max, M->kmax, MODE);
     const int nt = M->ntypes;
     const int o0 = nb_off[i], o1 = nb_off[i + 1];
     const float rcut = sp.rc_max;
     const int row = row_of[i];
+    const int ng = sp.ng;
 
-    // --- bucket the neighbours by species ---------------------------------------------------
-    int cnt[ACE_MAXT];
+    // --- wave 0: bucket the neighbours by species (ballot order keeps the list order) -------------
+    if (wv == 0) {
+        int cnt[ACE_MAXT];
 #pragma unroll
-    for (int J = 0; J < ACE_MAXT; ++J) cnt[J] = 0;
-    for (int cb = o0; cb < o1; cb += 64) {
-        const int p = cb + lane;
-        int Jl = -1;
-        if (p < o1 && nb_d[p].w < rcut) Jl = type[nb_j[p]];
+        for (int J = 0; J < ACE_MAXT; ++J) cnt[J] = 0;
+        for (int cb = o0; cb < o1; cb += 64) {
+            const int p = cb + lane;
+            int Jl = -1;
+            if (p < o1 && nb_d[p].w < rcut) Jl = type[nb_j[p]];
 #pragma unroll
-        for (int J = 0; J < ACE_MAXT; ++J)
-            if (J < nt) cnt[J] += __popcll(__ballot(Jl == J));
-    }
-    int bstart[ACE_MAXT + 1];
-    bstart[0] = 0;
-#pragma unroll
-    for (int J = 0; J < ACE_MAXT; ++J) bstart[J + 1] = bstart[J] + ((J < nt) ? cnt[J] : 0);
-    const int ntot = bstart[ACE_MAXT];
-    if (ntot > nbm) {
-        if (lane == 0) atomicAdd(err, 1);
-        return;
-    }
-    int fill[ACE_MAXT];
-#pragma unroll
-    for (int J = 0; J < ACE_MAXT; ++J) fill[J] = bstart[J];
-    for (int cb = o0; cb < o1; cb += 64) {
-        const int p = cb + lane;
-        int Jl = -1;
-        float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (p < o1) {
-            d = nb_d[p];
-            if (d.w < rcut) Jl = type[nb_j[p]];
-            else if (ADJ) pair_g[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int J = 0; J < ACE_MAXT; ++J)
+                if (J < nt) cnt[J] += __popcll(__ballot(Jl == J));
         }
-        int slot = -1;
+        int fill[ACE_MAXT];
+        int acc = 0;
 #pragma unroll
         for (int J = 0; J < ACE_MAXT; ++J) {
-            if (J < nt) {
-                const unsigned long long m = __ballot(Jl == J);
-                if (Jl == J) slot = fill[J] + __popcll(m & lane_mask_lt());
-                fill[J] += __popcll(m);
-            }
+            fill[J] = acc;
+            acc += (J < nt) ? cnt[J] : 0;
         }
-        if (slot >= 0) {
-            const float r = d.w, ir = 1.f / r;
-            L.u[slot] = make_float4(d.x * ir, d.y * ir, d.z * ir, r);
-            if (ADJ) L.p[slot] = p;
-            float fc, gc;
-            cutoff_fg(sp.a_cut, sp.a_rc, r, fc, gc);
+        if (lane == 0) {
 #pragma unroll
-            for (int g = 0; g < ACE_MAXG; ++g) {
-                if (g < sp.ng) {
-                    const float e = fast_exp(-r * r * sp.ietar2[g]);
-                    L.E[g * nbm + slot] = fc * e;
-                    if (ADJ || TAN) L.Ed[g * nbm + slot] = (gc - 2.f * r * sp.ietar2[g] * fc) * e;
+            for (int J = 0; J <= ACE_MAXT; ++J)
+                if (J <= nt) s_cnt[J] = (J < nt) ? fill[J] : acc;
+        }
+        if (acc <= nbm) {
+            for (int cb = o0; cb < o1; cb += 64) {
+                const int p = cb + lane;
+                int Jl = -1;
+                if (p < o1) {
+                    if (nb_d[p].w < rcut) Jl = type[nb_j[p]];
+                    else if (ADJ) pair_g[p] = make_float4(0.f, 0.f, 0.f, 0.f);
                 }
-            }
-            if (ADJ) {
-                L.g[3 * slot] = 0.f;
-                L.g[3 * slot + 1] = 0.f;
-                L.g[3 * slot + 2] = 0.f;
-            }
-            if (TAN) {
-                const int jj = nb_j[p];
-                const float ex = (float)(eps[3 * i] - eps[3 * jj]);
-                const float ey = (float)(eps[3 * i + 1] - eps[3 * jj + 1]);
-                const float ez = (float)(eps[3 * i + 2] - eps[3 * jj + 2]);
-                L.te[slot] = make_float4(ex, ey, ez, (d.x * ex + d.y * ey + d.z * ez) * ir);
+#pragma unroll
+                for (int J = 0; J < ACE_MAXT; ++J) {
+                    if (J < nt) {
+                        const unsigned long long m = __ballot(Jl == J);
+                        if (Jl == J) L.p[fill[J] + __popcll(m & lane_mask_lt())] = p;
+                        fill[J] += __popcll(m);
+                    }
+                }
             }
         }
     }
-    // row buffer: zero (forward) or dE/dG (adjoint)
-    for (int k = lane; k < sp.K; k += 64) L.row[k] = ADJ ? dEdG[(long long)row * ldg + k] : 0.f;
-    wave_sync();
+    __syncthreads();
+    const int ntot = s_cnt[nt];
+    if (ntot > nbm) {
+        if (tid == 0) atomicAdd(err, 1);
+        return;
+    }
 
-    // --- radial blocks -------------------------------------------------------------------------
+    // --- per-slot quantities ---------------------------------------------------------------------
+    const StrucInfo* S = F64 ? &sinfo[struc_of[i]] : nullptr;
+    for (int q = tid; q < ntot; q += BT) {
+        const int p = L.p[q];
+        int J = 0;
+#pragma unroll
+        for (int jj = 1; jj < ACE_MAXT; ++jj)
+            if (jj < nt && q >= s_cnt[jj]) J = jj;
+        T dx, dy, dz;
+        if constexpr (F64) {      // r_ij = posn(i) - (posn(j) + R img) in FP64 (nnp.cpp:363)
+            int a, b, c;
+            img_unpack(nb_img[p], a, b, c);
+            const int j = nb_j[p];
+            dx = pos[3 * i] - (pos[3 * j] + (S->R[0] * a + S->R[3] * b + S->R[6] * c));
+            dy = pos[3 * i + 1] - (pos[3 * j + 1] + (S->R[1] * a + S->R[4] * b + S->R[7] * c));
+            dz = pos[3 * i + 2] - (pos[3 * j + 2] + (S->R[2] * a + S->R[5] * b + S->R[8] * c));
+        } else {
+            const float4 d = nb_d[p];
+            dx = d.x; dy = d.y; dz = d.z;
+        }
+        const T r = sqrt(dx * dx + dy * dy + dz * dz), ir = T(1) / r;
+        L.u[4 * q] = dx * ir;
+        L.u[4 * q + 1] = dy * ir;
+        L.u[4 * q + 2] = dz * ir;
+        L.u[4 * q + 3] = r;
+        T fc, gc;
+        t_cutoff<T>(sp.r_cut[J], F64 ? (T)sp.r_rc64[J] : (T)sp.r_rc[J], r, fc, gc);
+        L.fR[q] = fc;
+        L.gR[q] = gc;
+        t_cutoff<T>(sp.a_cut, F64 ? (T)sp.a_rc64 : (T)sp.a_rc, r, fc, gc);
+        for (int g = 0; g < ng; ++g) {
+            const T ie = F64 ? (T)sp.ietar2_64[g] : (T)sp.ietar2[g];
+            const T e = t_exp<T>(-r * r * ie);
+            L.E[g * nbm + q] = fc * e;
+            if (ADJ || TAN) L.Ed[g * nbm + q] = (gc - T(2) * r * ie * fc) * e;
+        }
+        if (ADJ) {
+            for (int w = 0; w < BW; ++w) {
+                L.g[w * 3 * nbm + 3 * q] = T(0);
+                L.g[w * 3 * nbm + 3 * q + 1] = T(0);
+                L.g[w * 3 * nbm + 3 * q + 2] = T(0);
+            }
+        }
+        if (TAN) {
+            const int jj = nb_j[p];
+            const T ex = (T)(eps[3 * i] - eps[3 * jj]);
+            const T ey = (T)(eps[3 * i + 1] - eps[3 * jj + 1]);
+            const T ez = (T)(eps[3 * i + 2] - eps[3 * jj + 2]);
+            L.te[4 * q] = ex;
+            L.te[4 * q + 1] = ey;
+            L.te[4 * q + 2] = ez;
+            L.te[4 * q + 3] = (dx * ex + dy * ey + dz * ez) * ir;
+        }
+    }
+    for (int k = tid; k < sp.K; k += BT) L.row[k] = ADJ ? (T)dEdG[(long long)row * ldg + k] : T(0);
+    __syncthreads();
+
+    // --- radial blocks: lane = function, lane group = neighbour subset ---------------------------
     for (int J = 0; J < nt; ++J) {
-        const int nJ = cnt[J];
+        const int bJ = s_cnt[J], nJ = s_cnt[J + 1] - s_cnt[J];
         const int nR = sp.r_n[J];
         if (nJ == 0 || nR == 0) continue;
-        const int form = sp.r_form[J], cut = sp.r_cut[J];
-        const float rc = sp.r_rc[J];
+        const int form = sp.r_form[J];
         const int goff = sp.r_goff[J], poff = sp.r_poff[J];
-        for (int n0 = 0; n0 < nR; n0 += MAXR) {
-            float acc[MAXR];
-#pragma unroll
-            for (int n = 0; n < MAXR; ++n) acc[n] = 0.f;
-            for (int q = lane; q < nJ; q += 64) {
-                const int slot = bstart[J] + q;
-                const float4 u = L.u[slot];
-                float fc, gc;
-                cutoff_fg(cut, rc, u.w, fc, gc);
-                float dEdr = 0.f;
-#pragma unroll
-                for (int n = 0; n < MAXR; ++n) {
-                    if (n0 + n < nR) {
-                        float v, dv;
-                        radial_fd(form, P.r_eta[poff + n0 + n], P.r_rs[poff + n0 + n], u.w, fc, gc, v, dv);
-                        if (ADJ) dEdr += L.row[goff + n0 + n] * dv;
-                        else if (TAN) acc[n] += dv * L.te[slot].w;
-                        else acc[n] += v;
-                    }
+        for (int n0 = 0; n0 < nR; n0 += 64) {
+            const int nc = min(64, nR - n0);
+            int nrp = 1;
+            while (nrp < nc) nrp <<= 1;
+            const int gpw = 64 / nrp, ngr = BW * gpw;
+            const int n = lane % nrp, grp = wv * gpw + lane / nrp;
+            const bool on = n < nc;
+            const T eta = on ? prm<T>(P.r_eta, P.r_eta64, poff + n0 + n) : T(0);
+            const T rs = on ? prm<T>(P.r_rs, P.r_rs64, poff + n0 + n) : T(0);
+            const T w = (ADJ && on) ? L.row[goff + n0 + n] : T(0);
+            T acc = T(0);
+            const int rounds = (nJ + ngr - 1) / ngr;
+            for (int t = 0; t < rounds; ++t) {
+                const int q = grp + t * ngr;
+                T val = T(0);
+                if (on && q < nJ) {
+                    const int slot = bJ + q;
+                    T v, dv;
+                    t_radial<T>(form, eta, rs, L.u[4 * slot + 3], L.fR[slot], L.gR[slot], v, dv);
+                    val = ADJ ? w * dv : (TAN ? dv * L.te[4 * slot + 3] : v);
                 }
                 if (ADJ) {
-                    L.g[3 * slot] += dEdr * u.x;
-                    L.g[3 * slot + 1] += dEdr * u.y;
-                    L.g[3 * slot + 2] += dEdr * u.z;
+                    // dE/dr of the slot: sum over the nrp function lanes of its group
+                    for (int m = 1; m < nrp; m <<= 1) val += __shfl_xor(val, m, 64);
+                    if (n == 0 && q < nJ) {
+                        const int slot = bJ + q;
+                        L.g[3 * slot] += val * L.u[4 * slot];          // wave-private array of wave 0..3
+                        L.g[3 * slot + 1] += val * L.u[4 * slot + 1];
+                        L.g[3 * slot + 2] += val * L.u[4 * slot + 2];
+                    }
+                } else {
+                    acc += val;
                 }
             }
             if (!ADJ) {
-#pragma unroll
-                for (int n = 0; n < MAXR; ++n) {
-                    if (n0 + n < nR) {
-                        const float t = wave_sum(acc[n]);
-                        if (lane == 0) L.row[goff + n0 + n] = t;
-                    }
+                for (int m = nrp; m < 64; m <<= 1) acc += __shfl_xor(acc, m, 64);
+                if (lane < nrp) L.red[wv * 64 + lane] = acc;
+                __syncthreads();
+                if (tid < nc) {
+                    T s = T(0);
+                    for (int ww = 0; ww < BW; ++ww) s += L.red[ww * 64 + tid];
+                    L.row[goff + n0 + tid] = s;
                 }
+                __syncthreads();
             }
         }
     }
-    wave_sync();
+    if (ADJ) __syncthreads();
 
-    // --- angular blocks (J <= K) ------------------------------------------------------------------
-    const int ng = sp.ng;
+    // --- angular blocks (J <= K) -------------------------------------------------------------------
+    T* gw = L.g + wv * 3 * nbm;
     for (int K = 0; K < nt; ++K) {
         for (int J = 0; J <= K; ++J) {
             const int b = K * (K + 1) / 2 + J;
             const int nA = sp.a_n[b];
             const bool same = (J == K);
-            const int nJ = cnt[J], nK = cnt[K];
-            const int T = same ? nJ * (nJ - 1) / 2 : nJ * nK;
-            if (nA == 0 || T == 0) continue;
+            const int bJ = s_cnt[J], bK = s_cnt[K];
+            const int nJ = s_cnt[J + 1] - bJ, nK = s_cnt[K + 1] - bK;
+            const int T3 = same ? nJ * (nJ - 1) / 2 : nJ * nK;
+            if (nA == 0 || T3 == 0) continue;
+            if (sp.a_canon[b]) {
+                angular_canon<T, NG, MODE>(L, nbm, sp.nz, ng, bJ, bK, nJ, nK, same, T3, P.a_cidx + sp.a_poff[b], nA,
+                                           sp.a_goff[b], gw, tid, wv, lane);
+                continue;
+            }
             const int goff = sp.a_goff[b], poff = sp.a_poff[b];
-            const int bJ = bstart[J], bK = bstart[K];
+            const float rnK = nK > 0 ? 1.f / (float)nK : 0.f;
             for (int m0 = 0; m0 < nA; m0 += MAXA) {
-                float acc[MAXA];
-                float w[MAXA];
+                T lamf[MAXA], zeta[MAXA], acc[MAXA], w[MAXA];
+                int zint[MAXA], grp[MAXA], sel[MAXA];
+                int zmax = 1;
 #pragma unroll
                 for (int m = 0; m < MAXA; ++m) {
-                    acc[m] = 0.f;
-                    w[m] = (ADJ && m0 + m < nA) ? L.row[goff + m0 + m] : 0.f;
+                    const bool on = m0 + m < nA;
+                    const int fi = poff + m0 + m;
+                    lamf[m] = on ? prm<T>(P.a_lamf, P.a_lamf64, fi) : T(0.5);
+                    zeta[m] = on ? prm<T>(P.a_zeta, P.a_zeta64, fi) : T(1);
+                    zint[m] = on ? P.a_zint[fi] : 1;
+                    grp[m] = on ? P.a_grp[fi] : 0;
+                    sel[m] = on ? P.a_sel[fi] : 0;
+                    acc[m] = T(0);
+                    w[m] = (ADJ && on) ? L.row[goff + m0 + m] : T(0);
+                    zmax = max(zmax, zint[m]);
                 }
-                for (int t = lane; t < T; t += 64) {
+                int nk = 1;
+                while (nk < 7 && (1 << nk) <= zmax) ++nk;
+                for (int t = tid; t < T3; t += BT) {
                     int a, c;
-                    tri_decode(t, same, nJ, nK, a, c);
+                    tri_decode(t, same, nJ, nK, rnK, a, c);
                     const int sa = bJ + a, sc = bK + c;
-                    const float4 ua = L.u[sa], uc = L.u[sc];
-                    const float cs = ua.x * uc.x + ua.y * uc.y + ua.z * uc.z;
-                    float Da[ACE_MAXG], Dc[ACE_MAXG], Dda[ACE_MAXG], Ddc[ACE_MAXG];
-                    load_groups<(MODE >= 1)>(L, ng, sa, sc, Da, Dc, Dda, Ddc);
-                    if (MODE == 0) {
-                        float D[ACE_MAXG];
+                    const T uax = L.u[4 * sa], uay = L.u[4 * sa + 1], uaz = L.u[4 * sa + 2], ra = L.u[4 * sa + 3];
+                    const T ucx = L.u[4 * sc], ucy = L.u[4 * sc + 1], ucz = L.u[4 * sc + 2], rc = L.u[4 * sc + 3];
+                    const T cs = uax * ucx + uay * ucy + uaz * ucz;
+                    T Da[ACE_MAXG], Dc[ACE_MAXG], Dda[ACE_MAXG], Ddc[ACE_MAXG];
 #pragma unroll
-                        for (int g = 0; g < ACE_MAXG; ++g) D[g] = Da[g] * Dc[g];
+                    for (int g = 0; g < ACE_MAXG; ++g) {
+                        const bool on = g < ng;
+                        Da[g] = on ? L.E[g * nbm + sa] : T(0);
+                        Dc[g] = on ? L.E[g * nbm + sc] : T(0);
+                        Dda[g] = (on && (ADJ || TAN)) ? L.Ed[g * nbm + sa] : T(0);
+                        Ddc[g] = (on && (ADJ || TAN)) ? L.Ed[g * nbm + sc] : T(0);
+                    }
+                    // cp[k + 7s] = b_s^(2^k), cq[k + 7s] = b_s^(2^k - 1), b_0 = (1+cos)/2, b_1 = (1-cos)/2
+                    T cp[14], cq[14];
+                    cp[0] = fabs(T(0.5) + T(0.5) * cs);
+                    cp[7] = fabs(T(0.5) - T(0.5) * cs);
+                    cq[0] = T(1);
+                    cq[7] = T(1);
 #pragma unroll
-                        for (int m = 0; m < MAXA; ++m) {
-                            if (m0 + m < nA) {
-                                const int fi = poff + m0 + m;
-                                const float base = fabsf(0.5f + P.a_lamf[fi] * cs);
-                                const float pw = pow_window(base, P.a_zeta[fi], P.a_zint[fi]);
-                                acc[m] += pw * pick4(D, P.a_grp[fi]);
-                            }
-                        }
-                    } else if (TAN) {
-                        const float4 ta = L.te[sa], tc = L.te[sc];
-                        const float dcs = (uc.x * ta.x + uc.y * ta.y + uc.z * ta.z - cs * ta.w) / ua.w +
-                                          (ua.x * tc.x + ua.y * tc.y + ua.z * tc.z - cs * tc.w) / uc.w;
-                        float X[ACE_MAXG], Y[ACE_MAXG];
+                    for (int k = 1; k < 7; ++k) {
+                        const bool on = k < nk;
+                        cq[k] = on ? cq[k - 1] * cp[k - 1] : T(0);
+                        cp[k] = on ? cp[k - 1] * cp[k - 1] : T(0);
+                        cq[k + 7] = on ? cq[k + 6] * cp[k + 6] : T(0);
+                        cp[k + 7] = on ? cp[k + 6] * cp[k + 6] : T(0);
+                    }
+                    T X[ACE_MAXG], Y[ACE_MAXG], S[ACE_MAXG], Sp[ACE_MAXG];
+#pragma unroll
+                    for (int g = 0; g < ACE_MAXG; ++g) {
+                        X[g] = Da[g] * Dc[g];
+                        Y[g] = T(0);
+                        S[g] = T(0);
+                        Sp[g] = T(0);
+                    }
+                    if (TAN) {
+                        const T tax = L.te[4 * sa], tay = L.te[4 * sa + 1], taz = L.te[4 * sa + 2], tap = L.te[4 * sa + 3];
+                        const T tcx = L.te[4 * sc], tcy = L.te[4 * sc + 1], tcz = L.te[4 * sc + 2], tcp = L.te[4 * sc + 3];
+                        const T dcs = (ucx * tax + ucy * tay + ucz * taz - cs * tap) / ra +
+                                      (uax * tcx + uay * tcy + uaz * tcz - cs * tcp) / rc;
 #pragma unroll
                         for (int g = 0; g < ACE_MAXG; ++g) {
-                            X[g] = Da[g] * Dc[g] * dcs;
-                            Y[g] = Dda[g] * Dc[g] * ta.w + Da[g] * Ddc[g] * tc.w;
+                            Y[g] = Dda[g] * Dc[g] * tap + Da[g] * Ddc[g] * tcp;
+                            X[g] *= dcs;
                         }
+                    }
 #pragma unroll
-                        for (int m = 0; m < MAXA; ++m) {
-                            if (m0 + m < nA) {
-                                const int fi = poff + m0 + m;
-                                const float lamf = P.a_lamf[fi], zeta = P.a_zeta[fi];
-                                const int zint = P.a_zint[fi];
-                                const float base = fabsf(0.5f + lamf * cs);
-                                const float p1 = (zint >= 0) ? powi_f(base, zint - 1)
-                                               : pow_window(base, zeta - 1.f, -1);
-                                const int g = P.a_grp[fi];
-                                acc[m] += zeta * lamf * p1 * pick4(X, g) + p1 * base * pick4(Y, g);
+                    for (int m = 0; m < MAXA; ++m) {
+                        if (m0 + m < nA) {
+                            T base, p1;          // window and window^(zeta-1)
+                            if (sel[m] >= 0) {
+                                base = (sel[m] < 7) ? cp[0] : cp[7];
+                                p1 = sel14<T>(cq, sel[m]);
+                            } else {
+                                base = fabs(T(0.5) + lamf[m] * cs);
+                                p1 = (zint[m] >= 1) ? t_pow_window<T>(base, T(0), zint[m] - 1)
+                                                    : t_pow_window<T>(base, zeta[m] - T(1), -1);
                             }
-                        }
-                    } else {
-                        float S[ACE_MAXG], Sp[ACE_MAXG];
-#pragma unroll
-                        for (int g = 0; g < ACE_MAXG; ++g) { S[g] = 0.f; Sp[g] = 0.f; }
-#pragma unroll
-                        for (int m = 0; m < MAXA; ++m) {
-                            if (m0 + m < nA) {
-                                const int fi = poff + m0 + m;
-                                const float lamf = P.a_lamf[fi], zeta = P.a_zeta[fi];
-                                const int zint = P.a_zint[fi];
-                                const float base = fabsf(0.5f + lamf * cs);
-                                const float p1 = (zint >= 0) ? powi_f(base, zint - 1)
-                                               : pow_window(base, zeta - 1.f, -1);
-                                const float s = w[m] * p1 * base;
-                                const float sp_ = w[m] * zeta * lamf * p1;
-                                const int g = P.a_grp[fi];
+                            if (MODE == 0) {
+                                acc[m] += p1 * base * pick4<T>(X, grp[m]);
+                            } else if (TAN) {
+                                acc[m] += zeta[m] * lamf[m] * p1 * pick4<T>(X, grp[m]) + p1 * base * pick4<T>(Y, grp[m]);
+                            } else {
+                                const T s = w[m] * p1 * base, s1 = w[m] * zeta[m] * lamf[m] * p1;
 #pragma unroll
                                 for (int gg = 0; gg < ACE_MAXG; ++gg) {
-                                    S[gg] += (g == gg) ? s : 0.f;
-                                    Sp[gg] += (g == gg) ? sp_ : 0.f;
+                                    S[gg] += (grp[m] == gg) ? s : T(0);
+                                    Sp[gg] += (grp[m] == gg) ? s1 : T(0);
                                 }
                             }
                         }
-                        float dEdc = 0.f, dEda = 0.f, dEdcc = 0.f;
+                    }
+                    if (ADJ) {
+                        T dEdc = T(0), dEda = T(0), dEdcc = T(0);
 #pragma unroll
                         for (int g = 0; g < ACE_MAXG; ++g) {
-                            dEdc += Sp[g] * Da[g] * Dc[g];
+                            dEdc += Sp[g] * X[g];
                             dEda += S[g] * Dda[g] * Dc[g];
                             dEdcc += S[g] * Da[g] * Ddc[g];
                         }
-                        const float ia = 1.f / ua.w, ic = 1.f / uc.w;
-                        const float ka = dEda - dEdc * cs * ia, kc = dEdcc - dEdc * cs * ic;
-                        const float xa = dEdc * ia, xc = dEdc * ic;
-                        atomicAdd(&L.g[3 * sa], ka * ua.x + xa * uc.x);
-                        atomicAdd(&L.g[3 * sa + 1], ka * ua.y + xa * uc.y);
-                        atomicAdd(&L.g[3 * sa + 2], ka * ua.z + xa * uc.z);
-                        atomicAdd(&L.g[3 * sc], kc * uc.x + xc * ua.x);
-                        atomicAdd(&L.g[3 * sc + 1], kc * uc.y + xc * ua.y);
-                        atomicAdd(&L.g[3 * sc + 2], kc * uc.z + xc * ua.z);
+                        const T ia = T(1) / ra, ic = T(1) / rc;
+                        const T ka = dEda - dEdc * cs * ia, kc = dEdcc - dEdc * cs * ic;
+                        const T xa = dEdc * ia, xc = dEdc * ic;
+                        atomicAdd(&gw[3 * sa], ka * uax + xa * ucx);
+                        atomicAdd(&gw[3 * sa + 1], ka * uay + xa * ucy);
+                        atomicAdd(&gw[3 * sa + 2], ka * uaz + xa * ucz);
+                        atomicAdd(&gw[3 * sc], kc * ucx + xc * uax);
+                        atomicAdd(&gw[3 * sc + 1], kc * ucy + xc * uay);
+                        atomicAdd(&gw[3 * sc + 2], kc * ucz + xc * uaz);
                     }
                 }
                 if (!ADJ) {
 #pragma unroll
                     for (int m = 0; m < MAXA; ++m) {
                         if (m0 + m < nA) {
-                            const float t = wave_sum(acc[m]);
-                            if (lane == 0) L.row[goff + m0 + m] = t;
+                            const T t = t_wave_sum<T>(acc[m]);
+                            if (lane == 0) L.red[wv * 64 + m] = t;
                         }
                     }
+                    __syncthreads();
+                    if (tid < MAXA && m0 + tid < nA) {
+                        T s = T(0);
+                        for (int ww = 0; ww < BW; ++ww) s += L.red[ww * 64 + tid];
+                        L.row[goff + m0 + tid] = s;
+                    }
+                    __syncthreads();
                 }
             }
         }
     }
-    wave_sync();
-    if (!ADJ) {
-        for (int k = lane; k < sp.K; k += 64) G[(long long)row * ldg + k] = L.row[k];   // G or v
+    __syncthreads();
+    if (MODE == 0) {
+        // network input row: the ANN's input scaling inpw*(G + inpb) (nn.cpp:1406) applied here, in T,
+        // so the FP32 row keeps full relative precision after the mean is removed
+        for (int k = tid; k < sp.K; k += BT) {
+            const T w = prm<T>(P.inpw, P.inpw64, sp.in_off + k), b = prm<T>(P.inpb, P.inpb64, sp.in_off + k);
+            G[(long long)row * ldg + k] = (float)(w * (L.row[k] + b));
+        }
+    } else if (TAN) {
+        for (int k = tid; k < sp.K; k += BT) G[(long long)row * ldg + k] = (float)L.row[k];   // v
     } else {
-        for (int q = lane; q < ntot; q += 64)
-            pair_g[L.p[q]] = make_float4(L.g[3 * q], L.g[3 * q + 1], L.g[3 * q + 2], 0.f);
+        for (int q = tid; q < ntot; q += BT) {
+            T gx = T(0), gy = T(0), gz = T(0);
+            for (int ww = 0; ww < BW; ++ww) {
+                gx += L.g[ww * 3 * nbm + 3 * q];
+                gy += L.g[ww * 3 * nbm + 3 * q + 1];
+                gz += L.g[ww * 3 * nbm + 3 * q + 2];
+            }
+            pair_g[L.p[q]] = make_float4((float)gx, (float)gy, (float)gz, 0.f);
+        }
     }
 }
-

@@ -1,6 +1,7 @@
 // mlp.hip -- per-species atomic networks over species-sorted atom tiles.
 //
-// Forward: ANN::fpbp (src/ml/nn.cpp:1402-1420): ins = inpw*(G+inpb), z_l = b_l + W_l a_{l-1},
+// Forward: ANN::fpbp (src/ml/nn.cpp:1402-1420): ins = inpw*(G+inpb) (applied by the basis kernel when it
+// writes the rows), z_l = b_l + W_l a_{l-1},
 // a_l = f(c z_l) (hidden), last layer linear, out = outb + outw*a_last.
 // Backward: DODZ::grad row 0 (src/ml/nn.cpp:1836-1862): dE/dG = dO/dz_0 W_0 diag(inpw).
 // A workgroup holds a tile of TM atoms of one species; activations and derivatives live in LDS.
@@ -70,9 +71,10 @@ __global__ __launch_bounds__(MLP_THREADS) void k_mlp_energy_grad(
     const float* iw = inpw + sp.in_off;
     const float* ib = inpb + sp.in_off;
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < tm * K; idx += MLP_THREADS) {
+    (void)ib;
+    for (int idx = tid; idx < tm * K; idx += MLP_THREADS) {   // rows already hold inpw*(G + inpb)
         const int r = idx / K, k = idx - r * K;
-        X[r * K + k] = (r < nr) ? iw[k] * (G[(long long)(r0 + r) * ldg + k] + ib[k]) : 0.f;
+        X[r * K + k] = (r < nr) ? G[(long long)(r0 + r) * ldg + k] : 0.f;
     }
     __syncthreads();
     for (int l = 0; l < L; ++l) {

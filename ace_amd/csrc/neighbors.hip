@@ -127,7 +127,7 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
     const StrucInfo* __restrict__ sinfo, const int* __restrict__ cell_of, const int* __restrict__ wrap_of,
     const int* __restrict__ cell_start, const int* __restrict__ cell_atoms, int* __restrict__ counts,
     int* __restrict__ maxcount, const int* __restrict__ nb_off, int* __restrict__ nb_j, int* __restrict__ nb_img,
-    float4* __restrict__ nb_d) {
+    float4* __restrict__ nb_d, int* __restrict__ nb_src) {
     __shared__ int s_cstart[NB_WAVES][64];
     __shared__ int s_cpref[NB_WAVES][65];
     __shared__ int s_cimg[NB_WAVES][64];
@@ -223,9 +223,9 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
         }
         return;
     }
-    // fill: rank sort of the keys, write in reference order
-    const int n = count < NB_LMAX ? count : NB_LMAX;
+    // fill: rank sort of the keys, write in reference order (an emptied list means overflow)
     const int base = nb_off[i];
+    const int n = min(count < NB_LMAX ? count : NB_LMAX, nb_off[i + 1] - base);
     wave_sync();
     for (int e = lane; e < n; e += 64) {
         const long long k = s_key[MODE ? wv : 0][e];
@@ -240,6 +240,7 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
         const int j = (int)jl + S.atom_off;
         const int p = base + rank;
         nb_j[p] = j;
+        nb_src[p] = i;
         nb_img[p] = img_pack(c0, c1, c2);
         // r_IJ = posn(i) - (posn(j) + R*img)  (nnp.cpp:363), FP64, stored as FP32
         double rv[3];
@@ -253,12 +254,12 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
 }
 
 // thread per pair: index of the reverse pair (j -> i, -n) in j's list (binary search on the key)
-__global__ void k_nb_reverse(int npairs, const int* __restrict__ nb_src, const int* __restrict__ nb_j,
+__global__ void k_nb_reverse(const int* __restrict__ total, const int* __restrict__ nb_src, const int* __restrict__ nb_j,
                              const int* __restrict__ nb_img, const int* __restrict__ nb_off,
                              const int* __restrict__ struc_of, const StrucInfo* __restrict__ sinfo,
                              int* __restrict__ nb_rev, int* __restrict__ err) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+    if (p >= *total) return;
     const int i = nb_src[p], j = nb_j[p];
     const StrucInfo& S = sinfo[struc_of[i]];
     const int ny = 2 * S.cf[1] + 1, nz = 2 * S.cf[2] + 1;
@@ -278,13 +279,20 @@ __global__ void k_nb_reverse(int npairs, const int* __restrict__ nb_src, const i
         if (k == want) { found = mid; break; }
         if (k < want) lo = mid + 1; else hi = mid - 1;
     }
-    nb_rev[p] = found;
+    nb_rev[p] = found < 0 ? p : found;
     if (found < 0) atomicAdd(err, 1);
 }
 
-// thread per atom: source atom of each pair
-__global__ void k_nb_src(int natoms, const int* __restrict__ nb_off, int* __restrict__ nb_src) {
+// thread per offset entry: if the new list exceeds the pair capacity or an atom exceeds the LDS
+// neighbour slots, every list is emptied and the overflow flag is raised.  The total is read from the
+// scan's own copy, which this kernel does not modify.
+__global__ void k_nb_check(int natoms, int* __restrict__ nb_off, const int* __restrict__ maxcount,
+                           const int* __restrict__ total, int pcap, int slots, int* __restrict__ bad) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natoms) return;
-    for (int p = nb_off[i]; p < nb_off[i + 1]; ++p) nb_src[p] = i;
+    if (i > natoms) return;
+    if (*total > pcap || *maxcount > slots) {
+        nb_off[i] = 0;
+        if (i == 0) *bad = 1;
+    }
 }
+

@@ -7,33 +7,51 @@
 // deterministic gather in FP64, without atomics.
 #include "ace_common.hpp"
 
-// thread per atom: force (FP64) and the atom's virial share -sum_p g_p (x) d_p (FP64, 9 values)
-__global__ void k_force_gather(int natoms, const int* __restrict__ nb_off, const int* __restrict__ nb_rev,
-                               const float4* __restrict__ nb_d, const float4* __restrict__ pair_g,
-                               double* __restrict__ F, double* __restrict__ vir_atom) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natoms) return;
-    double fx = 0, fy = 0, fz = 0;
+// GL lanes per atom: force (FP64) and the atom's virial share -sum_p g_p (x) d_p (FP64, 9 values);
+// each lane sums a strided subset of the atom's pairs, the GL partials are combined by shuffles in a
+// fixed order.
+#define GL 8
+__global__ __launch_bounds__(256) void k_force_gather(int natoms, const int* __restrict__ nb_off,
+                                                      const int* __restrict__ nb_rev, const float4* __restrict__ nb_d,
+                                                      const float4* __restrict__ pair_g, double* __restrict__ F,
+                                                      double* __restrict__ vir_atom) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gt / GL, sub = gt % GL;
+    const bool on = i < natoms;
+    double f[3] = {0, 0, 0};
     double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int p = nb_off[i]; p < nb_off[i + 1]; ++p) {
-        const float4 g = pair_g[p];
-        const float4 gr = pair_g[nb_rev[p]];
-        fx += (double)gr.x - (double)g.x;
-        fy += (double)gr.y - (double)g.y;
-        fz += (double)gr.z - (double)g.z;
-        const float4 d = nb_d[p];
-        const double ga[3] = {g.x, g.y, g.z}, da[3] = {d.x, d.y, d.z};
+    if (on) {
+        for (int p = nb_off[i] + sub; p < nb_off[i + 1]; p += GL) {
+            const float4 g = pair_g[p];
+            const float4 gr = pair_g[nb_rev[p]];
+            const float4 d = nb_d[p];
+            f[0] += (double)gr.x - (double)g.x;
+            f[1] += (double)gr.y - (double)g.y;
+            f[2] += (double)gr.z - (double)g.z;
+            const double ga[3] = {g.x, g.y, g.z}, da[3] = {d.x, d.y, d.z};
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
+            for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int b = 0; b < 3; ++b) w[3 * a + b] -= ga[a] * da[b];
+                for (int b = 0; b < 3; ++b) w[3 * a + b] -= ga[a] * da[b];
+        }
     }
-    F[3 * i] = fx;
-    F[3 * i + 1] = fy;
-    F[3 * i + 2] = fz;
-    if (vir_atom) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) vir_atom[9 * i + q] = w[q];
+    for (int m = GL / 2; m >= 1; m >>= 1) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) f[q] += __shfl_xor(f[q], m, GL);
+        if (vir_atom) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) w[q] += __shfl_xor(w[q], m, GL);
+        }
+    }
+    if (on && sub == 0) {
+        F[3 * i] = f[0];
+        F[3 * i + 1] = f[1];
+        F[3 * i + 2] = f[2];
+        if (vir_atom) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) vir_atom[9 * i + q] = w[q];
+        }
     }
 }
 
@@ -66,12 +84,12 @@ __global__ __launch_bounds__(RED_THREADS) void k_struct_reduce(
 }
 
 // thread per pair: d_p = r_i - (r_j + R n) in FP64 (nnp.cpp:363), stored as FP32 with |d|
-__global__ void k_nb_disp(int npairs, const int* __restrict__ nb_src, const int* __restrict__ nb_j,
+__global__ void k_nb_disp(const int* __restrict__ npairs, const int* __restrict__ nb_src, const int* __restrict__ nb_j,
                           const int* __restrict__ nb_img, const int* __restrict__ struc_of,
                           const StrucInfo* __restrict__ sinfo, const double* __restrict__ pos,
                           float4* __restrict__ nb_d) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npairs) return;
+    if (p >= *npairs) return;
     const int i = nb_src[p], j = nb_j[p];
     const StrucInfo& S = sinfo[struc_of[i]];
     int c0, c1, c2;
@@ -86,10 +104,19 @@ __global__ void k_nb_disp(int npairs, const int* __restrict__ nb_src, const int*
 }
 
 // G rows (species-sorted, FP32) -> FP64 rows in atom order with stride ldo
-__global__ void k_export_rows(int natoms, int kmax, const int* __restrict__ row_of, int ldg,
-                              const float* __restrict__ G, int ldo, double* __restrict__ out) {
+// network-input rows x = inpw*(G + inpb) (species-sorted, FP32) -> G = x/inpw - inpb in atom order
+__global__ void k_export_rows(int natoms, const int* __restrict__ row_of, const int* __restrict__ type,
+                              const DevModel* __restrict__ M, const double* __restrict__ inpw,
+                              const double* __restrict__ inpb, int ldg, const float* __restrict__ G, int ldo,
+                              double* __restrict__ out) {
     const int i = blockIdx.y;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= natoms || k >= ldo) return;
-    out[(long long)i * ldo + k] = (k < kmax) ? (double)G[(long long)row_of[i] * ldg + k] : 0.0;
+    const DevSpecies& sp = M->sp[type[i]];
+    double g = 0.0;
+    if (k < sp.K) {
+        const int o = sp.in_off + k;
+        g = (double)G[(long long)row_of[i] * ldg + k] / inpw[o] - inpb[o];
+    }
+    out[(long long)i * ldo + k] = g;
 }

@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(
     for (int idx = tid; idx < tm * K; idx += 256) {
         const int r = idx / K, k = idx - r * K;
         const long long gi = (long long)(r0 + r) * ldg + k;
-        X[idx] = (r < nr) ? iw[k] * (G[gi] + ib[k]) : 0.f;
+        X[idx] = (r < nr) ? G[gi] : 0.f;          // rows already hold inpw*(G + inpb)
         Xt[idx] = (r < nr && tan) ? iw[k] * V[gi] : 0.f;
     }
     __syncthreads();

@@ -74,6 +74,7 @@ int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* 
     if (Ld->norm < ACE_NORM_IDENTITY || Ld->norm > ACE_NORM_LOG) return fail(ctx, "Invalid normalization method.");
     if ((Ld->loss == ACE_LOSS_HUBER || Ld->loss == ACE_LOSS_ASINH) && !(Ld->delta > 0))
         return fail(ctx, "ace_train_step: delta must be positive");
+    ctx->basis_f64 = ctx->precision == ACE_PREC_FP64_BASIS;
     if (require_nb(ctx)) return -1;
     if (train_tiles(ctx)) return -1;
     const int N = ctx->N, S = ctx->S;
@@ -89,7 +90,7 @@ int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* 
     if (wf) {
         if (run_basis(ctx, 1)) return -1;
         if (timed(ctx, "force_gather", [&] {
-                k_force_gather<<<cdiv(N, 256), 256, 0, ctx->stream>>>(N, ctx->d_nb_off.p, ctx->d_nb_rev.p, ctx->d_nb_d.p,
+                k_force_gather<<<cdiv((long long)N * GL, 256), 256, 0, ctx->stream>>>(N, ctx->d_nb_off.p, ctx->d_nb_rev.p, ctx->d_nb_d.p,
                                                                        ctx->d_pair_g.p, ctx->d_F.p, nullptr);
             }))
             return -1;
@@ -144,6 +145,7 @@ int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* 
     HC(hipMemcpyAsync(out.data(), T.d_out.p, 3 * (size_t)S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HC(hipStreamSynchronize(ctx->stream));
     if (check_err_flag(ctx, "ace_train_step")) return -1;
+    if (ctx->nb_pending && nb_verify(ctx)) return -1;
     double le = 0, lf = 0, rm = 0;
     for (int s = 0; s < S; ++s) {
         le += out[3 * s];

@@ -11,7 +11,7 @@ column-major (src/ml/nn.cpp:881-896).  :meth:`Model.desc` packs it into ``ace_mo
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -114,7 +114,6 @@ class Model:
     angular: list           # angular[I][b] -> Angular, b = tri_index(J, K)
     nets: list              # nets[I] -> Network
     E0: np.ndarray = None
-    _keep: list = field(default_factory=list, repr=False)
 
     def input_width(self, I: int) -> int:
         return sum(len(r.rs) for r in self.radial[I]) + sum(len(a.eta) for a in self.angular[I])
@@ -139,7 +138,7 @@ class Model:
             off += k
 
     def desc(self) -> ModelDesc:
-        """Pack into ``ace_model_desc``; the returned struct borrows arrays kept on the model."""
+        """Pack into ``ace_model_desc``; the returned struct owns the arrays its pointers refer to."""
         nt = self.ntypes
         nab = nt * (nt + 1) // 2
         keep = []
@@ -189,7 +188,7 @@ class Model:
             arr(iw, C.c_double), arr(ib, C.c_double), arr(ow, C.c_double), arr(ob, C.c_double),
             arr(e0, C.c_double), arr(po, C.c_int), arr(prm, C.c_double),
         )
-        self._keep = keep
+        d._keep = keep          # the arrays live exactly as long as this descriptor
         return d
 
 

@@ -15,7 +15,7 @@ import numpy as np
 from . import _lib
 from .model import LossDesc, Model
 
-PREC_FP32, PREC_BF16L1 = 0, 1
+PREC_FP32, PREC_BF16L1, PREC_FP64_BASIS = 0, 1, 2
 
 
 def _dp(a):
@@ -130,6 +130,10 @@ class NNPDevice:
         return off, j[: n.value], img[: n.value]
 
     # -- NNP methods -----------------------------------------------------------------------------
+    def set_precision(self, precision):
+        """Precision of compute_symm and train_step (PREC_FP32 or PREC_FP64_BASIS)."""
+        self._check(self.lib.ace_set_precision(self.ctx, precision), "ace_set_precision")
+
     def compute_symm(self, ldg=None):
         """NNP::compute_symm: G row per atom (FP32 on the GPU, exported as FP64)."""
         kmax = max(self.model.input_width(I) for I in range(self.model.ntypes))

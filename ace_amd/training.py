@@ -1,0 +1,94 @@
+"""Host side of training around the GPU step: Taylor augmentation, ADAM, and the iteration loop.
+
+``taylor_augment`` restates the perturbed-copy augmentation of NNPTEFR (src/nnp/nnptefr.cpp:2733-2840):
+each structure whose mean |F| exceeds 1e-12 gets ``nadd`` copies; every atom of a copy moves by
+dr = r (sin t cos p, sin t sin p, cos t) with r drawn from the chosen distribution (scale ``rdelta``),
+cos t ~ U(-1,1), p ~ U(-pi,pi), and the copy's energy becomes E - sum_atoms dr . F (first-order Taylor
+expansion); the copy's forces are dropped.
+``Adam`` restates ADAM::step (src/opt/algo.cpp:280-296, beta1 0.9, beta2 0.999, eps 1e-16).
+``Trainer.step`` is one iteration of the NNPTEFR::train loop (src/nnp/nnptefr.cpp:977-1081) on one rank:
+loss and gradient from the GPU, then the optimiser step and the parameter upload.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .configs import Structure
+
+
+def taylor_augment(strucs, nadd, rdelta, rng, dist="normal"):
+    """Originals first, then ``nadd`` perturbed copies after each structure with forces."""
+    draw = {
+        "uniform": lambda: rng.uniform(0.0, rdelta),
+        "normal": lambda: rng.normal(0.0, rdelta),
+        "laplace": lambda: rng.laplace(0.0, rdelta),
+        "logistic": lambda: rng.logistic(0.0, rdelta),
+    }
+    if dist not in draw:
+        raise ValueError("Invalid position distribution.")
+    out = []
+    for s in strucs:
+        out.append(s)
+        if s.forces is None:
+            continue
+        favg = np.mean(np.linalg.norm(s.forces, axis=1))
+        if favg <= 1.0e-12:
+            continue
+        for _ in range(nadd):
+            pos = s.pos.copy()
+            e = float(s.energy)
+            for j in range(s.natoms):
+                vr = draw[dist]()
+                ct = rng.uniform(-1.0, 1.0)
+                st = np.sqrt(1.0 - ct * ct)
+                ph = rng.uniform(-1.0, 1.0) * np.pi
+                dr = np.array([vr * st * np.cos(ph), vr * st * np.sin(ph), vr * ct])
+                pos[j] += dr
+                e -= dr @ s.forces[j]
+            out.append(Structure(s.cell, pos, s.type, energy=e, forces=None))
+    return out
+
+
+class Adam:
+    """ADAM::step: m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2, p -= gamma m/(1-b1^t) / (sqrt(v/(1-b2^t)) + eps)."""
+
+    def __init__(self, dim, gamma=1e-3, eps=1e-16):
+        self.m = np.zeros(dim)
+        self.v = np.zeros(dim)
+        self.b1t, self.b2t = 0.9, 0.999
+        self.gamma, self.eps = gamma, eps
+
+    def step(self, p, g):
+        b1, b2 = 0.9, 0.999
+        b1ii = 1.0 / (1.0 - self.b1t)
+        b2ii = 1.0 / (1.0 - self.b2t)
+        self.m = self.m * b1 + (1.0 - b1) * g
+        self.v = self.v * b2 + (1.0 - b2) * g * g
+        p -= self.gamma * b1ii * self.m / (np.sqrt(b2ii * self.v) + self.eps)
+        self.b1t *= b1
+        self.b2t *= b2
+        return p
+
+
+class Trainer:
+    """Full-batch training on one GPU: every uploaded structure is in the batch."""
+
+    def __init__(self, dev, model, strucs, gamma=1e-3, w_force=0.1, loss=0, norm=0, delta=1.0):
+        self.dev, self.model = dev, model
+        self.nat = sum(s.natoms for s in strucs)
+        self.nbatch = len(strucs)
+        self.opt = Adam(sum(model.param_counts()), gamma)
+        self.params = model.all_params()
+        self.w_force, self.loss, self.norm, self.delta = w_force, loss, norm, delta
+        have_f = all(s.forces is not None for s in strucs)
+        dev.set_targets([s.energy for s in strucs],
+                        np.concatenate([s.forces for s in strucs]) if have_f else None)
+
+    def step(self):
+        """Loss terms and the updated parameter vector of one iteration."""
+        wf = self.w_force / (3 * self.nat) if self.w_force else 0.0
+        out, g = self.dev.train_step(self.loss, self.norm, self.delta, 1.0 / self.nbatch, wf)
+        self.params = self.opt.step(self.params, g)
+        self.dev.set_all_params(self.params)
+        self.model.set_all_params(self.params)
+        return out

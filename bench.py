@@ -70,6 +70,51 @@ def cpu_baseline(model, s, samples):
     return s.natoms / dt, out
 
 
+def train_bench(steps):
+    """Force-matching training throughput (structures/s): 64 structures of 32-1024 atoms (log-uniform),
+    3 elements, random packing at 0.08 atoms/A^3, 24 G2 + 32 G4, 56-64-64-1; targets from a seeded teacher
+    network of the same architecture (evaluated on the GPU in FP64-basis mode).  One step = loss and
+    gradient (energy MSE + 0.1 force MSE, d(F)/dW by forward-over-reverse), host ADAM update, parameter
+    upload."""
+    from ace_amd import PREC_FP64_BASIS, NNPDevice, configs
+    from ace_amd.training import Trainer
+    model, teacher, strucs = configs.config3()
+    tdev = NNPDevice(teacher)
+    tdev.upload(strucs)
+    tdev.build_neighbors()
+    tdev.set_precision(PREC_FP64_BASIS)
+    G = tdev.compute_symm()
+    configs.set_input_scaling(teacher, strucs, G)
+    tdev.sync_model_scaling()
+    E, F, _ = tdev.compute(PREC_FP64_BASIS)
+    off = np.concatenate([[0], np.cumsum([s.natoms for s in strucs])])
+    for k, s in enumerate(strucs):
+        s.energy, s.forces = E[k], F[off[k]:off[k + 1]]
+    tdev.close()
+    for I in range(model.ntypes):
+        model.nets[I].inpw, model.nets[I].inpb = teacher.nets[I].inpw, teacher.nets[I].inpb
+    dev = NNPDevice(model)
+    dev.upload(strucs)
+    dev.build_neighbors()
+    tr = Trainer(dev, model, strucs, gamma=1e-3, w_force=0.1)
+    for _ in range(3):
+        tr.step()
+    dev.sync()
+    t0 = time.perf_counter()
+    first = last = None
+    for k in range(steps):
+        out = tr.step()
+        first = out if k == 0 else first
+        last = out
+    dev.sync()
+    dt = (time.perf_counter() - t0) / steps
+    dev.close()
+    return {"metric": "train structs/s", "value": len(strucs) / dt, "unit": "structures/s",
+            "ms_per_step": dt * 1e3, "structures": len(strucs), "atoms": int(off[-1]), "steps": steps,
+            "loss_first": float(first[0]), "loss_last": float(last[0]), "dtype": "fp32",
+            "workload": "64 structures, N log-uniform [32,1024], 3 elements, 56-64-64-1, energy + 0.1 force MSE"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,13 +122,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-samples", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--train-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
               "'How the path would shard')", file=sys.stderr)
         sys.exit(2)
 
-    from ace_amd import NNPDevice, configs
+    from ace_amd import PREC_FP32, PREC_FP64_BASIS, NNPDevice, configs
 
     model, [s] = configs.config2()
     dev = NNPDevice(model)
@@ -93,33 +139,34 @@ def main():
     configs.set_input_scaling(model, [s], G)       # PreBias MEAN / PreScale DEV from the GPU's G
     dev.sync_model_scaling()
 
-    def step():
-        dev.build_neighbors()
-        dev.compute_device()
+    def run(prec, steps, events):
+        """K steps of neighbour build + energy/forces; returns ms per step and per-kernel event times."""
+        dev.profile(events)
+        dev.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dev.build_neighbors()
+            dev.compute_device(prec)
+        dev.sync()
+        t1 = time.perf_counter()
+        kern = {}
+        if events:
+            sfx = "_f64" if prec == PREC_FP64_BASIS else ""
+            for k in ("nb_assign", "nb_count", "nb_fill", "nb_reverse", "basis_symm" + sfx, "mlp_energy_grad",
+                      "basis_adjoint" + sfx, "force_gather", "struct_reduce"):
+                kern[k.replace(sfx, "") if sfx else k] = dev.profile_read(k)
+            dev.profile(False)
+        return (t1 - t0) / steps * 1e3, kern
 
-    for _ in range(args.warmup):
-        step()
-    dev.sync()
-    dev.profile(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    dev.sync()
-    t1 = time.perf_counter()
-    kern = {}
-    for k in ("nb_assign", "nb_count", "nb_fill", "nb_reverse", "basis_symm", "mlp_energy_grad",
-              "basis_adjoint", "force_gather", "struct_reduce"):
-        kern[k] = dev.profile_read(k)
-    dev.profile(False)
-    ms = (t1 - t0) / args.steps * 1e3
+    for prec in (PREC_FP32, PREC_FP64_BASIS):
+        run(prec, args.warmup, False)
+    # timed region: K steps, no per-kernel events
+    ms, _ = run(PREC_FP32, args.steps, False)
     value = s.natoms / (ms * 1e-3)
-
-    # same K steps without per-kernel events (event overhead)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    dev.sync()
-    ms_noev = (time.perf_counter() - t0) / args.steps * 1e3
+    # the same K steps again with HIP events around every kernel (per-kernel durations)
+    ms_ev, kern = run(PREC_FP32, args.steps, True)
+    ms64, _ = run(PREC_FP64_BASIS, args.steps, False)
+    _, kern64 = run(PREC_FP64_BASIS, args.steps, True)
 
     # algorithmic work of the dominant kernel
     off, j, img = dev.neighbors()
@@ -149,7 +196,10 @@ def main():
     plain_ms = (time.perf_counter() - t0) / nplain * 1e3
     plain_value = s.natoms / (plain_ms * 1e-3)
 
-    E, F, _ = dev.compute()
+    E, F, _ = dev.compute(PREC_FP32)
+    E64, F64, _ = dev.compute(PREC_FP64_BASIS)
+    train = train_bench(args.train_steps) if args.train_steps > 0 else None
+    avg64 = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kern64.items()}
     res = {
         "metric": METRIC,
         "value": value,
@@ -171,9 +221,12 @@ def main():
                      "flops_per_launch": int(flops[dom]), "avg_launch_ms": avg[dom],
                      "note": "FP32 vector-ALU kernel priced against the FP32 dense peak"},
         "kernels_avg_us": {k: round(v * 1e3, 2) for k, v in avg.items()},
-        "ms_per_step_without_events": ms_noev,
+        "ms_per_step_with_kernel_events": ms_ev,
+        "fp64_basis": {"value": s.natoms / (ms64 * 1e-3), "unit": "atoms/s", "ms_per_step": ms64,
+                       "kernels_avg_us": {k: round(v * 1e3, 2) for k, v in avg64.items()}},
         "plain_fp64_port": {"value": plain_value, "unit": "atoms/s", "ms_per_step": plain_ms,
                             "speedup_batched_vs_plain": value / plain_value},
+        "train": train,
     }
     if not args.no_cpu:
         cpu_value, (E_ref, F_ref, _) = cpu_baseline(model, s, args.cpu_samples)
@@ -183,6 +236,8 @@ def main():
         res["speedup_vs_cpu"] = value / cpu_value
         res["error_vs_fp64_oracle"] = {"energy_per_atom": abs(E[0] - E_ref) / s.natoms,
                                        "force_max_abs": float(np.max(np.abs(F - F_ref)))}
+        res["fp64_basis"]["error_vs_fp64_oracle"] = {"energy_per_atom": abs(E64[0] - E_ref) / s.natoms,
+                                                     "force_max_abs": float(np.max(np.abs(F64 - F_ref)))}
     dev.close()
     print(json.dumps(res))
 

@@ -41,7 +41,10 @@ enum { ACE_ANG_NONE = 0, ACE_ANG_GAUSS = 1, ACE_ANG_GAUSS2 = 2, ACE_ANG_SECH = 3
        ACE_ANG_STUDENT3 = 4, ACE_ANG_STUDENT4 = 5, ACE_ANG_STUDENT5 = 6 };
 enum { ACE_ACT_LINEAR = 1, ACE_ACT_SIGMOID = 2, ACE_ACT_TANH = 3, ACE_ACT_SWISH = 10,
        ACE_ACT_SOFTPLUS = 15 };
-enum { ACE_PREC_FP32 = 0, ACE_PREC_BF16L1 = 1 };
+/* ACE_PREC_FP32: symmetry functions, networks and adjoints in FP32 (FP64 reductions).
+ * ACE_PREC_FP64_BASIS: symmetry functions and their adjoint/tangent in FP64, networks in FP32.
+ * ACE_PREC_BF16L1: reserved (first network layer in BF16). */
+enum { ACE_PREC_FP32 = 0, ACE_PREC_BF16L1 = 1, ACE_PREC_FP64_BASIS = 2 };
 enum { ACE_LOSS_MSE = 0, ACE_LOSS_MAE = 1, ACE_LOSS_HUBER = 2, ACE_LOSS_ASINH = 3 };
 enum { ACE_NORM_IDENTITY = 0, ACE_NORM_LINEAR = 1, ACE_NORM_SQRT = 2, ACE_NORM_CBRT = 3,
        ACE_NORM_LOG = 4 };
@@ -142,8 +145,12 @@ int  ace_neighbor_count(const ace_ctx* ctx, long long* npairs);
  * lexicographic. */
 int  ace_export_neighbors(ace_ctx* ctx, long long* offsets, int* j, int* img);
 
-/* Symmetry functions G (FP32 on the GPU); G exported row per atom with stride ldg (>= max width),
- * zero-padded, as FP64.  G may be NULL (computed and kept on the device). */
+/* Precision of ace_symm and ace_train_step (ACE_PREC_*; default ACE_PREC_FP32).  No reference
+ * counterpart: the reference computes everything in FP64. */
+int  ace_set_precision(ace_ctx* ctx, int precision);
+
+/* Symmetry functions G (on the GPU, stored FP32); G exported row per atom with stride ldg (>= max
+ * width), zero-padded, as FP64.  G may be NULL (computed and kept on the device). */
 int  ace_symm(ace_ctx* ctx, double* G, int ldg);
 
 /*

@@ -42,8 +42,8 @@ def oracle_loss(model, strucs, G, ld):
     return tot, grad
 
 
-@pytest.mark.parametrize("loss,norm,wf", [(0, 0, 0.0), (0, 1, 0.1), (2, 2, 0.1)])
-def test_train_step_matches_oracle(loss, norm, wf):
+@pytest.mark.parametrize("loss,norm,wf,prec", [(0, 0, 0.0, 0), (0, 1, 0.1, 0), (2, 2, 0.1, 0), (0, 1, 0.1, 2)])
+def test_train_step_matches_oracle(loss, norm, wf, prec):
     model, strucs, G = batch()
     nat = sum(s.natoms for s in strucs)
     ld = LossDesc(loss, norm, 0.5, 1.0 / len(strucs), wf / (3 * nat))
@@ -51,12 +51,13 @@ def test_train_step_matches_oracle(loss, norm, wf):
     dev = NNPDevice(model)
     dev.upload(strucs)
     dev.build_neighbors()
+    dev.set_precision(prec)
     dev.set_targets([s.energy for s in strucs], np.concatenate([s.forces for s in strucs]))
     out, g = dev.train_step(loss, norm, 0.5, 1.0 / len(strucs), wf / (3 * nat))
     le = abs(out[1] - ref[0]) / max(abs(ref[0]), 1e-12)
     lf = abs(out[2] - ref[1]) / max(abs(ref[1]), 1e-12) if wf else 0.0
     ge = np.max(np.abs(g - gref)) / np.max(np.abs(gref))
-    log(f"train loss={loss} norm={norm} wf={wf}: loss_E rel {le:.1e} loss_F rel {lf:.1e} grad {ge:.1e}")
+    log(f"train loss={loss} norm={norm} wf={wf} prec={prec}: loss_E rel {le:.1e} loss_F rel {lf:.1e} grad {ge:.1e}")
     assert le < 1e-5 and lf < 1e-5 and ge < 1e-3
     # a second step with unchanged parameters reproduces the first (cached G, deterministic reductions)
     out2, g2 = dev.train_step(loss, norm, 0.5, 1.0 / len(strucs), wf / (3 * nat))

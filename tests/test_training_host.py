@@ -1,0 +1,48 @@
+"""Host-side training pieces: Taylor augmentation (nnptefr.cpp:2733-2840) and ADAM (algo.cpp:280-296)
+against the oracle.  CPU only."""
+import numpy as np
+
+from ace_amd import configs
+from ace_amd.training import Adam, taylor_augment
+from oracle import oracle as O
+
+
+def test_taylor_augmentation_first_order_energy():
+    rng = np.random.default_rng(0)
+    s1 = configs.Structure(np.eye(3) * 10, rng.uniform(0, 10, (7, 3)), np.zeros(7, dtype=np.int32),
+                           energy=-3.2, forces=rng.normal(size=(7, 3)))
+    s0 = configs.Structure(np.eye(3) * 10, rng.uniform(0, 10, (5, 3)), np.zeros(5, dtype=np.int32),
+                           energy=1.0, forces=np.zeros((5, 3)))          # zero forces: no copies
+    out = taylor_augment([s1, s0], nadd=3, rdelta=0.05, rng=rng)
+    assert len(out) == 1 + 3 + 1
+    assert out[0] is s1 and out[4] is s0
+    for c in out[1:4]:
+        dr = c.pos - s1.pos
+        assert c.forces is None
+        assert np.isclose(c.energy, s1.energy - np.sum(dr * s1.forces), rtol=0, atol=1e-13)
+        assert np.all(np.linalg.norm(dr, axis=1) < 0.05 * 8)
+
+
+def test_taylor_direction_is_isotropic():
+    rng = np.random.default_rng(1)
+    s = configs.Structure(np.eye(3) * 10, np.zeros((4000, 3)), np.zeros(4000, dtype=np.int32),
+                          energy=0.0, forces=np.ones((4000, 3)))
+    c = taylor_augment([s], nadd=1, rdelta=1.0, rng=rng, dist="uniform")[1]
+    u = c.pos / np.linalg.norm(c.pos, axis=1)[:, None]
+    assert np.all(np.abs(u.mean(axis=0)) < 0.05)          # uniform on the sphere
+    assert np.allclose((u ** 2).mean(axis=0), 1 / 3, atol=0.03)
+
+
+def test_adam_matches_oracle():
+    rng = np.random.default_rng(2)
+    dim = 17
+    p = rng.normal(size=dim)
+    q = p.copy()
+    a = Adam(dim, gamma=0.01)
+    m, v, b = np.zeros(dim), np.zeros(dim), np.array([0.9, 0.999])
+    L = O.lib()
+    for _ in range(5):
+        g = rng.normal(size=dim)
+        p = a.step(p, g)
+        L.or_adam_step(dim, O._d(q), O._d(g), 0.01, 1e-16, O._d(m), O._d(v), O._d(b))
+    assert np.allclose(p, q, rtol=0, atol=1e-15)

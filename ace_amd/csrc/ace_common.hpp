@@ -11,7 +11,7 @@
 #define ACE_WAVE 64
 
 // One species (NNH): offsets follow NNH::init_input (nnh.cpp:183-201).
-struct DevSpecies {
+struct __attribute__((aligned(16))) DevSpecies {
     int K;                         // input width
     int nR;                        // radial inputs
     int ldp;                       // parameter count
@@ -37,6 +37,10 @@ struct DevSpecies {
     float c;
     int boff[ACE_MAXL], woff[ACE_MAXL];   // offsets in the species' parameter block
     int poff;                      // offset of the species' parameter block in the global array
+    // matrix-core layout of the same parameters (16-byte aligned block at wp_off, wp_n floats, a multiple
+    // of 4): W_l as [k][o] with row stride w_{l+1} + 1 at wl_off[l], biases b_l at bl_off[l]
+    int wp_off, wp_n;
+    int wl_off[ACE_MAXL], bl_off[ACE_MAXL];
     int in_off;                    // offset of inpw/inpb
     float outw, outb;
     double E0;

@@ -54,9 +54,11 @@ struct ace_ctx {
     double rc = 0;
     std::vector<HSpecies> sp;
     int kmax = 0, ldg = 0, maxA = 0, maxR = 0, ngmax = 1, ngk = 1;
+    bool all_canon = true;
     DevModel hdm;
     DBuf<DevModel> d_model;
-    DBuf<float> d_r_rs, d_r_eta, d_a_lamf, d_a_zeta, d_params, d_inpw, d_inpb;
+    DBuf<float> d_r_rs, d_r_eta, d_a_lamf, d_a_zeta, d_params, d_inpw, d_inpb, d_wpad;
+    int wp_total = 0;
     DBuf<int> d_a_zint, d_a_grp, d_a_sel, d_a_cidx;
     bool plain_ok = false;
     std::string plain_why;
@@ -81,11 +83,13 @@ struct ace_ctx {
     bool nb_valid = false;
     bool disp_dirty = false;
     double nb_rc = 0;
-    std::vector<StrucInfo> sinfo_h;
+    std::vector<StrucInfo> sinfo_h, sinfo_up;   // host tables; the copy last uploaded
     DBuf<StrucInfo> d_sinfo;
     int ncells = 0;
-    DBuf<int> d_cell_of, d_wrap_of, d_cell_count, d_cell_start, d_cell_cursor, d_cell_atoms, d_nb_count, d_nb_off,
-        d_scan_blk, d_scal;
+    DBuf<int> d_cell_of, d_wrap_of, d_slot_of, d_cell_start, d_nb_count, d_nb_off, d_scan_blk, d_scal;
+    DBuf<double4> d_cpos;      // cell-ordered positions
+    DBuf<int2> d_cinfo;        // cell-ordered (atom, wrap)
+    DBuf<long long> d_nb_keys; // per-atom rows of sorted neighbour keys
     long long npairs = 0;       // pairs of the current list (exact after a synchronising call)
     long long pcap = 0;         // pair capacity of the device arrays
     int slots_cap = 32;         // neighbour slots per atom held in LDS by the basis kernels
@@ -187,6 +191,8 @@ int exclusive_scan(ace_ctx* ctx, const int* in, int* out, int n) {
     if (ensure(ctx, ctx->d_scan_blk, nb + 1)) return -1;
     int* blk = ctx->d_scan_blk.p;
     int* total = ctx->d_scal.p + 1;
+    if (n <= SCAN_BLOCK * SCAN_RUN)
+        return timed(ctx, "scan", [&] { k_scan_single<<<1, SCAN_BLOCK, 0, ctx->stream>>>(n, in, out, total); });
     if (timed(ctx, "scan_local", [&] { k_scan_local<<<nb, SCAN_BLOCK, 0, ctx->stream>>>(n, in, out, blk); })) return -1;
     if (timed(ctx, "scan_sums", [&] { k_scan_sums<<<1, SCAN_BLOCK, 0, ctx->stream>>>(nb, blk, total, n, out); })) return -1;
     if (timed(ctx, "scan_add", [&] { k_scan_add<<<nb, SCAN_BLOCK, 0, ctx->stream>>>(n, out, blk); })) return -1;
@@ -267,6 +273,24 @@ int parse_model(ace_ctx* ctx, const ace_model_desc* m) {
     return 0;
 }
 
+// matrix-core copy of the network parameters (layout in DevSpecies::wl_off / bl_off)
+int upload_wpad(ace_ctx* ctx) {
+    std::vector<float> w(std::max(ctx->wp_total, 4), 0.f);
+    for (int I = 0; I < ctx->nt; ++I) {
+        const HSpecies& s = ctx->sp[I];
+        const DevSpecies& d = ctx->hdm.sp[I];
+        for (int l = 0; l < s.L; ++l) {
+            const int ni = s.width[l], no = s.width[l + 1];
+            for (int k = 0; k < ni; ++k)
+                for (int o = 0; o < no; ++o)
+                    w[d.wp_off + d.wl_off[l] + k * (no + 1) + o] = (float)s.params[d.woff[l] + o + k * no];
+            for (int o = 0; o < no; ++o) w[d.wp_off + d.bl_off[l] + o] = (float)s.params[d.boff[l] + o];
+        }
+    }
+    if (upload(ctx, ctx->d_wpad, w.data(), w.size())) return -1;
+    return 0;
+}
+
 // BasisA::init run grouping (basis_angular.cpp:178-205), merged over the blocks of one species
 int build_tables(ace_ctx* ctx) {
     const int nt = ctx->nt, nab = nt * (nt + 1) / 2;
@@ -275,7 +299,7 @@ int build_tables(ace_ctx* ctx) {
     dm.ntypes = nt;
     std::vector<float> r_rs, r_eta, a_lamf, a_zeta, prm, iw, ib;
     std::vector<int> a_zint, a_grp, a_sel;
-    int kmax = 0, maxA = 1, maxR = 1, ngmax = 1;
+    int kmax = 0, maxA = 1, maxR = 1, ngmax = 1, wp_total = 0;
     for (int I = 0; I < nt; ++I) {
         const HSpecies& s = ctx->sp[I];
         DevSpecies& d = dm.sp[I];
@@ -369,6 +393,14 @@ int build_tables(ace_ctx* ctx) {
         for (int l = 0; l < s.L; ++l) { d.woff[l] = cnt; cnt += s.width[l + 1] * s.width[l]; }
         d.ldp = cnt;
         d.poff = (int)prm.size();
+        {
+            int off = 0;
+            for (int l = 0; l < s.L; ++l) { d.wl_off[l] = off; off += s.width[l] * (s.width[l + 1] + 1); }
+            for (int l = 0; l < s.L; ++l) { d.bl_off[l] = off; off += s.width[l + 1]; }
+            d.wp_n = (off + 3) & ~3;
+            d.wp_off = wp_total;
+            wp_total += d.wp_n;
+        }
         for (double v : s.params) prm.push_back((float)v);
         d.in_off = (int)iw.size();
         for (int k = 0; k < s.K; ++k) {
@@ -383,6 +415,7 @@ int build_tables(ace_ctx* ctx) {
     // (sign, level, eta group); their sums become a register outer product in the basis kernel
     const int ngk = ngmax <= 1 ? 1 : (ngmax <= 2 ? 2 : 4);
     std::vector<int> a_cidx(a_sel.size(), 0);
+    bool all_canon = true;
     for (int I = 0; I < nt; ++I) {
         DevSpecies& d = dm.sp[I];
         int nz = 1;
@@ -398,12 +431,14 @@ int build_tables(ace_ctx* ctx) {
                 a_cidx[fi] = c;
             }
             d.a_canon[b] = canon ? 1 : 0;
+            if (d.a_n[b] > 0 && !canon) all_canon = false;
             if (canon)
                 for (int q = 0; q < d.a_n[b]; ++q) nz = std::max(nz, a_sel[d.a_poff[b] + q] % 7 + 1);
         }
         d.nz = nz;
     }
     ctx->ngk = ngk;
+    ctx->all_canon = all_canon;
     dm.kmax = kmax;
     dm.ng_max = ngmax;
     ctx->kmax = kmax;
@@ -423,6 +458,8 @@ int build_tables(ace_ctx* ctx) {
     if (upload(ctx, ctx->d_params, prm.data(), prm.size())) return -1;
     if (upload(ctx, ctx->d_inpw, iw.data(), iw.size())) return -1;
     if (upload(ctx, ctx->d_inpb, ib.data(), ib.size())) return -1;
+    ctx->wp_total = wp_total;
+    if (upload_wpad(ctx)) return -1;
 
     // FP64 tables of the plain port
     ctx->plain_ok = true;
@@ -491,6 +528,7 @@ int refresh_params(ace_ctx* ctx) {
         }
     }
     if (upload(ctx, ctx->d_params, prm.data(), prm.size())) return -1;
+    if (upload_wpad(ctx)) return -1;
     if (upload(ctx, ctx->d_inpw, iw.data(), iw.size())) return -1;
     if (upload(ctx, ctx->d_inpb, ib.data(), ib.size())) return -1;
     if (upload(ctx, ctx->d_nn64, nn.data(), nn.size())) return -1;
@@ -526,25 +564,33 @@ DevBasisParams basis_params(ace_ctx* ctx) {
 
 typedef void (*BasisFn)(int, const DevModel*, DevBasisParams, const int*, const int*, const int*, const float4*,
                         const int*, const double*, const int*, const StrucInfo*, const int*, int, float*,
-                        const float*, float4*, int*, int, const double*);
+                        const float*, float4*, int*, int, const double*, int, int, int);
 
 template <class T, int NG>
-void pick_basis(BasisFn* f) {
-    f[0] = k_basis<T, NG, 0>;
-    f[1] = k_basis<T, NG, 1>;
-    f[2] = k_basis<T, NG, 2>;
+void pick_basis(BasisFn* f, bool gen) {
+    if (gen) {
+        f[0] = k_basis<T, NG, 0, 1>;
+        f[1] = k_basis<T, NG, 1, 1>;
+        f[2] = k_basis<T, NG, 2, 1>;
+    } else {
+        f[0] = k_basis<T, NG, 0, 0>;
+        f[1] = k_basis<T, NG, 1, 0>;
+        f[2] = k_basis<T, NG, 2, 0>;
+    }
 }
 
-// the canonical angular accumulators are sized by the largest number of eta groups (1, 2 or 4)
+// the canonical angular accumulators are sized by the largest number of eta groups (1, 2 or 4); the
+// generic per-function path is compiled in only when some angular block is not canonical
 void basis_fns(ace_ctx* ctx, bool f64, BasisFn* f) {
+    const bool gen = !ctx->all_canon;
     if (f64) {
-        if (ctx->ngk == 1) pick_basis<double, 1>(f);
-        else if (ctx->ngk == 2) pick_basis<double, 2>(f);
-        else pick_basis<double, 4>(f);
+        if (ctx->ngk == 1) pick_basis<double, 1>(f, gen);
+        else if (ctx->ngk == 2) pick_basis<double, 2>(f, gen);
+        else pick_basis<double, 4>(f, gen);
     } else {
-        if (ctx->ngk == 1) pick_basis<float, 1>(f);
-        else if (ctx->ngk == 2) pick_basis<float, 2>(f);
-        else pick_basis<float, 4>(f);
+        if (ctx->ngk == 1) pick_basis<float, 1>(f, gen);
+        else if (ctx->ngk == 2) pick_basis<float, 2>(f, gen);
+        else pick_basis<float, 4>(f, gen);
     }
 }
 
@@ -592,19 +638,25 @@ int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = 
         f<<<ctx->N, BT, bytes, ctx->stream>>>(ctx->N, ctx->d_model.p, P, ctx->d_type.p, ctx->d_nb_off.p, ctx->d_nb_j.p,
                                               ctx->d_nb_d.p, ctx->d_nb_img.p, ctx->d_pos.p, ctx->d_struc_of.p,
                                               ctx->d_sinfo.p, ctx->d_row_of.p, ctx->ldg, rows, ctx->d_dEdG.p,
-                                              ctx->d_pair_g.p, err, nbm, eps);
+                                              ctx->d_pair_g.p, err, nbm, eps, ctx->nt, ctx->ngmax, ctx->kmax);
     });
 }
 
 int run_mlp(ace_ctx* ctx) {
-    int lds = 0;
-    for (int I = 0; I < ctx->nt; ++I) lds = std::max(lds, mlp_lds_floats(ctx->hdm.sp[I], ctx->tm) * 4);
+    int lds = 0, lds_w = 0;
+    for (int I = 0; I < ctx->nt; ++I) {
+        lds = std::max(lds, mlp_lds_floats(ctx->hdm.sp[I]) * 4);
+        lds_w = std::max(lds_w, (mlp_lds_floats(ctx->hdm.sp[I]) + ctx->hdm.sp[I].wp_n) * 4);
+    }
+    const bool stage = lds_w <= 64 * 1024;
+    if (stage) lds = lds_w;
     if (lds > 160 * 1024) return fail(ctx, "mlp: network too wide for LDS");
-    if (lds > 64 * 1024) HC(hipFuncSetAttribute((const void*)k_mlp_energy_grad, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    auto f = stage ? k_mlp_energy_grad<true> : k_mlp_energy_grad<false>;
+    if (lds > 64 * 1024) HC(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
     return timed(ctx, "mlp_energy_grad", [&] {
-        k_mlp_energy_grad<<<(int)ctx->tiles_h.size(), MLP_THREADS, lds, ctx->stream>>>(
-            ctx->d_model.p, ctx->d_params.p, ctx->d_inpw.p, ctx->d_inpb.p, ctx->d_tiles.p, ctx->ldg, ctx->d_G.p,
-            ctx->d_dEdG.p, ctx->d_atom_of_row.p, ctx->d_e_atom.p, ctx->tm);
+        f<<<(int)ctx->tiles_h.size(), 64, lds, ctx->stream>>>(ctx->d_model.p, ctx->d_wpad.p, ctx->d_inpw.p,
+                                                              ctx->d_tiles.p, ctx->ldg, ctx->d_G.p, ctx->d_dEdG.p,
+                                                              ctx->d_atom_of_row.p, ctx->d_e_atom.p);
     });
 }
 
@@ -667,7 +719,7 @@ void ace_destroy(ace_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& r : ctx->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     release(ctx->d_model); release(ctx->d_r_rs); release(ctx->d_r_eta); release(ctx->d_a_lamf);
-    release(ctx->d_a_zeta); release(ctx->d_params); release(ctx->d_inpw); release(ctx->d_inpb);
+    release(ctx->d_a_zeta); release(ctx->d_params); release(ctx->d_wpad); release(ctx->d_inpw); release(ctx->d_inpb);
     release(ctx->d_a_zint); release(ctx->d_a_grp); release(ctx->d_a_sel); release(ctx->d_a_cidx); release(ctx->d_pmodel); release(ctx->d_r_rs64);
     release(ctx->d_r_eta64); release(ctx->d_a_eta64); release(ctx->d_a_zeta64); release(ctx->d_a_lamf64);
     release(ctx->d_nn64); release(ctx->d_inpw64); release(ctx->d_inpb64);
@@ -675,8 +727,9 @@ void ace_destroy(ace_ctx* ctx) {
     release(ctx->d_row_of); release(ctx->d_atom_of_row); release(ctx->d_pos); release(ctx->d_cell);
     release(ctx->d_tiles); release(ctx->d_G); release(ctx->d_dEdG); release(ctx->d_e_atom);
     release(ctx->d_vir_atom); release(ctx->d_F); release(ctx->d_E); release(ctx->d_vir);
-    release(ctx->d_sinfo); release(ctx->d_cell_of); release(ctx->d_wrap_of); release(ctx->d_cell_count);
-    release(ctx->d_cell_start); release(ctx->d_cell_cursor); release(ctx->d_cell_atoms); release(ctx->d_nb_count);
+    release(ctx->d_sinfo); release(ctx->d_cell_of); release(ctx->d_wrap_of); release(ctx->d_slot_of);
+    release(ctx->d_cell_start); release(ctx->d_cpos); release(ctx->d_cinfo); release(ctx->d_nb_keys);
+    release(ctx->d_nb_count);
     release(ctx->d_nb_off); release(ctx->d_scan_blk); release(ctx->d_scal); release(ctx->d_nb_j);
     release(ctx->d_nb_img); release(ctx->d_nb_src); release(ctx->d_nb_rev); release(ctx->d_nb_d);
     release(ctx->d_pair_g); release(ctx->d_G64);
@@ -746,6 +799,7 @@ int ace_upload(ace_ctx* ctx, int n_struc, const int* natoms, const double* pos, 
     ctx->struc_of_h.resize(N);
     for (int s = 0; s < n_struc; ++s)
         for (int i = ctx->atom_off[s]; i < ctx->atom_off[s + 1]; ++i) ctx->struc_of_h[i] = s;
+    ctx->sinfo_up.clear();
     // species-sorted rows (stable), tiles of tm rows of one species
     std::vector<int> cnt(ctx->nt, 0), start(ctx->nt + 1, 0);
     for (long long i = 0; i < N; ++i) cnt[type[i]]++;
@@ -757,17 +811,8 @@ int ace_upload(ace_ctx* ctx, int n_struc, const int* natoms, const double* pos, 
         ctx->row_of_h[i] = r;
         atom_of_row[r] = (int)i;
     }
-    // rows per network tile: 32 when there are enough atoms to give every CU several tiles, fewer
-    // (down to 8) for small batches, and never more than 64 KB of LDS
-    int tm = 32;
-    while (tm > 8 && N / tm < 1024) tm >>= 1;
-    for (;;) {
-        int lds = 0;
-        for (int I = 0; I < ctx->nt; ++I) lds = std::max(lds, mlp_lds_floats(ctx->hdm.sp[I], tm) * 4);
-        if (lds <= 64 * 1024 || tm <= 4) break;
-        tm >>= 1;
-    }
-    ctx->tm = tm;
+    // network tiles: 16 rows of one species (one matrix-core wave each)
+    ctx->tm = MLP_ROWS;
     ctx->tiles_h.clear();
     for (int t = 0; t < ctx->nt; ++t)
         for (int r = start[t]; r < start[t + 1]; r += ctx->tm)
@@ -847,63 +892,66 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
         inv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c) I.Rinv[r + 3 * c] = inv[r][c];
-        long long ncell = 1;
+        double hgt[3];
         for (int d = 0; d < 3; ++d) {
             const double coln = std::sqrt(R[3 * d] * R[3 * d] + R[3 * d + 1] * R[3 * d + 1] + R[3 * d + 2] * R[3 * d + 2]);
             I.cf[d] = (int)std::ceil(3.0 * rcl / coln);                       // neighbor.cpp:49-53
             const double rown = std::sqrt(inv[d][0] * inv[d][0] + inv[d][1] * inv[d][1] + inv[d][2] * inv[d][2]);
-            const double h = 1.0 / rown;                                      // perpendicular height
-            int nc = (int)std::floor(h / rsearch);
-            nc = std::max(1, std::min(nc, 1024));
-            I.nc[d] = nc;
-            I.st[d] = std::max(1, (int)std::ceil(rsearch * nc / h));
+            hgt[d] = 1.0 / rown;                                              // perpendicular height
+            I.nc[d] = std::max(1, std::min((int)std::floor(hgt[d] / rsearch), 1024));
             if (I.cf[d] > 511) return fail(ctx, "ace_build_neighbors: image range too large for this cell");
-            ncell *= nc;
         }
+        // the binning workgroup holds one counter per cell: coarsen the grid to NB_MAXC cells
+        while ((long long)I.nc[0] * I.nc[1] * I.nc[2] > NB_MAXC) {
+            int d = 0;
+            for (int e = 1; e < 3; ++e)
+                if (I.nc[e] > I.nc[d]) d = e;
+            I.nc[d] -= 1;
+        }
+        for (int d = 0; d < 3; ++d) I.st[d] = std::max(1, (int)std::ceil(rsearch * I.nc[d] / hgt[d]));
+        const int ncell = I.nc[0] * I.nc[1] * I.nc[2];
         I.atom_off = ctx->atom_off[s];
         I.natoms = ctx->natoms[s];
         I.cell_off = cell_off;
-        if (cell_off + ncell > (1LL << 30)) return fail(ctx, "ace_build_neighbors: too many cells");
-        cell_off += (int)ncell;
+        if ((long long)cell_off + ncell + ctx->S > (1LL << 30)) return fail(ctx, "ace_build_neighbors: too many cells");
+        cell_off += ncell;
     }
     ctx->ncells = cell_off;
-    const int N = ctx->N, nc = ctx->ncells;
-    if (upload(ctx, ctx->d_sinfo, ctx->sinfo_h.data(), ctx->S)) return -1;
-    if (ensure(ctx, ctx->d_cell_of, N) || ensure(ctx, ctx->d_wrap_of, N) || ensure(ctx, ctx->d_cell_count, nc + 1) ||
-        ensure(ctx, ctx->d_cell_start, nc + 1) || ensure(ctx, ctx->d_cell_cursor, nc + 1) ||
-        ensure(ctx, ctx->d_cell_atoms, N) || ensure(ctx, ctx->d_nb_count, N + 1) || ensure(ctx, ctx->d_nb_off, N + 1))
+    const int N = ctx->N, ncs = ctx->ncells + ctx->S;
+    // the cell geometry changes only with the cells or the cutoff: skip the upload when it is unchanged
+    if (ctx->sinfo_up.size() != ctx->sinfo_h.size() ||
+        memcmp(ctx->sinfo_up.data(), ctx->sinfo_h.data(), ctx->S * sizeof(StrucInfo)) != 0 || !ctx->d_sinfo.p) {
+        if (upload(ctx, ctx->d_sinfo, ctx->sinfo_h.data(), ctx->S)) return -1;
+        ctx->sinfo_up = ctx->sinfo_h;
+    }
+    if (ensure(ctx, ctx->d_cell_of, N) || ensure(ctx, ctx->d_wrap_of, N) || ensure(ctx, ctx->d_slot_of, N) ||
+        ensure(ctx, ctx->d_cell_start, ncs) || ensure(ctx, ctx->d_cpos, N) || ensure(ctx, ctx->d_cinfo, N) ||
+        ensure(ctx, ctx->d_nb_count, N + 1) || ensure(ctx, ctx->d_nb_off, N + 1))
         return -1;
-    HC(hipMemsetAsync(ctx->d_cell_count.p, 0, (nc + 1) * sizeof(int), ctx->stream));
-    HC(hipMemsetAsync(ctx->d_cell_cursor.p, 0, (nc + 1) * sizeof(int), ctx->stream));
-    HC(hipMemsetAsync(ctx->d_scal.p, 0, 16 * sizeof(int), ctx->stream));
-    if (timed(ctx, "nb_assign", [&] {
-            k_nb_assign<<<cdiv(N, 256), 256, 0, ctx->stream>>>(N, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p,
-                                                                ctx->d_cell_of.p, ctx->d_wrap_of.p, ctx->d_cell_count.p);
-        }))
-        return -1;
-    if (exclusive_scan(ctx, ctx->d_cell_count.p, ctx->d_cell_start.p, nc)) return -1;
-    if (timed(ctx, "nb_scatter", [&] {
-            k_nb_scatter<<<cdiv(N, 256), 256, 0, ctx->stream>>>(N, ctx->d_cell_of.p, ctx->d_cell_start.p,
-                                                                 ctx->d_cell_cursor.p, ctx->d_cell_atoms.p);
+    if (timed(ctx, "nb_bin", [&] {
+            k_nb_bin<<<ctx->S, NB_BIN_THREADS, 0, ctx->stream>>>(ctx->d_pos.p, ctx->d_sinfo.p, ctx->d_cell_of.p,
+                                                                 ctx->d_wrap_of.p, ctx->d_slot_of.p,
+                                                                 ctx->d_cell_start.p, ctx->d_cpos.p, ctx->d_cinfo.p,
+                                                                 ctx->d_scal.p);
         }))
         return -1;
     const double rc2 = rcl * rcl;
-    if (timed(ctx, "nb_count", [&] {
-            k_nb_search<0><<<cdiv(N, NB_WAVES), NB_WAVES * 64, 0, ctx->stream>>>(
-                N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
-                ctx->d_cell_start.p, ctx->d_cell_atoms.p, ctx->d_nb_count.p, ctx->d_scal.p, nullptr, nullptr, nullptr,
-                nullptr, nullptr);
-        }))
-        return -1;
-    if (exclusive_scan(ctx, ctx->d_nb_count.p, ctx->d_nb_off.p, N)) return -1;
-    // Capacities: the first build (or one after an overflow) reads the exact sizes back; later builds
-    // reuse the capacities with a margin and run without a host round trip.  An overflow empties every
-    // list on the device (k_nb_check) and is reported at the next synchronising call.
+    // Capacities: the first build (or one after an overflow) counts first and reads the sizes back;
+    // later builds reuse the capacities (pairs with a 10% margin, key rows = LDS slots of the basis
+    // kernels) and run without a host round trip.  An overflow empties every list on the device and is
+    // reported at the next synchronising call.
     const bool async = ctx->nb_async_ok && ctx->nb_same_layout;
     if (!async) {
+        if (timed(ctx, "nb_count", [&] {
+                k_nb_search<0><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
+                    N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
+                    ctx->d_cell_start.p, ctx->d_cpos.p, ctx->d_cinfo.p, ctx->d_nb_count.p, ctx->d_scal.p, nullptr, 0,
+                    nullptr);
+            }))
+            return -1;
+        if (exclusive_scan(ctx, ctx->d_nb_count.p, ctx->d_nb_off.p, N)) return -1;
         int h[2] = {0, 0};
-        HC(hipMemcpyAsync(&h[0], ctx->d_scal.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        HC(hipMemcpyAsync(&h[1], ctx->d_nb_off.p + N, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HC(hipMemcpyAsync(&h[0], ctx->d_scal.p, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HC(hipStreamSynchronize(ctx->stream));
         if (h[0] > NB_LMAX) return fail(ctx, "ace_build_neighbors: an atom has more than 1024 neighbours");
         ctx->pcap = (long long)(1.1 * h[1]) + 1024;
@@ -913,24 +961,22 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
     }
     const long long P = ctx->pcap;
     if (ensure(ctx, ctx->d_nb_j, P) || ensure(ctx, ctx->d_nb_img, P) || ensure(ctx, ctx->d_nb_src, P) ||
-        ensure(ctx, ctx->d_nb_rev, P) || ensure(ctx, ctx->d_nb_d, P) || ensure(ctx, ctx->d_pair_g, P))
+        ensure(ctx, ctx->d_nb_rev, P) || ensure(ctx, ctx->d_nb_d, P) || ensure(ctx, ctx->d_pair_g, P) ||
+        ensure(ctx, ctx->d_nb_keys, (size_t)N * ctx->slots_cap))
         return -1;
-    if (timed(ctx, "nb_check", [&] {
-            k_nb_check<<<cdiv(N + 1, 256), 256, 0, ctx->stream>>>(N, ctx->d_nb_off.p, ctx->d_scal.p, ctx->d_scal.p + 1,
-                                                                   (int)P, ctx->slots_cap, ctx->d_scal.p + 4);
-        }))
-        return -1;
-    if (timed(ctx, "nb_fill", [&] {
-            k_nb_search<1><<<cdiv(N, NB_WAVES), NB_WAVES * 64, 0, ctx->stream>>>(
+    if (timed(ctx, "nb_search", [&] {
+            k_nb_search<1><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
                 N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
-                ctx->d_cell_start.p, ctx->d_cell_atoms.p, nullptr, nullptr, ctx->d_nb_off.p, ctx->d_nb_j.p,
-                ctx->d_nb_img.p, ctx->d_nb_d.p, ctx->d_nb_src.p);
+                ctx->d_cell_start.p, ctx->d_cpos.p, ctx->d_cinfo.p, ctx->d_nb_count.p, ctx->d_scal.p,
+                ctx->d_nb_keys.p, ctx->slots_cap, ctx->d_scal.p + 5);
         }))
         return -1;
-    if (timed(ctx, "nb_reverse", [&] {
-            k_nb_reverse<<<cdiv(P, 256), 256, 0, ctx->stream>>>(ctx->d_nb_off.p + N, ctx->d_nb_src.p, ctx->d_nb_j.p,
-                                                                 ctx->d_nb_img.p, ctx->d_nb_off.p, ctx->d_struc_of.p,
-                                                                 ctx->d_sinfo.p, ctx->d_nb_rev.p, ctx->d_scal.p + 2);
+    if (exclusive_scan(ctx, ctx->d_nb_count.p, ctx->d_nb_off.p, N)) return -1;
+    if (timed(ctx, "nb_compact", [&] {
+            k_nb_compact<<<cdiv(N + 1, 4), 256, 0, ctx->stream>>>(
+                N, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_nb_keys.p, ctx->slots_cap,
+                ctx->d_nb_count.p, ctx->d_nb_off.p, ctx->d_scal.p + 1, ctx->d_scal.p + 5, P, ctx->d_nb_j.p,
+                ctx->d_nb_img.p, ctx->d_nb_src.p, ctx->d_nb_d.p, ctx->d_nb_rev.p, ctx->d_scal.p + 4, ctx->d_scal.p + 2);
         }))
         return -1;
     ctx->nb_valid = true;
@@ -1016,7 +1062,6 @@ int ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, d
         return fail(ctx, "ace_energy_forces: precision mode not available");
     ctx->basis_f64 = precision == ACE_PREC_FP64_BASIS;
     if (require_nb(ctx)) return -1;
-    HC(hipMemsetAsync(ctx->d_scal.p + 3, 0, sizeof(int), ctx->stream));
     if (run_basis(ctx, 0)) return -1;
     if (run_mlp(ctx)) return -1;
     if (run_basis(ctx, 1)) return -1;

@@ -26,16 +26,18 @@
 // Modes: 0 = network input rows inpw*(G + inpb); 1 = adjoint (per-pair dE/dd from dE/dG rows);
 // 2 = tangent rows v = sum_a eps_a . dG/dr_a.
 // Dynamic LDS per workgroup (nbm = neighbour slots, kp = kmax rounded up to 4), in units of T:
-//   u[4 nbm], te[4 nbm] (mode 2), E[ng nbm], Ed[ng nbm] (modes 1,2), fR[nbm], gR[nbm],
-//   g[BW 3 nbm] (mode 1), row[kp], red[BW 64]; then p[nbm] (int).
+//   u[4 nbm], te[4 nbm] (mode 2), E[ng nbm], Ed[ng nbm] (modes 1,2), fR[nbm], gR[nbm], ri[nbm],
+//   row[kp], red[BW 64], rp[2 kp] (radial eta, rs of the species), sc[2 kp] (mode 0: inpw, inpb);
+//   then p[nbm] and ci[kp] (int).  Mode 1 puts g[BW 3 nbm] (double) in front.
 template <class T>
 __host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int mode) {
     const int kp = (kmax + 3) & ~3;
-    int n = 4 * nbm + ng * nbm + 2 * nbm + kp + BW * 64;
+    int n = 4 * nbm + ng * nbm + 3 * nbm + kp + BW * 64 + 2 * kp;
+    if (mode == 0) n += 2 * kp;
     if (mode == 2) n += 4 * nbm;
     if (mode >= 1) n += ng * nbm;
-    if (mode == 1) n += BW * 3 * nbm;
-    return ((int)sizeof(T) * n + 4 * nbm + 15) & ~15;
+    const int gb = (mode == 1) ? 8 * BW * 3 * nbm : 0;
+    return (gb + (int)sizeof(T) * n + 4 * (nbm + kp) + 15) & ~15;
 }
 
 template <class T>
@@ -46,46 +48,56 @@ struct BasisLds {
     T* Ed;     // [ng][nbm]
     T* fR;
     T* gR;
-    T* g;      // [BW][3 nbm]
+    T* ri;     // 1 / r
+    double* g; // [BW][3 nbm] per-wave dE/dd accumulators (FP64: LDS FP64 atomics, exact-order-free)
     T* row;
     T* red;    // [BW][64]
+    T* rp;     // radial eta [nR], rs [nR] of the centre species
+    T* sc;     // network input scaling inpw [K], inpb [K] (mode 0)
     int* p;
+    int* ci;   // canonical slot of each angular function of the species
 };
 
 template <class T>
 __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, int mode) {
-    T* c = (T*)base;
     BasisLds<T> L;
+    L.g = (double*)base;
+    T* c = (T*)((char*)base + ((mode == 1) ? 8 * BW * 3 * nbm : 0));
     L.u = c; c += 4 * nbm;
     L.te = c; if (mode == 2) c += 4 * nbm;
     L.E = c; c += ng * nbm;
     L.Ed = c; if (mode >= 1) c += ng * nbm;
     L.fR = c; c += nbm;
     L.gR = c; c += nbm;
-    L.g = c; if (mode == 1) c += BW * 3 * nbm;
+    L.ri = c; c += nbm;
     L.row = c; c += (kmax + 3) & ~3;
     L.red = c; c += BW * 64;
+    const int kp = (kmax + 3) & ~3;
+    L.rp = c; c += 2 * kp;
+    L.sc = c; if (mode == 0) c += 2 * kp;
     L.p = (int*)c;
+    L.ci = L.p + nbm;
     return L;
 }
 
 // (a, c) of flattened triplet t in block (J,K): rectangle a in J, c in K; triangle a<c in J.
-__device__ inline void tri_decode(int t, bool same, int nJ, int nK, float rnK, int& a, int& c) {
-    if (!same) {
-        a = (int)(((float)t + 0.5f) * rnK);
-        c = t - a * nK;
-        if (c < 0) { --a; c += nK; }
-        else if (c >= nK) { ++a; c -= nK; }
-    } else {
-        const float n2 = 2.f * nJ - 1.f;
-        int r = (int)floorf(0.5f * (n2 - sqrtf(fmaxf(n2 * n2 - 8.f * (float)t, 0.f))));
-        if (r < 0) r = 0;
-        // S(r) = r (2 nJ - r - 1) / 2 pairs precede row r
-        while (r > 0 && (r * (2 * nJ - r - 1)) / 2 > t) --r;
-        while (((r + 1) * (2 * nJ - r - 2)) / 2 <= t) ++r;
-        a = r;
-        c = t - (r * (2 * nJ - r - 1)) / 2 + r + 1;
-    }
+// Triplet t of a block -> neighbour slots (a, c) by rotation: row d pairs x with (x + d) mod n, so the
+// lanes of one row touch distinct a and distinct c (no LDS address shared by two lanes of an atomic).
+//   same species (n = nJ):  d = 1 + t / n, a = t mod n, c = (a + d) mod n; rows d <= (n - 1) / 2 are full
+//                           and, for even n, row n / 2 holds a < n / 2 only -- n (n - 1) / 2 triplets.
+//   J != K (m = min, M = max of nJ, nK): d = t / m, x = t mod m, y = (x + d) mod M -- nJ nK triplets.
+// rn = 1 / n (same) or 1 / m.
+__device__ inline void tri_decode(int t, bool same, int nJ, int nK, float rn, int& a, int& c) {
+    const int m = same ? nJ : min(nJ, nK), M = same ? nJ : max(nJ, nK);
+    int d = (int)(((float)t + 0.5f) * rn);
+    int x = t - d * m;
+    if (x < 0) { --d; x += m; }
+    else if (x >= m) { ++d; x -= m; }
+    if (same) ++d;
+    int y = x + d;
+    if (y >= M) y -= M;
+    if (same || nJ <= nK) { a = x; c = y; }
+    else { a = y; c = x; }
 }
 
 // pick v[g] for a workgroup-uniform g < ACE_MAXG
@@ -122,7 +134,7 @@ __device__ inline T prm(const float* f, const double* d, int i) {
 // held in registers with static indices; cidx maps each function of the block to (s*7 + k)*NG + g.
 template <class T, int NG, int MODE>
 __device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int ng, int bJ, int bK, int nJ, int nK,
-                                     bool same, int T3, const int* __restrict__ cidx, int nA, int goff, T* gw,
+                                     bool same, int T3, const int* __restrict__ cidx, int nA, int goff, double* gw,
                                      int tid, int wv, int lane) {
     constexpr int NZ = 7;
     constexpr int ADJ = (MODE == 1);
@@ -151,13 +163,13 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int 
                 for (int g = 0; g < NG; ++g) wgt[s][k][g] = L.red[(s * NZ + k) * NG + g];
         __syncthreads();
     }
-    const float rnK = nK > 0 ? 1.f / (float)nK : 0.f;
+    const float rnK = 1.f / (float)(same ? nJ : min(nJ, nK));
     for (int t = tid; t < T3; t += BT) {
         int a, c;
         tri_decode(t, same, nJ, nK, rnK, a, c);
         const int sa = bJ + a, sc = bK + c;
-        const T uax = L.u[4 * sa], uay = L.u[4 * sa + 1], uaz = L.u[4 * sa + 2], ra = L.u[4 * sa + 3];
-        const T ucx = L.u[4 * sc], ucy = L.u[4 * sc + 1], ucz = L.u[4 * sc + 2], rc = L.u[4 * sc + 3];
+        const T uax = L.u[4 * sa], uay = L.u[4 * sa + 1], uaz = L.u[4 * sa + 2];
+        const T ucx = L.u[4 * sc], ucy = L.u[4 * sc + 1], ucz = L.u[4 * sc + 2];
         const T cs = uax * ucx + uay * ucy + uaz * ucz;
         T Da[NG], Dc[NG], Dda[NG], Ddc[NG], D[NG];
 #pragma unroll
@@ -195,8 +207,8 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int 
         } else if (TAN) {
             const T tax = L.te[4 * sa], tay = L.te[4 * sa + 1], taz = L.te[4 * sa + 2], tap = L.te[4 * sa + 3];
             const T tcx = L.te[4 * sc], tcy = L.te[4 * sc + 1], tcz = L.te[4 * sc + 2], tcp = L.te[4 * sc + 3];
-            const T dcs = (ucx * tax + ucy * tay + ucz * taz - cs * tap) / ra +
-                          (uax * tcx + uay * tcy + uaz * tcz - cs * tcp) / rc;
+            const T dcs = (ucx * tax + ucy * tay + ucz * taz - cs * tap) * L.ri[sa] +
+                          (uax * tcx + uay * tcy + uaz * tcz - cs * tcp) * L.ri[sc];
             T X[NG], Y[NG];
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
@@ -235,15 +247,15 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int 
                 dEda += S[g] * Dda[g] * Dc[g];
                 dEdcc += S[g] * Da[g] * Ddc[g];
             }
-            const T ia = T(1) / ra, ic = T(1) / rc;
+            const T ia = L.ri[sa], ic = L.ri[sc];
             const T ka = dEda - dEdc * cs * ia, kc = dEdcc - dEdc * cs * ic;
             const T xa = dEdc * ia, xc = dEdc * ic;
-            atomicAdd(&gw[3 * sa], ka * uax + xa * ucx);
-            atomicAdd(&gw[3 * sa + 1], ka * uay + xa * ucy);
-            atomicAdd(&gw[3 * sa + 2], ka * uaz + xa * ucz);
-            atomicAdd(&gw[3 * sc], kc * ucx + xc * uax);
-            atomicAdd(&gw[3 * sc + 1], kc * ucy + xc * uay);
-            atomicAdd(&gw[3 * sc + 2], kc * ucz + xc * uaz);
+            atomicAdd(&gw[3 * sa], (double)(ka * uax + xa * ucx));
+            atomicAdd(&gw[3 * sa + 1], (double)(ka * uay + xa * ucy));
+            atomicAdd(&gw[3 * sa + 2], (double)(ka * uaz + xa * ucz));
+            atomicAdd(&gw[3 * sc], (double)(kc * ucx + xc * uax));
+            atomicAdd(&gw[3 * sc + 1], (double)(kc * ucy + xc * uay));
+            atomicAdd(&gw[3 * sc + 2], (double)(kc * ucz + xc * uaz));
         }
     }
     if (!ADJ) {
@@ -267,31 +279,60 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int 
     }
 }
 
-template <class T, int NG, int MODE>
+// GEN = 0 compiles the kernel for models whose angular blocks are all canonical (no generic path).
+template <class T, int NG, int MODE, int GEN>
 __global__ __launch_bounds__(BT) void k_basis(
     int natoms, const DevModel* __restrict__ M, DevBasisParams P, const int* __restrict__ type,
     const int* __restrict__ nb_off, const int* __restrict__ nb_j, const float4* __restrict__ nb_d,
     const int* __restrict__ nb_img, const double* __restrict__ pos, const int* __restrict__ struc_of,
     const StrucInfo* __restrict__ sinfo, const int* __restrict__ row_of, int ldg, float* __restrict__ G,
     const float* __restrict__ dEdG, float4* __restrict__ pair_g, int* __restrict__ err, int nbm,
-    const double* __restrict__ eps) {
+    const double* __restrict__ eps, int nt, int ngmax, int kmax) {
     constexpr int ADJ = (MODE == 1);
     constexpr int TAN = (MODE == 2);
     constexpr bool F64 = sizeof(T) == 8;
     constexpr int MAXA = 16;   // functions per register chunk of the generic path
     extern __shared__ double4 basis_smem[];
     __shared__ int s_cnt[ACE_MAXT + 1];
+    __shared__ DevSpecies s_sp;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int i = blockIdx.x;
     if (i >= natoms) return;
     const int I = type[i];
-    const DevSpecies& sp = M->sp[I];
-    const BasisLds<T> L = basis_lds<T>(basis_smem, nbm, M->ng_max, M->kmax, MODE);
-    const int nt = M->ntypes;
+    const DevSpecies& gsp = M->sp[I];
+    const BasisLds<T> L = basis_lds<T>(basis_smem, nbm, ngmax, kmax, MODE);
     const int o0 = nb_off[i], o1 = nb_off[i + 1];
-    const float rcut = sp.rc_max;
+    const float rcut = gsp.rc_max;
     const int row = row_of[i];
-    const int ng = sp.ng;
+
+    // --- waves 1-3 stage the species tables in LDS while wave 0 buckets the neighbours --------------
+    if (wv == 1) {
+        constexpr int n16 = (int)(sizeof(DevSpecies) / 16);
+        static_assert(sizeof(DevSpecies) % 16 == 0, "DevSpecies is copied in 16-byte words");
+        const int4* src = (const int4*)&gsp;
+        int4* dst = (int4*)&s_sp;
+        const int4 v0 = src[lane];
+        const int4 v1 = (64 + lane < n16) ? src[64 + lane] : v0;
+        dst[lane] = v0;
+        if (64 + lane < n16) dst[64 + lane] = v1;
+        static_assert(n16 >= 64 && n16 <= 128, "DevSpecies copy covers 64 to 128 words");
+    } else if (wv == 2) {
+        const int nR = gsp.nR, r0 = gsp.r_poff[0];
+        for (int q = lane; q < nR; q += 64) {
+            L.rp[q] = prm<T>(P.r_eta, P.r_eta64, r0 + q);
+            L.rp[nR + q] = prm<T>(P.r_rs, P.r_rs64, r0 + q);
+        }
+        if (ADJ)
+            for (int k = lane; k < gsp.K; k += 64) L.row[k] = (T)dEdG[(long long)row * ldg + k];
+    } else if (wv == 3) {
+        const int nA = gsp.K - gsp.nR, a0 = gsp.a_poff[0];
+        for (int q = lane; q < nA; q += 64) L.ci[q] = P.a_cidx[a0 + q];
+        if (MODE == 0)
+            for (int k = lane; k < gsp.K; k += 64) {
+                L.sc[k] = prm<T>(P.inpw, P.inpw64, gsp.in_off + k);
+                L.sc[gsp.K + k] = prm<T>(P.inpb, P.inpb64, gsp.in_off + k);
+            }
+    }
 
     // --- wave 0: bucket the neighbours by species (ballot order keeps the list order) -------------
     if (wv == 0) {
@@ -338,6 +379,8 @@ __global__ __launch_bounds__(BT) void k_basis(
         }
     }
     __syncthreads();
+    const DevSpecies& sp = s_sp;
+    const int ng = sp.ng;
     const int ntot = s_cnt[nt];
     if (ntot > nbm) {
         if (tid == 0) atomicAdd(err, 1);
@@ -369,6 +412,7 @@ __global__ __launch_bounds__(BT) void k_basis(
         L.u[4 * q + 1] = dy * ir;
         L.u[4 * q + 2] = dz * ir;
         L.u[4 * q + 3] = r;
+        L.ri[q] = ir;
         T fc, gc;
         t_cutoff<T>(sp.r_cut[J], F64 ? (T)sp.r_rc64[J] : (T)sp.r_rc[J], r, fc, gc);
         L.fR[q] = fc;
@@ -382,9 +426,9 @@ __global__ __launch_bounds__(BT) void k_basis(
         }
         if (ADJ) {
             for (int w = 0; w < BW; ++w) {
-                L.g[w * 3 * nbm + 3 * q] = T(0);
-                L.g[w * 3 * nbm + 3 * q + 1] = T(0);
-                L.g[w * 3 * nbm + 3 * q + 2] = T(0);
+                L.g[w * 3 * nbm + 3 * q] = 0.0;
+                L.g[w * 3 * nbm + 3 * q + 1] = 0.0;
+                L.g[w * 3 * nbm + 3 * q + 2] = 0.0;
             }
         }
         if (TAN) {
@@ -398,7 +442,8 @@ __global__ __launch_bounds__(BT) void k_basis(
             L.te[4 * q + 3] = (dx * ex + dy * ey + dz * ez) * ir;
         }
     }
-    for (int k = tid; k < sp.K; k += BT) L.row[k] = ADJ ? (T)dEdG[(long long)row * ldg + k] : T(0);
+    if (!ADJ)
+        for (int k = tid; k < sp.K; k += BT) L.row[k] = T(0);
     __syncthreads();
 
     // --- radial blocks: lane = function, lane group = neighbour subset ---------------------------
@@ -415,8 +460,9 @@ __global__ __launch_bounds__(BT) void k_basis(
             const int gpw = 64 / nrp, ngr = BW * gpw;
             const int n = lane % nrp, grp = wv * gpw + lane / nrp;
             const bool on = n < nc;
-            const T eta = on ? prm<T>(P.r_eta, P.r_eta64, poff + n0 + n) : T(0);
-            const T rs = on ? prm<T>(P.r_rs, P.r_rs64, poff + n0 + n) : T(0);
+            const int rq = poff - sp.r_poff[0] + n0 + n;
+            const T eta = on ? L.rp[rq] : T(0);
+            const T rs = on ? L.rp[sp.nR + rq] : T(0);
             const T w = (ADJ && on) ? L.row[goff + n0 + n] : T(0);
             T acc = T(0);
             const int rounds = (nJ + ngr - 1) / ngr;
@@ -431,24 +477,24 @@ __global__ __launch_bounds__(BT) void k_basis(
                 }
                 if (ADJ) {
                     // dE/dr of the slot: sum over the nrp function lanes of its group
-                    for (int m = 1; m < nrp; m <<= 1) val += __shfl_xor(val, m, 64);
+                    val = t_group_sum<T>(val, nrp);
                     if (n == 0 && q < nJ) {
                         const int slot = bJ + q;
-                        L.g[3 * slot] += val * L.u[4 * slot];          // wave-private array of wave 0..3
-                        L.g[3 * slot + 1] += val * L.u[4 * slot + 1];
-                        L.g[3 * slot + 2] += val * L.u[4 * slot + 2];
+                        L.g[3 * slot] += (double)(val * L.u[4 * slot]);          // wave-private array of wave 0..3
+                        L.g[3 * slot + 1] += (double)(val * L.u[4 * slot + 1]);
+                        L.g[3 * slot + 2] += (double)(val * L.u[4 * slot + 2]);
                     }
                 } else {
                     acc += val;
                 }
             }
             if (!ADJ) {
-                for (int m = nrp; m < 64; m <<= 1) acc += __shfl_xor(acc, m, 64);
-                if (lane < nrp) L.red[wv * 64 + lane] = acc;
+                L.red[wv * 64 + lane] = acc;   // lane n + nrp * group of each wave
                 __syncthreads();
                 if (tid < nc) {
                     T s = T(0);
-                    for (int ww = 0; ww < BW; ++ww) s += L.red[ww * 64 + tid];
+                    for (int ww = 0; ww < BW; ++ww)
+                        for (int g = tid; g < 64; g += nrp) s += L.red[ww * 64 + g];
                     L.row[goff + n0 + tid] = s;
                 }
                 __syncthreads();
@@ -458,7 +504,7 @@ __global__ __launch_bounds__(BT) void k_basis(
     if (ADJ) __syncthreads();
 
     // --- angular blocks (J <= K) -------------------------------------------------------------------
-    T* gw = L.g + wv * 3 * nbm;
+    double* gw = L.g + wv * 3 * nbm;
     for (int K = 0; K < nt; ++K) {
         for (int J = 0; J <= K; ++J) {
             const int b = K * (K + 1) / 2 + J;
@@ -468,13 +514,13 @@ __global__ __launch_bounds__(BT) void k_basis(
             const int nJ = s_cnt[J + 1] - bJ, nK = s_cnt[K + 1] - bK;
             const int T3 = same ? nJ * (nJ - 1) / 2 : nJ * nK;
             if (nA == 0 || T3 == 0) continue;
-            if (sp.a_canon[b]) {
-                angular_canon<T, NG, MODE>(L, nbm, sp.nz, ng, bJ, bK, nJ, nK, same, T3, P.a_cidx + sp.a_poff[b], nA,
+            if (!GEN || sp.a_canon[b]) {
+                angular_canon<T, NG, MODE>(L, nbm, sp.nz, ng, bJ, bK, nJ, nK, same, T3, L.ci + (sp.a_poff[b] - sp.a_poff[0]), nA,
                                            sp.a_goff[b], gw, tid, wv, lane);
                 continue;
             }
             const int goff = sp.a_goff[b], poff = sp.a_poff[b];
-            const float rnK = nK > 0 ? 1.f / (float)nK : 0.f;
+            const float rnK = 1.f / (float)(same ? nJ : min(nJ, nK));
             for (int m0 = 0; m0 < nA; m0 += MAXA) {
                 T lamf[MAXA], zeta[MAXA], acc[MAXA], w[MAXA];
                 int zint[MAXA], grp[MAXA], sel[MAXA];
@@ -498,8 +544,8 @@ __global__ __launch_bounds__(BT) void k_basis(
                     int a, c;
                     tri_decode(t, same, nJ, nK, rnK, a, c);
                     const int sa = bJ + a, sc = bK + c;
-                    const T uax = L.u[4 * sa], uay = L.u[4 * sa + 1], uaz = L.u[4 * sa + 2], ra = L.u[4 * sa + 3];
-                    const T ucx = L.u[4 * sc], ucy = L.u[4 * sc + 1], ucz = L.u[4 * sc + 2], rc = L.u[4 * sc + 3];
+                    const T uax = L.u[4 * sa], uay = L.u[4 * sa + 1], uaz = L.u[4 * sa + 2];
+                    const T ucx = L.u[4 * sc], ucy = L.u[4 * sc + 1], ucz = L.u[4 * sc + 2];
                     const T cs = uax * ucx + uay * ucy + uaz * ucz;
                     T Da[ACE_MAXG], Dc[ACE_MAXG], Dda[ACE_MAXG], Ddc[ACE_MAXG];
 #pragma unroll
@@ -535,8 +581,8 @@ __global__ __launch_bounds__(BT) void k_basis(
                     if (TAN) {
                         const T tax = L.te[4 * sa], tay = L.te[4 * sa + 1], taz = L.te[4 * sa + 2], tap = L.te[4 * sa + 3];
                         const T tcx = L.te[4 * sc], tcy = L.te[4 * sc + 1], tcz = L.te[4 * sc + 2], tcp = L.te[4 * sc + 3];
-                        const T dcs = (ucx * tax + ucy * tay + ucz * taz - cs * tap) / ra +
-                                      (uax * tcx + uay * tcy + uaz * tcz - cs * tcp) / rc;
+                        const T dcs = (ucx * tax + ucy * tay + ucz * taz - cs * tap) * L.ri[sa] +
+                                      (uax * tcx + uay * tcy + uaz * tcz - cs * tcp) * L.ri[sc];
 #pragma unroll
                         for (int g = 0; g < ACE_MAXG; ++g) {
                             Y[g] = Dda[g] * Dc[g] * tap + Da[g] * Ddc[g] * tcp;
@@ -577,15 +623,15 @@ __global__ __launch_bounds__(BT) void k_basis(
                             dEda += S[g] * Dda[g] * Dc[g];
                             dEdcc += S[g] * Da[g] * Ddc[g];
                         }
-                        const T ia = T(1) / ra, ic = T(1) / rc;
+                        const T ia = L.ri[sa], ic = L.ri[sc];
                         const T ka = dEda - dEdc * cs * ia, kc = dEdcc - dEdc * cs * ic;
                         const T xa = dEdc * ia, xc = dEdc * ic;
-                        atomicAdd(&gw[3 * sa], ka * uax + xa * ucx);
-                        atomicAdd(&gw[3 * sa + 1], ka * uay + xa * ucy);
-                        atomicAdd(&gw[3 * sa + 2], ka * uaz + xa * ucz);
-                        atomicAdd(&gw[3 * sc], kc * ucx + xc * uax);
-                        atomicAdd(&gw[3 * sc + 1], kc * ucy + xc * uay);
-                        atomicAdd(&gw[3 * sc + 2], kc * ucz + xc * uaz);
+                        atomicAdd(&gw[3 * sa], (double)(ka * uax + xa * ucx));
+                        atomicAdd(&gw[3 * sa + 1], (double)(ka * uay + xa * ucy));
+                        atomicAdd(&gw[3 * sa + 2], (double)(ka * uaz + xa * ucz));
+                        atomicAdd(&gw[3 * sc], (double)(kc * ucx + xc * uax));
+                        atomicAdd(&gw[3 * sc + 1], (double)(kc * ucy + xc * uay));
+                        atomicAdd(&gw[3 * sc + 2], (double)(kc * ucz + xc * uaz));
                     }
                 }
                 if (!ADJ) {
@@ -612,14 +658,14 @@ __global__ __launch_bounds__(BT) void k_basis(
         // network input row: the ANN's input scaling inpw*(G + inpb) (nn.cpp:1406) applied here, in T,
         // so the FP32 row keeps full relative precision after the mean is removed
         for (int k = tid; k < sp.K; k += BT) {
-            const T w = prm<T>(P.inpw, P.inpw64, sp.in_off + k), b = prm<T>(P.inpb, P.inpb64, sp.in_off + k);
+            const T w = L.sc[k], b = L.sc[sp.K + k];
             G[(long long)row * ldg + k] = (float)(w * (L.row[k] + b));
         }
     } else if (TAN) {
         for (int k = tid; k < sp.K; k += BT) G[(long long)row * ldg + k] = (float)L.row[k];   // v
     } else {
         for (int q = tid; q < ntot; q += BT) {
-            T gx = T(0), gy = T(0), gz = T(0);
+            double gx = 0.0, gy = 0.0, gz = 0.0;
             for (int ww = 0; ww < BW; ++ww) {
                 gx += L.g[ww * 3 * nbm + 3 * q];
                 gy += L.g[ww * 3 * nbm + 3 * q + 1];

@@ -107,9 +107,62 @@ __device__ inline T t_pow_window(T x, T z, int zint) {
     return (x > T(0)) ? pow(x, z) : T(0);
 }
 
+// Cross-lane moves through DPP (data-parallel primitives: a VALU operand modifier, no LDS round trip).
+// CTRL: 0xB1 quad_perm(1,0,3,2), 0x4E quad_perm(2,3,0,1), 0x141 row_half_mirror, 0x140 row_mirror,
+// 0x142 row_bcast:15, 0x143 row_bcast:31.  Rows outside ROWMASK receive 0.
+template <int CTRL, int ROWMASK>
+__device__ inline float dpp_mov(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWMASK, 0xF, false));
+}
+template <int CTRL, int ROWMASK>
+__device__ inline double dpp_mov(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWMASK, 0xF, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// sum over the 16 lanes of each row, returned in every lane of the row
+template <class T>
+__device__ inline T t_row_sum(T v) {
+    v += dpp_mov<0xB1, 0xF>(v);
+    v += dpp_mov<0x4E, 0xF>(v);
+    v += dpp_mov<0x141, 0xF>(v);
+    v += dpp_mov<0x140, 0xF>(v);
+    return v;
+}
+
+// sum over the wave, valid in lane 63 (row sums, then row_bcast:15 into rows 1 and 3 and
+// row_bcast:31 into rows 2 and 3)
+template <class T>
+__device__ inline T t_wave_sum63(T v) {
+    v = t_row_sum<T>(v);
+    v += dpp_mov<0x142, 0xA>(v);
+    v += dpp_mov<0x143, 0xC>(v);
+    return v;
+}
+
+// sum over aligned groups of n lanes (n a power of two, wave-uniform), returned in every lane of the group
+template <class T>
+__device__ inline T t_group_sum(T v, int n) {
+    if (n >= 2) v += dpp_mov<0xB1, 0xF>(v);
+    if (n >= 4) v += dpp_mov<0x4E, 0xF>(v);
+    if (n >= 8) v += dpp_mov<0x141, 0xF>(v);
+    if (n >= 16) v += dpp_mov<0x140, 0xF>(v);
+    if (n >= 32) v += __shfl_xor(v, 16, 64);
+    if (n >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
 template <class T>
 __device__ inline T t_wave_sum(T v) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
+    v = t_wave_sum63<T>(v);
+    if constexpr (sizeof(T) == 8) {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+        return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+    } else {
+        return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+    }
 }

@@ -40,92 +40,209 @@ __device__ inline void act_fd(int act, float c, float z, float& a, float& d) {
     }
 }
 
-// dynamic LDS: x [TM][K], then per layer l: a_l [TM][w_{l+1}], d_l [TM][w_{l+1}]
-__host__ __device__ inline int mlp_lds_floats(const DevSpecies& sp, int tm) {
-    int n = tm * sp.K;
-    for (int l = 0; l < sp.L; ++l) n += 2 * tm * sp.width[l + 1];
+// ---- matrix-core network pass -------------------------------------------------------------------
+// One wave per tile of 16 atoms of one species.  Every layer, forward and backward, is a chain of
+// v_mfma_f32_16x16x4f32 (D[16x16] += A[16x4] B[4x16], FP32 products and sums):
+//   lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16], and receives D[4 (l / 16) + r][l % 16] in
+//   register r.  A (activations, deltas) comes from LDS; B (weights) from LDS when the species' weights
+//   fit next to the activations (one batched copy), else from global memory.
+// LDS rows have an odd stride (width rounded up to 4, plus 1) so the 16 rows of one A read fall in
+// distinct banks; the padding columns hold zeros, so a k-step past the width adds nothing.  LDS is
+// addressed through integer offsets into mlp_smem so every access compiles to a ds_ instruction.
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+#define MLP_ROWS 16
+
+extern __shared__ float mlp_smem[];
+
+__host__ __device__ inline int mlp_ld(int w) { return ((w + 3) & ~3) + 1; }
+
+// floats of LDS for one tile: X [16][ld(K)], then per layer A_l, D_l [16][ld(w_{l+1})]
+__host__ __device__ inline int mlp_lds_floats(const DevSpecies& sp) {
+    int n = MLP_ROWS * mlp_ld(sp.K);
+    for (int l = 0; l < sp.L; ++l) n += 2 * MLP_ROWS * mlp_ld(sp.width[l + 1]);
     return n;
 }
 
-// tiles: (species, first row, rows, 0)
-__global__ __launch_bounds__(MLP_THREADS) void k_mlp_energy_grad(
-    const DevModel* __restrict__ M, const float* __restrict__ params, const float* __restrict__ inpw,
-    const float* __restrict__ inpb, const int4* __restrict__ tiles, int ldg, const float* __restrict__ G,
-    float* __restrict__ dEdG, const int* __restrict__ atom_of_row, double* __restrict__ e_atom, int tm) {
-    extern __shared__ float smem[];
+// C[16 x n] = A[16 x k] * B for the 16-wide output tile starting at o0; A at LDS offset a (row stride
+// lda), B[kk][o] at LDS offset b (BLDS) or global Bg, element o * so + kk * sk.  Returns the lane's
+// 4 accumulators.
+template <bool BLDS>
+__device__ inline mfma_f4 mfma_tile(int a, int lda, int k, int b, const float* __restrict__ Bg, int so, int sk,
+                                    int n, int o0, int lane) {
+    mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int r = lane & 15, q = lane >> 4;
+    const int o = o0 + r;
+    const bool on = o < n;
+    const int ks = (k + 3) >> 2;
+    for (int s0 = 0; s0 < ks; s0 += 8) {
+        float av[8], bv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int kk = 4 * (s0 + u) + q;
+            av[u] = (s0 + u < ks) ? mlp_smem[a + r * lda + kk] : 0.f;
+            if (BLDS) bv[u] = (on && kk < k) ? mlp_smem[b + o * so + kk * sk] : 0.f;
+            else bv[u] = (on && kk < k) ? Bg[o * so + kk * sk] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// tiles: (species, first row, rows, 0), rows <= 16.  wpad: the parameters in the matrix-core layout
+// (DevSpecies::wl_off / bl_off).  STAGE: the species' block is first copied to LDS with batched 16-byte
+// loads (one memory round trip instead of one per k-step); otherwise it is read from global memory.
+template <bool STAGE>
+__global__ __launch_bounds__(64) void k_mlp_energy_grad(
+    const DevModel* __restrict__ M, const float* __restrict__ wpad, const float* __restrict__ inpw,
+    const int4* __restrict__ tiles, int ldg, const float* __restrict__ G, float* __restrict__ dEdG,
+    const int* __restrict__ atom_of_row, double* __restrict__ e_atom) {
+    __shared__ int s_off[2 * ACE_MAXL + 2];   // A_l, D_l offsets; weights offset
+    __shared__ DevSpecies s_sp;                 // the species' tables, read from LDS in the layer loops
     const int4 tile = tiles[blockIdx.x];
     const int I = tile.x, r0 = tile.y, nr = tile.z;
-    const DevSpecies& sp = M->sp[I];
-    const int K = sp.K, L = sp.L;
-    float* X = smem;
-    float* A[ACE_MAXL];
-    float* D[ACE_MAXL];
+    const int lane = threadIdx.x;
     {
-        float* q = smem + tm * K;
+        constexpr int n16 = (int)(sizeof(DevSpecies) / 16);
+        static_assert(n16 >= 64 && n16 <= 128, "DevSpecies copy covers 64 to 128 words");
+        const int4* src = (const int4*)&M->sp[I];
+        int4* dst = (int4*)&s_sp;
+        const int4 v0 = src[lane];
+        const int4 v1 = (64 + lane < n16) ? src[64 + lane] : v0;
+        dst[lane] = v0;
+        if (64 + lane < n16) dst[64 + lane] = v1;
+    }
+    wave_sync();
+    const DevSpecies& sp = s_sp;
+    const int K = sp.K, L = sp.L;
+    const int ldx = mlp_ld(K);
+    if (lane == 0) {
+        int q = MLP_ROWS * ldx;
         for (int l = 0; l < L; ++l) {
-            A[l] = q; q += tm * sp.width[l + 1];
-            D[l] = q; q += tm * sp.width[l + 1];
+            const int w = MLP_ROWS * mlp_ld(sp.width[l + 1]);
+            s_off[2 * l] = q;
+            s_off[2 * l + 1] = q + w;
+            q += 2 * w;
+        }
+        s_off[2 * ACE_MAXL] = q;   // 16-byte aligned: every ld is odd and the row count 16
+    }
+    wave_sync();
+    const int wofs = s_off[2 * ACE_MAXL];
+    const float* Wg = wpad + sp.wp_off;
+    if (STAGE) {
+        const float4* src = (const float4*)Wg;
+        float4* dst = (float4*)(mlp_smem + wofs);
+        const int n4 = sp.wp_n >> 2;
+        for (int b0 = 0; b0 < n4; b0 += 64 * 8) {
+            float4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = b0 + 64 * u + lane;
+                v[u] = (e < n4) ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = b0 + 64 * u + lane;
+                if (e < n4) dst[e] = v[u];
+            }
         }
     }
-    const float* p = params + sp.poff;
-    const float* iw = inpw + sp.in_off;
-    const float* ib = inpb + sp.in_off;
-    const int tid = threadIdx.x;
-    (void)ib;
-    for (int idx = tid; idx < tm * K; idx += MLP_THREADS) {   // rows already hold inpw*(G + inpb)
-        const int r = idx / K, k = idx - r * K;
-        X[r * K + k] = (r < nr) ? G[(long long)(r0 + r) * ldg + k] : 0.f;
+    // input rows (already inpw (G + inpb)): 16-byte loads of the row prefix [0, round4(K)); the row tail
+    // past K holds zeros in G; rows past the tile are zero
+    {
+        const int k4 = (K + 3) >> 2;
+        const int n = MLP_ROWS * k4;
+        for (int b0 = 0; b0 < n; b0 += 64 * 8) {
+            float4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = b0 + 64 * u + lane;
+                const int r = e / k4, c = e - r * k4;
+                v[u] = (e < n && r < nr) ? *(const float4*)(G + (long long)(r0 + r) * ldg + 4 * c)
+                                         : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = b0 + 64 * u + lane;
+                if (e < n) {
+                    const int r = e / k4, c = e - r * k4;
+                    const int x = r * ldx + 4 * c;
+                    mlp_smem[x] = v[u].x; mlp_smem[x + 1] = v[u].y; mlp_smem[x + 2] = v[u].z; mlp_smem[x + 3] = v[u].w;
+                }
+            }
+        }
+        for (int r = lane; r < MLP_ROWS; r += 64) mlp_smem[r * ldx + ldx - 1] = 0.f;
     }
-    __syncthreads();
+    for (int idx = MLP_ROWS * ldx + lane; idx < wofs; idx += 64) mlp_smem[idx] = 0.f;
+    wave_sync();
+    const int cr = lane & 15, rq = 4 * (lane >> 4);
+    // forward: z_l = b_l + W_l a_{l-1}; W_l[o][k] at wl_off + k (no + 1) + o
+    int ain = 0, lda = ldx;
     for (int l = 0; l < L; ++l) {
         const int no = sp.width[l + 1], ni = sp.width[l];
-        const float* W = p + sp.woff[l];
-        const float* b = p + sp.boff[l];
+        const int ao = s_off[2 * l], dofs = s_off[2 * l + 1], ldo = mlp_ld(no);
         const int act = (l == L - 1) ? ACE_ACT_LINEAR : sp.act;
-        for (int idx = tid; idx < tm * no; idx += MLP_THREADS) {
-            const int r = idx / no, o = idx - r * no;
-            float z = b[o];
-            if (l == 0) {
-                for (int k = 0; k < ni; ++k) z += W[o + k * no] * X[r * ni + k];
-            } else {
-                for (int k = 0; k < ni; ++k) z += W[o + k * no] * A[l - 1][r * ni + k];
+        for (int o0 = 0; o0 < no; o0 += 16) {
+            const mfma_f4 acc = mfma_tile<STAGE>(ain, lda, ni, wofs + sp.wl_off[l], Wg + sp.wl_off[l], 1, no + 1, no,
+                                                 o0, lane);
+            const int o = o0 + cr;
+            if (o < no) {
+                const float bo = STAGE ? mlp_smem[wofs + sp.bl_off[l] + o] : Wg[sp.bl_off[l] + o];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    float a, d;
+                    act_fd(act, sp.c, acc[r] + bo, a, d);
+                    mlp_smem[ao + (rq + r) * ldo + o] = a;
+                    mlp_smem[dofs + (rq + r) * ldo + o] = d;
+                }
             }
-            float a, d;
-            act_fd(act, sp.c, z, a, d);
-            A[l][r * no + o] = a;
-            D[l][r * no + o] = d;
         }
-        __syncthreads();
+        ain = ao;
+        lda = ldo;
+        wave_sync();
     }
-    // energies
-    for (int r = tid; r < nr; r += MLP_THREADS) {
-        const float out = sp.outb + sp.outw * A[L - 1][r];
-        e_atom[atom_of_row[r0 + r]] = (double)out + sp.E0;
-    }
-    __syncthreads();
-    // backward: delta of the output layer, then hidden layers; deltas overwrite a[]
-    for (int r = tid; r < tm; r += MLP_THREADS) A[L - 1][r] = sp.outw * D[L - 1][r];
-    __syncthreads();
+    // energies: out = outb + outw a_last (the last layer has width 1)
+    const int aL = s_off[2 * (L - 1)], dL = s_off[2 * (L - 1) + 1], ldL = mlp_ld(1);
+    if (lane < nr) e_atom[atom_of_row[r0 + lane]] = (double)(sp.outb + sp.outw * mlp_smem[aL + lane * ldL]) + sp.E0;
+    wave_sync();
+    // backward: delta_l overwrites A_l; delta_{l-1} = (delta_l W_l) * f'(z_{l-1})
+    if (lane < MLP_ROWS) mlp_smem[aL + lane * ldL] = sp.outw * mlp_smem[dL + lane * ldL];
+    wave_sync();
     for (int l = L - 1; l > 0; --l) {
         const int no = sp.width[l + 1], ni = sp.width[l];
-        const float* W = p + sp.woff[l];
-        for (int idx = tid; idx < tm * ni; idx += MLP_THREADS) {
-            const int r = idx / ni, k = idx - r * ni;
-            float s = 0.f;
-            for (int o = 0; o < no; ++o) s += W[o + k * no] * A[l][r * no + o];
-            A[l - 1][r * ni + k] = s * D[l - 1][r * ni + k];
+        const int al = s_off[2 * l], am = s_off[2 * (l - 1)], dm = s_off[2 * (l - 1) + 1];
+        const int ldl = mlp_ld(no), ldm = mlp_ld(ni);
+        for (int k0 = 0; k0 < ni; k0 += 16) {
+            // B[o][k] = W_l[o][k]: output column k, reduction index o
+            const mfma_f4 acc = mfma_tile<STAGE>(al, ldl, no, wofs + sp.wl_off[l], Wg + sp.wl_off[l], no + 1, 1, ni,
+                                                 k0, lane);
+            const int k = k0 + cr;
+            if (k < ni) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int e = (rq + r) * ldm + k;
+                    mlp_smem[am + e] = acc[r] * mlp_smem[dm + e];
+                }
+            }
         }
-        __syncthreads();
+        wave_sync();
     }
+    // dE/dG = (delta_0 W_0) diag(inpw)
     {
         const int no = sp.width[1];
-        const float* W = p + sp.woff[0];
-        for (int idx = tid; idx < tm * K; idx += MLP_THREADS) {
-            const int r = idx / K, k = idx - r * K;
-            if (r >= nr) continue;
-            float s = 0.f;
-            for (int o = 0; o < no; ++o) s += W[o + k * no] * A[0][r * no + o];
-            dEdG[(long long)(r0 + r) * ldg + k] = s * iw[k];
+        const float* iw = inpw + sp.in_off;
+        const int a0 = s_off[0], ld0 = mlp_ld(no);
+        for (int k0 = 0; k0 < K; k0 += 16) {
+            const mfma_f4 acc = mfma_tile<STAGE>(a0, ld0, no, wofs + sp.wl_off[0], Wg + sp.wl_off[0], no + 1, 1, K, k0,
+                                                 lane);
+            const int k = k0 + cr;
+            if (k < K) {
+                const float w = iw[k];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (rq + r < nr) dEdG[(long long)(r0 + rq + r) * ldg + k] = acc[r] * w;
+            }
         }
     }
 }

@@ -9,42 +9,88 @@
 // lattice vector d); the stencil half-width st_d covers rc along the perpendicular height, and a
 // stencil cell past the grid edge stands for a translated copy of the cell (image shift), so small
 // cells (L < 2 rc) yield every periodic image of a neighbour once.
+//
+// Pipeline (one build): k_nb_bin (workgroup per structure: cell counts, scan and a cell-ordered copy of
+// the positions, all in LDS) -> k_nb_search (wave per atom; candidates are contiguous runs of the
+// cell-ordered copy, loaded four chunks at a time; keys rank-sorted in LDS into the reference order and
+// written to a fixed-capacity per-atom row) -> exclusive scan of the counts -> k_nb_compact (wave per
+// atom: CSR arrays, FP32 displacements and the index of each pair's reverse pair).
 #include "ace_common.hpp"
 
-// ---- binning -------------------------------------------------------------------------------------
+// ---- binning: one workgroup per structure ---------------------------------------------------------
 
-// thread per atom: fractional coordinates, wrap vector, cell index
-__global__ void k_nb_assign(int natoms, const double* __restrict__ pos, const int* __restrict__ struc_of,
-                            const StrucInfo* __restrict__ sinfo, int* __restrict__ cell_of,
-                            int* __restrict__ wrap_of, int* __restrict__ cell_count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natoms) return;
-    const StrucInfo& S = sinfo[struc_of[i]];
-    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
-    int c[3], w[3];
-    for (int d = 0; d < 3; ++d) {
-        const double f = S.Rinv[d] * x + S.Rinv[d + 3] * y + S.Rinv[d + 6] * z;
-        const double fl = floor(f);
-        double fw = f - fl;
-        int cc = (int)(fw * S.nc[d]);
-        if (cc >= S.nc[d]) cc = S.nc[d] - 1;
-        if (cc < 0) cc = 0;
-        c[d] = cc;
-        w[d] = (int)fl;
+#define NB_BIN_THREADS 1024
+#define NB_MAXC 8192   // cells per structure (the host coarsens the grid to stay within it)
+
+// cell_start[cell_off + c] (global index into the cell-ordered arrays, per structure, with the end entry
+// cell_start[cell_off + ncl] written by the structure's own workgroup at index cell_off + ncl; the
+// host lays structures out with one spare entry each), cpos / cinfo: cell-ordered positions and
+// (atom, wrap) pairs.  cell_of / wrap_of: per atom, in the atom's own slot.
+__global__ __launch_bounds__(NB_BIN_THREADS) void k_nb_bin(const double* __restrict__ pos,
+                                                           const StrucInfo* __restrict__ sinfo,
+                                                           int* __restrict__ cell_of, int* __restrict__ wrap_of,
+                                                           int* __restrict__ slot_of, int* __restrict__ cell_start,
+                                                           double4* __restrict__ cpos, int2* __restrict__ cinfo,
+                                                           int* __restrict__ flags) {
+    __shared__ int cnt[NB_MAXC];
+    __shared__ int wsum[NB_BIN_THREADS / 64];
+    const StrucInfo& S = sinfo[blockIdx.x];
+    const int t = threadIdx.x;
+    const int ncl = S.nc[0] * S.nc[1] * S.nc[2];
+    const int a0 = S.atom_off, na = S.natoms;
+    const int cs0 = S.cell_off + blockIdx.x;          // this structure's cell_start block (ncl + 1 entries)
+    if (blockIdx.x == 0 && t < 16) flags[t] = 0;      // build flags and counters (maximum, totals, errors)
+    for (int c = t; c < ncl; c += NB_BIN_THREADS) cnt[c] = 0;
+    __syncthreads();
+    for (int k = t; k < na; k += NB_BIN_THREADS) {
+        const int i = a0 + k;
+        const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        int c[3], w[3];
+        for (int d = 0; d < 3; ++d) {
+            const double f = S.Rinv[d] * x + S.Rinv[d + 3] * y + S.Rinv[d + 6] * z;
+            const double fl = floor(f);
+            int cc = (int)((f - fl) * S.nc[d]);
+            cc = min(max(cc, 0), S.nc[d] - 1);
+            c[d] = cc;
+            w[d] = (int)fl;
+        }
+        const int cell = (c[2] * S.nc[1] + c[1]) * S.nc[0] + c[0];
+        cell_of[i] = cell;
+        wrap_of[i] = img_pack(w[0], w[1], w[2]);
+        slot_of[i] = atomicAdd(&cnt[cell], 1);
     }
-    const int cell = S.cell_off + (c[2] * S.nc[1] + c[1]) * S.nc[0] + c[0];
-    cell_of[i] = cell;
-    wrap_of[i] = img_pack(w[0], w[1], w[2]);
-    atomicAdd(&cell_count[cell], 1);
-}
-
-__global__ void k_nb_scatter(int natoms, const int* __restrict__ cell_of, const int* __restrict__ cell_start,
-                             int* __restrict__ cell_cursor, int* __restrict__ cell_atoms) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= natoms) return;
-    const int c = cell_of[i];
-    const int slot = atomicAdd(&cell_cursor[c], 1);
-    cell_atoms[cell_start[c] + slot] = i;
+    __syncthreads();
+    // exclusive scan of cnt: each thread sums a run of ceil(ncl / threads) cells, then a wave/block scan
+    const int per = (ncl + NB_BIN_THREADS - 1) / NB_BIN_THREADS;
+    const int c0 = t * per, c1 = min(c0 + per, ncl);
+    int run = 0;
+    for (int c = c0; c < c1; ++c) run += cnt[c];
+    int incl = run;
+    const int lane = t & 63, wv = t >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int wbase = 0;
+    for (int q = 0; q < wv; ++q) wbase += wsum[q];
+    int acc = a0 + wbase + incl - run;
+    for (int c = c0; c < c1; ++c) {
+        const int v = cnt[c];
+        cnt[c] = acc;
+        cell_start[cs0 + c] = acc;
+        acc += v;
+    }
+    if (t == 0) cell_start[cs0 + ncl] = a0 + na;
+    __syncthreads();
+    for (int k = t; k < na; k += NB_BIN_THREADS) {
+        const int i = a0 + k;
+        const int dst = cnt[cell_of[i]] + slot_of[i];
+        cpos[dst] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.0);
+        cinfo[dst] = make_int2(i, wrap_of[i]);
+    }
 }
 
 // ---- exclusive scan (int), three passes --------------------------------------------------------
@@ -102,6 +148,37 @@ __global__ void k_scan_add(int n, int* __restrict__ out, const int* __restrict__
     if (i < n) out[i] += block_sums[blockIdx.x];
 }
 
+// one workgroup: exclusive scan of n <= SCAN_BLOCK * SCAN_RUN ints; writes out[0..n] and *total
+#define SCAN_RUN 64
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(int n, const int* __restrict__ in, int* __restrict__ out,
+                                                            int* __restrict__ total) {
+    __shared__ int wsum[SCAN_BLOCK / 64];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int per = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    const int c0 = min(t * per, n), c1 = min(c0 + per, n);
+    int run = 0;
+    for (int c = c0; c < c1; ++c) run += in[c];
+    int incl = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int acc = incl - run;
+    for (int q = 0; q < wv; ++q) acc += wsum[q];
+    for (int c = c0; c < c1; ++c) {
+        const int v = in[c];
+        out[c] = acc;
+        acc += v;
+    }
+    if (t == SCAN_BLOCK - 1) {
+        out[n] = acc;
+        *total = acc;
+    }
+}
+
 // ---- search ---------------------------------------------------------------------------------------
 
 __device__ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
@@ -119,25 +196,30 @@ __device__ inline double ref_ds2(const double* R, double dx, double dy, double d
 
 #define NB_WAVES 4
 #define NB_LMAX 1024   // neighbours per atom held while sorting
+#define NB_U 4         // candidate chunks loaded together by one wave
 
-// Wave per atom.  mode 0: count (writes counts[i], max into *maxcount); mode 1: fill, sorted.
+// Workgroup (4 waves) per atom; the waves take interleaved groups of NB_U x 64 candidates.
+// MODE 0: count only (counts[i], running maximum in *maxcount).  MODE 1: also sort the keys
+// (j_local * nimg + image code: the reference order) and write them to row i of keys (cap slots); an
+// atom with more than cap neighbours raises *over and writes none.
 template <int MODE>
 __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
     int natoms, double rc2, const double* __restrict__ pos, const int* __restrict__ struc_of,
     const StrucInfo* __restrict__ sinfo, const int* __restrict__ cell_of, const int* __restrict__ wrap_of,
-    const int* __restrict__ cell_start, const int* __restrict__ cell_atoms, int* __restrict__ counts,
-    int* __restrict__ maxcount, const int* __restrict__ nb_off, int* __restrict__ nb_j, int* __restrict__ nb_img,
-    float4* __restrict__ nb_d, int* __restrict__ nb_src) {
-    __shared__ int s_cstart[NB_WAVES][64];
-    __shared__ int s_cpref[NB_WAVES][65];
-    __shared__ int s_cimg[NB_WAVES][64];
-    __shared__ long long s_key[MODE ? NB_WAVES : 1][MODE ? NB_LMAX : 1];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = blockIdx.x * NB_WAVES + wv;
-    if (i >= natoms) return;
+    const int* __restrict__ cell_start, const double4* __restrict__ cpos, const int2* __restrict__ cinfo,
+    int* __restrict__ counts, int* __restrict__ maxcount, long long* __restrict__ keys, int cap,
+    int* __restrict__ over) {
+    __shared__ int s_cstart[64];
+    __shared__ int s_cpref[65];
+    __shared__ int s_cimg[64];
+    __shared__ int s_n;
+    __shared__ long long s_key[MODE ? NB_LMAX : 1];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int i = blockIdx.x;
     const int s = struc_of[i];
     const StrucInfo& S = sinfo[s];
-    const int loc = cell_of[i] - S.cell_off;
+    const int cs0 = S.cell_off + s;
+    const int loc = cell_of[i];
     const int ci[3] = {loc % S.nc[0], (loc / S.nc[0]) % S.nc[1], loc / (S.nc[0] * S.nc[1])};
     int wi0, wi1, wi2;
     img_unpack(wrap_of[i], wi0, wi1, wi2);
@@ -146,91 +228,143 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
     const int nst = sx * sy * sz;
     const int ny = 2 * S.cf[1] + 1, nz = 2 * S.cf[2] + 1;
     const long long nimg = (long long)(2 * S.cf[0] + 1) * ny * nz;
-    int count = 0;
+    if (tid == 0) s_n = 0;
     for (int cb = 0; cb < nst; cb += 64) {
-        // stencil cells of this chunk
-        const int q = cb + lane;
-        int cnt = 0, start = 0, shift = 0;
-        if (q < nst) {
-            const int d0 = q % sx - S.st[0], d1 = (q / sx) % sy - S.st[1], d2 = q / (sx * sy) - S.st[2];
-            const int a0 = ci[0] + d0, a1 = ci[1] + d1, a2 = ci[2] + d2;
-            const int n0 = floordiv(a0, S.nc[0]), n1 = floordiv(a1, S.nc[1]), n2 = floordiv(a2, S.nc[2]);
-            const int c0 = a0 - n0 * S.nc[0], c1 = a1 - n1 * S.nc[1], c2 = a2 - n2 * S.nc[2];
-            const int cell = S.cell_off + (c2 * S.nc[1] + c1) * S.nc[0] + c0;
-            start = cell_start[cell];
-            cnt = cell_start[cell + 1] - start;
-            shift = img_pack(n0, n1, n2);
-        }
-        // inclusive prefix over lanes
-        int pre = cnt;
+        // stencil cells of this chunk: start, size and image shift, with an inclusive prefix (wave 0)
+        __syncthreads();
+        if (wv == 0) {
+            const int q = cb + lane;
+            int cnt = 0, start = 0, shift = 0;
+            if (q < nst) {
+                const int d0 = q % sx - S.st[0], d1 = (q / sx) % sy - S.st[1], d2 = q / (sx * sy) - S.st[2];
+                const int a0 = ci[0] + d0, a1 = ci[1] + d1, a2 = ci[2] + d2;
+                const int n0 = floordiv(a0, S.nc[0]), n1 = floordiv(a1, S.nc[1]), n2 = floordiv(a2, S.nc[2]);
+                const int c0 = a0 - n0 * S.nc[0], c1 = a1 - n1 * S.nc[1], c2 = a2 - n2 * S.nc[2];
+                const int cell = cs0 + (c2 * S.nc[1] + c1) * S.nc[0] + c0;
+                start = cell_start[cell];
+                cnt = cell_start[cell + 1] - start;
+                shift = img_pack(n0, n1, n2);
+            }
+            int pre = cnt;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(pre, o, 64);
-            if (lane >= o) pre += t;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(pre, o, 64);
+                if (lane >= o) pre += t;
+            }
+            s_cstart[lane] = start;
+            s_cimg[lane] = shift;
+            s_cpref[lane + 1] = pre;
+            if (lane == 0) s_cpref[0] = 0;
         }
-        s_cstart[wv][lane] = start;
-        s_cimg[wv][lane] = shift;
-        s_cpref[wv][lane + 1] = pre;
-        if (lane == 0) s_cpref[wv][0] = 0;
-        const int total = __shfl(pre, 63, 64);
-        wave_sync();
-        for (int tb = 0; tb < total; tb += 64) {
-            const int x = tb + lane;
-            bool acc = false;
-            long long key = 0;
-            if (x < total) {
-                // cell slot: largest k with pref[k] <= x
-                int lo = 0, hi = 63;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_cpref[wv][mid] <= x) lo = mid; else hi = mid - 1;
+        __syncthreads();
+        const int total = s_cpref[64];
+        for (int tb = wv * 64 * NB_U; tb < total; tb += NB_WAVES * 64 * NB_U) {
+            double4 pj[NB_U];
+            int2 inf[NB_U];
+            int sl[NB_U];
+#pragma unroll
+            for (int u = 0; u < NB_U; ++u) {
+                const int x = tb + 64 * u + lane;
+                // stencil slot of candidate x: largest c with s_cpref[c] <= x
+                int c = 0;
+#pragma unroll
+                for (int st = 32; st > 0; st >>= 1)
+                    if (s_cpref[c + st] <= x) c += st;
+                sl[u] = c;
+                if (x < total) {
+                    const int e = s_cstart[c] + (x - s_cpref[c]);
+                    pj[u] = cpos[e];
+                    inf[u] = cinfo[e];
                 }
-                const int j = cell_atoms[s_cstart[wv][lo] + (x - s_cpref[wv][lo])];
-                int sh0, sh1, sh2, wj0, wj1, wj2;
-                img_unpack(s_cimg[wv][lo], sh0, sh1, sh2);
-                img_unpack(wrap_of[j], wj0, wj1, wj2);
-                const int n0 = sh0 + wi0 - wj0, n1 = sh1 + wi1 - wj1, n2 = sh2 + wi2 - wj2;
-                if (n0 >= -S.cf[0] && n0 <= S.cf[0] && n1 >= -S.cf[1] && n1 <= S.cf[1] &&
-                    n2 >= -S.cf[2] && n2 <= S.cf[2]) {
-                    double dx, dy, dz;
-                    {
+            }
+#pragma unroll
+            for (int u = 0; u < NB_U; ++u) {
+                const int x = tb + 64 * u + lane;
+                bool acc = false;
+                long long key = 0;
+                if (x < total) {
+                    const int j = inf[u].x;
+                    int sh0, sh1, sh2, wj0, wj1, wj2;
+                    img_unpack(s_cimg[sl[u]], sh0, sh1, sh2);
+                    img_unpack(inf[u].y, wj0, wj1, wj2);
+                    const int n0 = sh0 + wi0 - wj0, n1 = sh1 + wi1 - wj1, n2 = sh2 + wi2 - wj2;
+                    if (n0 >= -S.cf[0] && n0 <= S.cf[0] && n1 >= -S.cf[1] && n1 <= S.cf[1] &&
+                        n2 >= -S.cf[2] && n2 <= S.cf[2]) {
+                        double dx, dy, dz;
+                        {
 #pragma clang fp contract(off)
-                        dx = pix - pos[3 * j];
-                        dy = piy - pos[3 * j + 1];
-                        dz = piz - pos[3 * j + 2];
-                    }
-                    const double ds2 = ref_ds2(S.R, dx, dy, dz, n0, n1, n2);
-                    if ((j != i || ds2 > 1e-12) && ds2 < rc2) {
-                        acc = true;
-                        const long long code = ((long long)(n0 + S.cf[0]) * ny + (n1 + S.cf[1])) * nz + (n2 + S.cf[2]);
-                        key = (long long)(j - S.atom_off) * nimg + code;
+                            dx = pix - pj[u].x;
+                            dy = piy - pj[u].y;
+                            dz = piz - pj[u].z;
+                        }
+                        const double ds2 = ref_ds2(S.R, dx, dy, dz, n0, n1, n2);
+                        if ((j != i || ds2 > 1e-12) && ds2 < rc2) {
+                            acc = true;
+                            const long long code =
+                                ((long long)(n0 + S.cf[0]) * ny + (n1 + S.cf[1])) * nz + (n2 + S.cf[2]);
+                            key = (long long)(j - S.atom_off) * nimg + code;
+                        }
                     }
                 }
+                const unsigned long long m = __ballot(acc);
+                int base = 0;
+                if (lane == 0 && m) base = atomicAdd(&s_n, __popcll(m));
+                base = __shfl(base, 0, 64);
+                if (MODE == 1 && acc) {
+                    const int r = base + __popcll(m & lane_mask_lt());
+                    if (r < NB_LMAX) s_key[r] = key;
+                }
             }
-            const unsigned long long m = __ballot(acc);
-            if (MODE == 1 && acc) {
-                const int r = count + __popcll(m & lane_mask_lt());
-                if (r < NB_LMAX) s_key[MODE ? wv : 0][r] = key;
-            }
-            count += __popcll(m);
         }
-        wave_sync();
     }
-    if (MODE == 0) {
-        if (lane == 0) {
-            counts[i] = count;
-            atomicMax(maxcount, count);
-        }
+    __syncthreads();
+    const int count = s_n;
+    if (tid == 0) {
+        counts[i] = count;
+        atomicMax(maxcount, count);
+    }
+    if (MODE == 0) return;
+    if (count > cap || count > NB_LMAX) {
+        if (tid == 0) atomicMax(over, 1);
         return;
     }
-    // fill: rank sort of the keys, write in reference order (an emptied list means overflow)
-    const int base = nb_off[i];
-    const int n = min(count < NB_LMAX ? count : NB_LMAX, nb_off[i + 1] - base);
-    wave_sync();
-    for (int e = lane; e < n; e += 64) {
-        const long long k = s_key[MODE ? wv : 0][e];
+    // rank sort into the reference order (the keys of one atom are distinct)
+    long long* row = keys + (long long)i * cap;
+    for (int e = tid; e < count; e += NB_WAVES * 64) {
+        const long long k = s_key[e];
         int rank = 0;
-        for (int f = 0; f < n; ++f) rank += (s_key[MODE ? wv : 0][f] < k);
+        for (int f = 0; f < count; ++f) rank += (s_key[f] < k);
+        row[rank] = k;
+    }
+}
+
+// Wave per atom: CSR arrays from the sorted key rows.  nb_j, nb_img, nb_src; the FP64 displacement
+// r_ij = posn(i) - (posn(j) + R n) (nnp.cpp:363) stored as FP32 with its length; nb_rev = index of the
+// reverse pair (j -> i, -n), found by binary search in row j.  When the pair total exceeds pcap or a row
+// overflowed, every list is emptied (offsets zeroed) and *bad is raised; total is the scan's own copy.
+__global__ __launch_bounds__(256) void k_nb_compact(
+    int natoms, const double* __restrict__ pos, const int* __restrict__ struc_of, const StrucInfo* __restrict__ sinfo,
+    const long long* __restrict__ keys, int cap, const int* __restrict__ counts, int* __restrict__ nb_off,
+    const int* __restrict__ total, const int* __restrict__ over, long long pcap, int* __restrict__ nb_j,
+    int* __restrict__ nb_img, int* __restrict__ nb_src, float4* __restrict__ nb_d, int* __restrict__ nb_rev,
+    int* __restrict__ bad, int* __restrict__ rev_err) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + wv;
+    if (i > natoms) return;
+    if (*total > pcap || *over) {
+        if (lane == 0) nb_off[i] = 0;
+        if (i == 0 && lane == 0) *bad = 1;
+        return;
+    }
+    if (i == natoms) return;
+    const StrucInfo& S = sinfo[struc_of[i]];
+    const int ny = 2 * S.cf[1] + 1, nz = 2 * S.cf[2] + 1;
+    const long long nimg = (long long)(2 * S.cf[0] + 1) * ny * nz;
+    const int base = nb_off[i], n = counts[i];
+    const long long* row = keys + (long long)i * cap;
+    const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
+    for (int e = lane; e < n; e += 64) {
+        const long long k = row[e];
         const long long jl = k / nimg;
         long long code = k - jl * nimg;
         const int c2 = (int)(code % nz) - S.cf[2];
@@ -238,61 +372,28 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
         const int c1 = (int)(code % ny) - S.cf[1];
         const int c0 = (int)(code / ny) - S.cf[0];
         const int j = (int)jl + S.atom_off;
-        const int p = base + rank;
+        const int p = base + e;
         nb_j[p] = j;
         nb_src[p] = i;
         nb_img[p] = img_pack(c0, c1, c2);
-        // r_IJ = posn(i) - (posn(j) + R*img)  (nnp.cpp:363), FP64, stored as FP32
-        double rv[3];
-        for (int d = 0; d < 3; ++d) {
-            const double sh = S.R[d] * c0 + S.R[d + 3] * c1 + S.R[d + 6] * c2;
-            rv[d] = pos[3 * i + d] - (pos[3 * j + d] + sh);
+        const double sx = S.R[0] * c0 + S.R[3] * c1 + S.R[6] * c2;
+        const double sy = S.R[1] * c0 + S.R[4] * c1 + S.R[7] * c2;
+        const double sz = S.R[2] * c0 + S.R[5] * c1 + S.R[8] * c2;
+        const double rx = pix - (pos[3 * j] + sx), ry = piy - (pos[3 * j + 1] + sy), rz = piz - (pos[3 * j + 2] + sz);
+        const double r = sqrt(rx * rx + ry * ry + rz * rz);
+        nb_d[p] = make_float4((float)rx, (float)ry, (float)rz, (float)r);
+        // reverse pair: key (i_local, -n) in row j
+        const long long want = (long long)(i - S.atom_off) * nimg +
+                               ((long long)(-c0 + S.cf[0]) * ny + (-c1 + S.cf[1])) * nz + (-c2 + S.cf[2]);
+        const long long* rj = keys + (long long)j * cap;
+        int lo = 0, hi = counts[j] - 1, found = -1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            const long long km = rj[mid];
+            if (km == want) { found = mid; break; }
+            if (km < want) lo = mid + 1; else hi = mid - 1;
         }
-        const double r = sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
-        nb_d[p] = make_float4((float)rv[0], (float)rv[1], (float)rv[2], (float)r);
+        if (found < 0) atomicAdd(rev_err, 1);
+        nb_rev[p] = found < 0 ? p : nb_off[j] + found;
     }
 }
-
-// thread per pair: index of the reverse pair (j -> i, -n) in j's list (binary search on the key)
-__global__ void k_nb_reverse(const int* __restrict__ total, const int* __restrict__ nb_src, const int* __restrict__ nb_j,
-                             const int* __restrict__ nb_img, const int* __restrict__ nb_off,
-                             const int* __restrict__ struc_of, const StrucInfo* __restrict__ sinfo,
-                             int* __restrict__ nb_rev, int* __restrict__ err) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= *total) return;
-    const int i = nb_src[p], j = nb_j[p];
-    const StrucInfo& S = sinfo[struc_of[i]];
-    const int ny = 2 * S.cf[1] + 1, nz = 2 * S.cf[2] + 1;
-    const long long nimg = (long long)(2 * S.cf[0] + 1) * ny * nz;
-    int a, b, c;
-    img_unpack(nb_img[p], a, b, c);
-    const long long want = (long long)(i - S.atom_off) * nimg +
-                           ((long long)(-a + S.cf[0]) * ny + (-b + S.cf[1])) * nz + (-c + S.cf[2]);
-    int lo = nb_off[j], hi = nb_off[j + 1] - 1;
-    int found = -1;
-    while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        int x, y, z;
-        img_unpack(nb_img[mid], x, y, z);
-        const long long k = (long long)(nb_j[mid] - S.atom_off) * nimg +
-                            ((long long)(x + S.cf[0]) * ny + (y + S.cf[1])) * nz + (z + S.cf[2]);
-        if (k == want) { found = mid; break; }
-        if (k < want) lo = mid + 1; else hi = mid - 1;
-    }
-    nb_rev[p] = found < 0 ? p : found;
-    if (found < 0) atomicAdd(err, 1);
-}
-
-// thread per offset entry: if the new list exceeds the pair capacity or an atom exceeds the LDS
-// neighbour slots, every list is emptied and the overflow flag is raised.  The total is read from the
-// scan's own copy, which this kernel does not modify.
-__global__ void k_nb_check(int natoms, int* __restrict__ nb_off, const int* __restrict__ maxcount,
-                           const int* __restrict__ total, int pcap, int slots, int* __restrict__ bad) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > natoms) return;
-    if (*total > pcap || *maxcount > slots) {
-        nb_off[i] = 0;
-        if (i == 0) *bad = 1;
-    }
-}
-

@@ -6,6 +6,7 @@
 // This is the scatter F_i +=, F_j -= of NNP::compute_force (src/nnp/nnp.cpp:442-471) done as a
 // deterministic gather in FP64, without atomics.
 #include "ace_common.hpp"
+#include "basis_math.hpp"
 
 // GL lanes per atom: force (FP64) and the atom's virial share -sum_p g_p (x) d_p (FP64, 9 values);
 // each lane sums a strided subset of the atom's pairs, the GL partials are combined by shuffles in a
@@ -35,14 +36,12 @@ __global__ __launch_bounds__(256) void k_force_gather(int natoms, const int* __r
                 for (int b = 0; b < 3; ++b) w[3 * a + b] -= ga[a] * da[b];
         }
     }
+    // sums over the GL = 8 lanes of the atom (DPP quad and half-row moves)
 #pragma unroll
-    for (int m = GL / 2; m >= 1; m >>= 1) {
+    for (int q = 0; q < 3; ++q) f[q] = t_group_sum<double>(f[q], GL);
+    if (vir_atom) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) f[q] += __shfl_xor(f[q], m, GL);
-        if (vir_atom) {
-#pragma unroll
-            for (int q = 0; q < 9; ++q) w[q] += __shfl_xor(w[q], m, GL);
-        }
+        for (int q = 0; q < 9; ++q) w[q] = t_group_sum<double>(w[q], GL);
     }
     if (on && sub == 0) {
         F[3 * i] = f[0];

@@ -115,6 +115,16 @@ def train_bench(steps):
             "workload": "64 structures, N log-uniform [32,1024], 3 elements, 56-64-64-1, energy + 0.1 force MSE"}
 
 
+def copy_calibration(mib):
+    import torch
+    a = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    log(f"copy calibration: 5 x neg over {mib} MiB (read {mib} MiB, write {mib} MiB per launch)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,7 +133,13 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-steps", type=int, default=20)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 elementwise copies of MIB MiB (reads MIB, writes MIB): the reference point "
+                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
               "'How the path would shard')", file=sys.stderr)
@@ -152,8 +168,8 @@ def main():
         kern = {}
         if events:
             sfx = "_f64" if prec == PREC_FP64_BASIS else ""
-            for k in ("nb_assign", "nb_count", "nb_fill", "nb_reverse", "basis_symm" + sfx, "mlp_energy_grad",
-                      "basis_adjoint" + sfx, "force_gather", "struct_reduce"):
+            for k in ("nb_bin", "nb_search", "scan", "nb_compact", "basis_symm" + sfx,
+                      "mlp_energy_grad", "basis_adjoint" + sfx, "force_gather", "struct_reduce"):
                 kern[k.replace(sfx, "") if sfx else k] = dev.profile_read(k)
             dev.profile(False)
         return (t1 - t0) / steps * 1e3, kern

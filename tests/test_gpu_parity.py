@@ -168,6 +168,24 @@ def test_every_radial_and_cutoff_form_and_generic_angular():
     check_against_oracle(m, strucs, "forms", check_plain=False)
 
 
+@pytest.mark.parametrize("hidden,act", [((128, 128), 3), ((40, 24, 17), 2), ((8,), 10), ((33, 31), 15)])
+def test_network_shapes_and_activations(hidden, act):
+    """Widths that are not multiples of the 16x16 matrix tiles, 1 to 3 hidden layers, each activation,
+    and a network too large to stage its weights in LDS (128-128)."""
+    rng = np.random.default_rng(23)
+    m = configs.c3_model(rng)
+    for I in range(3):
+        K = m.input_width(I)
+        w = [K] + list(hidden) + [1]
+        m.nets[I] = configs.Network(w, act, 1.0, configs.xavier_params(w, rng), np.ones(K), np.zeros(K))
+    strucs = []
+    for n in (17, 40):
+        cell, pos = configs.random_packed(n, 0.08, 1.6, rng)
+        strucs.append(structure(cell, pos, rng.integers(0, 3, n)))
+    scaled_model(m, strucs)
+    check_against_oracle(m, strucs, f"net{hidden}", check_plain=False)
+
+
 @pytest.mark.parametrize("prec,tol", [(PREC_FP32, 1e-4), (PREC_FP64_BASIS, 2e-5)])
 def test_c2_forces_central_fd(prec, tol):
     """Adjoint forces vs central finite differences (h = 1e-4 A) of the FP64 energy."""

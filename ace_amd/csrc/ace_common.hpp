@@ -38,7 +38,8 @@ struct __attribute__((aligned(16))) DevSpecies {
     int boff[ACE_MAXL], woff[ACE_MAXL];   // offsets in the species' parameter block
     int poff;                      // offset of the species' parameter block in the global array
     // matrix-core layout of the same parameters (16-byte aligned block at wp_off, wp_n floats, a multiple
-    // of 4): W_l as [k][o] with row stride w_{l+1} + 1 at wl_off[l], biases b_l at bl_off[l]
+    // of 4): W_l as [k][o] at wl_off[l], round16(w_l) rows of stride round16(w_{l+1}) + 1, zero padded;
+    // biases b_l at bl_off[l]
     int wp_off, wp_n;
     int wl_off[ACE_MAXL], bl_off[ACE_MAXL];
     int in_off;                    // offset of inpw/inpb

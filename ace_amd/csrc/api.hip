@@ -53,7 +53,7 @@ struct ace_ctx {
     int nt = 0;
     double rc = 0;
     std::vector<HSpecies> sp;
-    int kmax = 0, ldg = 0, maxA = 0, maxR = 0, ngmax = 1, ngk = 1;
+    int kmax = 0, ldg = 0, maxA = 0, maxR = 0, ngmax = 1, ngk = 1, nzmax = 1;
     bool all_canon = true;
     DevModel hdm;
     DBuf<DevModel> d_model;
@@ -78,7 +78,11 @@ struct ace_ctx {
     DBuf<int4> d_tiles;
     int tm = 32;
     DBuf<float> d_G, d_dEdG;
-    DBuf<double> d_e_atom, d_vir_atom, d_F, d_E, d_vir;
+    DBuf<double> d_e_atom, d_F, d_E, d_vir;
+    DBuf<int> d_ticket;          // per-structure completion tickets of the force gather
+    DBuf<int2> d_gtab, d_sblk;   // gather workgroups (structure, first atom); per structure (first, count)
+    DBuf<double> d_part;         // per-workgroup (energy, virial) partial sums
+    int ngblk = 0;
     // neighbours
     bool nb_valid = false;
     bool disp_dirty = false;
@@ -283,7 +287,7 @@ int upload_wpad(ace_ctx* ctx) {
             const int ni = s.width[l], no = s.width[l + 1];
             for (int k = 0; k < ni; ++k)
                 for (int o = 0; o < no; ++o)
-                    w[d.wp_off + d.wl_off[l] + k * (no + 1) + o] = (float)s.params[d.woff[l] + o + k * no];
+                    w[d.wp_off + d.wl_off[l] + k * mlp_ld(no) + o] = (float)s.params[d.woff[l] + o + k * no];
             for (int o = 0; o < no; ++o) w[d.wp_off + d.bl_off[l] + o] = (float)s.params[d.boff[l] + o];
         }
     }
@@ -395,8 +399,12 @@ int build_tables(ace_ctx* ctx) {
         d.poff = (int)prm.size();
         {
             int off = 0;
-            for (int l = 0; l < s.L; ++l) { d.wl_off[l] = off; off += s.width[l] * (s.width[l + 1] + 1); }
-            for (int l = 0; l < s.L; ++l) { d.bl_off[l] = off; off += s.width[l + 1]; }
+            // W_l as [k][o]: rows padded to a multiple of 16, row stride = mlp_ld(w_{l+1}); zero padding
+            for (int l = 0; l < s.L; ++l) {
+                d.wl_off[l] = off;
+                off += ((s.width[l] + 15) & ~15) * mlp_ld(s.width[l + 1]);
+            }
+            for (int l = 0; l < s.L; ++l) { d.bl_off[l] = off; off += (s.width[l + 1] + 15) & ~15; }
             d.wp_n = (off + 3) & ~3;
             d.wp_off = wp_total;
             wp_total += d.wp_n;
@@ -416,6 +424,7 @@ int build_tables(ace_ctx* ctx) {
     const int ngk = ngmax <= 1 ? 1 : (ngmax <= 2 ? 2 : 4);
     std::vector<int> a_cidx(a_sel.size(), 0);
     bool all_canon = true;
+    int nzmax_all = 1;
     for (int I = 0; I < nt; ++I) {
         DevSpecies& d = dm.sp[I];
         int nz = 1;
@@ -436,7 +445,9 @@ int build_tables(ace_ctx* ctx) {
                 for (int q = 0; q < d.a_n[b]; ++q) nz = std::max(nz, a_sel[d.a_poff[b] + q] % 7 + 1);
         }
         d.nz = nz;
+        nzmax_all = std::max(nzmax_all, nz);
     }
+    ctx->nzmax = nzmax_all;
     ctx->ngk = ngk;
     ctx->all_canon = all_canon;
     dm.kmax = kmax;
@@ -566,31 +577,39 @@ typedef void (*BasisFn)(int, const DevModel*, DevBasisParams, const int*, const 
                         const int*, const double*, const int*, const StrucInfo*, const int*, int, float*,
                         const float*, float4*, int*, int, const double*, int, int, int);
 
-template <class T, int NG>
+template <class T, int NG, int NZ>
 void pick_basis(BasisFn* f, bool gen) {
     if (gen) {
-        f[0] = k_basis<T, NG, 0, 1>;
-        f[1] = k_basis<T, NG, 1, 1>;
-        f[2] = k_basis<T, NG, 2, 1>;
+        f[0] = k_basis<T, NG, NZ, 0, 1>;
+        f[1] = k_basis<T, NG, NZ, 1, 1>;
+        f[2] = k_basis<T, NG, NZ, 2, 1>;
     } else {
-        f[0] = k_basis<T, NG, 0, 0>;
-        f[1] = k_basis<T, NG, 1, 0>;
-        f[2] = k_basis<T, NG, 2, 0>;
+        f[0] = k_basis<T, NG, NZ, 0, 0>;
+        f[1] = k_basis<T, NG, NZ, 1, 0>;
+        f[2] = k_basis<T, NG, NZ, 2, 0>;
     }
 }
 
-// the canonical angular accumulators are sized by the largest number of eta groups (1, 2 or 4); the
-// generic per-function path is compiled in only when some angular block is not canonical
+template <class T, int NG>
+void pick_basis_nz(BasisFn* f, bool gen, int nz) {
+    if (nz <= 4) pick_basis<T, NG, 4>(f, gen);
+    else pick_basis<T, NG, 7>(f, gen);
+}
+
+// the canonical angular accumulators are sized by the largest number of eta groups (1, 2 or 4) and of
+// power levels (4 or 7); the generic per-function path is compiled in only when some angular block is
+// not canonical
 void basis_fns(ace_ctx* ctx, bool f64, BasisFn* f) {
     const bool gen = !ctx->all_canon;
+    const int nz = ctx->nzmax;
     if (f64) {
-        if (ctx->ngk == 1) pick_basis<double, 1>(f, gen);
-        else if (ctx->ngk == 2) pick_basis<double, 2>(f, gen);
-        else pick_basis<double, 4>(f, gen);
+        if (ctx->ngk == 1) pick_basis_nz<double, 1>(f, gen, nz);
+        else if (ctx->ngk == 2) pick_basis_nz<double, 2>(f, gen, nz);
+        else pick_basis_nz<double, 4>(f, gen, nz);
     } else {
-        if (ctx->ngk == 1) pick_basis<float, 1>(f, gen);
-        else if (ctx->ngk == 2) pick_basis<float, 2>(f, gen);
-        else pick_basis<float, 4>(f, gen);
+        if (ctx->ngk == 1) pick_basis_nz<float, 1>(f, gen, nz);
+        else if (ctx->ngk == 2) pick_basis_nz<float, 2>(f, gen, nz);
+        else pick_basis_nz<float, 4>(f, gen, nz);
     }
 }
 
@@ -624,8 +643,8 @@ int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = 
     const bool f64 = ctx->basis_f64;
     basis_fns(ctx, f64, fns);
     const int nbm = nb_slots(ctx);
-    const int bytes = f64 ? basis_lds_bytes<double>(nbm, ctx->ngmax, ctx->kmax, mode)
-                          : basis_lds_bytes<float>(nbm, ctx->ngmax, ctx->kmax, mode);
+    const int bytes = f64 ? basis_lds_bytes<double>(nbm, ctx->ngk, ctx->kmax, mode)
+                          : basis_lds_bytes<float>(nbm, ctx->ngk, ctx->kmax, mode);
     if (bytes > 160 * 1024) return fail(ctx, "basis: neighbour list too long for LDS");
     BasisFn f = fns[mode];
     if (bytes > 64 * 1024) HC(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
@@ -657,6 +676,16 @@ int run_mlp(ace_ctx* ctx) {
         f<<<(int)ctx->tiles_h.size(), 64, lds, ctx->stream>>>(ctx->d_model.p, ctx->d_wpad.p, ctx->d_inpw.p,
                                                               ctx->d_tiles.p, ctx->ldg, ctx->d_G.p, ctx->d_dEdG.p,
                                                               ctx->d_atom_of_row.p, ctx->d_e_atom.p);
+    });
+}
+
+// forces, per-structure energies and virials from the per-pair gradients
+int run_gather(ace_ctx* ctx) {
+    return timed(ctx, "force_gather", [&] {
+        k_force_gather<<<ctx->ngblk, RED_THREADS, 0, ctx->stream>>>(
+            ctx->d_gtab.p, ctx->d_sblk.p, ctx->d_atom_off.p, ctx->d_natoms.p, ctx->d_nb_off.p, ctx->d_nb_rev.p,
+            ctx->d_nb_d.p, ctx->d_pair_g.p, ctx->d_e_atom.p, ctx->d_F.p, ctx->d_part.p, ctx->d_ticket.p, ctx->d_E.p,
+            ctx->d_vir.p);
     });
 }
 
@@ -726,7 +755,7 @@ void ace_destroy(ace_ctx* ctx) {
     release(ctx->d_type); release(ctx->d_struc_of); release(ctx->d_atom_off); release(ctx->d_natoms);
     release(ctx->d_row_of); release(ctx->d_atom_of_row); release(ctx->d_pos); release(ctx->d_cell);
     release(ctx->d_tiles); release(ctx->d_G); release(ctx->d_dEdG); release(ctx->d_e_atom);
-    release(ctx->d_vir_atom); release(ctx->d_F); release(ctx->d_E); release(ctx->d_vir);
+    release(ctx->d_F); release(ctx->d_E); release(ctx->d_vir); release(ctx->d_ticket); release(ctx->d_gtab); release(ctx->d_sblk); release(ctx->d_part);
     release(ctx->d_sinfo); release(ctx->d_cell_of); release(ctx->d_wrap_of); release(ctx->d_slot_of);
     release(ctx->d_cell_start); release(ctx->d_cpos); release(ctx->d_cinfo); release(ctx->d_nb_keys);
     release(ctx->d_nb_count);
@@ -831,10 +860,24 @@ int ace_upload(ace_ctx* ctx, int n_struc, const int* natoms, const double* pos, 
     HC(hipMemsetAsync(ctx->d_G.p, 0, (size_t)N * ctx->ldg * sizeof(float), ctx->stream));
     HC(hipMemsetAsync(ctx->d_dEdG.p, 0, (size_t)N * ctx->ldg * sizeof(float), ctx->stream));
     if (ensure(ctx, ctx->d_e_atom, N)) return -1;
-    if (ensure(ctx, ctx->d_vir_atom, 9 * N)) return -1;
     if (ensure(ctx, ctx->d_F, 3 * N)) return -1;
     if (ensure(ctx, ctx->d_E, n_struc)) return -1;
     if (ensure(ctx, ctx->d_vir, 9 * n_struc)) return -1;
+    if (ensure(ctx, ctx->d_ticket, n_struc)) return -1;
+    {
+        // gather workgroups: GATHER_ATOMS atoms of one structure each
+        std::vector<int2> gtab, sblk(n_struc);
+        for (int s = 0; s < n_struc; ++s) {
+            sblk[s] = make_int2((int)gtab.size(), 0);
+            for (int a = 0; a < natoms[s]; a += GATHER_ATOMS) gtab.push_back(make_int2(s, ctx->atom_off[s] + a));
+            sblk[s].y = (int)gtab.size() - sblk[s].x;
+        }
+        ctx->ngblk = (int)gtab.size();
+        if (upload(ctx, ctx->d_gtab, gtab.data(), gtab.size()) || upload(ctx, ctx->d_sblk, sblk.data(), sblk.size()) ||
+            ensure(ctx, ctx->d_part, 10 * (size_t)ctx->ngblk))
+            return -1;
+    }
+    HC(hipMemsetAsync(ctx->d_ticket.p, 0, n_struc * sizeof(int), ctx->stream));   // reset by their last users
     HC(hipStreamSynchronize(ctx->stream));
     ctx->nb_valid = false;
     ctx->nb_same_layout = false;
@@ -1066,16 +1109,7 @@ int ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, d
     if (run_mlp(ctx)) return -1;
     if (run_basis(ctx, 1)) return -1;
     const int N = ctx->N;
-    if (timed(ctx, "force_gather", [&] {
-            k_force_gather<<<cdiv((long long)N * GL, 256), 256, 0, ctx->stream>>>(N, ctx->d_nb_off.p, ctx->d_nb_rev.p, ctx->d_nb_d.p,
-                                                                   ctx->d_pair_g.p, ctx->d_F.p, ctx->d_vir_atom.p);
-        }))
-        return -1;
-    if (timed(ctx, "struct_reduce", [&] {
-            k_struct_reduce<<<ctx->S, RED_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p, ctx->d_e_atom.p,
-                                                                     ctx->d_vir_atom.p, ctx->d_E.p, ctx->d_vir.p);
-        }))
-        return -1;
+    if (run_gather(ctx)) return -1;
     if (E_struc) HC(hipMemcpyAsync(E_struc, ctx->d_E.p, ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (F) HC(hipMemcpyAsync(F, ctx->d_F.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (virial) HC(hipMemcpyAsync(virial, ctx->d_vir.p, 9 * ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

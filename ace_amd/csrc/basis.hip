@@ -26,7 +26,7 @@
 // Modes: 0 = network input rows inpw*(G + inpb); 1 = adjoint (per-pair dE/dd from dE/dG rows);
 // 2 = tangent rows v = sum_a eps_a . dG/dr_a.
 // Dynamic LDS per workgroup (nbm = neighbour slots, kp = kmax rounded up to 4), in units of T:
-//   u[4 nbm], te[4 nbm] (mode 2), E[ng nbm], Ed[ng nbm] (modes 1,2), fR[nbm], gR[nbm], ri[nbm],
+//   u[4 nbm], te[4 nbm] (mode 2), E[nbm ng], Ed[nbm ng] (modes 1,2), fR[nbm], gR[nbm], ri[nbm],
 //   row[kp], red[BW 64], rp[2 kp] (radial eta, rs of the species), sc[2 kp] (mode 0: inpw, inpb);
 //   then p[nbm] and ci[kp] (int).  Mode 1 puts g[BW 3 nbm] (double) in front.
 template <class T>
@@ -44,8 +44,8 @@ template <class T>
 struct BasisLds {
     T* u;      // [nbm][4]
     T* te;     // [nbm][4]
-    T* E;      // [ng][nbm]
-    T* Ed;     // [ng][nbm]
+    T* E;      // [nbm][ng]: fc exp(-r^2 / eta_g^2) of each slot and eta group
+    T* Ed;     // [nbm][ng]: its r derivative
     T* fR;
     T* gR;
     T* ri;     // 1 / r
@@ -90,14 +90,14 @@ __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, i
 __device__ inline void tri_decode(int t, bool same, int nJ, int nK, float rn, int& a, int& c) {
     const int m = same ? nJ : min(nJ, nK), M = same ? nJ : max(nJ, nK);
     int d = (int)(((float)t + 0.5f) * rn);
-    int x = t - d * m;
-    if (x < 0) { --d; x += m; }
-    else if (x >= m) { ++d; x -= m; }
-    if (same) ++d;
-    int y = x + d;
-    if (y >= M) y -= M;
-    if (same || nJ <= nK) { a = x; c = y; }
-    else { a = y; c = x; }
+    const int x0 = t - d * m;
+    d += (x0 >= m) - (x0 < 0);
+    const int x = t - d * m;
+    int y = x + d + (same ? 1 : 0);
+    y -= (y >= M) ? M : 0;
+    const bool sw = !same && nJ > nK;
+    a = sw ? y : x;
+    c = sw ? x : y;
 }
 
 // pick v[g] for a workgroup-uniform g < ACE_MAXG
@@ -127,16 +127,17 @@ __device__ inline T prm(const float* f, const double* d, int i) {
     else return f[i];
 }
 
-// One angular block whose functions are all canonical: lambda = +-1 and zeta = 2^k, k < nz <= 7, one
+// One angular block whose functions are all canonical: lambda = +-1 and zeta = 2^k, k < NZ <= 7, one
 // function per (sign s, level k, eta group g).  Per triplet the windows b_s = |1/2 + s cos/2| are raised
 // to 2^k (cp) and 2^k - 1 (cq) by squaring, and the block's sums are the outer products
 //   G[s][k][g] += b_s^(2^k) E_j[g] E_k[g]
 // held in registers with static indices; cidx maps each function of the block to (s*7 + k)*NG + g.
-template <class T, int NG, int MODE>
-__device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int ng, int bJ, int bK, int nJ, int nK,
-                                     bool same, int T3, const int* __restrict__ cidx, int nA, int goff, double* gw,
-                                     int tid, int wv, int lane) {
-    constexpr int NZ = 7;
+// NZ (a compile-time bound on the levels in use) keeps the loop free of branches; unused levels are
+// computed and discarded.  E / Ed are interleaved per slot: E[slot * NG + g], zero for g >= ng.
+template <class T, int NG, int NZ, int MODE>
+__device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int nJ, int nK, bool same, int T3,
+                                     const int* __restrict__ cidx, int nA, int goff, double* gw, int tid, int wv,
+                                     int lane) {
     constexpr int ADJ = (MODE == 1);
     constexpr int TAN = (MODE == 2);
     T acc[2][NZ][NG], wgt[2][NZ][NG];
@@ -151,7 +152,7 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int 
             }
     if (ADJ) {
         // dE/dG of the block scattered to the (s, k, g) table, then read back into registers
-        for (int q = tid; q < 2 * NZ * NG; q += BT) L.red[q] = T(0);
+        for (int q = tid; q < 14 * NG; q += BT) L.red[q] = T(0);
         __syncthreads();
         for (int m = tid; m < nA; m += BT) L.red[cidx[m]] = L.row[goff + m];
         __syncthreads();
@@ -160,13 +161,13 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int 
 #pragma unroll
             for (int k = 0; k < NZ; ++k)
 #pragma unroll
-                for (int g = 0; g < NG; ++g) wgt[s][k][g] = L.red[(s * NZ + k) * NG + g];
+                for (int g = 0; g < NG; ++g) wgt[s][k][g] = L.red[(s * 7 + k) * NG + g];
         __syncthreads();
     }
-    const float rnK = 1.f / (float)(same ? nJ : min(nJ, nK));
+    const float rn = 1.f / (float)(same ? nJ : min(nJ, nK));
     for (int t = tid; t < T3; t += BT) {
         int a, c;
-        tri_decode(t, same, nJ, nK, rnK, a, c);
+        tri_decode(t, same, nJ, nK, rn, a, c);
         const int sa = bJ + a, sc = bK + c;
         const T uax = L.u[4 * sa], uay = L.u[4 * sa + 1], uaz = L.u[4 * sa + 2];
         const T ucx = L.u[4 * sc], ucy = L.u[4 * sc + 1], ucz = L.u[4 * sc + 2];
@@ -174,11 +175,10 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int 
         T Da[NG], Dc[NG], Dda[NG], Ddc[NG], D[NG];
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            const bool on = g < ng;
-            Da[g] = on ? L.E[g * nbm + sa] : T(0);
-            Dc[g] = on ? L.E[g * nbm + sc] : T(0);
-            Dda[g] = (on && (ADJ || TAN)) ? L.Ed[g * nbm + sa] : T(0);
-            Ddc[g] = (on && (ADJ || TAN)) ? L.Ed[g * nbm + sc] : T(0);
+            Da[g] = L.E[sa * NG + g];
+            Dc[g] = L.E[sc * NG + g];
+            Dda[g] = (ADJ || TAN) ? L.Ed[sa * NG + g] : T(0);
+            Ddc[g] = (ADJ || TAN) ? L.Ed[sc * NG + g] : T(0);
             D[g] = Da[g] * Dc[g];
         }
         T cp[2][NZ], cq[2][NZ];
@@ -187,23 +187,19 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int 
         cq[0][0] = T(1);
         cq[1][0] = T(1);
 #pragma unroll
-        for (int k = 1; k < NZ; ++k) {
-            if (k < nz) {
+        for (int k = 1; k < NZ; ++k)
 #pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    cq[s][k] = cq[s][k - 1] * cp[s][k - 1];
-                    cp[s][k] = cp[s][k - 1] * cp[s][k - 1];
-                }
+            for (int s = 0; s < 2; ++s) {
+                cq[s][k] = cq[s][k - 1] * cp[s][k - 1];
+                cp[s][k] = cp[s][k - 1] * cp[s][k - 1];
             }
-        }
         if (MODE == 0) {
 #pragma unroll
             for (int k = 0; k < NZ; ++k)
-                if (k < nz)
 #pragma unroll
-                    for (int s = 0; s < 2; ++s)
+                for (int s = 0; s < 2; ++s)
 #pragma unroll
-                        for (int g = 0; g < NG; ++g) acc[s][k][g] += cp[s][k] * D[g];
+                    for (int g = 0; g < NG; ++g) acc[s][k][g] += cp[s][k] * D[g];
         } else if (TAN) {
             const T tax = L.te[4 * sa], tay = L.te[4 * sa + 1], taz = L.te[4 * sa + 2], tap = L.te[4 * sa + 3];
             const T tcx = L.te[4 * sc], tcy = L.te[4 * sc + 1], tcz = L.te[4 * sc + 2], tcp = L.te[4 * sc + 3];
@@ -217,29 +213,27 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int 
             }
 #pragma unroll
             for (int k = 0; k < NZ; ++k)
-                if (k < nz)
 #pragma unroll
-                    for (int s = 0; s < 2; ++s) {
-                        const T dp = T(1 << k) * (s ? T(-0.5) : T(0.5)) * cq[s][k];   // d b^(2^k) / d cos
+                for (int s = 0; s < 2; ++s) {
+                    const T dp = T(1 << k) * (s ? T(-0.5) : T(0.5)) * cq[s][k];   // d b^(2^k) / d cos
 #pragma unroll
-                        for (int g = 0; g < NG; ++g) acc[s][k][g] += dp * X[g] + cp[s][k] * Y[g];
-                    }
+                    for (int g = 0; g < NG; ++g) acc[s][k][g] += dp * X[g] + cp[s][k] * Y[g];
+                }
         } else {
             T S[NG], Sp[NG];
 #pragma unroll
             for (int g = 0; g < NG; ++g) { S[g] = T(0); Sp[g] = T(0); }
 #pragma unroll
             for (int k = 0; k < NZ; ++k)
-                if (k < nz)
 #pragma unroll
-                    for (int s = 0; s < 2; ++s) {
-                        const T dp = T(1 << k) * (s ? T(-0.5) : T(0.5)) * cq[s][k];
+                for (int s = 0; s < 2; ++s) {
+                    const T dp = T(1 << k) * (s ? T(-0.5) : T(0.5)) * cq[s][k];
 #pragma unroll
-                        for (int g = 0; g < NG; ++g) {
-                            S[g] += wgt[s][k][g] * cp[s][k];
-                            Sp[g] += wgt[s][k][g] * dp;
-                        }
+                    for (int g = 0; g < NG; ++g) {
+                        S[g] += wgt[s][k][g] * cp[s][k];
+                        Sp[g] += wgt[s][k][g] * dp;
                     }
+                }
             T dEdc = T(0), dEda = T(0), dEdcc = T(0);
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
@@ -263,12 +257,11 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int 
         for (int s = 0; s < 2; ++s)
 #pragma unroll
             for (int k = 0; k < NZ; ++k)
-                if (k < nz)
 #pragma unroll
-                    for (int g = 0; g < NG; ++g) {
-                        const T v = t_wave_sum<T>(acc[s][k][g]);
-                        if (lane == 0) L.red[wv * 64 + (s * NZ + k) * NG + g] = v;
-                    }
+                for (int g = 0; g < NG; ++g) {
+                    const T v = t_wave_sum63<T>(acc[s][k][g]);
+                    if (lane == 63) L.red[wv * 64 + (s * 7 + k) * NG + g] = v;
+                }
         __syncthreads();
         for (int m = tid; m < nA; m += BT) {
             T v = T(0);
@@ -280,7 +273,9 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int nbm, int nz, int 
 }
 
 // GEN = 0 compiles the kernel for models whose angular blocks are all canonical (no generic path).
-template <class T, int NG, int MODE, int GEN>
+// NG: eta groups per species (1, 2 or 4, at least the model's largest count); NZ: canonical power levels
+// (4 or 7, at least the model's largest log2(zeta) + 1).
+template <class T, int NG, int NZ, int MODE, int GEN>
 __global__ __launch_bounds__(BT) void k_basis(
     int natoms, const DevModel* __restrict__ M, DevBasisParams P, const int* __restrict__ type,
     const int* __restrict__ nb_off, const int* __restrict__ nb_j, const float4* __restrict__ nb_d,
@@ -300,7 +295,7 @@ __global__ __launch_bounds__(BT) void k_basis(
     if (i >= natoms) return;
     const int I = type[i];
     const DevSpecies& gsp = M->sp[I];
-    const BasisLds<T> L = basis_lds<T>(basis_smem, nbm, ngmax, kmax, MODE);
+    const BasisLds<T> L = basis_lds<T>(basis_smem, nbm, NG, kmax, MODE);
     const int o0 = nb_off[i], o1 = nb_off[i + 1];
     const float rcut = gsp.rc_max;
     const int row = row_of[i];
@@ -418,11 +413,17 @@ __global__ __launch_bounds__(BT) void k_basis(
         L.fR[q] = fc;
         L.gR[q] = gc;
         t_cutoff<T>(sp.a_cut, F64 ? (T)sp.a_rc64 : (T)sp.a_rc, r, fc, gc);
-        for (int g = 0; g < ng; ++g) {
-            const T ie = F64 ? (T)sp.ietar2_64[g] : (T)sp.ietar2[g];
-            const T e = t_exp<T>(-r * r * ie);
-            L.E[g * nbm + q] = fc * e;
-            if (ADJ || TAN) L.Ed[g * nbm + q] = (gc - T(2) * r * ie * fc) * e;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            T ev = T(0), edv = T(0);
+            if (g < ng) {
+                const T ie = F64 ? (T)sp.ietar2_64[g] : (T)sp.ietar2[g];
+                const T e = t_exp<T>(-r * r * ie);
+                ev = fc * e;
+                edv = (gc - T(2) * r * ie * fc) * e;
+            }
+            L.E[q * NG + g] = ev;
+            if (ADJ || TAN) L.Ed[q * NG + g] = edv;
         }
         if (ADJ) {
             for (int w = 0; w < BW; ++w) {
@@ -515,8 +516,8 @@ __global__ __launch_bounds__(BT) void k_basis(
             const int T3 = same ? nJ * (nJ - 1) / 2 : nJ * nK;
             if (nA == 0 || T3 == 0) continue;
             if (!GEN || sp.a_canon[b]) {
-                angular_canon<T, NG, MODE>(L, nbm, sp.nz, ng, bJ, bK, nJ, nK, same, T3, L.ci + (sp.a_poff[b] - sp.a_poff[0]), nA,
-                                           sp.a_goff[b], gw, tid, wv, lane);
+                angular_canon<T, NG, NZ, MODE>(L, bJ, bK, nJ, nK, same, T3, L.ci + (sp.a_poff[b] - sp.a_poff[0]), nA,
+                                               sp.a_goff[b], gw, tid, wv, lane);
                 continue;
             }
             const int goff = sp.a_goff[b], poff = sp.a_poff[b];
@@ -551,10 +552,10 @@ __global__ __launch_bounds__(BT) void k_basis(
 #pragma unroll
                     for (int g = 0; g < ACE_MAXG; ++g) {
                         const bool on = g < ng;
-                        Da[g] = on ? L.E[g * nbm + sa] : T(0);
-                        Dc[g] = on ? L.E[g * nbm + sc] : T(0);
-                        Dda[g] = (on && (ADJ || TAN)) ? L.Ed[g * nbm + sa] : T(0);
-                        Ddc[g] = (on && (ADJ || TAN)) ? L.Ed[g * nbm + sc] : T(0);
+                        Da[g] = on ? L.E[sa * NG + g] : T(0);
+                        Dc[g] = on ? L.E[sc * NG + g] : T(0);
+                        Dda[g] = (on && (ADJ || TAN)) ? L.Ed[sa * NG + g] : T(0);
+                        Ddc[g] = (on && (ADJ || TAN)) ? L.Ed[sc * NG + g] : T(0);
                     }
                     // cp[k + 7s] = b_s^(2^k), cq[k + 7s] = b_s^(2^k - 1), b_0 = (1+cos)/2, b_1 = (1-cos)/2
                     T cp[14], cq[14];

@@ -46,8 +46,9 @@ __device__ inline void act_fd(int act, float c, float z, float& a, float& d) {
 //   lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16], and receives D[4 (l / 16) + r][l % 16] in
 //   register r.  A (activations, deltas) comes from LDS; B (weights) from LDS when the species' weights
 //   fit next to the activations (one batched copy), else from global memory.
-// LDS rows have an odd stride (width rounded up to 4, plus 1) so the 16 rows of one A read fall in
-// distinct banks; the padding columns hold zeros, so a k-step past the width adds nothing.  LDS is
+// LDS rows have an odd stride (width rounded up to 16, plus 1) so the 16 rows of one A read fall in
+// distinct banks; padding columns and the padded weight rows and columns hold zeros, so every operand
+// load is unconditional and a k-step or column past the width adds nothing.  LDS is
 // addressed through integer offsets into mlp_smem so every access compiles to a ds_ instruction.
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
@@ -55,7 +56,8 @@ typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
 extern __shared__ float mlp_smem[];
 
-__host__ __device__ inline int mlp_ld(int w) { return ((w + 3) & ~3) + 1; }
+// row stride of an activation tile: width rounded up to 16 (four k-steps) plus 1
+__host__ __device__ inline int mlp_ld(int w) { return ((w + 15) & ~15) + 1; }
 
 // floats of LDS for one tile: X [16][ld(K)], then per layer A_l, D_l [16][ld(w_{l+1})]
 __host__ __device__ inline int mlp_lds_floats(const DevSpecies& sp) {
@@ -64,30 +66,70 @@ __host__ __device__ inline int mlp_lds_floats(const DevSpecies& sp) {
     return n;
 }
 
-// C[16 x n] = A[16 x k] * B for the 16-wide output tile starting at o0; A at LDS offset a (row stride
-// lda), B[kk][o] at LDS offset b (BLDS) or global Bg, element o * so + kk * sk.  Returns the lane's
-// 4 accumulators.
-template <bool BLDS>
-__device__ inline mfma_f4 mfma_tile(int a, int lda, int k, int b, const float* __restrict__ Bg, int so, int sk,
-                                    int n, int o0, int lane) {
-    mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
-    const int r = lane & 15, q = lane >> 4;
-    const int o = o0 + r;
-    const bool on = o < n;
-    const int ks = (k + 3) >> 2;
-    for (int s0 = 0; s0 < ks; s0 += 8) {
-        float av[8], bv[8];
+// C[16 x n] = A[16 x k] * B for NT 16-wide output tiles starting at o0, o0 + 16, ...; A at LDS offset a
+// (row stride lda), B[kk][o] at LDS offset b (BLDS) or global Bg, element o * so + kk * sk.  Two
+// accumulators per tile (even and odd k-steps) give 2 NT independent matrix-core chains; the A operand
+// of a k-step is shared by the NT tiles.  acc[t] receives the lane's 4 values of tile t.
+template <bool BLDS, int NT>
+__device__ inline void mfma_group(int a, int lda, int k, int b, const float* __restrict__ Bg, int so, int sk, int n,
+                                  int o0, int lane, mfma_f4* acc) {
+    mfma_f4 c0[NT], c1[NT];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int kk = 4 * (s0 + u) + q;
-            av[u] = (s0 + u < ks) ? mlp_smem[a + r * lda + kk] : 0.f;
-            if (BLDS) bv[u] = (on && kk < k) ? mlp_smem[b + o * so + kk * sk] : 0.f;
-            else bv[u] = (on && kk < k) ? Bg[o * so + kk * sk] : 0.f;
+    for (int t = 0; t < NT; ++t) {
+        c0[t] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+        c1[t] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+    }
+    const int r = lane & 15, q = lane >> 4;
+    const int k16 = (k + 15) & ~15;   // rows [k, k16) of A and B are zero padding
+    (void)n;
+    for (int kb = 0; kb < k16; kb += 16) {
+        float av[4], bv[4][NT];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int kk = kb + 4 * u + q;
+            av[u] = mlp_smem[a + r * lda + kk];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int o = o0 + 16 * t + r;
+                if (BLDS) bv[u][t] = mlp_smem[b + o * so + kk * sk];
+                else bv[u][t] = Bg[o * so + kk * sk];
+            }
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc, 0, 0, 0);
+        for (int u = 0; u < 4; u += 2)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                c0[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u][t], c0[t], 0, 0, 0);
+                c1[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u + 1], bv[u + 1][t], c1[t], 0, 0, 0);
+            }
     }
-    return acc;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = c0[t] + c1[t];
+}
+
+// every output tile of a layer, in groups of 4, 2 or 1 tiles; epi(acc, tile count, first column)
+template <bool BLDS, class EPI>
+__device__ inline void mfma_layer(int a, int lda, int k, int b, const float* __restrict__ Bg, int so, int sk, int n,
+                                  int lane, EPI epi) {
+    for (int o0 = 0; o0 < n;) {
+        const int rem = (n - o0 + 15) >> 4;
+        if (rem >= 4) {
+            mfma_f4 acc[4];
+            mfma_group<BLDS, 4>(a, lda, k, b, Bg, so, sk, n, o0, lane, acc);
+            epi(acc, 4, o0);
+            o0 += 64;
+        } else if (rem >= 2) {
+            mfma_f4 acc[2];
+            mfma_group<BLDS, 2>(a, lda, k, b, Bg, so, sk, n, o0, lane, acc);
+            epi(acc, 2, o0);
+            o0 += 32;
+        } else {
+            mfma_f4 acc[1];
+            mfma_group<BLDS, 1>(a, lda, k, b, Bg, so, sk, n, o0, lane, acc);
+            epi(acc, 1, o0);
+            o0 += 16;
+        }
+    }
 }
 
 // tiles: (species, first row, rows, 0), rows <= 16.  wpad: the parameters in the matrix-core layout
@@ -172,32 +214,37 @@ __global__ __launch_bounds__(64) void k_mlp_energy_grad(
                 }
             }
         }
-        for (int r = lane; r < MLP_ROWS; r += 64) mlp_smem[r * ldx + ldx - 1] = 0.f;
+        const int pad = ldx - 4 * k4;   // columns [4 k4, ldx) of every row
+        for (int e = lane; e < MLP_ROWS * pad; e += 64) mlp_smem[(e / pad) * ldx + 4 * k4 + e % pad] = 0.f;
     }
     for (int idx = MLP_ROWS * ldx + lane; idx < wofs; idx += 64) mlp_smem[idx] = 0.f;
     wave_sync();
     const int cr = lane & 15, rq = 4 * (lane >> 4);
-    // forward: z_l = b_l + W_l a_{l-1}; W_l[o][k] at wl_off + k (no + 1) + o
+    // forward: z_l = b_l + W_l a_{l-1}; W_l[o][k] at wl_off + k mlp_ld(no) + o
     int ain = 0, lda = ldx;
     for (int l = 0; l < L; ++l) {
         const int no = sp.width[l + 1], ni = sp.width[l];
         const int ao = s_off[2 * l], dofs = s_off[2 * l + 1], ldo = mlp_ld(no);
         const int act = (l == L - 1) ? ACE_ACT_LINEAR : sp.act;
-        for (int o0 = 0; o0 < no; o0 += 16) {
-            const mfma_f4 acc = mfma_tile<STAGE>(ain, lda, ni, wofs + sp.wl_off[l], Wg + sp.wl_off[l], 1, no + 1, no,
-                                                 o0, lane);
-            const int o = o0 + cr;
-            if (o < no) {
-                const float bo = STAGE ? mlp_smem[wofs + sp.bl_off[l] + o] : Wg[sp.bl_off[l] + o];
+        const float* bias = STAGE ? nullptr : Wg + sp.bl_off[l];
+        const int boff = wofs + sp.bl_off[l];
+        const float cc = sp.c;
+        mfma_layer<STAGE>(ain, lda, ni, wofs + sp.wl_off[l], Wg + sp.wl_off[l], 1, mlp_ld(no), no, lane,
+                          [&](const mfma_f4* acc, int nt, int o0) {
+                              for (int t = 0; t < nt; ++t) {
+                                  const int o = o0 + 16 * t + cr;
+                                  if (o < no) {
+                                      const float bo = STAGE ? mlp_smem[boff + o] : bias[o];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    float a, d;
-                    act_fd(act, sp.c, acc[r] + bo, a, d);
-                    mlp_smem[ao + (rq + r) * ldo + o] = a;
-                    mlp_smem[dofs + (rq + r) * ldo + o] = d;
-                }
-            }
-        }
+                                      for (int r = 0; r < 4; ++r) {
+                                          float a, d;
+                                          act_fd(act, cc, acc[t][r] + bo, a, d);
+                                          mlp_smem[ao + (rq + r) * ldo + o] = a;
+                                          mlp_smem[dofs + (rq + r) * ldo + o] = d;
+                                      }
+                                  }
+                              }
+                          });
         ain = ao;
         lda = ldo;
         wave_sync();
@@ -213,19 +260,20 @@ __global__ __launch_bounds__(64) void k_mlp_energy_grad(
         const int no = sp.width[l + 1], ni = sp.width[l];
         const int al = s_off[2 * l], am = s_off[2 * (l - 1)], dm = s_off[2 * (l - 1) + 1];
         const int ldl = mlp_ld(no), ldm = mlp_ld(ni);
-        for (int k0 = 0; k0 < ni; k0 += 16) {
-            // B[o][k] = W_l[o][k]: output column k, reduction index o
-            const mfma_f4 acc = mfma_tile<STAGE>(al, ldl, no, wofs + sp.wl_off[l], Wg + sp.wl_off[l], no + 1, 1, ni,
-                                                 k0, lane);
-            const int k = k0 + cr;
-            if (k < ni) {
+        // B[o][k] = W_l[o][k]: output column k, reduction index o
+        mfma_layer<STAGE>(al, ldl, no, wofs + sp.wl_off[l], Wg + sp.wl_off[l], mlp_ld(no), 1, ni, lane,
+                          [&](const mfma_f4* acc, int nt, int k0) {
+                              for (int t = 0; t < nt; ++t) {
+                                  const int k = k0 + 16 * t + cr;
+                                  if (k < ni) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int e = (rq + r) * ldm + k;
-                    mlp_smem[am + e] = acc[r] * mlp_smem[dm + e];
-                }
-            }
-        }
+                                      for (int r = 0; r < 4; ++r) {
+                                          const int e = (rq + r) * ldm + k;
+                                          mlp_smem[am + e] = acc[t][r] * mlp_smem[dm + e];
+                                      }
+                                  }
+                              }
+                          });
         wave_sync();
     }
     // dE/dG = (delta_0 W_0) diag(inpw)
@@ -233,16 +281,17 @@ __global__ __launch_bounds__(64) void k_mlp_energy_grad(
         const int no = sp.width[1];
         const float* iw = inpw + sp.in_off;
         const int a0 = s_off[0], ld0 = mlp_ld(no);
-        for (int k0 = 0; k0 < K; k0 += 16) {
-            const mfma_f4 acc = mfma_tile<STAGE>(a0, ld0, no, wofs + sp.wl_off[0], Wg + sp.wl_off[0], no + 1, 1, K, k0,
-                                                 lane);
-            const int k = k0 + cr;
-            if (k < K) {
-                const float w = iw[k];
+        mfma_layer<STAGE>(a0, ld0, no, wofs + sp.wl_off[0], Wg + sp.wl_off[0], mlp_ld(no), 1, K, lane,
+                          [&](const mfma_f4* acc, int nt, int k0) {
+                              for (int t = 0; t < nt; ++t) {
+                                  const int k = k0 + 16 * t + cr;
+                                  if (k < K) {
+                                      const float w = iw[k];
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    if (rq + r < nr) dEdG[(long long)(r0 + rq + r) * ldg + k] = acc[r] * w;
-            }
-        }
+                                      for (int r = 0; r < 4; ++r)
+                                          if (rq + r < nr) dEdG[(long long)(r0 + rq + r) * ldg + k] = acc[t][r] * w;
+                                  }
+                              }
+                          });
     }
 }

@@ -194,13 +194,26 @@ __device__ inline double ref_ds2(const double* R, double dx, double dy, double d
     return (v0 * v0 + v1 * v1) + v2 * v2;
 }
 
+// neighbour order key: j_local in the high bits, then the image (n0, n1, n2) offset by 512 in three
+// 10-bit fields; integer order = the reference order (j ascending, then n lexicographic)
+__device__ inline long long nb_key(int jl, int n0, int n1, int n2) {
+    return ((long long)jl << 30) | ((long long)(n0 + 512) << 20) | ((n1 + 512) << 10) | (n2 + 512);
+}
+__device__ inline void nb_key_decode(long long k, int& jl, int& n0, int& n1, int& n2) {
+    jl = (int)(k >> 30);
+    const int c = (int)(k & 0x3fffffff);
+    n0 = (c >> 20) - 512;
+    n1 = ((c >> 10) & 1023) - 512;
+    n2 = (c & 1023) - 512;
+}
+
 #define NB_WAVES 4
 #define NB_LMAX 1024   // neighbours per atom held while sorting
 #define NB_U 4         // candidate chunks loaded together by one wave
 
 // Workgroup (4 waves) per atom; the waves take interleaved groups of NB_U x 64 candidates.
 // MODE 0: count only (counts[i], running maximum in *maxcount).  MODE 1: also sort the keys
-// (j_local * nimg + image code: the reference order) and write them to row i of keys (cap slots); an
+// (j_local << 30 | image order code: the reference order) and write them to row i of keys (cap slots); an
 // atom with more than cap neighbours raises *over and writes none.
 template <int MODE>
 __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
@@ -226,8 +239,6 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
     const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
     const int sx = 2 * S.st[0] + 1, sy = 2 * S.st[1] + 1, sz = 2 * S.st[2] + 1;
     const int nst = sx * sy * sz;
-    const int ny = 2 * S.cf[1] + 1, nz = 2 * S.cf[2] + 1;
-    const long long nimg = (long long)(2 * S.cf[0] + 1) * ny * nz;
     if (tid == 0) s_n = 0;
     for (int cb = 0; cb < nst; cb += 64) {
         // stencil cells of this chunk: start, size and image shift, with an inclusive prefix (wave 0)
@@ -300,9 +311,7 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
                         const double ds2 = ref_ds2(S.R, dx, dy, dz, n0, n1, n2);
                         if ((j != i || ds2 > 1e-12) && ds2 < rc2) {
                             acc = true;
-                            const long long code =
-                                ((long long)(n0 + S.cf[0]) * ny + (n1 + S.cf[1])) * nz + (n2 + S.cf[2]);
-                            key = (long long)(j - S.atom_off) * nimg + code;
+                            key = nb_key(j - S.atom_off, n0, n1, n2);
                         }
                     }
                 }
@@ -358,20 +367,14 @@ __global__ __launch_bounds__(256) void k_nb_compact(
     }
     if (i == natoms) return;
     const StrucInfo& S = sinfo[struc_of[i]];
-    const int ny = 2 * S.cf[1] + 1, nz = 2 * S.cf[2] + 1;
-    const long long nimg = (long long)(2 * S.cf[0] + 1) * ny * nz;
     const int base = nb_off[i], n = counts[i];
     const long long* row = keys + (long long)i * cap;
     const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
     for (int e = lane; e < n; e += 64) {
         const long long k = row[e];
-        const long long jl = k / nimg;
-        long long code = k - jl * nimg;
-        const int c2 = (int)(code % nz) - S.cf[2];
-        code /= nz;
-        const int c1 = (int)(code % ny) - S.cf[1];
-        const int c0 = (int)(code / ny) - S.cf[0];
-        const int j = (int)jl + S.atom_off;
+        int jl, c0, c1, c2;
+        nb_key_decode(k, jl, c0, c1, c2);
+        const int j = jl + S.atom_off;
         const int p = base + e;
         nb_j[p] = j;
         nb_src[p] = i;
@@ -383,8 +386,7 @@ __global__ __launch_bounds__(256) void k_nb_compact(
         const double r = sqrt(rx * rx + ry * ry + rz * rz);
         nb_d[p] = make_float4((float)rx, (float)ry, (float)rz, (float)r);
         // reverse pair: key (i_local, -n) in row j
-        const long long want = (long long)(i - S.atom_off) * nimg +
-                               ((long long)(-c0 + S.cf[0]) * ny + (-c1 + S.cf[1])) * nz + (-c2 + S.cf[2]);
+        const long long want = nb_key(i - S.atom_off, -c0, -c1, -c2);
         const long long* rj = keys + (long long)j * cap;
         int lo = 0, hi = counts[j] - 1, found = -1;
         while (lo <= hi) {

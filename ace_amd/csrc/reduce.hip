@@ -8,79 +8,135 @@
 #include "ace_common.hpp"
 #include "basis_math.hpp"
 
-// GL lanes per atom: force (FP64) and the atom's virial share -sum_p g_p (x) d_p (FP64, 9 values);
-// each lane sums a strided subset of the atom's pairs, the GL partials are combined by shuffles in a
-// fixed order.
-#define GL 8
-__global__ __launch_bounds__(256) void k_force_gather(int natoms, const int* __restrict__ nb_off,
-                                                      const int* __restrict__ nb_rev, const float4* __restrict__ nb_d,
-                                                      const float4* __restrict__ pair_g, double* __restrict__ F,
-                                                      double* __restrict__ vir_atom) {
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = gt / GL, sub = gt % GL;
-    const bool on = i < natoms;
+// GL lanes per atom, 16 atoms of one structure per workgroup.  Each lane sums a strided subset of the
+// atom's pairs (four pairs per round: the reverse indices, then the gradients, are loaded together) into
+// the force (FP64) and the atom's virial share -sum_p g_p (x) d_p; DPP row sums combine the GL lanes.
+// The workgroup's 16 (energy, virial) values are summed in a fixed order into one partial per
+// workgroup; the last workgroup of a structure (per-structure ticket) sums the structure's partials in
+// workgroup order.  Every result is independent of block timing.
+#define GL 16
+#define GB 4
+#define RED_THREADS 256
+#define GATHER_ATOMS (RED_THREADS / GL)
+
+// gtab[b] = (structure, first atom) of workgroup b; sblk[s] = (first workgroup, workgroups) of structure s
+__global__ __launch_bounds__(RED_THREADS) void k_force_gather(
+    const int2* __restrict__ gtab, const int2* __restrict__ sblk, const int* __restrict__ atom_off,
+    const int* __restrict__ natoms_of, const int* __restrict__ nb_off, const int* __restrict__ nb_rev,
+    const float4* __restrict__ nb_d, const float4* __restrict__ pair_g, const double* __restrict__ e_atom,
+    double* __restrict__ F, double* __restrict__ part, int* __restrict__ ticket, double* __restrict__ E,
+    double* __restrict__ vir) {
+    __shared__ double s_v[GATHER_ATOMS][10];
+    __shared__ int s_last;
+    const int2 gb = gtab[blockIdx.x];
+    const int st = gb.x;
+    const int aend = atom_off[st] + natoms_of[st];
+    const int i = gb.y + threadIdx.x / GL, sub = threadIdx.x % GL;
+    const bool on = i < aend;
     double f[3] = {0, 0, 0};
     double w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (on) {
-        for (int p = nb_off[i] + sub; p < nb_off[i + 1]; p += GL) {
-            const float4 g = pair_g[p];
-            const float4 gr = pair_g[nb_rev[p]];
-            const float4 d = nb_d[p];
-            f[0] += (double)gr.x - (double)g.x;
-            f[1] += (double)gr.y - (double)g.y;
-            f[2] += (double)gr.z - (double)g.z;
-            const double ga[3] = {g.x, g.y, g.z}, da[3] = {d.x, d.y, d.z};
+        const int p1 = nb_off[i + 1];
+        for (int p0 = nb_off[i] + sub; p0 < p1; p0 += GL * GB) {
+            int rv[GB];
+            float4 g[GB], d[GB], gr[GB];
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
+            for (int u = 0; u < GB; ++u) {
+                const int p = p0 + GL * u;
+                rv[u] = (p < p1) ? nb_rev[p] : 0;
+                g[u] = (p < p1) ? pair_g[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+                d[u] = (p < p1) ? nb_d[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
 #pragma unroll
-                for (int b = 0; b < 3; ++b) w[3 * a + b] -= ga[a] * da[b];
+            for (int u = 0; u < GB; ++u) gr[u] = (p0 + GL * u < p1) ? pair_g[rv[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < GB; ++u) {
+                f[0] += (double)gr[u].x - (double)g[u].x;
+                f[1] += (double)gr[u].y - (double)g[u].y;
+                f[2] += (double)gr[u].z - (double)g[u].z;
+                const double ga[3] = {g[u].x, g[u].y, g[u].z}, da[3] = {d[u].x, d[u].y, d[u].z};
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) w[3 * a + b] -= ga[a] * da[b];
+            }
         }
     }
-    // sums over the GL = 8 lanes of the atom (DPP quad and half-row moves)
 #pragma unroll
     for (int q = 0; q < 3; ++q) f[q] = t_group_sum<double>(f[q], GL);
-    if (vir_atom) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) w[q] = t_group_sum<double>(w[q], GL);
+    for (int q = 0; q < 9; ++q) w[q] = t_group_sum<double>(w[q], GL);
+    const int la = threadIdx.x / GL;
+    if (sub == 0) {
+        if (on) {
+            F[3 * i] = f[0];
+            F[3 * i + 1] = f[1];
+            F[3 * i + 2] = f[2];
+        }
+        s_v[la][0] = on ? e_atom[i] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) s_v[la][1 + q] = on ? w[q] : 0.0;
     }
-    if (on && sub == 0) {
-        F[3 * i] = f[0];
-        F[3 * i + 1] = f[1];
-        F[3 * i + 2] = f[2];
-        if (vir_atom) {
+    __syncthreads();
+    if (threadIdx.x < 10) {
+        double acc = 0.0;
+        for (int a = 0; a < GATHER_ATOMS; ++a) acc += s_v[a][threadIdx.x];
+        part[10 * (long long)blockIdx.x + threadIdx.x] = acc;
+    }
+    __threadfence();
+    __syncthreads();
+    const int2 sb = sblk[st];
+    if (threadIdx.x == 0) {
+        const int last = atomicAdd(&ticket[st], 1) == sb.y - 1;
+        if (last) ticket[st] = 0;
+        s_last = last;
+    }
+    __syncthreads();
+    if (s_last) {
+        // thread t sums partials t, t + 256, ... in order; then fixed-order wave (DPP) and block sums
+        __threadfence();
+        double acc[10];
 #pragma unroll
-            for (int q = 0; q < 9; ++q) vir_atom[9 * i + q] = w[q];
+        for (int q = 0; q < 10; ++q) acc[q] = 0.0;
+        for (int b = threadIdx.x; b < sb.y; b += RED_THREADS) {
+            const double* pb = part + 10 * (long long)(sb.x + b);
+#pragma unroll
+            for (int q = 0; q < 10; ++q) acc[q] += pb[q];
+        }
+        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+        for (int q = 0; q < 10; ++q) {
+            const double v = t_wave_sum63<double>(acc[q]);
+            if (lane == 63) s_v[wv][q] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 10) {
+            double v = 0.0;
+            for (int w = 0; w < RED_THREADS / 64; ++w) v += s_v[w][threadIdx.x];
+            if (threadIdx.x == 0) E[st] = v;
+            else vir[9 * st + threadIdx.x - 1] = v;
         }
     }
 }
 
-// block per structure: E_s = sum e_atom, W_s = sum vir_atom (fixed-order FP64 tree)
-#define RED_THREADS 256
-__global__ __launch_bounds__(RED_THREADS) void k_struct_reduce(
-    const int* __restrict__ atom_off, const int* __restrict__ natoms_of, const double* __restrict__ e_atom,
-    const double* __restrict__ vir_atom, double* __restrict__ E, double* __restrict__ vir) {
-    __shared__ double s[RED_THREADS][10];
+// block per structure: E_s = sum e_atom (fixed-order FP64 tree); the training loss without forces
+__global__ __launch_bounds__(RED_THREADS) void k_struct_reduce(const int* __restrict__ atom_off,
+                                                                const int* __restrict__ natoms_of,
+                                                                const double* __restrict__ e_atom, double* __restrict__ E) {
+    __shared__ double s[RED_THREADS];
     const int st = blockIdx.x, t = threadIdx.x;
     const int a0 = atom_off[st], n = natoms_of[st];
-    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int k = t; k < n; k += RED_THREADS) {
-        acc[0] += e_atom[a0 + k];
-        if (vir_atom)
-            for (int q = 0; q < 9; ++q) acc[1 + q] += vir_atom[9 * (a0 + k) + q];
-    }
-    for (int q = 0; q < 10; ++q) s[t][q] = acc[q];
+    double acc = 0.0;
+    for (int k = t; k < n; k += RED_THREADS) acc += e_atom[a0 + k];
+    s[t] = acc;
     __syncthreads();
     for (int h = RED_THREADS / 2; h > 0; h >>= 1) {
-        if (t < h)
-            for (int q = 0; q < 10; ++q) s[t][q] += s[t + h][q];
+        if (t < h) s[t] += s[t + h];
         __syncthreads();
     }
-    if (t == 0) {
-        E[st] = s[0][0];
-        if (vir)
-            for (int q = 0; q < 9; ++q) vir[9 * st + q] = s[0][1 + q];
-    }
+    if (t == 0) E[st] = s[0];
 }
+
 
 // thread per pair: d_p = r_i - (r_j + R n) in FP64 (nnp.cpp:363), stored as FP32 with |d|
 __global__ void k_nb_disp(const int* __restrict__ npairs, const int* __restrict__ nb_src, const int* __restrict__ nb_j,

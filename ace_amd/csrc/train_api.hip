@@ -89,17 +89,13 @@ int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* 
     if (run_mlp(ctx)) return -1;
     if (wf) {
         if (run_basis(ctx, 1)) return -1;
-        if (timed(ctx, "force_gather", [&] {
-                k_force_gather<<<cdiv((long long)N * GL, 256), 256, 0, ctx->stream>>>(N, ctx->d_nb_off.p, ctx->d_nb_rev.p, ctx->d_nb_d.p,
-                                                                       ctx->d_pair_g.p, ctx->d_F.p, nullptr);
-            }))
-            return -1;
-    }
-    if (timed(ctx, "struct_reduce", [&] {
-            k_struct_reduce<<<S, RED_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p, ctx->d_e_atom.p,
-                                                                nullptr, ctx->d_E.p, nullptr);
-        }))
+        if (run_gather(ctx)) return -1;
+    } else if (timed(ctx, "struct_reduce", [&] {
+                   k_struct_reduce<<<S, RED_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p,
+                                                                       ctx->d_e_atom.p, ctx->d_E.p);
+               })) {
         return -1;
+    }
     const ace_loss_desc L = *Ld;
     if (timed(ctx, "loss", [&] {
             k_loss<<<S, LOSS_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p, ctx->d_E.p, T.d_Eref.p,

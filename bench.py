@@ -169,7 +169,7 @@ def main():
         if events:
             sfx = "_f64" if prec == PREC_FP64_BASIS else ""
             for k in ("nb_bin", "nb_search", "scan", "nb_compact", "basis_symm" + sfx,
-                      "mlp_energy_grad", "basis_adjoint" + sfx, "force_gather", "struct_reduce"):
+                      "mlp_energy_grad", "basis_adjoint" + sfx, "force_gather"):
                 kern[k.replace(sfx, "") if sfx else k] = dev.profile_read(k)
             dev.profile(False)
         return (t1 - t0) / steps * 1e3, kern

@@ -130,7 +130,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-samples", type=int, default=2)
+    ap.add_argument("--cpu-samples", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -195,10 +195,12 @@ def main():
     avg = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kern.items()}
     dom = max(("basis_symm", "basis_adjoint"), key=lambda k: avg[k])
     achieved = flops[dom] / (avg[dom] * 1e-3) / 1e12
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"{dom}_traffic.json")
+    traffic = None   # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary, when present
+    tf = os.path.join(ROOT, "profiles", f"{dom}_c2_traffic.txt")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("bytes_per_launch")
+        for line in open(tf):
+            if line.startswith("bytes_per_launch:"):
+                traffic = int(float(line.split(":")[1]))
 
     # plain unbatched FP64 GPU port (the 5x denominator), same step shape
     for _ in range(3):

@@ -101,6 +101,42 @@ __device__ inline float fast_exp(float x) {   // e^x through the hardware exp2
     return __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
 }
 
+// row stride of a network activation tile and of a padded weight block: width rounded up to 16 (four
+// matrix-core k-steps) plus 1 (odd, so rows fall in distinct LDS banks)
+__host__ __device__ inline int mlp_ld(int w) { return ((w + 15) & ~15) + 1; }
+
+// activation value and derivative of the hidden layers (NN::ANN af_* of ml/nn.cpp:149-240), FP32
+__device__ inline void act_fd(int act, float c, float z, float& a, float& d) {
+    switch (act) {
+        case ACE_ACT_TANH: {
+            const float t = tanhf(c * z);
+            a = t;
+            d = c * (1.f - t * t);
+        } break;
+        case ACE_ACT_SIGMOID: {
+            const float e = fast_exp(-c * fabsf(z));
+            const float f = 1.f / (1.f + e);
+            a = (z >= 0.f) ? f : e * f;
+            d = c * e * f * f;
+        } break;
+        case ACE_ACT_SWISH: {
+            const float zz = c * z;
+            const float e = fast_exp(-fabsf(zz));
+            const float den = 1.f / (1.f + e);
+            if (zz > 0.f) { a = z * den; d = (1.f + e * (1.f + zz)) * den * den; }
+            else { a = z * e * den; d = e * (1.f + zz + e) * den * den; }
+        } break;
+        case ACE_ACT_SOFTPLUS: {
+            const float e = fast_exp(-c * fabsf(z));
+            const float den = 1.f / (e + 1.f);
+            const float l = (log1pf(e) - 0.693147180559945f) / c;
+            a = (z >= 0.f) ? z + l : l;
+            d = (z >= 0.f) ? den : e * den;
+        } break;
+        default: a = z; d = 1.f;
+    }
+}
+
 // orders LDS traffic between the lanes of one wave (no cross-wave barrier)
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -575,7 +575,7 @@ DevBasisParams basis_params(ace_ctx* ctx) {
 
 typedef void (*BasisFn)(int, const DevModel*, DevBasisParams, const int*, const int*, const int*, const float4*,
                         const int*, const double*, const int*, const StrucInfo*, const int*, int, float*,
-                        const float*, float4*, int*, int, const double*, int, int, int);
+                        const float*, float4*, int*, int, const double*, int, int, const float*, double*);
 
 template <class T, int NG, int NZ>
 void pick_basis(BasisFn* f, bool gen) {
@@ -583,10 +583,12 @@ void pick_basis(BasisFn* f, bool gen) {
         f[0] = k_basis<T, NG, NZ, 0, 1>;
         f[1] = k_basis<T, NG, NZ, 1, 1>;
         f[2] = k_basis<T, NG, NZ, 2, 1>;
+        f[3] = k_basis<T, NG, NZ, 3, 1>;
     } else {
         f[0] = k_basis<T, NG, NZ, 0, 0>;
         f[1] = k_basis<T, NG, NZ, 1, 0>;
         f[2] = k_basis<T, NG, NZ, 2, 0>;
+        f[3] = k_basis<T, NG, NZ, 3, 0>;
     }
 }
 
@@ -638,26 +640,49 @@ int nb_verify(ace_ctx* ctx) {
 }
 
 // mode 0: G -> out; mode 1: per-pair gradients from dE/dG; mode 2: tangent rows along eps -> out
+// floats of the fused kernel's network area: staged parameters, activations and derivatives, partials
+int net_lds_floats(const ace_ctx* ctx) {
+    int m = 0;
+    for (int I = 0; I < ctx->nt; ++I) {
+        const DevSpecies& d = ctx->hdm.sp[I];
+        int n = d.wp_n + BT;
+        for (int l = 0; l < d.L; ++l) n += 2 * ((d.width[l + 1] + 3) & ~3);
+        m = std::max(m, n);
+    }
+    return m;
+}
+
+int basis_bytes(ace_ctx* ctx, int mode) {
+    const int nbm = nb_slots(ctx), netf = mode == 3 ? net_lds_floats(ctx) : 0;
+    return ctx->basis_f64 ? basis_lds_bytes<double>(nbm, ctx->ngk, ctx->kmax, mode, netf)
+                          : basis_lds_bytes<float>(nbm, ctx->ngk, ctx->kmax, mode, netf);
+}
+
+// the fused symmetry-function + network + adjoint kernel is used when its LDS keeps two workgroups per CU
+bool use_fused(ace_ctx* ctx) { return basis_bytes(ctx, 3) <= 64 * 1024; }
+
+// mode 0: G -> out; mode 1: per-pair gradients from dE/dG; mode 2: tangent rows along eps -> out;
+// mode 3: per-atom energies and per-pair gradients in one pass
 int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = nullptr) {
-    BasisFn fns[3];
+    BasisFn fns[4];
     const bool f64 = ctx->basis_f64;
     basis_fns(ctx, f64, fns);
     const int nbm = nb_slots(ctx);
-    const int bytes = f64 ? basis_lds_bytes<double>(nbm, ctx->ngk, ctx->kmax, mode)
-                          : basis_lds_bytes<float>(nbm, ctx->ngk, ctx->kmax, mode);
+    const int bytes = basis_bytes(ctx, mode);
     if (bytes > 160 * 1024) return fail(ctx, "basis: neighbour list too long for LDS");
     BasisFn f = fns[mode];
     if (bytes > 64 * 1024) HC(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     DevBasisParams P = basis_params(ctx);
     int* err = ctx->d_scal.p + 3;
     float* rows = out ? out : ctx->d_G.p;
-    static const char* names[2][3] = {{"basis_symm", "basis_adjoint", "basis_tangent"},
-                                      {"basis_symm_f64", "basis_adjoint_f64", "basis_tangent_f64"}};
+    static const char* names[2][4] = {{"basis_symm", "basis_adjoint", "basis_tangent", "basis_fused"},
+                                      {"basis_symm_f64", "basis_adjoint_f64", "basis_tangent_f64", "basis_fused_f64"}};
     return timed(ctx, names[f64][mode], [&] {
         f<<<ctx->N, BT, bytes, ctx->stream>>>(ctx->N, ctx->d_model.p, P, ctx->d_type.p, ctx->d_nb_off.p, ctx->d_nb_j.p,
                                               ctx->d_nb_d.p, ctx->d_nb_img.p, ctx->d_pos.p, ctx->d_struc_of.p,
                                               ctx->d_sinfo.p, ctx->d_row_of.p, ctx->ldg, rows, ctx->d_dEdG.p,
-                                              ctx->d_pair_g.p, err, nbm, eps, ctx->nt, ctx->ngmax, ctx->kmax);
+                                              ctx->d_pair_g.p, err, nbm, eps, ctx->nt, ctx->kmax, ctx->d_wpad.p,
+                                              ctx->d_e_atom.p);
     });
 }
 
@@ -1105,9 +1130,13 @@ int ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, d
         return fail(ctx, "ace_energy_forces: precision mode not available");
     ctx->basis_f64 = precision == ACE_PREC_FP64_BASIS;
     if (require_nb(ctx)) return -1;
-    if (run_basis(ctx, 0)) return -1;
-    if (run_mlp(ctx)) return -1;
-    if (run_basis(ctx, 1)) return -1;
+    if (use_fused(ctx)) {
+        if (run_basis(ctx, 3)) return -1;
+    } else {
+        if (run_basis(ctx, 0)) return -1;
+        if (run_mlp(ctx)) return -1;
+        if (run_basis(ctx, 1)) return -1;
+    }
     const int N = ctx->N;
     if (run_gather(ctx)) return -1;
     if (E_struc) HC(hipMemcpyAsync(E_struc, ctx->d_E.p, ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

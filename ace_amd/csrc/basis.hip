@@ -26,18 +26,20 @@
 // Modes: 0 = network input rows inpw*(G + inpb); 1 = adjoint (per-pair dE/dd from dE/dG rows);
 // 2 = tangent rows v = sum_a eps_a . dG/dr_a.
 // Dynamic LDS per workgroup (nbm = neighbour slots, kp = kmax rounded up to 4), in units of T:
-//   u[4 nbm], te[4 nbm] (mode 2), E[nbm ng], Ed[nbm ng] (modes 1,2), fR[nbm], gR[nbm], ri[nbm],
-//   row[kp], red[BW 64], rp[2 kp] (radial eta, rs of the species), sc[2 kp] (mode 0: inpw, inpb);
-//   then p[nbm] and ci[kp] (int).  Mode 1 puts g[BW 3 nbm] (double) in front.
+//   u[4 nbm], te[4 nbm] (mode 2), E[nbm ng], Ed[nbm ng] (modes 1-3), fR[nbm], gR[nbm], ri[nbm],
+//   row[kp], red[BW 64], rp[2 kp] (radial eta, rs of the species), sc[2 kp] (modes 0, 3: inpw, inpb);
+//   then p[nbm] and ci[kp] (int).  Modes 1 and 3 put g[BW 3 nbm] (double) in front; mode 3 appends the
+//   network area (netf floats: staged weights, activations, partial sums), 16-byte aligned.
 template <class T>
-__host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int mode) {
+__host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int mode, int netf = 0) {
     const int kp = (kmax + 3) & ~3;
     int n = 4 * nbm + ng * nbm + 3 * nbm + kp + BW * 64 + 2 * kp;
-    if (mode == 0) n += 2 * kp;
+    if (mode == 0 || mode == 3) n += 2 * kp;
     if (mode == 2) n += 4 * nbm;
     if (mode >= 1) n += ng * nbm;
-    const int gb = (mode == 1) ? 8 * BW * 3 * nbm : 0;
-    return (gb + (int)sizeof(T) * n + 4 * (nbm + kp) + 15) & ~15;
+    const int gb = (mode == 1 || mode == 3) ? 8 * BW * 3 * nbm : 0;
+    const int base = (gb + (int)sizeof(T) * n + 4 * (nbm + kp) + 15) & ~15;
+    return base + (mode == 3 ? 4 * netf : 0);
 }
 
 template <class T>
@@ -53,16 +55,17 @@ struct BasisLds {
     T* row;
     T* red;    // [BW][64]
     T* rp;     // radial eta [nR], rs [nR] of the centre species
-    T* sc;     // network input scaling inpw [K], inpb [K] (mode 0)
+    T* sc;     // network input scaling inpw [K], inpb [K] (modes 0, 3)
     int* p;
     int* ci;   // canonical slot of each angular function of the species
+    float* nw; // mode 3: the species' network parameters (matrix-core layout), then activations
 };
 
 template <class T>
 __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, int mode) {
     BasisLds<T> L;
     L.g = (double*)base;
-    T* c = (T*)((char*)base + ((mode == 1) ? 8 * BW * 3 * nbm : 0));
+    T* c = (T*)((char*)base + ((mode == 1 || mode == 3) ? 8 * BW * 3 * nbm : 0));
     L.u = c; c += 4 * nbm;
     L.te = c; if (mode == 2) c += 4 * nbm;
     L.E = c; c += ng * nbm;
@@ -74,9 +77,11 @@ __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, i
     L.red = c; c += BW * 64;
     const int kp = (kmax + 3) & ~3;
     L.rp = c; c += 2 * kp;
-    L.sc = c; if (mode == 0) c += 2 * kp;
+    L.sc = c; if (mode == 0 || mode == 3) c += 2 * kp;
     L.p = (int*)c;
     L.ci = L.p + nbm;
+    const long long end = ((long long)((char*)(L.ci + kp) - (char*)base) + 15) & ~15ll;
+    L.nw = (float*)((char*)base + end);
     return L;
 }
 
@@ -272,181 +277,17 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
     }
 }
 
-// GEN = 0 compiles the kernel for models whose angular blocks are all canonical (no generic path).
-// NG: eta groups per species (1, 2 or 4, at least the model's largest count); NZ: canonical power levels
-// (4 or 7, at least the model's largest log2(zeta) + 1).
-template <class T, int NG, int NZ, int MODE, int GEN>
-__global__ __launch_bounds__(BT) void k_basis(
-    int natoms, const DevModel* __restrict__ M, DevBasisParams P, const int* __restrict__ type,
-    const int* __restrict__ nb_off, const int* __restrict__ nb_j, const float4* __restrict__ nb_d,
-    const int* __restrict__ nb_img, const double* __restrict__ pos, const int* __restrict__ struc_of,
-    const StrucInfo* __restrict__ sinfo, const int* __restrict__ row_of, int ldg, float* __restrict__ G,
-    const float* __restrict__ dEdG, float4* __restrict__ pair_g, int* __restrict__ err, int nbm,
-    const double* __restrict__ eps, int nt, int ngmax, int kmax) {
-    constexpr int ADJ = (MODE == 1);
-    constexpr int TAN = (MODE == 2);
-    constexpr bool F64 = sizeof(T) == 8;
+// Radial and angular sums of one atom (slots, tables and row already in LDS).  PM 0: G into row;
+// PM 1: per-slot dE/dd into the per-wave accumulators g from the dE/dG weights in row; PM 2: tangent
+// rows into row.  Ends with a workgroup barrier.
+template <class T, int NG, int NZ, int PM, int GEN>
+__device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, const DevBasisParams& P,
+                                  const int* s_cnt, int nt, int nbm, int tid, int wv, int lane) {
+    constexpr int MODE = PM;
+    constexpr int ADJ = (PM == 1);
+    constexpr int TAN = (PM == 2);
     constexpr int MAXA = 16;   // functions per register chunk of the generic path
-    extern __shared__ double4 basis_smem[];
-    __shared__ int s_cnt[ACE_MAXT + 1];
-    __shared__ DevSpecies s_sp;
-    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-    const int i = blockIdx.x;
-    if (i >= natoms) return;
-    const int I = type[i];
-    const DevSpecies& gsp = M->sp[I];
-    const BasisLds<T> L = basis_lds<T>(basis_smem, nbm, NG, kmax, MODE);
-    const int o0 = nb_off[i], o1 = nb_off[i + 1];
-    const float rcut = gsp.rc_max;
-    const int row = row_of[i];
-
-    // --- waves 1-3 stage the species tables in LDS while wave 0 buckets the neighbours --------------
-    if (wv == 1) {
-        constexpr int n16 = (int)(sizeof(DevSpecies) / 16);
-        static_assert(sizeof(DevSpecies) % 16 == 0, "DevSpecies is copied in 16-byte words");
-        const int4* src = (const int4*)&gsp;
-        int4* dst = (int4*)&s_sp;
-        const int4 v0 = src[lane];
-        const int4 v1 = (64 + lane < n16) ? src[64 + lane] : v0;
-        dst[lane] = v0;
-        if (64 + lane < n16) dst[64 + lane] = v1;
-        static_assert(n16 >= 64 && n16 <= 128, "DevSpecies copy covers 64 to 128 words");
-    } else if (wv == 2) {
-        const int nR = gsp.nR, r0 = gsp.r_poff[0];
-        for (int q = lane; q < nR; q += 64) {
-            L.rp[q] = prm<T>(P.r_eta, P.r_eta64, r0 + q);
-            L.rp[nR + q] = prm<T>(P.r_rs, P.r_rs64, r0 + q);
-        }
-        if (ADJ)
-            for (int k = lane; k < gsp.K; k += 64) L.row[k] = (T)dEdG[(long long)row * ldg + k];
-    } else if (wv == 3) {
-        const int nA = gsp.K - gsp.nR, a0 = gsp.a_poff[0];
-        for (int q = lane; q < nA; q += 64) L.ci[q] = P.a_cidx[a0 + q];
-        if (MODE == 0)
-            for (int k = lane; k < gsp.K; k += 64) {
-                L.sc[k] = prm<T>(P.inpw, P.inpw64, gsp.in_off + k);
-                L.sc[gsp.K + k] = prm<T>(P.inpb, P.inpb64, gsp.in_off + k);
-            }
-    }
-
-    // --- wave 0: bucket the neighbours by species (ballot order keeps the list order) -------------
-    if (wv == 0) {
-        int cnt[ACE_MAXT];
-#pragma unroll
-        for (int J = 0; J < ACE_MAXT; ++J) cnt[J] = 0;
-        for (int cb = o0; cb < o1; cb += 64) {
-            const int p = cb + lane;
-            int Jl = -1;
-            if (p < o1 && nb_d[p].w < rcut) Jl = type[nb_j[p]];
-#pragma unroll
-            for (int J = 0; J < ACE_MAXT; ++J)
-                if (J < nt) cnt[J] += __popcll(__ballot(Jl == J));
-        }
-        int fill[ACE_MAXT];
-        int acc = 0;
-#pragma unroll
-        for (int J = 0; J < ACE_MAXT; ++J) {
-            fill[J] = acc;
-            acc += (J < nt) ? cnt[J] : 0;
-        }
-        if (lane == 0) {
-#pragma unroll
-            for (int J = 0; J <= ACE_MAXT; ++J)
-                if (J <= nt) s_cnt[J] = (J < nt) ? fill[J] : acc;
-        }
-        if (acc <= nbm) {
-            for (int cb = o0; cb < o1; cb += 64) {
-                const int p = cb + lane;
-                int Jl = -1;
-                if (p < o1) {
-                    if (nb_d[p].w < rcut) Jl = type[nb_j[p]];
-                    else if (ADJ) pair_g[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-                }
-#pragma unroll
-                for (int J = 0; J < ACE_MAXT; ++J) {
-                    if (J < nt) {
-                        const unsigned long long m = __ballot(Jl == J);
-                        if (Jl == J) L.p[fill[J] + __popcll(m & lane_mask_lt())] = p;
-                        fill[J] += __popcll(m);
-                    }
-                }
-            }
-        }
-    }
-    __syncthreads();
-    const DevSpecies& sp = s_sp;
     const int ng = sp.ng;
-    const int ntot = s_cnt[nt];
-    if (ntot > nbm) {
-        if (tid == 0) atomicAdd(err, 1);
-        return;
-    }
-
-    // --- per-slot quantities ---------------------------------------------------------------------
-    const StrucInfo* S = F64 ? &sinfo[struc_of[i]] : nullptr;
-    for (int q = tid; q < ntot; q += BT) {
-        const int p = L.p[q];
-        int J = 0;
-#pragma unroll
-        for (int jj = 1; jj < ACE_MAXT; ++jj)
-            if (jj < nt && q >= s_cnt[jj]) J = jj;
-        T dx, dy, dz;
-        if constexpr (F64) {      // r_ij = posn(i) - (posn(j) + R img) in FP64 (nnp.cpp:363)
-            int a, b, c;
-            img_unpack(nb_img[p], a, b, c);
-            const int j = nb_j[p];
-            dx = pos[3 * i] - (pos[3 * j] + (S->R[0] * a + S->R[3] * b + S->R[6] * c));
-            dy = pos[3 * i + 1] - (pos[3 * j + 1] + (S->R[1] * a + S->R[4] * b + S->R[7] * c));
-            dz = pos[3 * i + 2] - (pos[3 * j + 2] + (S->R[2] * a + S->R[5] * b + S->R[8] * c));
-        } else {
-            const float4 d = nb_d[p];
-            dx = d.x; dy = d.y; dz = d.z;
-        }
-        const T r = sqrt(dx * dx + dy * dy + dz * dz), ir = T(1) / r;
-        L.u[4 * q] = dx * ir;
-        L.u[4 * q + 1] = dy * ir;
-        L.u[4 * q + 2] = dz * ir;
-        L.u[4 * q + 3] = r;
-        L.ri[q] = ir;
-        T fc, gc;
-        t_cutoff<T>(sp.r_cut[J], F64 ? (T)sp.r_rc64[J] : (T)sp.r_rc[J], r, fc, gc);
-        L.fR[q] = fc;
-        L.gR[q] = gc;
-        t_cutoff<T>(sp.a_cut, F64 ? (T)sp.a_rc64 : (T)sp.a_rc, r, fc, gc);
-#pragma unroll
-        for (int g = 0; g < NG; ++g) {
-            T ev = T(0), edv = T(0);
-            if (g < ng) {
-                const T ie = F64 ? (T)sp.ietar2_64[g] : (T)sp.ietar2[g];
-                const T e = t_exp<T>(-r * r * ie);
-                ev = fc * e;
-                edv = (gc - T(2) * r * ie * fc) * e;
-            }
-            L.E[q * NG + g] = ev;
-            if (ADJ || TAN) L.Ed[q * NG + g] = edv;
-        }
-        if (ADJ) {
-            for (int w = 0; w < BW; ++w) {
-                L.g[w * 3 * nbm + 3 * q] = 0.0;
-                L.g[w * 3 * nbm + 3 * q + 1] = 0.0;
-                L.g[w * 3 * nbm + 3 * q + 2] = 0.0;
-            }
-        }
-        if (TAN) {
-            const int jj = nb_j[p];
-            const T ex = (T)(eps[3 * i] - eps[3 * jj]);
-            const T ey = (T)(eps[3 * i + 1] - eps[3 * jj + 1]);
-            const T ez = (T)(eps[3 * i + 2] - eps[3 * jj + 2]);
-            L.te[4 * q] = ex;
-            L.te[4 * q + 1] = ey;
-            L.te[4 * q + 2] = ez;
-            L.te[4 * q + 3] = (dx * ex + dy * ey + dz * ez) * ir;
-        }
-    }
-    if (!ADJ)
-        for (int k = tid; k < sp.K; k += BT) L.row[k] = T(0);
-    __syncthreads();
-
     // --- radial blocks: lane = function, lane group = neighbour subset ---------------------------
     for (int J = 0; J < nt; ++J) {
         const int bJ = s_cnt[J], nJ = s_cnt[J + 1] - s_cnt[J];
@@ -655,6 +496,293 @@ __global__ __launch_bounds__(BT) void k_basis(
         }
     }
     __syncthreads();
+}
+
+// The atom's network inside the workgroup (energy and dE/dG, FP32; ANN::fpbp and DODZ row 0,
+// ml/nn.cpp:1402-1420, 1836-1862).  On entry row holds G; on exit e_out holds the atomic energy (with E0)
+// and row holds dE/dG (the input scaling inpw folded in).  The species' parameters are in L.nw in the
+// matrix-core layout (W_l as [k][o], row stride mlp_ld(w_{l+1}), biases at bl_off); after them: per layer
+// the activations and derivatives, then BT partial sums.  Each output is split over up to BT / outputs
+// threads, whose partial sums are added in a fixed order.
+template <class T>
+__device__ inline void block_network(const BasisLds<T>& L, const DevSpecies& sp, int tid, double* e_out) {
+    const int K = sp.K, NL = sp.L;
+    const float* W = L.nw;
+    float* act = L.nw + sp.wp_n;
+    int tot = 0;
+    for (int l = 0; l < NL; ++l) tot += 2 * ((sp.width[l + 1] + 3) & ~3);
+    float* red = act + tot;
+    for (int k = tid; k < K; k += BT) L.row[k] = L.sc[k] * (L.row[k] + L.sc[K + k]);   // inpw (G + inpb)
+    __syncthreads();
+    // forward
+    int ain = -1, aoff = 0;
+    for (int l = 0; l < NL; ++l) {
+        const int ni = sp.width[l], no = sp.width[l + 1], ldw = mlp_ld(no), wp = (no + 3) & ~3;
+        const float* Wl = W + sp.wl_off[l];
+        const float* bl = W + sp.bl_off[l];
+        float* A = act + aoff;
+        float* D = A + wp;
+        const int fa = (l == NL - 1) ? ACE_ACT_LINEAR : sp.act;
+        for (int o0 = 0; o0 < no; o0 += BT) {
+            const int nout = min(BT, no - o0);
+            const int S = max(1, min(BT / nout, (ni + 3) / 4));
+            const int chunk = (ni + S - 1) / S;
+            const int oo = tid % nout, sl = tid / nout;
+            if (sl < S) {
+                const int o = o0 + oo, k0 = sl * chunk, k1 = min(ni, k0 + chunk);
+                float part = 0.f;
+                if (ain < 0)
+                    for (int k = k0; k < k1; ++k) part += Wl[k * ldw + o] * (float)L.row[k];
+                else
+                    for (int k = k0; k < k1; ++k) part += Wl[k * ldw + o] * act[ain + k];
+                red[sl * nout + oo] = part;
+            }
+            __syncthreads();
+            if (tid < nout) {
+                float z = bl[o0 + tid];
+                for (int q = 0; q < S; ++q) z += red[q * nout + tid];
+                float a, d;
+                act_fd(fa, sp.c, z, a, d);
+                A[o0 + tid] = a;
+                D[o0 + tid] = d;
+            }
+            __syncthreads();
+        }
+        ain = aoff;
+        aoff += 2 * wp;
+    }
+    // energy (the last layer has width 1) and the output delta
+    {
+        float* A = act + ain;
+        const int wp = (sp.width[NL] + 3) & ~3;
+        if (tid == 0) {
+            *e_out = (double)(sp.outb + sp.outw * A[0]) + sp.E0;
+            A[0] = sp.outw * A[wp];   // delta = outw * f'(z), f linear
+        }
+        __syncthreads();
+    }
+    // backward: delta_{l-1}[k] = (sum_o W_l[k][o] delta_l[o]) f'(z_{l-1})[k]; layer 0 gives dE/dG
+    int dl = ain;   // offset of delta_l (overwrites the activations of layer l)
+    for (int l = NL - 1; l >= 0; --l) {
+        const int ni = sp.width[l], no = sp.width[l + 1], ldw = mlp_ld(no);
+        const float* Wl = W + sp.wl_off[l];
+        const int wpi = (ni + 3) & ~3;
+        const int dm = (l > 0) ? dl - 2 * wpi : 0;   // offset of layer l-1's activations
+        for (int k0 = 0; k0 < ni; k0 += BT) {
+            const int nout = min(BT, ni - k0);
+            const int S = max(1, min(BT / nout, (no + 3) / 4));
+            const int chunk = (no + S - 1) / S;
+            const int kk = tid % nout, sl = tid / nout;
+            if (sl < S) {
+                const int k = k0 + kk, q0 = sl * chunk, q1 = min(no, q0 + chunk);
+                float part = 0.f;
+                for (int o = q0; o < q1; ++o) part += Wl[k * ldw + o] * act[dl + o];
+                red[sl * nout + kk] = part;
+            }
+            __syncthreads();
+            if (tid < nout) {
+                float v = 0.f;
+                for (int q = 0; q < S; ++q) v += red[q * nout + tid];
+                const int k = k0 + tid;
+                if (l > 0) act[dm + k] = v * act[dm + wpi + k];
+                else L.row[k] = (T)v * L.sc[k];
+            }
+            __syncthreads();
+        }
+        dl = dm;
+    }
+}
+
+// GEN = 0 compiles the kernel for models whose angular blocks are all canonical (no generic path).
+// NG: eta groups per species (1, 2 or 4, at least the model's largest count); NZ: canonical power levels
+// (4 or 7, at least the model's largest log2(zeta) + 1).
+template <class T, int NG, int NZ, int MODE, int GEN>
+__global__ __launch_bounds__(BT) void k_basis(
+    int natoms, const DevModel* __restrict__ M, DevBasisParams P, const int* __restrict__ type,
+    const int* __restrict__ nb_off, const int* __restrict__ nb_j, const float4* __restrict__ nb_d,
+    const int* __restrict__ nb_img, const double* __restrict__ pos, const int* __restrict__ struc_of,
+    const StrucInfo* __restrict__ sinfo, const int* __restrict__ row_of, int ldg, float* __restrict__ G,
+    const float* __restrict__ dEdG, float4* __restrict__ pair_g, int* __restrict__ err, int nbm,
+    const double* __restrict__ eps, int nt, int kmax, const float* __restrict__ wpad, double* __restrict__ e_atom) {
+    constexpr int ADJ = (MODE == 1);
+    constexpr int TAN = (MODE == 2);
+    constexpr int FUSED = (MODE == 3);        // symmetry functions, network and adjoint in one pass
+    constexpr int GACC = (ADJ || FUSED);      // per-slot gradient accumulators
+    constexpr bool F64 = sizeof(T) == 8;
+    extern __shared__ double4 basis_smem[];
+    __shared__ int s_cnt[ACE_MAXT + 1];
+    __shared__ DevSpecies s_sp;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int i = blockIdx.x;
+    if (i >= natoms) return;
+    const int I = type[i];
+    const DevSpecies& gsp = M->sp[I];
+    const BasisLds<T> L = basis_lds<T>(basis_smem, nbm, NG, kmax, MODE);
+    const int o0 = nb_off[i], o1 = nb_off[i + 1];
+    const float rcut = gsp.rc_max;
+    const int row = row_of[i];
+
+    // --- waves 1-3 stage the species tables in LDS while wave 0 buckets the neighbours --------------
+    if (wv == 1) {
+        constexpr int n16 = (int)(sizeof(DevSpecies) / 16);
+        static_assert(sizeof(DevSpecies) % 16 == 0, "DevSpecies is copied in 16-byte words");
+        const int4* src = (const int4*)&gsp;
+        int4* dst = (int4*)&s_sp;
+        const int4 v0 = src[lane];
+        const int4 v1 = (64 + lane < n16) ? src[64 + lane] : v0;
+        dst[lane] = v0;
+        if (64 + lane < n16) dst[64 + lane] = v1;
+        static_assert(n16 >= 64 && n16 <= 128, "DevSpecies copy covers 64 to 128 words");
+    } else if (wv == 2) {
+        const int nR = gsp.nR, r0 = gsp.r_poff[0];
+        for (int q = lane; q < nR; q += 64) {
+            L.rp[q] = prm<T>(P.r_eta, P.r_eta64, r0 + q);
+            L.rp[nR + q] = prm<T>(P.r_rs, P.r_rs64, r0 + q);
+        }
+        if (ADJ)
+            for (int k = lane; k < gsp.K; k += 64) L.row[k] = (T)dEdG[(long long)row * ldg + k];
+    } else if (wv == 3) {
+        const int nA = gsp.K - gsp.nR, a0 = gsp.a_poff[0];
+        for (int q = lane; q < nA; q += 64) L.ci[q] = P.a_cidx[a0 + q];
+        if (MODE == 0 || FUSED)
+            for (int k = lane; k < gsp.K; k += 64) {
+                L.sc[k] = prm<T>(P.inpw, P.inpw64, gsp.in_off + k);
+                L.sc[gsp.K + k] = prm<T>(P.inpb, P.inpb64, gsp.in_off + k);
+            }
+    }
+    if (FUSED && wv > 0) {
+        // the network parameters of the species, staged by waves 1-3 (16-byte copies)
+        const float4* src = (const float4*)(wpad + gsp.wp_off);
+        float4* dst = (float4*)L.nw;
+        const int n4 = gsp.wp_n >> 2;
+        for (int e = tid - 64; e < n4; e += BT - 64) dst[e] = src[e];
+    }
+
+    // --- wave 0: bucket the neighbours by species (ballot order keeps the list order) -------------
+    if (wv == 0) {
+        int cnt[ACE_MAXT];
+#pragma unroll
+        for (int J = 0; J < ACE_MAXT; ++J) cnt[J] = 0;
+        for (int cb = o0; cb < o1; cb += 64) {
+            const int p = cb + lane;
+            int Jl = -1;
+            if (p < o1 && nb_d[p].w < rcut) Jl = type[nb_j[p]];
+#pragma unroll
+            for (int J = 0; J < ACE_MAXT; ++J)
+                if (J < nt) cnt[J] += __popcll(__ballot(Jl == J));
+        }
+        int fill[ACE_MAXT];
+        int acc = 0;
+#pragma unroll
+        for (int J = 0; J < ACE_MAXT; ++J) {
+            fill[J] = acc;
+            acc += (J < nt) ? cnt[J] : 0;
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int J = 0; J <= ACE_MAXT; ++J)
+                if (J <= nt) s_cnt[J] = (J < nt) ? fill[J] : acc;
+        }
+        if (acc <= nbm) {
+            for (int cb = o0; cb < o1; cb += 64) {
+                const int p = cb + lane;
+                int Jl = -1;
+                if (p < o1) {
+                    if (nb_d[p].w < rcut) Jl = type[nb_j[p]];
+                    else if (GACC) pair_g[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+#pragma unroll
+                for (int J = 0; J < ACE_MAXT; ++J) {
+                    if (J < nt) {
+                        const unsigned long long m = __ballot(Jl == J);
+                        if (Jl == J) L.p[fill[J] + __popcll(m & lane_mask_lt())] = p;
+                        fill[J] += __popcll(m);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const DevSpecies& sp = s_sp;
+    const int ng = sp.ng;
+    const int ntot = s_cnt[nt];
+    if (ntot > nbm) {
+        if (tid == 0) atomicAdd(err, 1);
+        return;
+    }
+
+    // --- per-slot quantities ---------------------------------------------------------------------
+    const StrucInfo* S = F64 ? &sinfo[struc_of[i]] : nullptr;
+    for (int q = tid; q < ntot; q += BT) {
+        const int p = L.p[q];
+        int J = 0;
+#pragma unroll
+        for (int jj = 1; jj < ACE_MAXT; ++jj)
+            if (jj < nt && q >= s_cnt[jj]) J = jj;
+        T dx, dy, dz;
+        if constexpr (F64) {      // r_ij = posn(i) - (posn(j) + R img) in FP64 (nnp.cpp:363)
+            int a, b, c;
+            img_unpack(nb_img[p], a, b, c);
+            const int j = nb_j[p];
+            dx = pos[3 * i] - (pos[3 * j] + (S->R[0] * a + S->R[3] * b + S->R[6] * c));
+            dy = pos[3 * i + 1] - (pos[3 * j + 1] + (S->R[1] * a + S->R[4] * b + S->R[7] * c));
+            dz = pos[3 * i + 2] - (pos[3 * j + 2] + (S->R[2] * a + S->R[5] * b + S->R[8] * c));
+        } else {
+            const float4 d = nb_d[p];
+            dx = d.x; dy = d.y; dz = d.z;
+        }
+        const T r = sqrt(dx * dx + dy * dy + dz * dz), ir = T(1) / r;
+        L.u[4 * q] = dx * ir;
+        L.u[4 * q + 1] = dy * ir;
+        L.u[4 * q + 2] = dz * ir;
+        L.u[4 * q + 3] = r;
+        L.ri[q] = ir;
+        T fc, gc;
+        t_cutoff<T>(sp.r_cut[J], F64 ? (T)sp.r_rc64[J] : (T)sp.r_rc[J], r, fc, gc);
+        L.fR[q] = fc;
+        L.gR[q] = gc;
+        t_cutoff<T>(sp.a_cut, F64 ? (T)sp.a_rc64 : (T)sp.a_rc, r, fc, gc);
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            T ev = T(0), edv = T(0);
+            if (g < ng) {
+                const T ie = F64 ? (T)sp.ietar2_64[g] : (T)sp.ietar2[g];
+                const T e = t_exp<T>(-r * r * ie);
+                ev = fc * e;
+                edv = (gc - T(2) * r * ie * fc) * e;
+            }
+            L.E[q * NG + g] = ev;
+            if (MODE >= 1) L.Ed[q * NG + g] = edv;
+        }
+        if (GACC) {
+            for (int w = 0; w < BW; ++w) {
+                L.g[w * 3 * nbm + 3 * q] = 0.0;
+                L.g[w * 3 * nbm + 3 * q + 1] = 0.0;
+                L.g[w * 3 * nbm + 3 * q + 2] = 0.0;
+            }
+        }
+        if (TAN) {
+            const int jj = nb_j[p];
+            const T ex = (T)(eps[3 * i] - eps[3 * jj]);
+            const T ey = (T)(eps[3 * i + 1] - eps[3 * jj + 1]);
+            const T ez = (T)(eps[3 * i + 2] - eps[3 * jj + 2]);
+            L.te[4 * q] = ex;
+            L.te[4 * q + 1] = ey;
+            L.te[4 * q + 2] = ez;
+            L.te[4 * q + 3] = (dx * ex + dy * ey + dz * ez) * ir;
+        }
+    }
+    if (!ADJ)
+        for (int k = tid; k < sp.K; k += BT) L.row[k] = T(0);
+    __syncthreads();
+
+    if (FUSED) {
+        basis_sums<T, NG, NZ, 0, GEN>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
+        block_network<T>(L, sp, tid, &e_atom[i]);
+        basis_sums<T, NG, NZ, 1, GEN>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
+    } else {
+        basis_sums<T, NG, NZ, MODE, GEN>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
+    }
     if (MODE == 0) {
         // network input row: the ANN's input scaling inpw*(G + inpb) (nn.cpp:1406) applied here, in T,
         // so the FP32 row keeps full relative precision after the mean is removed

@@ -9,37 +9,6 @@
 
 #define MLP_THREADS 256
 
-__device__ inline void act_fd(int act, float c, float z, float& a, float& d) {
-    switch (act) {
-        case ACE_ACT_TANH: {
-            const float t = tanhf(c * z);
-            a = t;
-            d = c * (1.f - t * t);
-        } break;
-        case ACE_ACT_SIGMOID: {
-            const float e = fast_exp(-c * fabsf(z));
-            const float f = 1.f / (1.f + e);
-            a = (z >= 0.f) ? f : e * f;
-            d = c * e * f * f;
-        } break;
-        case ACE_ACT_SWISH: {
-            const float zz = c * z;
-            const float e = fast_exp(-fabsf(zz));
-            const float den = 1.f / (1.f + e);
-            if (zz > 0.f) { a = z * den; d = (1.f + e * (1.f + zz)) * den * den; }
-            else { a = z * e * den; d = e * (1.f + zz + e) * den * den; }
-        } break;
-        case ACE_ACT_SOFTPLUS: {
-            const float e = fast_exp(-c * fabsf(z));
-            const float den = 1.f / (e + 1.f);
-            const float l = (log1pf(e) - 0.693147180559945f) / c;
-            a = (z >= 0.f) ? z + l : l;
-            d = (z >= 0.f) ? den : e * den;
-        } break;
-        default: a = z; d = 1.f;
-    }
-}
-
 // ---- matrix-core network pass -------------------------------------------------------------------
 // One wave per tile of 16 atoms of one species.  Every layer, forward and backward, is a chain of
 // v_mfma_f32_16x16x4f32 (D[16x16] += A[16x4] B[4x16], FP32 products and sums):
@@ -56,8 +25,6 @@ typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
 extern __shared__ float mlp_smem[];
 
-// row stride of an activation tile: width rounded up to 16 (four k-steps) plus 1
-__host__ __device__ inline int mlp_ld(int w) { return ((w + 15) & ~15) + 1; }
 
 // floats of LDS for one tile: X [16][ld(K)], then per layer A_l, D_l [16][ld(w_{l+1})]
 __host__ __device__ inline int mlp_lds_floats(const DevSpecies& sp) {

@@ -258,15 +258,25 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
         }
     }
     if (!ADJ) {
+        // wave reduce-scatter of the 2 NZ NG sums (padded to a power of two); the lanes holding sum f write
+        // it to its (sign, level, group) slot
+        constexpr int M0 = 2 * NZ * NG;
+        constexpr int LM = M0 <= 2 ? 1 : M0 <= 4 ? 2 : M0 <= 8 ? 3 : M0 <= 16 ? 4 : M0 <= 32 ? 5 : 6;
+        T v[1 << LM];
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll
             for (int k = 0; k < NZ; ++k)
 #pragma unroll
-                for (int g = 0; g < NG; ++g) {
-                    const T v = t_wave_sum63<T>(acc[s][k][g]);
-                    if (lane == 63) L.red[wv * 64 + (s * 7 + k) * NG + g] = v;
-                }
+                for (int g = 0; g < NG; ++g) v[(s * NZ + k) * NG + g] = acc[s][k][g];
+#pragma unroll
+        for (int f = M0; f < (1 << LM); ++f) v[f] = T(0);
+        const T tot = wave_reduce_scatter<LM, T>(v, lane);
+        const int f = lane >> (6 - LM);
+        if ((lane & ((1 << (6 - LM)) - 1)) == 0 && f < M0) {
+            const int g = f % NG, k = (f / NG) % NZ, s = f / (NG * NZ);
+            L.red[wv * 64 + (s * 7 + k) * NG + g] = tot;
+        }
         __syncthreads();
         for (int m = tid; m < nA; m += BT) {
             T v = T(0);
@@ -502,16 +512,20 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
 // ml/nn.cpp:1402-1420, 1836-1862).  On entry row holds G; on exit e_out holds the atomic energy (with E0)
 // and row holds dE/dG (the input scaling inpw folded in).  The species' parameters are in L.nw in the
 // matrix-core layout (W_l as [k][o], row stride mlp_ld(w_{l+1}), biases at bl_off); after them: per layer
-// the activations and derivatives, then BT partial sums.  Each output is split over up to BT / outputs
-// threads, whose partial sums are added in a fixed order.
+// the activations and derivatives.  Each output of a layer belongs to an aligned group of G lanes of one
+// wave (G = the power of two that spreads the outputs over the workgroup); the group's strided partial
+// sums are combined by DPP moves in a fixed order, so a layer needs one workgroup barrier.
+__device__ inline int net_group(int nout) {
+    int g = 1;
+    while (g < 64 && 2 * g * nout <= BT) g <<= 1;
+    return g;
+}
+
 template <class T>
 __device__ inline void block_network(const BasisLds<T>& L, const DevSpecies& sp, int tid, double* e_out) {
     const int K = sp.K, NL = sp.L;
     const float* W = L.nw;
     float* act = L.nw + sp.wp_n;
-    int tot = 0;
-    for (int l = 0; l < NL; ++l) tot += 2 * ((sp.width[l + 1] + 3) & ~3);
-    float* red = act + tot;
     for (int k = tid; k < K; k += BT) L.row[k] = L.sc[k] * (L.row[k] + L.sc[K + k]);   // inpw (G + inpb)
     __syncthreads();
     // forward
@@ -524,30 +538,27 @@ __device__ inline void block_network(const BasisLds<T>& L, const DevSpecies& sp,
         float* D = A + wp;
         const int fa = (l == NL - 1) ? ACE_ACT_LINEAR : sp.act;
         for (int o0 = 0; o0 < no; o0 += BT) {
-            const int nout = min(BT, no - o0);
-            const int S = max(1, min(BT / nout, (ni + 3) / 4));
-            const int chunk = (ni + S - 1) / S;
-            const int oo = tid % nout, sl = tid / nout;
-            if (sl < S) {
-                const int o = o0 + oo, k0 = sl * chunk, k1 = min(ni, k0 + chunk);
-                float part = 0.f;
-                if (ain < 0)
-                    for (int k = k0; k < k1; ++k) part += Wl[k * ldw + o] * (float)L.row[k];
-                else
-                    for (int k = k0; k < k1; ++k) part += Wl[k * ldw + o] * act[ain + k];
-                red[sl * nout + oo] = part;
+            const int nout = min(BT, no - o0), G = net_group(nout);
+            const int o = o0 + tid / G, r = tid % G;
+            float part = 0.f;
+            if (o < o0 + nout) {
+                if (ain < 0) {
+#pragma unroll 4
+                    for (int k = r; k < ni; k += G) part += Wl[k * ldw + o] * (float)L.row[k];
+                } else {
+#pragma unroll 4
+                    for (int k = r; k < ni; k += G) part += Wl[k * ldw + o] * act[ain + k];
+                }
             }
-            __syncthreads();
-            if (tid < nout) {
-                float z = bl[o0 + tid];
-                for (int q = 0; q < S; ++q) z += red[q * nout + tid];
+            part = t_group_sum<float>(part, G);
+            if (r == 0 && o < o0 + nout) {
                 float a, d;
-                act_fd(fa, sp.c, z, a, d);
-                A[o0 + tid] = a;
-                D[o0 + tid] = d;
+                act_fd(fa, sp.c, part + bl[o], a, d);
+                A[o] = a;
+                D[o] = d;
             }
-            __syncthreads();
         }
+        __syncthreads();
         ain = aoff;
         aoff += 2 * wp;
     }
@@ -569,26 +580,20 @@ __device__ inline void block_network(const BasisLds<T>& L, const DevSpecies& sp,
         const int wpi = (ni + 3) & ~3;
         const int dm = (l > 0) ? dl - 2 * wpi : 0;   // offset of layer l-1's activations
         for (int k0 = 0; k0 < ni; k0 += BT) {
-            const int nout = min(BT, ni - k0);
-            const int S = max(1, min(BT / nout, (no + 3) / 4));
-            const int chunk = (no + S - 1) / S;
-            const int kk = tid % nout, sl = tid / nout;
-            if (sl < S) {
-                const int k = k0 + kk, q0 = sl * chunk, q1 = min(no, q0 + chunk);
-                float part = 0.f;
-                for (int o = q0; o < q1; ++o) part += Wl[k * ldw + o] * act[dl + o];
-                red[sl * nout + kk] = part;
+            const int nout = min(BT, ni - k0), G = net_group(nout);
+            const int k = k0 + tid / G, r = tid % G;
+            float part = 0.f;
+            if (k < k0 + nout) {
+#pragma unroll 4
+                for (int o = r; o < no; o += G) part += Wl[k * ldw + o] * act[dl + o];
             }
-            __syncthreads();
-            if (tid < nout) {
-                float v = 0.f;
-                for (int q = 0; q < S; ++q) v += red[q * nout + tid];
-                const int k = k0 + tid;
-                if (l > 0) act[dm + k] = v * act[dm + wpi + k];
-                else L.row[k] = (T)v * L.sc[k];
+            part = t_group_sum<float>(part, G);
+            if (r == 0 && k < k0 + nout) {
+                if (l > 0) act[dm + k] = part * act[dm + wpi + k];
+                else L.row[k] = (T)part * L.sc[k];
             }
-            __syncthreads();
         }
+        __syncthreads();
         dl = dm;
     }
 }

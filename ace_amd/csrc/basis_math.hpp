@@ -154,6 +154,50 @@ __device__ inline T t_group_sum(T v, int n) {
     return v;
 }
 
+// lane exchange across lane bit B: bits 5 and 4 by permutes, bit 3 by row_mirror (lane ^ 15 in a row),
+// bit 2 by row_half_mirror (lane ^ 7), bits 1 and 0 by quad permutes.  Each is an involution whose
+// partner differs in bit B, which is what the reductions below need.
+template <int B, class T>
+__device__ inline T lane_swap(T v) {
+    if constexpr (B == 5) return __shfl_xor(v, 32, 64);
+    else if constexpr (B == 4) return __shfl_xor(v, 16, 64);
+    else if constexpr (B == 3) return dpp_mov<0x140, 0xF>(v);
+    else if constexpr (B == 2) return dpp_mov<0x141, 0xF>(v);
+    else if constexpr (B == 1) return dpp_mov<0x4E, 0xF>(v);
+    else return dpp_mov<0xB1, 0xF>(v);
+}
+
+// halving steps from bit B down: lanes with bit B set keep the upper half of v, the others the lower half,
+// and add the partner's copy of the half they keep
+template <int B, int H, class T>
+__device__ inline void rs_steps(T* v, int lane) {
+    if constexpr (H >= 1) {
+        const bool hi = (lane >> B) & 1;
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+            const T send = hi ? v[j] : v[j + H];
+            const T keep = hi ? v[j + H] : v[j];
+            v[j] = keep + lane_swap<B>(send);
+        }
+        rs_steps<B - 1, H / 2>(v, lane);
+    }
+}
+
+template <int B, class T>
+__device__ inline T sum_steps(T v) {
+    if constexpr (B >= 0) return sum_steps<B - 1>(v + lane_swap<B>(v));
+    else return v;
+}
+
+// Reduce-scatter of M = 2^LM values per lane over the wave: returns the wave total of value
+// lane >> (6 - LM), held by every lane of its group of 2^(6 - LM) lanes.  M - 1 + 6 - LM exchanges
+// instead of 6 per value.
+template <int LM, class T>
+__device__ inline T wave_reduce_scatter(T* v, int lane) {
+    rs_steps<5, (1 << LM) / 2>(v, lane);
+    return sum_steps<5 - LM>(v[0]);
+}
+
 template <class T>
 __device__ inline T t_wave_sum(T v) {
     v = t_wave_sum63<T>(v);

@@ -26,7 +26,7 @@
 // Modes: 0 = network input rows inpw*(G + inpb); 1 = adjoint (per-pair dE/dd from dE/dG rows);
 // 2 = tangent rows v = sum_a eps_a . dG/dr_a.
 // Dynamic LDS per workgroup (nbm = neighbour slots, kp = kmax rounded up to 4), in units of T:
-//   u[4 nbm], te[4 nbm] (mode 2), E[nbm ng], Ed[nbm ng] (modes 1-3), fR[nbm], gR[nbm], ri[nbm],
+//   u[4 nbm], te[4 nbm] (mode 2), E[nbm ng], Ed[nbm ng] (modes 1-3), fR[nbm], gR[nbm], ri[nbm], fr[nbm] (1, 3),
 //   row[kp], red[BW 64], rp[2 kp] (radial eta, rs of the species), sc[2 kp] (modes 0, 3: inpw, inpb);
 //   then p[nbm] and ci[kp] (int).  Modes 1 and 3 put g[BW 3 nbm] (double) in front; mode 3 appends the
 //   network area (netf floats: staged weights, activations, partial sums), 16-byte aligned.
@@ -37,6 +37,7 @@ __host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int mo
     if (mode == 0 || mode == 3) n += 2 * kp;
     if (mode == 2) n += 4 * nbm;
     if (mode >= 1) n += ng * nbm;
+    if (mode == 1 || mode == 3) n += nbm;
     const int gb = (mode == 1 || mode == 3) ? 8 * BW * 3 * nbm : 0;
     const int base = (gb + (int)sizeof(T) * n + 4 * (nbm + kp) + 15) & ~15;
     return base + (mode == 3 ? 4 * netf : 0);
@@ -51,6 +52,7 @@ struct BasisLds {
     T* fR;
     T* gR;
     T* ri;     // 1 / r
+    T* fr;     // modes 1, 3: radial dE/dr of each slot
     double* g; // [BW][3 nbm] per-wave dE/dd accumulators (FP64: LDS FP64 atomics, exact-order-free)
     T* row;
     T* red;    // [BW][64]
@@ -73,6 +75,7 @@ __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, i
     L.fR = c; c += nbm;
     L.gR = c; c += nbm;
     L.ri = c; c += nbm;
+    L.fr = c; if (mode == 1 || mode == 3) c += nbm;
     L.row = c; c += (kmax + 3) & ~3;
     L.red = c; c += BW * 64;
     const int kp = (kmax + 3) & ~3;
@@ -328,18 +331,14 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
                     val = ADJ ? w * dv : (TAN ? dv * L.te[4 * slot + 3] : v);
                 }
                 if (ADJ) {
-                    // dE/dr of the slot: sum over the nrp function lanes of its group
+                    // dE/dr of the slot: sum over the nrp function lanes of its group; one lane group per slot
                     val = t_group_sum<T>(val, nrp);
-                    if (n == 0 && q < nJ) {
-                        const int slot = bJ + q;
-                        L.g[3 * slot] += (double)(val * L.u[4 * slot]);          // wave-private array of wave 0..3
-                        L.g[3 * slot + 1] += (double)(val * L.u[4 * slot + 1]);
-                        L.g[3 * slot + 2] += (double)(val * L.u[4 * slot + 2]);
-                    }
+                    if (n == 0 && q < nJ) L.fr[bJ + q] += val;
                 } else {
                     acc += val;
                 }
             }
+            if (ADJ && nR > 64) __syncthreads();   // the next chunk maps slots to other lane groups
             if (!ADJ) {
                 L.red[wv * 64 + lane] = acc;   // lane n + nrp * group of each wave
                 __syncthreads();
@@ -760,6 +759,7 @@ __global__ __launch_bounds__(BT) void k_basis(
             if (MODE >= 1) L.Ed[q * NG + g] = edv;
         }
         if (GACC) {
+            L.fr[q] = T(0);
             for (int w = 0; w < BW; ++w) {
                 L.g[w * 3 * nbm + 3 * q] = 0.0;
                 L.g[w * 3 * nbm + 3 * q + 1] = 0.0;
@@ -799,7 +799,8 @@ __global__ __launch_bounds__(BT) void k_basis(
         for (int k = tid; k < sp.K; k += BT) G[(long long)row * ldg + k] = (float)L.row[k];   // v
     } else {
         for (int q = tid; q < ntot; q += BT) {
-            double gx = 0.0, gy = 0.0, gz = 0.0;
+            const double fr = (double)L.fr[q];
+            double gx = fr * (double)L.u[4 * q], gy = fr * (double)L.u[4 * q + 1], gz = fr * (double)L.u[4 * q + 2];
             for (int ww = 0; ww < BW; ++ww) {
                 gx += L.g[ww * 3 * nbm + 3 * q];
                 gy += L.g[ww * 3 * nbm + 3 * q + 1];

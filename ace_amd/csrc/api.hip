@@ -79,13 +79,11 @@ struct ace_ctx {
     int tm = 32;
     DBuf<float> d_G, d_dEdG;
     DBuf<double> d_e_atom, d_F, d_E, d_vir;
-    DBuf<int> d_ticket;          // per-structure completion tickets of the force gather
-    DBuf<int2> d_gtab, d_sblk;   // gather workgroups (structure, first atom); per structure (first, count)
-    DBuf<double> d_part;         // per-workgroup (energy, virial) partial sums
-    int ngblk = 0;
+    // fixed-point accumulators, two buffers: a gradient launch adds into buffer fbuf and zeroes the other
+    DBuf<unsigned long long> d_Ffix[2], d_Efix[2], d_Wfix[2];
+    int fbuf = 0;
     // neighbours
     bool nb_valid = false;
-    bool disp_dirty = false;
     double nb_rc = 0;
     std::vector<StrucInfo> sinfo_h, sinfo_up;   // host tables; the copy last uploaded
     DBuf<StrucInfo> d_sinfo;
@@ -94,15 +92,14 @@ struct ace_ctx {
     DBuf<double4> d_cpos;      // cell-ordered positions
     DBuf<int2> d_cinfo;        // cell-ordered (atom, wrap)
     DBuf<long long> d_nb_keys; // per-atom rows of sorted neighbour keys
-    long long npairs = 0;       // pairs of the current list (exact after a synchronising call)
-    long long pcap = 0;         // pair capacity of the device arrays
+    long long npairs = 0;       // pairs of the current list (set when the CSR arrays are built)
+    bool csr_valid = false;     // CSR arrays (export, unbatched port) match the key rows
     int slots_cap = 32;         // neighbour slots per atom held in LDS by the basis kernels
     bool nb_async_ok = false;   // capacities known: rebuilds need no host round trip
     bool nb_same_layout = false;
     bool nb_pending = false;    // a rebuild's overflow / symmetry flags are not checked yet
     double nb_skin = 0;
-    DBuf<int> d_nb_j, d_nb_img, d_nb_src, d_nb_rev;
-    DBuf<float4> d_nb_d, d_pair_g;
+    DBuf<int> d_nb_j, d_nb_img;
     // plain port scratch
     DBuf<double> d_G64;
     // training
@@ -573,9 +570,7 @@ DevBasisParams basis_params(ace_ctx* ctx) {
 
 // ---- basis kernel dispatch ------------------------------------------------------------------------
 
-typedef void (*BasisFn)(int, const DevModel*, DevBasisParams, const int*, const int*, const int*, const float4*,
-                        const int*, const double*, const int*, const StrucInfo*, const int*, int, float*,
-                        const float*, float4*, int*, int, const double*, int, int, const float*, double*);
+typedef void (*BasisFn)(int, const DevModel*, DevBasisParams, BasisIO, int, int, int);
 
 template <class T, int NG, int NZ>
 void pick_basis(BasisFn* f, bool gen) {
@@ -620,22 +615,37 @@ int nb_slots(ace_ctx* ctx) { return ctx->slots_cap; }
 // reads the neighbour-build flags (one host round trip); on overflow the next build is synchronous
 int nb_verify(ace_ctx* ctx) {
     if (!ctx->nb_pending) return 0;
-    int h[5] = {0, 0, 0, 0, 0};
-    HC(hipMemcpyAsync(h, ctx->d_scal.p, 5 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    int tot = 0;
-    HC(hipMemcpyAsync(&tot, ctx->d_nb_off.p + ctx->N, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    int over = 0;
+    HC(hipMemcpyAsync(&over, ctx->d_scal.p + 5, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HC(hipStreamSynchronize(ctx->stream));
     ctx->nb_pending = false;
-    if (h[4]) {
+    if (over) {
         ctx->nb_valid = false;
         ctx->nb_async_ok = false;
         return fail(ctx, "neighbour list outgrew its capacity in an asynchronous rebuild; build it again");
     }
-    if (h[2]) {
-        ctx->nb_valid = false;
-        return fail(ctx, "ace_build_neighbors: neighbour list is not symmetric");
-    }
+    return 0;
+}
+
+// CSR arrays from the key rows (export and the unbatched port); not on the energy/force path
+int ensure_csr(ace_ctx* ctx) {
+    if (nb_verify(ctx)) return -1;
+    if (ctx->csr_valid) return 0;
+    const int N = ctx->N;
+    if (exclusive_scan(ctx, ctx->d_nb_count.p, ctx->d_nb_off.p, N)) return -1;
+    int tot = 0;
+    HC(hipMemcpyAsync(&tot, ctx->d_nb_off.p + N, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    if (ensure(ctx, ctx->d_nb_j, std::max(tot, 1)) || ensure(ctx, ctx->d_nb_img, std::max(tot, 1))) return -1;
+    if (timed(ctx, "nb_csr", [&] {
+            k_nb_csr<<<cdiv(N, 4), 256, 0, ctx->stream>>>(N, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_nb_keys.p,
+                                                          ctx->slots_cap, ctx->d_nb_count.p, ctx->d_nb_off.p,
+                                                          ctx->d_nb_j.p, ctx->d_nb_img.p);
+        }))
+        return -1;
+    HC(hipStreamSynchronize(ctx->stream));
     ctx->npairs = tot;
+    ctx->csr_valid = true;
     return 0;
 }
 
@@ -677,12 +687,41 @@ int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = 
     float* rows = out ? out : ctx->d_G.p;
     static const char* names[2][4] = {{"basis_symm", "basis_adjoint", "basis_tangent", "basis_fused"},
                                       {"basis_symm_f64", "basis_adjoint_f64", "basis_tangent_f64", "basis_fused_f64"}};
-    return timed(ctx, names[f64][mode], [&] {
-        f<<<ctx->N, BT, bytes, ctx->stream>>>(ctx->N, ctx->d_model.p, P, ctx->d_type.p, ctx->d_nb_off.p, ctx->d_nb_j.p,
-                                              ctx->d_nb_d.p, ctx->d_nb_img.p, ctx->d_pos.p, ctx->d_struc_of.p,
-                                              ctx->d_sinfo.p, ctx->d_row_of.p, ctx->ldg, rows, ctx->d_dEdG.p,
-                                              ctx->d_pair_g.p, err, nbm, eps, ctx->nt, ctx->kmax, ctx->d_wpad.p,
-                                              ctx->d_e_atom.p);
+    BasisIO io;
+    io.type = ctx->d_type.p;
+    io.pos = ctx->d_pos.p;
+    io.struc_of = ctx->d_struc_of.p;
+    io.atom_off = ctx->d_atom_off.p;
+    io.sinfo = ctx->d_sinfo.p;
+    io.keys = ctx->d_nb_keys.p;
+    io.counts = ctx->d_nb_count.p;
+    io.cap = ctx->slots_cap;
+    io.row_of = ctx->d_row_of.p;
+    io.ldg = ctx->ldg;
+    io.G = rows;
+    io.dEdG = ctx->d_dEdG.p;
+    io.err = err;
+    io.eps = eps;
+    io.wpad = ctx->d_wpad.p;
+    io.e_atom = ctx->d_e_atom.p;
+    const int b = ctx->fbuf;
+    io.F_fix = ctx->d_Ffix[b].p; io.E_fix = ctx->d_Efix[b].p; io.W_fix = ctx->d_Wfix[b].p;
+    io.F_nxt = ctx->d_Ffix[1 - b].p; io.E_nxt = ctx->d_Efix[1 - b].p; io.W_nxt = ctx->d_Wfix[1 - b].p;
+    if (timed(ctx, names[f64][mode], [&] {
+            f<<<ctx->N, BT, bytes, ctx->stream>>>(ctx->N, ctx->d_model.p, P, io, nbm, ctx->nt, ctx->kmax);
+        }))
+        return -1;
+    if (mode == 1 || mode == 3) ctx->fbuf = 1 - b;   // the next gradient launch adds into the zeroed buffer
+    return 0;
+}
+
+// the accumulators of the last gradient launch as FP64 forces, energies and virials (device)
+int run_finalize(ace_ctx* ctx) {
+    const int b = 1 - ctx->fbuf;
+    const int n = std::max(3 * ctx->N, 9 * ctx->S);
+    return timed(ctx, "finalize", [&] {
+        k_finalize<<<cdiv(n, 256), 256, 0, ctx->stream>>>(3 * ctx->N, ctx->S, ctx->d_Ffix[b].p, ctx->d_Efix[b].p,
+                                                          ctx->d_Wfix[b].p, ctx->d_F.p, ctx->d_E.p, ctx->d_vir.p);
     });
 }
 
@@ -704,16 +743,6 @@ int run_mlp(ace_ctx* ctx) {
     });
 }
 
-// forces, per-structure energies and virials from the per-pair gradients
-int run_gather(ace_ctx* ctx) {
-    return timed(ctx, "force_gather", [&] {
-        k_force_gather<<<ctx->ngblk, RED_THREADS, 0, ctx->stream>>>(
-            ctx->d_gtab.p, ctx->d_sblk.p, ctx->d_atom_off.p, ctx->d_natoms.p, ctx->d_nb_off.p, ctx->d_nb_rev.p,
-            ctx->d_nb_d.p, ctx->d_pair_g.p, ctx->d_e_atom.p, ctx->d_F.p, ctx->d_part.p, ctx->d_ticket.p, ctx->d_E.p,
-            ctx->d_vir.p);
-    });
-}
-
 int check_err_flag(ace_ctx* ctx, const char* what) {
     int e = 0;
     HC(hipMemcpyAsync(&e, ctx->d_scal.p + 3, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -725,15 +754,6 @@ int check_err_flag(ace_ctx* ctx, const char* what) {
 int require_nb(ace_ctx* ctx) {
     if (ctx->S == 0) return fail(ctx, "no structures uploaded");
     if (!ctx->nb_valid) return fail(ctx, "neighbour list not built");
-    if (ctx->disp_dirty) {
-        if (timed(ctx, "nb_disp", [&] {
-                k_nb_disp<<<cdiv(ctx->pcap, 256), 256, 0, ctx->stream>>>(
-                    ctx->d_nb_off.p + ctx->N, ctx->d_nb_src.p, ctx->d_nb_j.p, ctx->d_nb_img.p, ctx->d_struc_of.p,
-                    ctx->d_sinfo.p, ctx->d_pos.p, ctx->d_nb_d.p);
-            }))
-            return -1;
-        ctx->disp_dirty = false;
-    }
     return refresh_params(ctx);
 }
 
@@ -780,13 +800,13 @@ void ace_destroy(ace_ctx* ctx) {
     release(ctx->d_type); release(ctx->d_struc_of); release(ctx->d_atom_off); release(ctx->d_natoms);
     release(ctx->d_row_of); release(ctx->d_atom_of_row); release(ctx->d_pos); release(ctx->d_cell);
     release(ctx->d_tiles); release(ctx->d_G); release(ctx->d_dEdG); release(ctx->d_e_atom);
-    release(ctx->d_F); release(ctx->d_E); release(ctx->d_vir); release(ctx->d_ticket); release(ctx->d_gtab); release(ctx->d_sblk); release(ctx->d_part);
+    release(ctx->d_F); release(ctx->d_E); release(ctx->d_vir);
+    for (int b = 0; b < 2; ++b) { release(ctx->d_Ffix[b]); release(ctx->d_Efix[b]); release(ctx->d_Wfix[b]); }
     release(ctx->d_sinfo); release(ctx->d_cell_of); release(ctx->d_wrap_of); release(ctx->d_slot_of);
     release(ctx->d_cell_start); release(ctx->d_cpos); release(ctx->d_cinfo); release(ctx->d_nb_keys);
     release(ctx->d_nb_count);
     release(ctx->d_nb_off); release(ctx->d_scan_blk); release(ctx->d_scal); release(ctx->d_nb_j);
-    release(ctx->d_nb_img); release(ctx->d_nb_src); release(ctx->d_nb_rev); release(ctx->d_nb_d);
-    release(ctx->d_pair_g); release(ctx->d_G64);
+    release(ctx->d_nb_img); release(ctx->d_G64);
     train_release(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -888,21 +908,15 @@ int ace_upload(ace_ctx* ctx, int n_struc, const int* natoms, const double* pos, 
     if (ensure(ctx, ctx->d_F, 3 * N)) return -1;
     if (ensure(ctx, ctx->d_E, n_struc)) return -1;
     if (ensure(ctx, ctx->d_vir, 9 * n_struc)) return -1;
-    if (ensure(ctx, ctx->d_ticket, n_struc)) return -1;
-    {
-        // gather workgroups: GATHER_ATOMS atoms of one structure each
-        std::vector<int2> gtab, sblk(n_struc);
-        for (int s = 0; s < n_struc; ++s) {
-            sblk[s] = make_int2((int)gtab.size(), 0);
-            for (int a = 0; a < natoms[s]; a += GATHER_ATOMS) gtab.push_back(make_int2(s, ctx->atom_off[s] + a));
-            sblk[s].y = (int)gtab.size() - sblk[s].x;
-        }
-        ctx->ngblk = (int)gtab.size();
-        if (upload(ctx, ctx->d_gtab, gtab.data(), gtab.size()) || upload(ctx, ctx->d_sblk, sblk.data(), sblk.size()) ||
-            ensure(ctx, ctx->d_part, 10 * (size_t)ctx->ngblk))
+    for (int b = 0; b < 2; ++b) {
+        if (ensure(ctx, ctx->d_Ffix[b], 3 * (size_t)N) || ensure(ctx, ctx->d_Efix[b], n_struc) ||
+            ensure(ctx, ctx->d_Wfix[b], 9 * (size_t)n_struc))
             return -1;
+        HC(hipMemsetAsync(ctx->d_Ffix[b].p, 0, 3 * (size_t)N * sizeof(unsigned long long), ctx->stream));
+        HC(hipMemsetAsync(ctx->d_Efix[b].p, 0, n_struc * sizeof(unsigned long long), ctx->stream));
+        HC(hipMemsetAsync(ctx->d_Wfix[b].p, 0, 9 * (size_t)n_struc * sizeof(unsigned long long), ctx->stream));
     }
-    HC(hipMemsetAsync(ctx->d_ticket.p, 0, n_struc * sizeof(int), ctx->stream));   // reset by their last users
+    ctx->fbuf = 0;
     HC(hipStreamSynchronize(ctx->stream));
     ctx->nb_valid = false;
     ctx->nb_same_layout = false;
@@ -916,7 +930,6 @@ int ace_update_positions(ace_ctx* ctx, const double* pos) {
     if (ctx->S == 0) return fail(ctx, "ace_update_positions: no structures uploaded");
     HC(hipMemcpyAsync(ctx->d_pos.p, pos, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HC(hipStreamSynchronize(ctx->stream));
-    ctx->disp_dirty = true;
     ctx->tr.g_valid = false;
     return 0;
 }
@@ -1004,10 +1017,10 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
         }))
         return -1;
     const double rc2 = rcl * rcl;
-    // Capacities: the first build (or one after an overflow) counts first and reads the sizes back;
-    // later builds reuse the capacities (pairs with a 10% margin, key rows = LDS slots of the basis
-    // kernels) and run without a host round trip.  An overflow empties every list on the device and is
-    // reported at the next synchronising call.
+    // Capacity: the first build (or one after an overflow) counts first and reads the largest count back;
+    // later builds reuse the key-row capacity (the LDS slots of the basis kernels, with a margin) and run
+    // without a host round trip.  An atom that outgrows it raises a flag checked at the next synchronising
+    // call, and the basis kernels refuse it.
     const bool async = ctx->nb_async_ok && ctx->nb_same_layout;
     if (!async) {
         if (timed(ctx, "nb_count", [&] {
@@ -1017,21 +1030,13 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
                     nullptr);
             }))
             return -1;
-        if (exclusive_scan(ctx, ctx->d_nb_count.p, ctx->d_nb_off.p, N)) return -1;
-        int h[2] = {0, 0};
-        HC(hipMemcpyAsync(&h[0], ctx->d_scal.p, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        int mx = 0;
+        HC(hipMemcpyAsync(&mx, ctx->d_scal.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HC(hipStreamSynchronize(ctx->stream));
-        if (h[0] > NB_LMAX) return fail(ctx, "ace_build_neighbors: an atom has more than 1024 neighbours");
-        ctx->pcap = (long long)(1.1 * h[1]) + 1024;
-        ctx->slots_cap = std::min(NB_LMAX, ((h[0] + 16 + 31) & ~31));
-        if (ctx->pcap > (1LL << 31) - 1) return fail(ctx, "ace_build_neighbors: too many pairs");
-        ctx->npairs = h[1];
+        if (mx > NB_LMAX) return fail(ctx, "ace_build_neighbors: an atom has more than 1024 neighbours");
+        ctx->slots_cap = std::min(NB_LMAX, ((mx + 16 + 31) & ~31));
     }
-    const long long P = ctx->pcap;
-    if (ensure(ctx, ctx->d_nb_j, P) || ensure(ctx, ctx->d_nb_img, P) || ensure(ctx, ctx->d_nb_src, P) ||
-        ensure(ctx, ctx->d_nb_rev, P) || ensure(ctx, ctx->d_nb_d, P) || ensure(ctx, ctx->d_pair_g, P) ||
-        ensure(ctx, ctx->d_nb_keys, (size_t)N * ctx->slots_cap))
-        return -1;
+    if (ensure(ctx, ctx->d_nb_keys, (size_t)N * ctx->slots_cap)) return -1;
     if (timed(ctx, "nb_search", [&] {
             k_nb_search<1><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
                 N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
@@ -1039,17 +1044,9 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
                 ctx->d_nb_keys.p, ctx->slots_cap, ctx->d_scal.p + 5);
         }))
         return -1;
-    if (exclusive_scan(ctx, ctx->d_nb_count.p, ctx->d_nb_off.p, N)) return -1;
-    if (timed(ctx, "nb_compact", [&] {
-            k_nb_compact<<<cdiv(N + 1, 4), 256, 0, ctx->stream>>>(
-                N, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_nb_keys.p, ctx->slots_cap,
-                ctx->d_nb_count.p, ctx->d_nb_off.p, ctx->d_scal.p + 1, ctx->d_scal.p + 5, P, ctx->d_nb_j.p,
-                ctx->d_nb_img.p, ctx->d_nb_src.p, ctx->d_nb_d.p, ctx->d_nb_rev.p, ctx->d_scal.p + 4, ctx->d_scal.p + 2);
-        }))
-        return -1;
     ctx->nb_valid = true;
     ctx->nb_pending = true;
-    ctx->disp_dirty = false;
+    ctx->csr_valid = false;
     if (!async) {
         if (nb_verify(ctx)) return -1;
         ctx->nb_async_ok = true;
@@ -1061,7 +1058,7 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
 int ace_neighbor_count(const ace_ctx* cctx, long long* npairs) {
     ace_ctx* ctx = const_cast<ace_ctx*>(cctx);
     if (!ctx || !npairs || !ctx->nb_valid) return -1;
-    if (nb_verify(ctx)) return -1;
+    if (ensure_csr(ctx)) return -1;
     *npairs = ctx->npairs;
     return 0;
 }
@@ -1069,7 +1066,7 @@ int ace_neighbor_count(const ace_ctx* cctx, long long* npairs) {
 int ace_export_neighbors(ace_ctx* ctx, long long* offsets, int* j, int* img) {
     if (!ctx) return -1;
     if (!ctx->nb_valid) return fail(ctx, "ace_export_neighbors: neighbour list not built");
-    if (nb_verify(ctx)) return -1;
+    if (ensure_csr(ctx)) return -1;
     const int N = ctx->N;
     const long long P = ctx->npairs;
     std::vector<int> off(N + 1), jj(P), im(P);
@@ -1137,19 +1134,21 @@ int ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, d
         if (run_mlp(ctx)) return -1;
         if (run_basis(ctx, 1)) return -1;
     }
-    const int N = ctx->N;
-    if (run_gather(ctx)) return -1;
-    if (E_struc) HC(hipMemcpyAsync(E_struc, ctx->d_E.p, ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    if (F) HC(hipMemcpyAsync(F, ctx->d_F.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    if (virial) HC(hipMemcpyAsync(virial, ctx->d_vir.p, 9 * ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (E_struc || F || virial || ctx->prof) {
-        if (check_err_flag(ctx, "ace_energy_forces")) return -1;
         if (ctx->nb_pending && nb_verify(ctx)) {
-            // an asynchronous rebuild outgrew its capacities: rebuild with exact sizes and recompute
+            // an asynchronous rebuild outgrew its capacity: rebuild with the exact size and recompute
             if (ctx->nb_async_ok) return -1;
             if (ace_build_neighbors(ctx, ctx->nb_skin)) return -1;
             return ace_energy_forces(ctx, precision, E_struc, F, virial);
         }
+        if (check_err_flag(ctx, "ace_energy_forces")) return -1;
+    }
+    if (E_struc || F || virial) {
+        if (run_finalize(ctx)) return -1;
+        if (E_struc) HC(hipMemcpyAsync(E_struc, ctx->d_E.p, ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        if (F) HC(hipMemcpyAsync(F, ctx->d_F.p, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        if (virial) HC(hipMemcpyAsync(virial, ctx->d_vir.p, 9 * ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HC(hipStreamSynchronize(ctx->stream));
     }
     return flush_prof(ctx);
 }
@@ -1158,6 +1157,7 @@ int ace_plain_energy_forces(ace_ctx* ctx, double* E_struc, double* F) {
     if (!ctx) return -1;
     if (!ctx->plain_ok) return fail(ctx, "ace_plain_energy_forces: model outside the port's forms (" + ctx->plain_why + ")");
     if (require_nb(ctx)) return -1;
+    if (ensure_csr(ctx)) return -1;
     int nmax = 0;
     for (int s = 0; s < ctx->S; ++s) nmax = std::max(nmax, ctx->natoms[s]);
     if (ensure(ctx, ctx->d_G64, (size_t)nmax * PL_KMAX)) return -1;

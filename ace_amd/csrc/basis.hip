@@ -597,17 +597,44 @@ __device__ inline void block_network(const BasisLds<T>& L, const DevSpecies& sp,
     }
 }
 
+// Fixed-point accumulation of forces, energies and virials: 64-bit integer atomics in units of 2^-34
+// (resolution 5.8e-11, range 5.4e8) are exact and order independent, so the sums do not depend on the
+// order in which workgroups add them.
+#define FIX_SCALE 17179869184.0
+__device__ inline unsigned long long fix_of(double v) { return (unsigned long long)__double2ll_rn(v * FIX_SCALE); }
+
+// Kernel inputs and outputs.  Neighbours of atom i: counts[i] keys in row keys[i * cap ...], sorted in
+// the reference order (key = j_local << 30 | image offset by 512 in three 10-bit fields).  The gradient
+// modes add -dE/dr into the fixed-point accumulators F_fix (per atom), E_fix and W_fix (per structure)
+// and zero the entries of the other buffer that this workgroup owns, which the next call uses.
+struct BasisIO {
+    const int* type;
+    const double* pos;
+    const int* struc_of;
+    const int* atom_off;
+    const StrucInfo* sinfo;
+    const long long* keys;
+    const int* counts;
+    int cap;
+    const int* row_of;
+    int ldg;
+    float* G;
+    const float* dEdG;
+    int* err;
+    const double* eps;
+    const float* wpad;
+    double* e_atom;
+    unsigned long long *F_fix, *E_fix, *W_fix;
+    unsigned long long *F_nxt, *E_nxt, *W_nxt;
+};
+
 // GEN = 0 compiles the kernel for models whose angular blocks are all canonical (no generic path).
 // NG: eta groups per species (1, 2 or 4, at least the model's largest count); NZ: canonical power levels
 // (4 or 7, at least the model's largest log2(zeta) + 1).
+// MODE 0: network input rows; 1: gradients from dE/dG rows; 2: tangent rows; 3: fused energy and forces.
 template <class T, int NG, int NZ, int MODE, int GEN>
-__global__ __launch_bounds__(BT) void k_basis(
-    int natoms, const DevModel* __restrict__ M, DevBasisParams P, const int* __restrict__ type,
-    const int* __restrict__ nb_off, const int* __restrict__ nb_j, const float4* __restrict__ nb_d,
-    const int* __restrict__ nb_img, const double* __restrict__ pos, const int* __restrict__ struc_of,
-    const StrucInfo* __restrict__ sinfo, const int* __restrict__ row_of, int ldg, float* __restrict__ G,
-    const float* __restrict__ dEdG, float4* __restrict__ pair_g, int* __restrict__ err, int nbm,
-    const double* __restrict__ eps, int nt, int kmax, const float* __restrict__ wpad, double* __restrict__ e_atom) {
+__global__ __launch_bounds__(BT) void k_basis(int natoms, const DevModel* __restrict__ M, DevBasisParams P, BasisIO io,
+                                              int nbm, int nt, int kmax) {
     constexpr int ADJ = (MODE == 1);
     constexpr int TAN = (MODE == 2);
     constexpr int FUSED = (MODE == 3);        // symmetry functions, network and adjoint in one pass
@@ -616,15 +643,26 @@ __global__ __launch_bounds__(BT) void k_basis(
     extern __shared__ double4 basis_smem[];
     __shared__ int s_cnt[ACE_MAXT + 1];
     __shared__ DevSpecies s_sp;
+    __shared__ double s_red[BW][16];
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int i = blockIdx.x;
     if (i >= natoms) return;
-    const int I = type[i];
+    const int st = io.struc_of[i];
+    if (GACC) {
+        // the other buffer's entries owned by this atom, for the next call
+        if (tid < 3) io.F_nxt[3 * i + tid] = 0ull;
+        if (i == io.atom_off[st] && tid < 10) {
+            if (tid == 0) io.E_nxt[st] = 0ull;
+            else io.W_nxt[9 * st + tid - 1] = 0ull;
+        }
+    }
+    const int I = io.type[i];
     const DevSpecies& gsp = M->sp[I];
     const BasisLds<T> L = basis_lds<T>(basis_smem, nbm, NG, kmax, MODE);
-    const int o0 = nb_off[i], o1 = nb_off[i + 1];
+    const int nn = io.counts[i];
+    const long long* krow = io.keys + (long long)i * io.cap;
     const float rcut = gsp.rc_max;
-    const int row = row_of[i];
+    const int row = io.row_of[i];
 
     // --- waves 1-3 stage the species tables in LDS while wave 0 buckets the neighbours --------------
     if (wv == 1) {
@@ -644,7 +682,7 @@ __global__ __launch_bounds__(BT) void k_basis(
             L.rp[nR + q] = prm<T>(P.r_rs, P.r_rs64, r0 + q);
         }
         if (ADJ)
-            for (int k = lane; k < gsp.K; k += 64) L.row[k] = (T)dEdG[(long long)row * ldg + k];
+            for (int k = lane; k < gsp.K; k += 64) L.row[k] = (T)io.dEdG[(long long)row * io.ldg + k];
     } else if (wv == 3) {
         const int nA = gsp.K - gsp.nR, a0 = gsp.a_poff[0];
         for (int q = lane; q < nA; q += 64) L.ci[q] = P.a_cidx[a0 + q];
@@ -656,90 +694,96 @@ __global__ __launch_bounds__(BT) void k_basis(
     }
     if (FUSED && wv > 0) {
         // the network parameters of the species, staged by waves 1-3 (16-byte copies)
-        const float4* src = (const float4*)(wpad + gsp.wp_off);
+        const float4* src = (const float4*)(io.wpad + gsp.wp_off);
         float4* dst = (float4*)L.nw;
         const int n4 = gsp.wp_n >> 2;
         for (int e = tid - 64; e < n4; e += BT - 64) dst[e] = src[e];
     }
 
-    // --- wave 0: bucket the neighbours by species (ballot order keeps the list order) -------------
+    // --- wave 0: decode the key row, displacements r_ij = posn(i) - (posn(j) + R n) in FP64
+    // (nnp.cpp:363), drop pairs at or beyond the species' largest cutoff, bucket by species (ballot order
+    // keeps the row order); slot q gets j in p[q] and (d, |d|) in u[q] ------------------------------------
     if (wv == 0) {
+        const StrucInfo& S = io.sinfo[st];
+        const double pix = io.pos[3 * i], piy = io.pos[3 * i + 1], piz = io.pos[3 * i + 2];
+        const bool fits = nn <= nbm;
         int cnt[ACE_MAXT];
 #pragma unroll
         for (int J = 0; J < ACE_MAXT; ++J) cnt[J] = 0;
-        for (int cb = o0; cb < o1; cb += 64) {
-            const int p = cb + lane;
-            int Jl = -1;
-            if (p < o1 && nb_d[p].w < rcut) Jl = type[nb_j[p]];
+        for (int pass = 0; pass < 2; ++pass) {
+            int fill[ACE_MAXT];
+            if (pass == 1) {
+                int acc = 0;
 #pragma unroll
-            for (int J = 0; J < ACE_MAXT; ++J)
-                if (J < nt) cnt[J] += __popcll(__ballot(Jl == J));
-        }
-        int fill[ACE_MAXT];
-        int acc = 0;
+                for (int J = 0; J < ACE_MAXT; ++J) {
+                    fill[J] = acc;
+                    acc += (J < nt) ? cnt[J] : 0;
+                }
+                if (lane == 0) {
 #pragma unroll
-        for (int J = 0; J < ACE_MAXT; ++J) {
-            fill[J] = acc;
-            acc += (J < nt) ? cnt[J] : 0;
-        }
-        if (lane == 0) {
-#pragma unroll
-            for (int J = 0; J <= ACE_MAXT; ++J)
-                if (J <= nt) s_cnt[J] = (J < nt) ? fill[J] : acc;
-        }
-        if (acc <= nbm) {
-            for (int cb = o0; cb < o1; cb += 64) {
-                const int p = cb + lane;
-                int Jl = -1;
-                if (p < o1) {
-                    if (nb_d[p].w < rcut) Jl = type[nb_j[p]];
-                    else if (GACC) pair_g[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    for (int J = 0; J <= ACE_MAXT; ++J)
+                        if (J <= nt) s_cnt[J] = (J < nt) ? fill[J] : acc;
+                }
+                if (!fits) break;
+            }
+            for (int cb = 0; cb < nn; cb += 64) {
+                const int e = cb + lane;
+                int Jl = -1, j = 0;
+                double dx = 0.0, dy = 0.0, dz = 0.0, r = 0.0;
+                if (e < nn && fits) {
+                    const long long key = krow[e];
+                    const int jl = (int)(key >> 30), c = (int)(key & 0x3fffffff);
+                    const int c0 = (c >> 20) - 512, c1 = ((c >> 10) & 1023) - 512, c2 = (c & 1023) - 512;
+                    j = jl + S.atom_off;
+                    dx = pix - (io.pos[3 * j] + (S.R[0] * c0 + S.R[3] * c1 + S.R[6] * c2));
+                    dy = piy - (io.pos[3 * j + 1] + (S.R[1] * c0 + S.R[4] * c1 + S.R[7] * c2));
+                    dz = piz - (io.pos[3 * j + 2] + (S.R[2] * c0 + S.R[5] * c1 + S.R[8] * c2));
+                    r = sqrt(dx * dx + dy * dy + dz * dz);
+                    if ((float)r < rcut) Jl = io.type[j];
                 }
 #pragma unroll
                 for (int J = 0; J < ACE_MAXT; ++J) {
                     if (J < nt) {
                         const unsigned long long m = __ballot(Jl == J);
-                        if (Jl == J) L.p[fill[J] + __popcll(m & lane_mask_lt())] = p;
-                        fill[J] += __popcll(m);
+                        if (pass == 0) {
+                            cnt[J] += __popcll(m);
+                        } else {
+                            if (Jl == J) {
+                                const int q = fill[J] + __popcll(m & lane_mask_lt());
+                                L.p[q] = j;
+                                L.u[4 * q] = (T)dx;
+                                L.u[4 * q + 1] = (T)dy;
+                                L.u[4 * q + 2] = (T)dz;
+                                L.u[4 * q + 3] = (T)r;
+                            }
+                            fill[J] += __popcll(m);
+                        }
                     }
                 }
             }
         }
+        if (!fits && lane == 0) s_cnt[nt] = nbm + 1;
     }
     __syncthreads();
     const DevSpecies& sp = s_sp;
     const int ng = sp.ng;
     const int ntot = s_cnt[nt];
     if (ntot > nbm) {
-        if (tid == 0) atomicAdd(err, 1);
+        if (tid == 0) atomicAdd(io.err, 1);
         return;
     }
 
     // --- per-slot quantities ---------------------------------------------------------------------
-    const StrucInfo* S = F64 ? &sinfo[struc_of[i]] : nullptr;
     for (int q = tid; q < ntot; q += BT) {
-        const int p = L.p[q];
         int J = 0;
 #pragma unroll
         for (int jj = 1; jj < ACE_MAXT; ++jj)
             if (jj < nt && q >= s_cnt[jj]) J = jj;
-        T dx, dy, dz;
-        if constexpr (F64) {      // r_ij = posn(i) - (posn(j) + R img) in FP64 (nnp.cpp:363)
-            int a, b, c;
-            img_unpack(nb_img[p], a, b, c);
-            const int j = nb_j[p];
-            dx = pos[3 * i] - (pos[3 * j] + (S->R[0] * a + S->R[3] * b + S->R[6] * c));
-            dy = pos[3 * i + 1] - (pos[3 * j + 1] + (S->R[1] * a + S->R[4] * b + S->R[7] * c));
-            dz = pos[3 * i + 2] - (pos[3 * j + 2] + (S->R[2] * a + S->R[5] * b + S->R[8] * c));
-        } else {
-            const float4 d = nb_d[p];
-            dx = d.x; dy = d.y; dz = d.z;
-        }
-        const T r = sqrt(dx * dx + dy * dy + dz * dz), ir = T(1) / r;
+        const T dx = L.u[4 * q], dy = L.u[4 * q + 1], dz = L.u[4 * q + 2], r = L.u[4 * q + 3];
+        const T ir = T(1) / r;
         L.u[4 * q] = dx * ir;
         L.u[4 * q + 1] = dy * ir;
         L.u[4 * q + 2] = dz * ir;
-        L.u[4 * q + 3] = r;
         L.ri[q] = ir;
         T fc, gc;
         t_cutoff<T>(sp.r_cut[J], F64 ? (T)sp.r_rc64[J] : (T)sp.r_rc[J], r, fc, gc);
@@ -767,10 +811,10 @@ __global__ __launch_bounds__(BT) void k_basis(
             }
         }
         if (TAN) {
-            const int jj = nb_j[p];
-            const T ex = (T)(eps[3 * i] - eps[3 * jj]);
-            const T ey = (T)(eps[3 * i + 1] - eps[3 * jj + 1]);
-            const T ez = (T)(eps[3 * i + 2] - eps[3 * jj + 2]);
+            const int jj = L.p[q];
+            const T ex = (T)(io.eps[3 * i] - io.eps[3 * jj]);
+            const T ey = (T)(io.eps[3 * i + 1] - io.eps[3 * jj + 1]);
+            const T ez = (T)(io.eps[3 * i + 2] - io.eps[3 * jj + 2]);
             L.te[4 * q] = ex;
             L.te[4 * q + 1] = ey;
             L.te[4 * q + 2] = ez;
@@ -783,7 +827,7 @@ __global__ __launch_bounds__(BT) void k_basis(
 
     if (FUSED) {
         basis_sums<T, NG, NZ, 0, GEN>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
-        block_network<T>(L, sp, tid, &e_atom[i]);
+        block_network<T>(L, sp, tid, &io.e_atom[i]);
         basis_sums<T, NG, NZ, 1, GEN>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
     } else {
         basis_sums<T, NG, NZ, MODE, GEN>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
@@ -793,20 +837,58 @@ __global__ __launch_bounds__(BT) void k_basis(
         // so the FP32 row keeps full relative precision after the mean is removed
         for (int k = tid; k < sp.K; k += BT) {
             const T w = L.sc[k], b = L.sc[sp.K + k];
-            G[(long long)row * ldg + k] = (float)(w * (L.row[k] + b));
+            io.G[(long long)row * io.ldg + k] = (float)(w * (L.row[k] + b));
         }
     } else if (TAN) {
-        for (int k = tid; k < sp.K; k += BT) G[(long long)row * ldg + k] = (float)L.row[k];   // v
+        for (int k = tid; k < sp.K; k += BT) io.G[(long long)row * io.ldg + k] = (float)L.row[k];   // v
     } else {
+        // g_q = dE_i/dd_q of slot q (pair i -> j): F_j -= -g_q, F_i -= g_q summed over the slots; the atom's
+        // virial share -sum_q g_q (x) d_q; the atom's energy into its structure's total
+        double fi[3] = {0.0, 0.0, 0.0}, w[9];
+#pragma unroll
+        for (int q9 = 0; q9 < 9; ++q9) w[q9] = 0.0;
         for (int q = tid; q < ntot; q += BT) {
             const double fr = (double)L.fr[q];
-            double gx = fr * (double)L.u[4 * q], gy = fr * (double)L.u[4 * q + 1], gz = fr * (double)L.u[4 * q + 2];
+            const double ux = (double)L.u[4 * q], uy = (double)L.u[4 * q + 1], uz = (double)L.u[4 * q + 2];
+            double g3[3] = {fr * ux, fr * uy, fr * uz};
             for (int ww = 0; ww < BW; ++ww) {
-                gx += L.g[ww * 3 * nbm + 3 * q];
-                gy += L.g[ww * 3 * nbm + 3 * q + 1];
-                gz += L.g[ww * 3 * nbm + 3 * q + 2];
+                g3[0] += L.g[ww * 3 * nbm + 3 * q];
+                g3[1] += L.g[ww * 3 * nbm + 3 * q + 1];
+                g3[2] += L.g[ww * 3 * nbm + 3 * q + 2];
             }
-            pair_g[L.p[q]] = make_float4((float)gx, (float)gy, (float)gz, 0.f);
+            const int j = L.p[q];
+            atomicAdd(&io.F_fix[3 * j], fix_of(g3[0]));
+            atomicAdd(&io.F_fix[3 * j + 1], fix_of(g3[1]));
+            atomicAdd(&io.F_fix[3 * j + 2], fix_of(g3[2]));
+            const double r = 1.0 / (double)L.ri[q];
+            const double d3[3] = {ux * r, uy * r, uz * r};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                fi[a] -= g3[a];
+#pragma unroll
+                for (int b = 0; b < 3; ++b) w[3 * a + b] -= g3[a] * d3[b];
+            }
+        }
+        double v[16];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = fi[a];
+#pragma unroll
+        for (int q9 = 0; q9 < 9; ++q9) v[3 + q9] = w[q9];
+#pragma unroll
+        for (int q4 = 12; q4 < 16; ++q4) v[q4] = 0.0;
+        const double tot = wave_reduce_scatter<4, double>(v, lane);
+        if ((lane & 3) == 0) s_red[wv][lane >> 2] = tot;
+        __syncthreads();
+        if (tid < 13) {
+            double a = 0.0;
+            if (tid < 12) {
+                for (int ww = 0; ww < BW; ++ww) a += s_red[ww][tid];
+            } else {
+                a = io.e_atom[i];
+            }
+            if (tid < 3) atomicAdd(&io.F_fix[3 * i + tid], fix_of(a));
+            else if (tid < 12) atomicAdd(&io.W_fix[9 * st + tid - 3], fix_of(a));
+            else atomicAdd(&io.E_fix[st], fix_of(a));
         }
     }
 }

@@ -11,10 +11,10 @@
 // cells (L < 2 rc) yield every periodic image of a neighbour once.
 //
 // Pipeline (one build): k_nb_bin (workgroup per structure: cell counts, scan and a cell-ordered copy of
-// the positions, all in LDS) -> k_nb_search (wave per atom; candidates are contiguous runs of the
+// the positions, all in LDS) -> k_nb_search (workgroup per atom; candidates are contiguous runs of the
 // cell-ordered copy, loaded four chunks at a time; keys rank-sorted in LDS into the reference order and
-// written to a fixed-capacity per-atom row) -> exclusive scan of the counts -> k_nb_compact (wave per
-// atom: CSR arrays, FP32 displacements and the index of each pair's reverse pair).
+// written to a fixed-capacity per-atom row, which the basis kernels read directly).  CSR arrays are
+// built from the rows on demand (export, unbatched port): exclusive scan of the counts -> k_nb_csr.
 #include "ace_common.hpp"
 
 // ---- binning: one workgroup per structure ---------------------------------------------------------
@@ -347,55 +347,22 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
     }
 }
 
-// Wave per atom: CSR arrays from the sorted key rows.  nb_j, nb_img, nb_src; the FP64 displacement
-// r_ij = posn(i) - (posn(j) + R n) (nnp.cpp:363) stored as FP32 with its length; nb_rev = index of the
-// reverse pair (j -> i, -n), found by binary search in row j.  When the pair total exceeds pcap or a row
-// overflowed, every list is emptied (offsets zeroed) and *bad is raised; total is the scan's own copy.
-__global__ __launch_bounds__(256) void k_nb_compact(
-    int natoms, const double* __restrict__ pos, const int* __restrict__ struc_of, const StrucInfo* __restrict__ sinfo,
-    const long long* __restrict__ keys, int cap, const int* __restrict__ counts, int* __restrict__ nb_off,
-    const int* __restrict__ total, const int* __restrict__ over, long long pcap, int* __restrict__ nb_j,
-    int* __restrict__ nb_img, int* __restrict__ nb_src, float4* __restrict__ nb_d, int* __restrict__ nb_rev,
-    int* __restrict__ bad, int* __restrict__ rev_err) {
+// Wave per atom: CSR arrays (neighbour index and packed image, the layout of NeighborList's accessors,
+// neighbor.hpp:34-40) from the sorted key rows, for export and for the unbatched port.
+__global__ __launch_bounds__(256) void k_nb_csr(int natoms, const int* __restrict__ struc_of,
+                                                const StrucInfo* __restrict__ sinfo, const long long* __restrict__ keys,
+                                                int cap, const int* __restrict__ counts, const int* __restrict__ nb_off,
+                                                int* __restrict__ nb_j, int* __restrict__ nb_img) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * 4 + wv;
-    if (i > natoms) return;
-    if (*total > pcap || *over) {
-        if (lane == 0) nb_off[i] = 0;
-        if (i == 0 && lane == 0) *bad = 1;
-        return;
-    }
-    if (i == natoms) return;
-    const StrucInfo& S = sinfo[struc_of[i]];
+    if (i >= natoms) return;
+    const int a0 = sinfo[struc_of[i]].atom_off;
     const int base = nb_off[i], n = counts[i];
     const long long* row = keys + (long long)i * cap;
-    const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
     for (int e = lane; e < n; e += 64) {
-        const long long k = row[e];
         int jl, c0, c1, c2;
-        nb_key_decode(k, jl, c0, c1, c2);
-        const int j = jl + S.atom_off;
-        const int p = base + e;
-        nb_j[p] = j;
-        nb_src[p] = i;
-        nb_img[p] = img_pack(c0, c1, c2);
-        const double sx = S.R[0] * c0 + S.R[3] * c1 + S.R[6] * c2;
-        const double sy = S.R[1] * c0 + S.R[4] * c1 + S.R[7] * c2;
-        const double sz = S.R[2] * c0 + S.R[5] * c1 + S.R[8] * c2;
-        const double rx = pix - (pos[3 * j] + sx), ry = piy - (pos[3 * j + 1] + sy), rz = piz - (pos[3 * j + 2] + sz);
-        const double r = sqrt(rx * rx + ry * ry + rz * rz);
-        nb_d[p] = make_float4((float)rx, (float)ry, (float)rz, (float)r);
-        // reverse pair: key (i_local, -n) in row j
-        const long long want = nb_key(i - S.atom_off, -c0, -c1, -c2);
-        const long long* rj = keys + (long long)j * cap;
-        int lo = 0, hi = counts[j] - 1, found = -1;
-        while (lo <= hi) {
-            const int mid = (lo + hi) >> 1;
-            const long long km = rj[mid];
-            if (km == want) { found = mid; break; }
-            if (km < want) lo = mid + 1; else hi = mid - 1;
-        }
-        if (found < 0) atomicAdd(rev_err, 1);
-        nb_rev[p] = found < 0 ? p : nb_off[j] + found;
+        nb_key_decode(row[e], jl, c0, c1, c2);
+        nb_j[base + e] = jl + a0;
+        nb_img[base + e] = img_pack(c0, c1, c2);
     }
 }

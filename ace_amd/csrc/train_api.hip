@@ -89,7 +89,7 @@ int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* 
     if (run_mlp(ctx)) return -1;
     if (wf) {
         if (run_basis(ctx, 1)) return -1;
-        if (run_gather(ctx)) return -1;
+        if (run_finalize(ctx)) return -1;
     } else if (timed(ctx, "struct_reduce", [&] {
                    k_struct_reduce<<<S, RED_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p,
                                                                        ctx->d_e_atom.p, ctx->d_E.p);

@@ -168,8 +168,8 @@ def main():
         kern = {}
         if events:
             sfx = "_f64" if prec == PREC_FP64_BASIS else ""
-            for k in ("nb_bin", "nb_search", "scan", "nb_compact", "basis_fused" + sfx, "basis_symm" + sfx,
-                      "mlp_energy_grad", "basis_adjoint" + sfx, "force_gather"):
+            for k in ("nb_bin", "nb_search", "basis_fused" + sfx, "basis_symm" + sfx, "mlp_energy_grad",
+                      "basis_adjoint" + sfx):
                 v = dev.profile_read(k)
                 if v[1]:
                     kern[k.replace(sfx, "") if sfx else k] = v

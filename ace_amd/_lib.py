@@ -35,6 +35,7 @@ SIGNATURES = {
     "ace_set_precision": (C.c_int, [P, C.c_int]),
     "ace_symm": (C.c_int, [P, dp, C.c_int]),
     "ace_energy_forces": (C.c_int, [P, C.c_int, dp, dp, dp]),
+    "ace_compute": (C.c_int, [P, C.c_int, C.c_double, dp, dp, dp]),
     "ace_plain_energy_forces": (C.c_int, [P, dp, dp]),
     "ace_set_targets": (C.c_int, [P, dp, dp]),
     "ace_train_step": (C.c_int, [P, P, dp, dp]),

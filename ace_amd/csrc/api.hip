@@ -1153,6 +1153,12 @@ int ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, d
     return flush_prof(ctx);
 }
 
+int ace_compute(ace_ctx* ctx, int precision, double skin, double* E_struc, double* F, double* virial) {
+    if (!ctx) return -1;
+    if (ace_build_neighbors(ctx, skin)) return -1;
+    return ace_energy_forces(ctx, precision, E_struc, F, virial);
+}
+
 int ace_plain_energy_forces(ace_ctx* ctx, double* E_struc, double* F) {
     if (!ctx) return -1;
     if (!ctx->plain_ok) return fail(ctx, "ace_plain_energy_forces: model outside the port's forms (" + ctx->plain_why + ")");

@@ -212,7 +212,7 @@ __device__ inline void nb_key_decode(long long k, int& jl, int& n0, int& n1, int
 #define NB_U 4         // candidate chunks loaded together by one wave
 
 // Workgroup (4 waves) per atom; the waves take interleaved groups of NB_U x 64 candidates.
-// MODE 0: count only (counts[i], running maximum in *maxcount).  MODE 1: also sort the keys
+// MODE 0: count only (counts[i], running maximum in *maxcount).  MODE 1: sort the keys
 // (j_local << 30 | image order code: the reference order) and write them to row i of keys (cap slots); an
 // atom with more than cap neighbours raises *over and writes none.
 template <int MODE>
@@ -330,7 +330,9 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
     const int count = s_n;
     if (tid == 0) {
         counts[i] = count;
-        atomicMax(maxcount, count);
+        // the running maximum is only read after the counting pass; one address updated by every
+        // workgroup serialises in L2 and delays the end of the kernel, so the filling pass skips it
+        if (MODE == 0) atomicMax(maxcount, count);
     }
     if (MODE == 0) return;
     if (count > cap || count > NB_LMAX) {
@@ -343,7 +345,7 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
         const long long k = s_key[e];
         int rank = 0;
         for (int f = 0; f < count; ++f) rank += (s_key[f] < k);
-        row[rank] = k;
+        __builtin_nontemporal_store(k, &row[rank]);   // read once, by the next kernel
     }
 }
 

@@ -167,6 +167,11 @@ class NNPDevice:
         """Energy/force pass whose results stay in HBM (timed loops)."""
         self._check(self.lib.ace_energy_forces(self.ctx, precision, None, None, None), "ace_energy_forces")
 
+    def compute_step(self, precision=PREC_FP32, skin=0.0):
+        """NNP::compute(struc) (nnp.cpp:489-494) for the whole batch: neighbour list, then energies and
+        forces, in one call; results stay in HBM."""
+        self._check(self.lib.ace_compute(self.ctx, precision, float(skin), None, None, None), "ace_compute")
+
     def plain_compute(self):
         """The unbatched FP64 GPU port: per-structure energies and forces."""
         E = np.zeros(len(self.natoms))

@@ -156,13 +156,13 @@ def main():
     dev.sync_model_scaling()
 
     def run(prec, steps, events):
-        """K steps of neighbour build + energy/forces; returns ms per step and per-kernel event times."""
+        """K steps of NNP::compute (neighbour build + energy/forces, one ace_compute call each); returns ms per
+        step and per-kernel event times."""
         dev.profile(events)
         dev.sync()
         t0 = time.perf_counter()
         for _ in range(steps):
-            dev.build_neighbors()
-            dev.compute_device(prec)
+            dev.compute_step(prec)
         dev.sync()
         t1 = time.perf_counter()
         kern = {}

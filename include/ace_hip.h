@@ -17,6 +17,7 @@
  *   ace_symm ....................... NNP::compute_symm (src/nnp/nnp.cpp:350-392)
  *   ace_energy_forces .............. NNP::compute / compute_force / compute_energy
  *                                   (src/nnp/nnp.cpp:329-338, 412-494)
+ *   ace_compute .................... NNP::compute(Structure&) (src/nnp/nnp.cpp:489-494): build + energy
  *   ace_train_step ................. NNPTEFR::error_cost_o1 (src/nnp/nnptefr.cpp:1292-1425)
  *                                   extended with the force-matching term d(F)/dW
  *                                   (network factor NN::D2ODZDI, src/ml/nn.cpp:2047-2140)
@@ -158,9 +159,13 @@ int  ace_symm(ace_ctx* ctx, double* G, int ldg);
  *   E_struc[s] = sum_i (NN_i + E0_i)            (NNP::compute_energy convention, FP64 reduction)
  *   F          = -dE/dr, sum(natoms) x 3         (NNP::compute_force)
  *   virial     = n_struc x 9, W_ab = -sum_pairs d_a dE/dd_b (FP64 reduction); may be NULL
- * precision: ACE_PREC_FP32 or ACE_PREC_BF16L1.  Output pointers may be NULL: results stay in HBM.
+ * precision: ACE_PREC_FP32 or ACE_PREC_FP64_BASIS.  Output pointers may be NULL: results stay in HBM.
  */
 int  ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, double* virial);
+
+/* NNP::compute(Structure&) (src/nnp/nnp.cpp:489-494): neighbour list with list radius rc + skin, then
+ * ace_energy_forces, in one call. */
+int  ace_compute(ace_ctx* ctx, int precision, double skin, double* E_struc, double* F, double* virial);
 
 /* Same quantities from the unbatched FP64 port: one structure per launch, one thread per atom. */
 int  ace_plain_energy_forces(ace_ctx* ctx, double* E_struc, double* F);

@@ -284,3 +284,24 @@ def test_async_rebuild_and_overflow_recovery():
     dev.close()
     ref.close()
     assert E2[0] == Er[0] and np.array_equal(F2, Fr)
+
+
+def test_compute_entry_matches_build_then_energy():
+    """ace_compute (NNP::compute with the list built inside) equals ace_build_neighbors + ace_energy_forces,
+    bit for bit (fixed-point force sums, fixed-order reductions)."""
+    m, [s] = configs.config2()
+    scaled_model(m, [s])
+    dev = NNPDevice(m)
+    dev.upload([s])
+    dev.build_neighbors()
+    E0, F0, W0 = dev.compute()
+    for _ in range(3):
+        dev.compute_step()                     # asynchronous steps, results in HBM
+    import ctypes as C
+    E = np.zeros(1)
+    F = np.zeros((s.natoms, 3))
+    W = np.zeros((1, 9))
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    assert dev.lib.ace_compute(dev.ctx, 0, 0.0, dp(E), dp(F), dp(W)) == 0
+    dev.close()
+    assert E[0] == E0[0] and np.array_equal(F, F0) and np.array_equal(W.reshape(W0.shape), W0)

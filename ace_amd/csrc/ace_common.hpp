@@ -42,6 +42,9 @@ struct __attribute__((aligned(16))) DevSpecies {
     // biases b_l at bl_off[l]
     int wp_off, wp_n;
     int wl_off[ACE_MAXL], bl_off[ACE_MAXL];
+    // BF16 copies of W_0 (offset in BF16 values): high parts, low parts of [o][k] (round16(w_1) rows of
+    // round32(K)), then high and low parts of [k][o] (round16(K) rows of round32(w_1)), zero padded
+    int bf_off;
     int in_off;                    // offset of inpw/inpb
     float outw, outb;
     double E0;

@@ -58,7 +58,8 @@ struct ace_ctx {
     DevModel hdm;
     DBuf<DevModel> d_model;
     DBuf<float> d_r_rs, d_r_eta, d_a_lamf, d_a_zeta, d_params, d_inpw, d_inpb, d_wpad;
-    int wp_total = 0;
+    DBuf<uint16_t> d_w0bf;           // BF16 split copies of every species' first-layer weights
+    int wp_total = 0, bf_total = 0;
     DBuf<int> d_a_zint, d_a_grp, d_a_sel, d_a_cidx;
     bool plain_ok = false;
     std::string plain_why;
@@ -275,6 +276,20 @@ int parse_model(ace_ctx* ctx, const ace_model_desc* m) {
 }
 
 // matrix-core copy of the network parameters (layout in DevSpecies::wl_off / bl_off)
+// FP32 -> BF16 bits, round to nearest even (finite inputs)
+uint16_t bf16_rne(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+float bf16_value(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float x;
+    memcpy(&x, &u, 4);
+    return x;
+}
+
 int upload_wpad(ace_ctx* ctx) {
     std::vector<float> w(std::max(ctx->wp_total, 4), 0.f);
     for (int I = 0; I < ctx->nt; ++I) {
@@ -289,6 +304,24 @@ int upload_wpad(ace_ctx* ctx) {
         }
     }
     if (upload(ctx, ctx->d_wpad, w.data(), w.size())) return -1;
+    // BF16 high/low parts of W_0 in the two layouts the BF16 first layer reads (DevSpecies::bf_off)
+    std::vector<uint16_t> b(std::max(ctx->bf_total, 8), 0);
+    for (int I = 0; I < ctx->nt; ++I) {
+        const HSpecies& s = ctx->sp[I];
+        const int K = s.width[0], n1 = s.width[1];
+        const int fwd = ((n1 + 15) & ~15) * ((K + 31) & ~31), bwd = ((K + 15) & ~15) * ((n1 + 31) & ~31);
+        uint16_t* f = b.data() + ctx->hdm.sp[I].bf_off;
+        uint16_t* g = f + 2 * fwd;
+        for (int o = 0; o < n1; ++o)
+            for (int k = 0; k < K; ++k) {
+                const float x = (float)s.params[ctx->hdm.sp[I].woff[0] + o + k * n1];
+                const uint16_t hi = bf16_rne(x), lo = bf16_rne(x - bf16_value(hi));
+                const int ef = o * ((K + 31) & ~31) + k, eb = k * ((n1 + 31) & ~31) + o;
+                f[ef] = hi; f[fwd + ef] = lo;
+                g[eb] = hi; g[bwd + eb] = lo;
+            }
+    }
+    if (upload(ctx, ctx->d_w0bf, b.data(), b.size())) return -1;
     return 0;
 }
 
@@ -300,7 +333,7 @@ int build_tables(ace_ctx* ctx) {
     dm.ntypes = nt;
     std::vector<float> r_rs, r_eta, a_lamf, a_zeta, prm, iw, ib;
     std::vector<int> a_zint, a_grp, a_sel;
-    int kmax = 0, maxA = 1, maxR = 1, ngmax = 1, wp_total = 0;
+    int kmax = 0, maxA = 1, maxR = 1, ngmax = 1, wp_total = 0, bf_total = 0;
     for (int I = 0; I < nt; ++I) {
         const HSpecies& s = ctx->sp[I];
         DevSpecies& d = dm.sp[I];
@@ -405,6 +438,9 @@ int build_tables(ace_ctx* ctx) {
             d.wp_n = (off + 3) & ~3;
             d.wp_off = wp_total;
             wp_total += d.wp_n;
+            const int K = s.width[0], n1 = s.width[1];
+            d.bf_off = bf_total;
+            bf_total += 2 * ((n1 + 15) & ~15) * ((K + 31) & ~31) + 2 * ((K + 15) & ~15) * ((n1 + 31) & ~31);
         }
         for (double v : s.params) prm.push_back((float)v);
         d.in_off = (int)iw.size();
@@ -467,6 +503,7 @@ int build_tables(ace_ctx* ctx) {
     if (upload(ctx, ctx->d_inpw, iw.data(), iw.size())) return -1;
     if (upload(ctx, ctx->d_inpb, ib.data(), ib.size())) return -1;
     ctx->wp_total = wp_total;
+    ctx->bf_total = bf_total;
     if (upload_wpad(ctx)) return -1;
 
     // FP64 tables of the plain port
@@ -725,7 +762,7 @@ int run_finalize(ace_ctx* ctx) {
     });
 }
 
-int run_mlp(ace_ctx* ctx) {
+int run_mlp(ace_ctx* ctx, bool bf16) {
     int lds = 0, lds_w = 0;
     for (int I = 0; I < ctx->nt; ++I) {
         lds = std::max(lds, mlp_lds_floats(ctx->hdm.sp[I]) * 4);
@@ -734,10 +771,12 @@ int run_mlp(ace_ctx* ctx) {
     const bool stage = lds_w <= 64 * 1024;
     if (stage) lds = lds_w;
     if (lds > 160 * 1024) return fail(ctx, "mlp: network too wide for LDS");
-    auto f = stage ? k_mlp_energy_grad<true> : k_mlp_energy_grad<false>;
+    auto f = stage ? (bf16 ? k_mlp_energy_grad<true, true> : k_mlp_energy_grad<true, false>)
+                   : (bf16 ? k_mlp_energy_grad<false, true> : k_mlp_energy_grad<false, false>);
     if (lds > 64 * 1024) HC(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-    return timed(ctx, "mlp_energy_grad", [&] {
-        f<<<(int)ctx->tiles_h.size(), 64, lds, ctx->stream>>>(ctx->d_model.p, ctx->d_wpad.p, ctx->d_inpw.p,
+    return timed(ctx, bf16 ? "mlp_energy_grad_bf16" : "mlp_energy_grad", [&] {
+        f<<<(int)ctx->tiles_h.size(), 64, lds, ctx->stream>>>(ctx->d_model.p, ctx->d_wpad.p,
+                                                              (const __bf16*)ctx->d_w0bf.p, ctx->d_inpw.p,
                                                               ctx->d_tiles.p, ctx->ldg, ctx->d_G.p, ctx->d_dEdG.p,
                                                               ctx->d_atom_of_row.p, ctx->d_e_atom.p);
     });
@@ -793,7 +832,7 @@ void ace_destroy(ace_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& r : ctx->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     release(ctx->d_model); release(ctx->d_r_rs); release(ctx->d_r_eta); release(ctx->d_a_lamf);
-    release(ctx->d_a_zeta); release(ctx->d_params); release(ctx->d_wpad); release(ctx->d_inpw); release(ctx->d_inpb);
+    release(ctx->d_a_zeta); release(ctx->d_params); release(ctx->d_wpad); release(ctx->d_w0bf); release(ctx->d_inpw); release(ctx->d_inpb);
     release(ctx->d_a_zint); release(ctx->d_a_grp); release(ctx->d_a_sel); release(ctx->d_a_cidx); release(ctx->d_pmodel); release(ctx->d_r_rs64);
     release(ctx->d_r_eta64); release(ctx->d_a_eta64); release(ctx->d_a_zeta64); release(ctx->d_a_lamf64);
     release(ctx->d_nn64); release(ctx->d_inpw64); release(ctx->d_inpb64);
@@ -1123,15 +1162,16 @@ int ace_symm(ace_ctx* ctx, double* G, int ldg) {
 
 int ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, double* virial) {
     if (!ctx) return -1;
-    if (precision != ACE_PREC_FP32 && precision != ACE_PREC_FP64_BASIS)
+    if (precision != ACE_PREC_FP32 && precision != ACE_PREC_FP64_BASIS && precision != ACE_PREC_BF16L1)
         return fail(ctx, "ace_energy_forces: precision mode not available");
     ctx->basis_f64 = precision == ACE_PREC_FP64_BASIS;
     if (require_nb(ctx)) return -1;
-    if (use_fused(ctx)) {
+    // the BF16 first layer runs on the matrix cores of the network kernel, so that mode takes the split path
+    if (precision != ACE_PREC_BF16L1 && use_fused(ctx)) {
         if (run_basis(ctx, 3)) return -1;
     } else {
         if (run_basis(ctx, 0)) return -1;
-        if (run_mlp(ctx)) return -1;
+        if (run_mlp(ctx, precision == ACE_PREC_BF16L1)) return -1;
         if (run_basis(ctx, 1)) return -1;
     }
     if (E_struc || F || virial || ctx->prof) {

@@ -74,6 +74,86 @@ __device__ inline void mfma_group(int a, int lda, int k, int b, const float* __r
     for (int t = 0; t < NT; ++t) acc[t] = c0[t] + c1[t];
 }
 
+// ---- BF16 first layer ---------------------------------------------------------------------------
+// v_mfma_f32_16x16x32_bf16 (D[16x16] += A[16x32] B[32x16], FP32 sums): lane l supplies the 8 values
+// A[l % 16][8 (l / 16) + i] and B[8 (l / 16) + i][l % 16], i = 0..7, and receives D as above.
+// Each FP32 operand x is split into x_hi = bf16(x) and x_lo = bf16(x - x_hi), so x = x_hi + x_lo to
+// about 2^-17 |x|, and the product is taken as A_hi B_hi + A_hi B_lo + A_lo B_hi (the dropped A_lo B_lo is
+// ~2^-16 of it).  A single BF16 product (2^-9 relative per operand) leaves errors of ~1e-3 eV/A in the
+// forces of a 128-wide layer; the three products keep the layer within ~1e-5 of FP32 at the BF16 rate.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline void bf_split8(const float* v, bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const __bf16 h = (__bf16)v[i];
+        hi[i] = h;
+        lo[i] = (__bf16)(v[i] - (float)h);
+    }
+}
+
+// C[16 x n] = A[16 x k] B for NT 16-wide output tiles from o0.  A: FP32 in LDS at offset a (row stride lda),
+// columns >= k read as zero.  B: BF16 pairs in global memory, B[kk][o] = Bh[o * sb + kk] (high parts) and
+// Bh[lo + o * sb + kk] (low parts), zero padded to sb = round32(k) values per row and to whole 16-row tiles,
+// so each lane's 8 values are one 16-byte load.
+template <int NT>
+__device__ inline void bf3_group(int a, int lda, int k, const __bf16* __restrict__ Bh, int lo, int sb, int o0,
+                                 int lane, mfma_f4* acc) {
+    mfma_f4 c0[NT], c1[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        c0[t] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+        c1[t] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+    }
+    const int r = lane & 15, q = lane >> 4;
+    for (int kb = 0; kb < sb; kb += 32) {
+        float av[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int kk = kb + 8 * q + i;
+            const float v = mlp_smem[a + r * lda + min(kk, k - 1)];
+            av[i] = kk < k ? v : 0.f;
+        }
+        bf16x8 ah, al;
+        bf_split8(av, ah, al);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int e = (o0 + 16 * t + r) * sb + kb + 8 * q;
+            const bf16x8 bh = *(const bf16x8*)(Bh + e);
+            const bf16x8 bl = *(const bf16x8*)(Bh + lo + e);
+            c0[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c0[t], 0, 0, 0);
+            c1[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c1[t], 0, 0, 0);
+            c1[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c1[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = c0[t] + c1[t];
+}
+
+template <class EPI>
+__device__ inline void bf3_layer(int a, int lda, int k, const __bf16* __restrict__ Bh, int lo, int n, int lane, EPI epi) {
+    const int sb = (k + 31) & ~31;
+    for (int o0 = 0; o0 < n;) {
+        const int rem = (n - o0 + 15) >> 4;
+        if (rem >= 4) {
+            mfma_f4 acc[4];
+            bf3_group<4>(a, lda, k, Bh, lo, sb, o0, lane, acc);
+            epi(acc, 4, o0);
+            o0 += 64;
+        } else if (rem >= 2) {
+            mfma_f4 acc[2];
+            bf3_group<2>(a, lda, k, Bh, lo, sb, o0, lane, acc);
+            epi(acc, 2, o0);
+            o0 += 32;
+        } else {
+            mfma_f4 acc[1];
+            bf3_group<1>(a, lda, k, Bh, lo, sb, o0, lane, acc);
+            epi(acc, 1, o0);
+            o0 += 16;
+        }
+    }
+}
+
 // every output tile of a layer, in groups of 4, 2 or 1 tiles; epi(acc, tile count, first column)
 template <bool BLDS, class EPI>
 __device__ inline void mfma_layer(int a, int lda, int k, int b, const float* __restrict__ Bg, int so, int sk, int n,
@@ -102,11 +182,13 @@ __device__ inline void mfma_layer(int a, int lda, int k, int b, const float* __r
 // tiles: (species, first row, rows, 0), rows <= 16.  wpad: the parameters in the matrix-core layout
 // (DevSpecies::wl_off / bl_off).  STAGE: the species' block is first copied to LDS with batched 16-byte
 // loads (one memory round trip instead of one per k-step); otherwise it is read from global memory.
-template <bool STAGE>
+// BF: the first layer's forward product and the dE/dG product run on BF16 matrix cores (three-product
+// split above) with the weights from w0bf (DevSpecies::bf_off); the other layers stay FP32.
+template <bool STAGE, bool BF>
 __global__ __launch_bounds__(64) void k_mlp_energy_grad(
-    const DevModel* __restrict__ M, const float* __restrict__ wpad, const float* __restrict__ inpw,
-    const int4* __restrict__ tiles, int ldg, const float* __restrict__ G, float* __restrict__ dEdG,
-    const int* __restrict__ atom_of_row, double* __restrict__ e_atom) {
+    const DevModel* __restrict__ M, const float* __restrict__ wpad, const __bf16* __restrict__ w0bf,
+    const float* __restrict__ inpw, const int4* __restrict__ tiles, int ldg, const float* __restrict__ G,
+    float* __restrict__ dEdG, const int* __restrict__ atom_of_row, double* __restrict__ e_atom) {
     __shared__ int s_off[2 * ACE_MAXL + 2];   // A_l, D_l offsets; weights offset
     __shared__ DevSpecies s_sp;                 // the species' tables, read from LDS in the layer loops
     const int4 tile = tiles[blockIdx.x];
@@ -196,22 +278,25 @@ __global__ __launch_bounds__(64) void k_mlp_energy_grad(
         const float* bias = STAGE ? nullptr : Wg + sp.bl_off[l];
         const int boff = wofs + sp.bl_off[l];
         const float cc = sp.c;
-        mfma_layer<STAGE>(ain, lda, ni, wofs + sp.wl_off[l], Wg + sp.wl_off[l], 1, mlp_ld(no), no, lane,
-                          [&](const mfma_f4* acc, int nt, int o0) {
-                              for (int t = 0; t < nt; ++t) {
-                                  const int o = o0 + 16 * t + cr;
-                                  if (o < no) {
-                                      const float bo = STAGE ? mlp_smem[boff + o] : bias[o];
+        auto epi = [&](const mfma_f4* acc, int nt, int o0) {
+            for (int t = 0; t < nt; ++t) {
+                const int o = o0 + 16 * t + cr;
+                if (o < no) {
+                    const float bo = STAGE ? mlp_smem[boff + o] : bias[o];
 #pragma unroll
-                                      for (int r = 0; r < 4; ++r) {
-                                          float a, d;
-                                          act_fd(act, cc, acc[t][r] + bo, a, d);
-                                          mlp_smem[ao + (rq + r) * ldo + o] = a;
-                                          mlp_smem[dofs + (rq + r) * ldo + o] = d;
-                                      }
-                                  }
-                              }
-                          });
+                    for (int r = 0; r < 4; ++r) {
+                        float a, d;
+                        act_fd(act, cc, acc[t][r] + bo, a, d);
+                        mlp_smem[ao + (rq + r) * ldo + o] = a;
+                        mlp_smem[dofs + (rq + r) * ldo + o] = d;
+                    }
+                }
+            }
+        };
+        if (BF && l == 0)   // forward block: B[k][o] = W_0[o][k] at [o][k], round16(w_1) x round32(K) per part
+            bf3_layer(ain, lda, ni, w0bf + sp.bf_off, ((no + 15) & ~15) * ((ni + 31) & ~31), no, lane, epi);
+        else
+            mfma_layer<STAGE>(ain, lda, ni, wofs + sp.wl_off[l], Wg + sp.wl_off[l], 1, mlp_ld(no), no, lane, epi);
         ain = ao;
         lda = ldo;
         wave_sync();
@@ -248,17 +333,22 @@ __global__ __launch_bounds__(64) void k_mlp_energy_grad(
         const int no = sp.width[1];
         const float* iw = inpw + sp.in_off;
         const int a0 = s_off[0], ld0 = mlp_ld(no);
-        mfma_layer<STAGE>(a0, ld0, no, wofs + sp.wl_off[0], Wg + sp.wl_off[0], mlp_ld(no), 1, K, lane,
-                          [&](const mfma_f4* acc, int nt, int k0) {
-                              for (int t = 0; t < nt; ++t) {
-                                  const int k = k0 + 16 * t + cr;
-                                  if (k < K) {
-                                      const float w = iw[k];
+        auto epi = [&](const mfma_f4* acc, int nt, int k0) {
+            for (int t = 0; t < nt; ++t) {
+                const int k = k0 + 16 * t + cr;
+                if (k < K) {
+                    const float w = iw[k];
 #pragma unroll
-                                      for (int r = 0; r < 4; ++r)
-                                          if (rq + r < nr) dEdG[(long long)(r0 + rq + r) * ldg + k] = acc[t][r] * w;
-                                  }
-                              }
-                          });
+                    for (int r = 0; r < 4; ++r)
+                        if (rq + r < nr) dEdG[(long long)(r0 + rq + r) * ldg + k] = acc[t][r] * w;
+                }
+            }
+        };
+        if (BF) {   // backward block after the forward one: B[o][k] = W_0[o][k] at [k][o], round16(K) x round32(w_1)
+            const int fwd = ((no + 15) & ~15) * ((K + 31) & ~31);
+            bf3_layer(a0, ld0, no, w0bf + sp.bf_off + 2 * fwd, ((K + 15) & ~15) * ((no + 31) & ~31), K, lane, epi);
+        } else {
+            mfma_layer<STAGE>(a0, ld0, no, wofs + sp.wl_off[0], Wg + sp.wl_off[0], mlp_ld(no), 1, K, lane, epi);
+        }
     }
 }

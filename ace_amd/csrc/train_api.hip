@@ -86,7 +86,7 @@ int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* 
         if (run_basis(ctx, 0)) return -1;
         T.g_valid = true;
     }
-    if (run_mlp(ctx)) return -1;
+    if (run_mlp(ctx, false)) return -1;
     if (wf) {
         if (run_basis(ctx, 1)) return -1;
         if (run_finalize(ctx)) return -1;

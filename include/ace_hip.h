@@ -44,7 +44,9 @@ enum { ACE_ACT_LINEAR = 1, ACE_ACT_SIGMOID = 2, ACE_ACT_TANH = 3, ACE_ACT_SWISH 
        ACE_ACT_SOFTPLUS = 15 };
 /* ACE_PREC_FP32: symmetry functions, networks and adjoints in FP32 (FP64 reductions).
  * ACE_PREC_FP64_BASIS: symmetry functions and their adjoint/tangent in FP64, networks in FP32.
- * ACE_PREC_BF16L1: reserved (first network layer in BF16). */
+ * ACE_PREC_BF16L1 (ace_energy_forces / ace_compute only): as FP32, with the first network layer's forward
+ *   product and its dE/dG product on BF16 matrix cores with FP32 sums; each operand is split into BF16
+ *   high and low parts and three products are summed (hi*hi + hi*lo + lo*hi). */
 enum { ACE_PREC_FP32 = 0, ACE_PREC_BF16L1 = 1, ACE_PREC_FP64_BASIS = 2 };
 enum { ACE_LOSS_MSE = 0, ACE_LOSS_MAE = 1, ACE_LOSS_HUBER = 2, ACE_LOSS_ASINH = 3 };
 enum { ACE_NORM_IDENTITY = 0, ACE_NORM_LINEAR = 1, ACE_NORM_SQRT = 2, ACE_NORM_CBRT = 3,
@@ -159,7 +161,8 @@ int  ace_symm(ace_ctx* ctx, double* G, int ldg);
  *   E_struc[s] = sum_i (NN_i + E0_i)            (NNP::compute_energy convention, FP64 reduction)
  *   F          = -dE/dr, sum(natoms) x 3         (NNP::compute_force)
  *   virial     = n_struc x 9, W_ab = -sum_pairs d_a dE/dd_b (FP64 reduction); may be NULL
- * precision: ACE_PREC_FP32 or ACE_PREC_FP64_BASIS.  Output pointers may be NULL: results stay in HBM.
+ * precision: ACE_PREC_FP32, ACE_PREC_FP64_BASIS or ACE_PREC_BF16L1.  Output pointers may be NULL: results
+ * stay in HBM.
  */
 int  ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, double* virial);
 

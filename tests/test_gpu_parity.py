@@ -7,13 +7,15 @@ Tolerances (written per mode):
       G 1e-6 relative; energy 1e-6 eV/atom; forces 2e-5 eV/A.
 The config-1 crystal (108 nearly identical fcc environments) is ill-conditioned for FP32 basis
 functions: its input preconditioning divides by a G spread of ~2% of G, so its FP32 energy bound is
-1e-5 eV/atom; the FP64_BASIS mode meets 1e-6.  The plain FP64 port must match to 1e-9."""
+1e-5 eV/atom; the FP64_BASIS mode meets 1e-6.  The plain FP64 port must match to 1e-9.
+  BF16L1 (FP32 basis; first network layer on BF16 matrix cores, hi/lo split operands, FP32 sums):
+      energy 1e-4 eV/atom; forces 1e-3 eV/A."""
 import os
 
 import numpy as np
 import pytest
 
-from ace_amd import PREC_FP32, PREC_FP64_BASIS, NNPDevice, configs
+from ace_amd import PREC_BF16L1, PREC_FP32, PREC_FP64_BASIS, NNPDevice, configs
 from oracle import oracle as O
 
 from conftest import log
@@ -305,3 +307,85 @@ def test_compute_entry_matches_build_then_energy():
     assert dev.lib.ace_compute(dev.ctx, 0, 0.0, dp(E), dp(F), dp(W)) == 0
     dev.close()
     assert E[0] == E0[0] and np.array_equal(F, F0) and np.array_equal(W.reshape(W0.shape), W0)
+
+
+BF16_TOL = dict(E=1e-4, F=1e-3)
+
+
+def check_bf16(model, strucs, name):
+    """BF16L1 energies and forces vs the FP64 oracle, next to the FP32 mode on the same batch."""
+    orc = O.Oracle(model)
+    dev = NNPDevice(model)
+    dev.upload(strucs)
+    dev.build_neighbors()
+    Eb, Fb, Wb = dev.compute(PREC_BF16L1)
+    Ef, Ff, _ = dev.compute(PREC_FP32)
+    a0 = 0
+    for k, s in enumerate(strucs):
+        n = s.natoms
+        Er, Fr, Wr = orc.compute(s)
+        de, df = abs(Eb[k] - Er) / n, np.max(np.abs(Fb[a0:a0 + n] - Fr))
+        de32, df32 = abs(Ef[k] - Er) / n, np.max(np.abs(Ff[a0:a0 + n] - Fr))
+        dw = np.max(np.abs(Wb[k] - Wr)) / max(1.0, np.max(np.abs(Wr)))
+        log(f"{name}[{k}] bf16l1 n={n} dE/atom {de:.1e} dF {df:.1e} dW {dw:.1e} (fp32: {de32:.1e} {df32:.1e})")
+        assert de < BF16_TOL["E"] and df < BF16_TOL["F"] and dw < 1e-3, (name, k)
+        a0 += n
+    dev.close()
+
+
+def test_bf16_first_layer():
+    """128-128-64-1 networks on 4 species (the mixed-precision inference model), random packings of
+    30 to 300 atoms, and the 80-32-32-1 model, which otherwise takes the fused FP32 kernel."""
+    rng = np.random.default_rng(21)
+    m = configs.c5_model(rng)
+    strucs = []
+    for n in (30, 111, 300):
+        cell, pos = configs.random_packed(n, 0.085, 1.5, rng)
+        strucs.append(structure(cell, pos, rng.integers(0, 4, n)))
+    scaled_model(m, strucs)
+    check_bf16(m, strucs, "bf16-c5model")
+    m2, strucs2 = configs.config2()
+    scaled_model(m2, strucs2)
+    check_bf16(m2, strucs2, "bf16-c2")
+
+
+def test_c5_full_size():
+    """The 256-structure, 568k-atom mixed-precision inference batch: FP32 and BF16L1 energies and forces
+    of the two smallest structures vs the oracle, and those two structures uploaded alone give the same
+    bits as inside the batch (every per-atom result is independent of the batch; sums are exact)."""
+    model, strucs = configs.config5()
+    N = sum(s.natoms for s in strucs)
+    dev = NNPDevice(model)
+    dev.upload(strucs)
+    dev.build_neighbors()
+    dev.set_precision(PREC_FP64_BASIS)
+    G = dev.compute_symm()
+    configs.set_input_scaling(model, strucs, G)     # PreBias MEAN / PreScale DEV over the whole batch
+    del G
+    dev.sync_model_scaling()
+    dev.set_precision(PREC_FP32)
+    res = {p: dev.compute(p) for p in (PREC_FP32, PREC_BF16L1)}
+    dev.close()
+    off = np.concatenate([[0], np.cumsum([s.natoms for s in strucs])])
+    small = [int(k) for k in np.argsort([s.natoms for s in strucs])[:2]]
+    orc = O.Oracle(model)
+    alone = NNPDevice(model)
+    alone.upload([strucs[k] for k in small])
+    alone.build_neighbors()
+    Ea = {p: alone.compute(p) for p in (PREC_FP32, PREC_BF16L1)}
+    alone.close()
+    a0 = 0
+    for q, k in enumerate(small):
+        s = strucs[k]
+        n = s.natoms
+        Er, Fr, _ = orc.compute(s)
+        for p, te, tf in ((PREC_FP32, TOL[PREC_FP32]["E"], TOL[PREC_FP32]["F"]),
+                          (PREC_BF16L1, BF16_TOL["E"], BF16_TOL["F"])):
+            E, F, W = res[p]
+            de = abs(E[k] - Er) / n
+            df = np.max(np.abs(F[off[k]:off[k] + n] - Fr))
+            log(f"c5 ({N} atoms) struct {k} n={n} prec {p}: dE/atom {de:.1e} dF {df:.1e}")
+            assert de < te and df < tf, (k, p)
+            e1, f1, w1 = Ea[p]
+            assert e1[q] == E[k] and np.array_equal(f1[a0:a0 + n], F[off[k]:off[k] + n]) and np.array_equal(w1[q], W[k])
+        a0 += n

@@ -36,6 +36,9 @@ SIGNATURES = {
     "ace_symm": (C.c_int, [P, dp, C.c_int]),
     "ace_energy_forces": (C.c_int, [P, C.c_int, dp, dp, dp]),
     "ace_compute": (C.c_int, [P, C.c_int, C.c_double, dp, dp, dp]),
+    "ace_md_init": (C.c_int, [P, dp, dp]),
+    "ace_md_run": (C.c_int, [P, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, dp]),
+    "ace_md_state": (C.c_int, [P, dp, dp]),
     "ace_plain_energy_forces": (C.c_int, [P, dp, dp]),
     "ace_set_targets": (C.c_int, [P, dp, dp]),
     "ace_train_step": (C.c_int, [P, P, dp, dp]),

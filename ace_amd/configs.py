@@ -196,6 +196,40 @@ def config4(seed=0xACE4, ncell=30):
     return m, Structure(cell, pos, t), masses
 
 
+KB_EV = 8.617333262e-5     # Boltzmann constant, eV/K
+MVV2E = 1.0364269e-4       # eV per amu (A/ps)^2 (metal units)
+
+
+def maxwell_boltzmann(atom_masses, T, rng):
+    """Velocities (A/ps) drawn from the Maxwell-Boltzmann distribution at T kelvin (per component
+    sigma = sqrt(kB T / m)), with the centre-of-mass velocity removed."""
+    m = np.asarray(atom_masses, dtype=float)[:, None]
+    v = rng.normal(size=(len(m), 3)) * np.sqrt(KB_EV * T / (m * MVV2E))
+    v -= (m * v).sum(axis=0) / m.sum()
+    return v
+
+
+def scale_output(model: Model, factor):
+    """Multiplies every network's output layer (weights and bias, the [biases][weights col-major] layout of
+    src/ml/nn.cpp:881-896) by factor, which scales NN energies (about outb) and all forces by factor."""
+    for n in model.nets:
+        w = n.widths
+        nb = sum(w[1:])
+        n.params = np.array(n.params, dtype=float)
+        n.params[nb - 1] *= factor                 # bias of the width-1 output layer
+        n.params[len(n.params) - w[-2]:] *= factor   # its 1 x w[-2] weights, stored last
+
+
+def calibrate_forces(model: Model, F, rms=0.1):
+    """Random-initialised networks on a nearly perfect crystal give forces of several eV/A (the input
+    scaling divides by the small spread of G); molecular dynamics with 1 fs steps needs the force scale of a
+    real solid.  Scales the output layers so that the force components of F have the given rms (eV/A);
+    returns the factor."""
+    f = rms / float(np.sqrt(np.mean(np.asarray(F) ** 2)))
+    scale_output(model, f)
+    return f
+
+
 def c5_model(rng):
     rad = 0.01 * 200.0 ** (np.arange(8) / 7.0)
     zl = zeta_lambda([1, 2, 4, 8, 16])

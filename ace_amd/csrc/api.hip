@@ -17,6 +17,7 @@
 #include "reduce.hip"
 #include "plain.hip"
 #include "train.hip"
+#include "md.hip"
 
 namespace {
 
@@ -105,6 +106,15 @@ struct ace_ctx {
     DBuf<double> d_G64;
     // training
     TrainState tr;
+    // molecular dynamics: velocities, per-species 1/(m mvv2e) then m mvv2e, per-step energies, step counter,
+    // and the captured two-step graphs (by the accumulator buffer the first step adds into)
+    DBuf<double> d_vel, d_minv;
+    DBuf<unsigned long long> d_md_rec;
+    DBuf<unsigned int> d_md_ctr;
+    bool md_ready = false;
+    hipGraphExec_t md_exec[2] = {nullptr, nullptr};
+    long long md_sig[2][8] = {};
+    unsigned long long gen = 0;   // counts device reallocations: captured graphs hold raw pointers
     // profiling
     bool prof = false;
     std::vector<ProfRec> pending;
@@ -133,6 +143,7 @@ int ensure(ace_ctx* ctx, DBuf<T>& b, size_t n) {
     if (b.p) HC(hipFree(b.p));
     b.p = nullptr;
     b.cap = 0;
+    ++ctx->gen;
     HC(hipMalloc((void**)&b.p, n * sizeof(T)));
     b.cap = n;
     return 0;
@@ -782,6 +793,74 @@ int run_mlp(ace_ctx* ctx, bool bf16) {
     });
 }
 
+// the energy/force launches (fused, or symm -> network -> adjoint); forces, energies and virials land in
+// the fixed-point accumulators.  The BF16 first layer runs on the matrix cores of the network kernel, so
+// that mode takes the split path.
+int run_ef(ace_ctx* ctx, int precision) {
+    if (precision != ACE_PREC_BF16L1 && use_fused(ctx)) return run_basis(ctx, 3);
+    if (run_basis(ctx, 0)) return -1;
+    if (run_mlp(ctx, precision == ACE_PREC_BF16L1)) return -1;
+    return run_basis(ctx, 1);
+}
+
+// one velocity-Verlet step: half kick with the forces of the last gradient launch, drift, energy/force
+// launches at the new positions (neighbour keys of the current list, which includes the skin), half kick
+int md_step(ace_ctx* ctx, int precision, double dt, int rec_cap) {
+    const int N = ctx->N, nb = cdiv(N, MD_THREADS);
+    if (timed(ctx, "md_kick_drift", [&] {
+            k_md_kick_drift<<<nb, MD_THREADS, 0, ctx->stream>>>(N, 0.5 * dt, dt, ctx->d_type.p, ctx->d_minv.p,
+                                                                ctx->d_Ffix[1 - ctx->fbuf].p, ctx->d_pos.p,
+                                                                ctx->d_vel.p, ctx->d_md_ctr.p);
+        }))
+        return -1;
+    if (run_ef(ctx, precision)) return -1;
+    const int b = 1 - ctx->fbuf;
+    return timed(ctx, "md_kick", [&] {
+        k_md_kick<<<nb, MD_THREADS, 0, ctx->stream>>>(N, ctx->S, 0.5 * dt, ctx->d_type.p, ctx->d_minv.p,
+                                                      ctx->d_minv.p + ctx->nt, ctx->d_Ffix[b].p, ctx->d_Efix[b].p,
+                                                      ctx->d_vel.p, ctx->d_md_rec.p, rec_cap, ctx->d_md_ctr.p);
+    });
+}
+
+// A graph of two steps leaves the accumulator buffers as it found them, so it can be replayed any number
+// of times; md_exec[p] starts with buffer p.  It is captured again when a buffer was reallocated or the
+// batch, precision, time step, list capacity or record size changed (all baked into its launches).
+int md_graph(ace_ctx* ctx, int precision, double dt, int rec_cap, hipGraphExec_t* out) {
+    const int p = ctx->fbuf;
+    long long dtbits;
+    memcpy(&dtbits, &dt, 8);
+    const long long sig[8] = {(long long)ctx->gen, precision, dtbits, rec_cap, ctx->slots_cap, ctx->N, ctx->S, 1};
+    if (ctx->md_exec[p] && memcmp(sig, ctx->md_sig[p], sizeof(sig)) == 0) {
+        *out = ctx->md_exec[p];
+        return 0;
+    }
+    if (ctx->md_exec[p]) {
+        HC(hipGraphExecDestroy(ctx->md_exec[p]));
+        ctx->md_exec[p] = nullptr;
+    }
+    const bool prof = ctx->prof;
+    ctx->prof = false;
+    HC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    const int rc = (md_step(ctx, precision, dt, rec_cap) || md_step(ctx, precision, dt, rec_cap)) ? -1 : 0;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    ctx->prof = prof;
+    ctx->fbuf = p;
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? -1 : fail_hip(ctx, "md: graph capture", e);
+    }
+    const hipError_t ei = hipGraphInstantiate(&ctx->md_exec[p], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        ctx->md_exec[p] = nullptr;
+        return fail_hip(ctx, "md: graph instantiate", ei);
+    }
+    memcpy(ctx->md_sig[p], sig, sizeof(sig));
+    *out = ctx->md_exec[p];
+    return 0;
+}
+
 int check_err_flag(ace_ctx* ctx, const char* what) {
     int e = 0;
     HC(hipMemcpyAsync(&e, ctx->d_scal.p + 3, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -846,6 +925,9 @@ void ace_destroy(ace_ctx* ctx) {
     release(ctx->d_nb_count);
     release(ctx->d_nb_off); release(ctx->d_scan_blk); release(ctx->d_scal); release(ctx->d_nb_j);
     release(ctx->d_nb_img); release(ctx->d_G64);
+    release(ctx->d_vel); release(ctx->d_minv); release(ctx->d_md_rec); release(ctx->d_md_ctr);
+    for (int q = 0; q < 2; ++q)
+        if (ctx->md_exec[q]) (void)hipGraphExecDestroy(ctx->md_exec[q]);
     train_release(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1166,14 +1248,7 @@ int ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, d
         return fail(ctx, "ace_energy_forces: precision mode not available");
     ctx->basis_f64 = precision == ACE_PREC_FP64_BASIS;
     if (require_nb(ctx)) return -1;
-    // the BF16 first layer runs on the matrix cores of the network kernel, so that mode takes the split path
-    if (precision != ACE_PREC_BF16L1 && use_fused(ctx)) {
-        if (run_basis(ctx, 3)) return -1;
-    } else {
-        if (run_basis(ctx, 0)) return -1;
-        if (run_mlp(ctx, precision == ACE_PREC_BF16L1)) return -1;
-        if (run_basis(ctx, 1)) return -1;
-    }
+    if (run_ef(ctx, precision)) return -1;
     if (E_struc || F || virial || ctx->prof) {
         if (ctx->nb_pending && nb_verify(ctx)) {
             // an asynchronous rebuild outgrew its capacity: rebuild with the exact size and recompute
@@ -1197,6 +1272,106 @@ int ace_compute(ace_ctx* ctx, int precision, double skin, double* E_struc, doubl
     if (!ctx) return -1;
     if (ace_build_neighbors(ctx, skin)) return -1;
     return ace_energy_forces(ctx, precision, E_struc, F, virial);
+}
+
+int ace_md_init(ace_ctx* ctx, const double* mass, const double* vel) {
+    if (!ctx) return -1;
+    if (ctx->S == 0) return fail(ctx, "ace_md_init: no structures uploaded");
+    if (!mass) return fail(ctx, "ace_md_init: masses required");
+    std::vector<double> mi(2 * ctx->nt);
+    for (int I = 0; I < ctx->nt; ++I) {
+        if (!(mass[I] > 0)) return fail(ctx, "ace_md_init: masses must be positive");
+        mi[I] = 1.0 / (mass[I] * MD_MVV2E);
+        mi[ctx->nt + I] = mass[I] * MD_MVV2E;
+    }
+    if (upload(ctx, ctx->d_minv, mi.data(), mi.size())) return -1;
+    if (ensure(ctx, ctx->d_vel, 3 * (size_t)ctx->N) || ensure(ctx, ctx->d_md_ctr, 2)) return -1;
+    if (vel)
+        HC(hipMemcpyAsync(ctx->d_vel.p, vel, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    else
+        HC(hipMemsetAsync(ctx->d_vel.p, 0, 3 * (size_t)ctx->N * sizeof(double), ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    ctx->md_ready = true;
+    return 0;
+}
+
+int ace_md_run(ace_ctx* ctx, int precision, int nsteps, double dt, int rebuild_every, double skin, int use_graph,
+               double* energy) {
+    if (!ctx) return -1;
+    if (!ctx->md_ready) return fail(ctx, "ace_md_run: call ace_md_init first");
+    if (precision != ACE_PREC_FP32 && precision != ACE_PREC_FP64_BASIS && precision != ACE_PREC_BF16L1)
+        return fail(ctx, "ace_md_run: precision mode not available");
+    if (nsteps < 0 || !(dt > 0) || rebuild_every < 1 || !(skin >= 0))
+        return fail(ctx, "ace_md_run: need nsteps >= 0, dt > 0, rebuild_every >= 1, skin >= 0");
+    ctx->basis_f64 = precision == ACE_PREC_FP64_BASIS;
+    const int rec_cap = std::max(nsteps, 1);
+    if (ensure(ctx, ctx->d_md_rec, 2 * (size_t)rec_cap)) return -1;
+    // record and step counter cleared outside any captured graph
+    HC(hipMemsetAsync(ctx->d_md_rec.p, 0, 2 * (size_t)rec_cap * sizeof(unsigned long long), ctx->stream));
+    HC(hipMemsetAsync(ctx->d_md_ctr.p, 0, 2 * sizeof(unsigned int), ctx->stream));   // step counter, flags
+    auto collect_flags = [&] {
+        return timed(ctx, "md_flags", [&] { k_md_flags<<<1, 64, 0, ctx->stream>>>(ctx->d_scal.p, ctx->d_md_ctr.p + 1); });
+    };
+    // A rebuild that reuses the row capacity is checked before any step uses it (one host round trip per
+    // rebuild); if an atom outgrew the rows, the list is built again with a counting pass and resized rows
+    // (the graphs are then captured again).  The flags left by the previous list are collected first, except
+    // before the run's first build (they belong to an earlier call).
+    auto rebuild = [&](bool collect) -> int {
+        if ((collect && collect_flags()) || ace_build_neighbors(ctx, skin)) return -1;
+        if (!ctx->nb_pending) return 0;
+        int over = 0;
+        HC(hipMemcpyAsync(&over, ctx->d_scal.p + 5, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HC(hipStreamSynchronize(ctx->stream));
+        ctx->nb_pending = false;
+        if (!over) return 0;
+        ctx->nb_async_ok = false;
+        return ace_build_neighbors(ctx, skin);
+    };
+    // list radius rc + skin; forces at the starting positions
+    if (rebuild(false)) return -1;
+    if (require_nb(ctx)) return -1;
+    if (run_ef(ctx, precision)) return -1;
+    ctx->tr.g_valid = false;
+    for (int k = 0; k < nsteps;) {
+        if (k > 0 && k % rebuild_every == 0 && rebuild(true)) return -1;
+        const int c = std::min(nsteps - k, rebuild_every - k % rebuild_every);
+        int done = 0;
+        if (use_graph) {
+            for (; done + 2 <= c; done += 2) {
+                hipGraphExec_t ex = nullptr;
+                if (md_graph(ctx, precision, dt, rec_cap, &ex)) return -1;
+                HC(hipGraphLaunch(ex, ctx->stream));
+            }
+        }
+        for (; done < c; ++done)
+            if (md_step(ctx, precision, dt, rec_cap)) return -1;
+        k += c;
+    }
+    if (collect_flags()) return -1;
+    unsigned int bad = 0;
+    HC(hipMemcpyAsync(&bad, ctx->d_md_ctr.p + 1, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    if (ctx->nb_pending && nb_verify(ctx)) return -1;
+    if (bad) {
+        ctx->nb_async_ok = false;   // the next build counts again and sizes the rows to the new maximum
+        return fail(ctx, "ace_md_run: an atom outgrew the neighbour-list capacity during the run; run again");
+    }
+    if (energy && nsteps > 0) {
+        std::vector<long long> r(2 * (size_t)nsteps);
+        HC(hipMemcpyAsync(r.data(), ctx->d_md_rec.p, r.size() * sizeof(long long), hipMemcpyDeviceToHost, ctx->stream));
+        HC(hipStreamSynchronize(ctx->stream));
+        for (size_t q = 0; q < r.size(); ++q) energy[q] = (double)r[q] * MD_FIX_INV;
+    }
+    return flush_prof(ctx);
+}
+
+int ace_md_state(ace_ctx* ctx, double* pos, double* vel) {
+    if (!ctx) return -1;
+    if (!ctx->md_ready) return fail(ctx, "ace_md_state: call ace_md_init first");
+    if (pos) HC(hipMemcpyAsync(pos, ctx->d_pos.p, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (vel) HC(hipMemcpyAsync(vel, ctx->d_vel.p, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    return 0;
 }
 
 int ace_plain_energy_forces(ace_ctx* ctx, double* E_struc, double* F) {

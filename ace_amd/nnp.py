@@ -172,6 +172,27 @@ class NNPDevice:
         forces, in one call; results stay in HBM."""
         self._check(self.lib.ace_compute(self.ctx, precision, float(skin), None, None, None), "ace_compute")
 
+    def md_init(self, masses, vel=None):
+        """Per-species masses (amu) and velocities (N x 3, A/ps; None = at rest) for md_run."""
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(self.N, 3)
+        self._check(self.lib.ace_md_init(self.ctx, _dp(m), _dp(v)), "ace_md_init")
+
+    def md_run(self, nsteps, dt=1e-3, rebuild_every=20, skin=1.0, precision=PREC_FP32, graph=True):
+        """nsteps velocity-Verlet steps of dt ps (metal units); returns per-step (potential, kinetic) energy in
+        eV, shape (nsteps, 2)."""
+        e = np.zeros((max(nsteps, 1), 2))
+        self._check(self.lib.ace_md_run(self.ctx, precision, nsteps, float(dt), rebuild_every, float(skin),
+                                        1 if graph else 0, _dp(e)), "ace_md_run")
+        return e[:nsteps]
+
+    def md_state(self):
+        """Current positions (A) and velocities (A/ps) on the device."""
+        pos = np.zeros((self.N, 3))
+        vel = np.zeros((self.N, 3))
+        self._check(self.lib.ace_md_state(self.ctx, _dp(pos), _dp(vel)), "ace_md_state")
+        return pos, vel
+
     def plain_compute(self):
         """The unbatched FP64 GPU port: per-structure energies and forces."""
         E = np.zeros(len(self.natoms))

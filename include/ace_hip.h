@@ -21,6 +21,7 @@
  *   ace_train_step ................. NNPTEFR::error_cost_o1 (src/nnp/nnptefr.cpp:1292-1425)
  *                                   extended with the force-matching term d(F)/dW
  *                                   (network factor NN::D2ODZDI, src/ml/nn.cpp:2047-2140)
+ *   ace_md_init / ace_md_run / ace_md_state  velocity-Verlet steps (src/sim/integrator.cpp:548-571), HIP graphs
  *   ace_plain_energy_forces ........ same as ace_energy_forces, unbatched FP64 one-thread-per-atom
  *                                   GPU port (the baseline the batched path is measured against)
  */
@@ -169,6 +170,21 @@ int  ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, 
 /* NNP::compute(Structure&) (src/nnp/nnp.cpp:489-494): neighbour list with list radius rc + skin, then
  * ace_energy_forces, in one call. */
 int  ace_compute(ace_ctx* ctx, int precision, double skin, double* E_struc, double* F, double* virial);
+
+/* Velocity-Verlet molecular dynamics of the uploaded batch, metal units (A, ps, amu, eV).  No counterpart in
+ * ace's NNP API: ace has no NNP molecular dynamics; the step follows its Verlet template
+ * (src/sim/integrator.cpp:548-571: half kick, drift, forces, half kick), with a = F / (m * 1.0364269e-4).
+ * ace_md_init: per-species masses (amu) and velocities (N x 3, A/ps; NULL = zero).
+ * ace_md_run: nsteps steps of dt (ps).  The neighbour list (radius rc + skin) is rebuilt before the first
+ *   step and every rebuild_every steps, outside any graph; with use_graph != 0 the steps in between are
+ *   replayed from a captured HIP graph of two steps (bit-identical to use_graph = 0).  energy: NULL or
+ *   nsteps x 2 (potential energy of the batch with E0, kinetic energy; eV) after each step.  Positions and
+ *   velocities stay on the device.
+ * ace_md_state: current positions and velocities (either pointer may be NULL). */
+int  ace_md_init(ace_ctx* ctx, const double* mass, const double* vel);
+int  ace_md_run(ace_ctx* ctx, int precision, int nsteps, double dt, int rebuild_every, double skin, int use_graph,
+                double* energy);
+int  ace_md_state(ace_ctx* ctx, double* pos, double* vel);
 
 /* Same quantities from the unbatched FP64 port: one structure per launch, one thread per atom. */
 int  ace_plain_energy_forces(ace_ctx* ctx, double* E_struc, double* F);

@@ -46,3 +46,18 @@ def test_adam_matches_oracle():
         p = a.step(p, g)
         L.or_adam_step(dim, O._d(q), O._d(g), 0.01, 1e-16, O._d(m), O._d(v), O._d(b))
     assert np.allclose(p, q, rtol=0, atol=1e-15)
+
+
+def test_output_scaling_scales_energies_and_forces():
+    """configs.scale_output multiplies the output layer, so NN energies (outb = 0, E0 = 0) and forces
+    scale by exactly the factor (oracle, FP64)."""
+    model, [s] = configs.config1()
+    orc = O.Oracle(model)
+    E1, F1, _ = orc.compute(s)
+    configs.scale_output(model, 0.37)
+    orc = O.Oracle(model)
+    E2, F2, _ = orc.compute(s)
+    assert np.isclose(E2, 0.37 * E1, rtol=1e-12, atol=0)
+    assert np.allclose(F2, 0.37 * F1, rtol=1e-10, atol=1e-14)
+    f = configs.calibrate_forces(model, F2, rms=0.1)
+    assert np.isclose(np.sqrt(np.mean((f * F2) ** 2)), 0.1, rtol=1e-12)

@@ -115,6 +115,91 @@ def train_bench(steps):
             "workload": "64 structures, N log-uniform [32,1024], 3 elements, 56-64-64-1, energy + 0.1 force MSE"}
 
 
+def md_bench(steps):
+    """Molecular dynamics throughput: 108,000-atom fcc cell (30^3 conventional cells, 0.085 atoms/A^3), 2
+    elements 70/30 with masses 27/16 amu, Maxwell-Boltzmann velocities at 300 K, list radius 6 + 1 A rebuilt
+    every 20 steps, 80-32-32-1 networks (output layers scaled to an rms force component of 0.1 eV/A),
+    velocity Verlet with dt = 1 fs, steps between rebuilds replayed from HIP graphs.  The FP32 run's energy
+    drift is reported next to the same trajectory length with FP64 symmetry functions."""
+    from ace_amd import PREC_FP32, PREC_FP64_BASIS, NNPDevice, configs
+    model, s, masses = configs.config4()
+    v0 = configs.maxwell_boltzmann(masses[s.type], 300.0, np.random.default_rng(0xACE4))
+    dev = NNPDevice(model)
+    dev.upload([s])
+    dev.build_neighbors()
+    configs.set_input_scaling(model, [s], dev.compute_symm())
+    dev.sync_model_scaling()
+    _, F, _ = dev.compute()
+    configs.calibrate_forces(model, F, rms=0.1)
+    for I in range(model.ntypes):
+        dev.set_params(I, model.nets[I].params)
+    out = {}
+    for prec, name in ((PREC_FP32, "fp32"), (PREC_FP64_BASIS, "fp64_basis")):
+        dev.update_positions(s.pos)
+        dev.md_init(masses, v0)
+        dev.md_run(20, precision=prec)                 # warm-up: list sizing, graph capture
+        dev.update_positions(s.pos)
+        dev.md_init(masses, v0)
+        dev.sync()
+        t0 = time.perf_counter()
+        e = dev.md_run(steps, precision=prec)
+        dt = time.perf_counter() - t0
+        tot = e.sum(axis=1)
+        out[name] = {"steps_per_s": steps / dt, "ms_per_step": dt / steps * 1e3,
+                     "atom_steps_per_s": s.natoms * steps / dt,
+                     "temperature_end_K": float(2 * e[-1, 1] / (3 * s.natoms * configs.KB_EV)),
+                     "drift": (tot - tot[0]) / s.natoms}
+    dev.close()
+    d32, d64 = out["fp32"].pop("drift"), out["fp64_basis"].pop("drift")
+    for name, d in (("fp32", d32), ("fp64_basis", d64)):
+        out[name]["max_abs_energy_drift_per_atom"] = float(np.abs(d).max())
+        out[name]["end_energy_drift_per_atom"] = float(d[-1])
+    return {"metric": "MD steps/s", "value": out["fp32"]["steps_per_s"], "unit": "steps/s", "atoms": int(s.natoms),
+            "steps": steps, "dt_fs": 1.0, "rebuild_every": 20, "skin_A": 1.0, "dtype": "fp32", "graph": True,
+            "fp32": out["fp32"], "fp64_basis": out["fp64_basis"],
+            "max_abs_drift_difference_fp32_vs_fp64_basis": float(np.abs(d32 - d64).max()),
+            "workload": "fcc 108,000 atoms, 2 elements 70/30, 300 K, velocity Verlet 1 fs, list 7 A rebuilt every 20 steps"}
+
+
+def c5_bench(steps):
+    """Mixed-precision batched inference: 256 structures of 256-8192 atoms (log-uniform, 568,089 atoms), 4
+    elements, rc 6.5 A, 32 G2 + 96 G4, 128-128-64-1; FP32, and the first layer on BF16 matrix cores.
+    Per-structure errors against the FP64 oracle for the two smallest structures."""
+    from ace_amd import PREC_BF16L1, PREC_FP32, PREC_FP64_BASIS, NNPDevice, configs
+    from oracle import oracle as O
+    model, strucs = configs.config5()
+    N = sum(x.natoms for x in strucs)
+    dev = NNPDevice(model)
+    dev.upload(strucs)
+    dev.build_neighbors()
+    dev.set_precision(PREC_FP64_BASIS)
+    configs.set_input_scaling(model, strucs, dev.compute_symm())
+    dev.sync_model_scaling()
+    dev.set_precision(PREC_FP32)
+    res = {"metric": "atoms/s E+F", "unit": "atoms/s", "structures": len(strucs), "atoms": int(N), "steps": steps,
+           "workload": "256 structures, N log-uniform [256,8192], 4 elements, rc 6.5, 128-128-64-1"}
+    off = np.concatenate([[0], np.cumsum([x.natoms for x in strucs])])
+    small = [int(k) for k in np.argsort([x.natoms for x in strucs])[:2]]
+    orc = O.Oracle(model)
+    ref = {k: orc.compute(strucs[k]) for k in small}
+    for prec, name in ((PREC_FP32, "fp32"), (PREC_BF16L1, "bf16_layer1")):
+        dev.compute_step(prec)
+        dev.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            dev.compute_step(prec)
+        dev.sync()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        E, F, _ = dev.compute(prec)
+        err = [{"structure": k, "atoms": strucs[k].natoms,
+                "energy_per_atom": abs(E[k] - ref[k][0]) / strucs[k].natoms,
+                "force_max_abs": float(np.max(np.abs(F[off[k]:off[k + 1]] - ref[k][1])))} for k in small]
+        res[name] = {"value": N / (ms * 1e-3), "ms_per_step": ms, "error_vs_fp64_oracle": err}
+    dev.close()
+    res["value"] = res["fp32"]["value"]
+    return res
+
+
 def copy_calibration(mib):
     import torch
     a = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
@@ -133,6 +218,8 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-steps", type=int, default=20)
+    ap.add_argument("--md-steps", type=int, default=1000)
+    ap.add_argument("--c5-steps", type=int, default=5)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 elementwise copies of MIB MiB (reads MIB, writes MIB): the reference point "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -141,8 +228,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, "
-              "'How the path would shard')", file=sys.stderr)
+        print("bench.py measures one GPU only: the multi-GPU path (structures over GPUs for training and "
+              "batched inference, spatial bricks with a ghost layer for MD) is not built", file=sys.stderr)
         sys.exit(2)
 
     from ace_amd import PREC_FP32, PREC_FP64_BASIS, NNPDevice, configs
@@ -221,6 +308,8 @@ def main():
     E, F, _ = dev.compute(PREC_FP32)
     E64, F64, _ = dev.compute(PREC_FP64_BASIS)
     train = train_bench(args.train_steps) if args.train_steps > 0 else None
+    md = md_bench(args.md_steps) if args.md_steps > 0 else None
+    c5 = c5_bench(args.c5_steps) if args.c5_steps > 0 else None
     avg64 = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kern64.items()}
     res = {
         "metric": METRIC,
@@ -249,6 +338,8 @@ def main():
         "plain_fp64_port": {"value": plain_value, "unit": "atoms/s", "ms_per_step": plain_ms,
                             "speedup_batched_vs_plain": value / plain_value},
         "train": train,
+        "md": md,
+        "mixed_precision_inference": c5,
     }
     if not args.no_cpu:
         cpu_value, (E_ref, F_ref, _) = cpu_baseline(model, s, args.cpu_samples)

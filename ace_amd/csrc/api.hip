@@ -712,8 +712,8 @@ int net_lds_floats(const ace_ctx* ctx) {
 
 int basis_bytes(ace_ctx* ctx, int mode) {
     const int nbm = nb_slots(ctx), netf = mode == 3 ? net_lds_floats(ctx) : 0;
-    return ctx->basis_f64 ? basis_lds_bytes<double>(nbm, ctx->ngk, ctx->kmax, mode, netf)
-                          : basis_lds_bytes<float>(nbm, ctx->ngk, ctx->kmax, mode, netf);
+    return ctx->basis_f64 ? basis_lds_bytes<double>(nbm, ctx->ngk, ctx->kmax, mode, netf, ctx->nt)
+                          : basis_lds_bytes<float>(nbm, ctx->ngk, ctx->kmax, mode, netf, ctx->nt);
 }
 
 // the fused symmetry-function + network + adjoint kernel is used when its LDS keeps two workgroups per CU

@@ -27,13 +27,17 @@
 // 2 = tangent rows v = sum_a eps_a . dG/dr_a.
 // Dynamic LDS per workgroup (nbm = neighbour slots, kp = kmax rounded up to 4), in units of T:
 //   u[4 nbm], te[4 nbm] (mode 2), E[nbm ng], Ed[nbm ng] (modes 1-3), fR[nbm], gR[nbm], ri[nbm], fr[nbm] (1, 3),
-//   row[kp], red[BW 64], rp[2 kp] (radial eta, rs of the species), sc[2 kp] (modes 0, 3: inpw, inpb);
+//   row[kp], red[nred], rp[2 kp] (radial eta, rs of the species), sc[2 kp] (modes 0, 3: inpw, inpb);
 //   then p[nbm] and ci[kp] (int).  Modes 1 and 3 put g[BW 3 nbm] (double) in front; mode 3 appends the
 //   network area (netf floats: staged weights, activations, partial sums), 16-byte aligned.
+// red holds the per-wave partial sums (BW 64) and, in the adjoint, the dE/dG weights of every canonical
+// angular block in (sign, level, group) order (14 ng per TensorTri block).
+__host__ __device__ inline int basis_nred(int nt, int ng) { return max(BW * 64, nt * (nt + 1) / 2 * 14 * ng); }
+
 template <class T>
-__host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int mode, int netf = 0) {
+__host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int mode, int netf, int nt) {
     const int kp = (kmax + 3) & ~3;
-    int n = 4 * nbm + ng * nbm + 3 * nbm + kp + BW * 64 + 2 * kp;
+    int n = 4 * nbm + ng * nbm + 3 * nbm + kp + basis_nred(nt, ng) + 2 * kp;
     if (mode == 0 || mode == 3) n += 2 * kp;
     if (mode == 2) n += 4 * nbm;
     if (mode >= 1) n += ng * nbm;
@@ -55,7 +59,7 @@ struct BasisLds {
     T* fr;     // modes 1, 3: radial dE/dr of each slot
     double* g; // [BW][3 nbm] per-wave dE/dd accumulators (FP64: LDS FP64 atomics, exact-order-free)
     T* row;
-    T* red;    // [BW][64]
+    T* red;    // [BW][64] partial sums; adjoint: canonical dE/dG weights [block][14 ng]
     T* rp;     // radial eta [nR], rs [nR] of the centre species
     T* sc;     // network input scaling inpw [K], inpb [K] (modes 0, 3)
     int* p;
@@ -64,7 +68,7 @@ struct BasisLds {
 };
 
 template <class T>
-__device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, int mode) {
+__device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, int mode, int nt) {
     BasisLds<T> L;
     L.g = (double*)base;
     T* c = (T*)((char*)base + ((mode == 1 || mode == 3) ? 8 * BW * 3 * nbm : 0));
@@ -77,7 +81,7 @@ __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, i
     L.ri = c; c += nbm;
     L.fr = c; if (mode == 1 || mode == 3) c += nbm;
     L.row = c; c += (kmax + 3) & ~3;
-    L.red = c; c += BW * 64;
+    L.red = c; c += basis_nred(nt, ng);
     const int kp = (kmax + 3) & ~3;
     L.rp = c; c += 2 * kp;
     L.sc = c; if (mode == 0 || mode == 3) c += 2 * kp;
@@ -144,8 +148,8 @@ __device__ inline T prm(const float* f, const double* d, int i) {
 // computed and discarded.  E / Ed are interleaved per slot: E[slot * NG + g], zero for g >= ng.
 template <class T, int NG, int NZ, int MODE>
 __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int nJ, int nK, bool same, int T3,
-                                     const int* __restrict__ cidx, int nA, int goff, double* gw, int tid, int wv,
-                                     int lane) {
+                                     const int* __restrict__ cidx, const T* wtab, int nA, int goff, double* gw,
+                                     int tid, int wv, int lane) {
     constexpr int ADJ = (MODE == 1);
     constexpr int TAN = (MODE == 2);
     T acc[2][NZ][NG], wgt[2][NZ][NG];
@@ -159,18 +163,13 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
                 wgt[s][k][g] = T(0);
             }
     if (ADJ) {
-        // dE/dG of the block scattered to the (s, k, g) table, then read back into registers
-        for (int q = tid; q < 14 * NG; q += BT) L.red[q] = T(0);
-        __syncthreads();
-        for (int m = tid; m < nA; m += BT) L.red[cidx[m]] = L.row[goff + m];
-        __syncthreads();
+        // the block's dE/dG weights in (s, k, g) order, from the table built before the angular loop
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll
             for (int k = 0; k < NZ; ++k)
 #pragma unroll
-                for (int g = 0; g < NG; ++g) wgt[s][k][g] = L.red[(s * 7 + k) * NG + g];
-        __syncthreads();
+                for (int g = 0; g < NG; ++g) wgt[s][k][g] = wtab[(s * 7 + k) * NG + g];
     }
     const float rn = 1.f / (float)(same ? nJ : min(nJ, nK));
     for (int t = tid; t < T3; t += BT) {
@@ -352,7 +351,19 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
             }
         }
     }
-    if (ADJ) __syncthreads();
+    if (ADJ) {
+        // dE/dG of every canonical block scattered to its (sign, level, group) table in red: two barriers
+        // for all blocks
+        const int nab = nt * (nt + 1) / 2;
+        for (int q = tid; q < nab * 14 * NG; q += BT) L.red[q] = T(0);
+        __syncthreads();
+        for (int b = 0; b < nab; ++b)
+            if (!GEN || sp.a_canon[b]) {
+                const int o = sp.a_poff[b] - sp.a_poff[0];
+                for (int m = tid; m < sp.a_n[b]; m += BT) L.red[14 * NG * b + L.ci[o + m]] = L.row[sp.a_goff[b] + m];
+            }
+        __syncthreads();
+    }
 
     // --- angular blocks (J <= K) -------------------------------------------------------------------
     double* gw = L.g + wv * 3 * nbm;
@@ -366,8 +377,8 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
             const int T3 = same ? nJ * (nJ - 1) / 2 : nJ * nK;
             if (nA == 0 || T3 == 0) continue;
             if (!GEN || sp.a_canon[b]) {
-                angular_canon<T, NG, NZ, MODE>(L, bJ, bK, nJ, nK, same, T3, L.ci + (sp.a_poff[b] - sp.a_poff[0]), nA,
-                                               sp.a_goff[b], gw, tid, wv, lane);
+                angular_canon<T, NG, NZ, MODE>(L, bJ, bK, nJ, nK, same, T3, L.ci + (sp.a_poff[b] - sp.a_poff[0]),
+                                               L.red + 14 * NG * b, nA, sp.a_goff[b], gw, tid, wv, lane);
                 continue;
             }
             const int goff = sp.a_goff[b], poff = sp.a_poff[b];
@@ -507,94 +518,113 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
     __syncthreads();
 }
 
-// The atom's network inside the workgroup (energy and dE/dG, FP32; ANN::fpbp and DODZ row 0,
+// The atom's network in wave 0 of the workgroup (energy and dE/dG, FP32; ANN::fpbp and DODZ row 0,
 // ml/nn.cpp:1402-1420, 1836-1862).  On entry row holds G; on exit e_out holds the atomic energy (with E0)
 // and row holds dE/dG (the input scaling inpw folded in).  The species' parameters are in L.nw in the
 // matrix-core layout (W_l as [k][o], row stride mlp_ld(w_{l+1}), biases at bl_off); after them: per layer
-// the activations and derivatives.  Each output of a layer belongs to an aligned group of G lanes of one
-// wave (G = the power of two that spreads the outputs over the workgroup); the group's strided partial
-// sums are combined by DPP moves in a fixed order, so a layer needs one workgroup barrier.
+// the activations and derivatives.  A chunk of a layer's outputs is spread over the 64 lanes, G lanes per
+// output (G a power of two); a lane's strided partial sum runs in four accumulators that are combined, and
+// then summed over the group by DPP moves, in a fixed order.  One wave needs no workgroup barrier between
+// layers (LDS accesses of a wave complete in order; wave_sync keeps the compiler from reordering them);
+// one barrier at the end publishes dE/dG to the other waves.
 __device__ inline int net_group(int nout) {
     int g = 1;
-    while (g < 64 && 2 * g * nout <= BT) g <<= 1;
+    while (g < 64 && 2 * g * nout <= 64) g <<= 1;
     return g;
+}
+
+// sum_{k = r, r + G, ... < n} w[k * ldw] x[k] in eight independent accumulators (16 LDS loads in flight per
+// round trip), combined in a fixed order
+template <class X>
+__device__ inline float net_dot(const float* w, int ldw, const X* x, int r, int G, int n) {
+    float p[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) p[u] = 0.f;
+    int k = r;
+    for (; k + 7 * G < n; k += 8 * G) {
+        float wv[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            wv[u] = w[(k + u * G) * ldw];
+            xv[u] = (float)x[k + u * G];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p[u] += wv[u] * xv[u];
+    }
+    for (int u = 0; k < n; k += G, ++u) p[u & 7] += w[k * ldw] * (float)x[k];
+    return ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
 }
 
 template <class T>
 __device__ inline void block_network(const BasisLds<T>& L, const DevSpecies& sp, int tid, double* e_out) {
-    const int K = sp.K, NL = sp.L;
-    const float* W = L.nw;
-    float* act = L.nw + sp.wp_n;
-    for (int k = tid; k < K; k += BT) L.row[k] = L.sc[k] * (L.row[k] + L.sc[K + k]);   // inpw (G + inpb)
-    __syncthreads();
-    // forward
-    int ain = -1, aoff = 0;
-    for (int l = 0; l < NL; ++l) {
-        const int ni = sp.width[l], no = sp.width[l + 1], ldw = mlp_ld(no), wp = (no + 3) & ~3;
-        const float* Wl = W + sp.wl_off[l];
-        const float* bl = W + sp.bl_off[l];
-        float* A = act + aoff;
-        float* D = A + wp;
-        const int fa = (l == NL - 1) ? ACE_ACT_LINEAR : sp.act;
-        for (int o0 = 0; o0 < no; o0 += BT) {
-            const int nout = min(BT, no - o0), G = net_group(nout);
-            const int o = o0 + tid / G, r = tid % G;
-            float part = 0.f;
-            if (o < o0 + nout) {
-                if (ain < 0) {
-#pragma unroll 4
-                    for (int k = r; k < ni; k += G) part += Wl[k * ldw + o] * (float)L.row[k];
-                } else {
-#pragma unroll 4
-                    for (int k = r; k < ni; k += G) part += Wl[k * ldw + o] * act[ain + k];
+    if (tid < 64) {
+        const int lane = tid;
+        const int K = sp.K, NL = sp.L;
+        const float* W = L.nw;
+        float* act = L.nw + sp.wp_n;
+        for (int k = lane; k < K; k += 64) L.row[k] = L.sc[k] * (L.row[k] + L.sc[K + k]);   // inpw (G + inpb)
+        wave_sync();
+        // forward
+        int ain = -1, aoff = 0;
+        for (int l = 0; l < NL; ++l) {
+            const int ni = sp.width[l], no = sp.width[l + 1], ldw = mlp_ld(no), wp = (no + 3) & ~3;
+            const float* Wl = W + sp.wl_off[l];
+            const float* bl = W + sp.bl_off[l];
+            float* A = act + aoff;
+            float* D = A + wp;
+            const int fa = (l == NL - 1) ? ACE_ACT_LINEAR : sp.act;
+            for (int o0 = 0; o0 < no; o0 += 64) {
+                const int nout = min(64, no - o0), G = net_group(nout);
+                const int o = o0 + lane / G, r = lane % G;
+                const bool on = o < o0 + nout;
+                const int oc = on ? o : o0;
+                float part = (ain < 0) ? net_dot(Wl + oc, ldw, L.row, r, G, ni) : net_dot(Wl + oc, ldw, act + ain, r, G, ni);
+                part = on ? part : 0.f;
+                part = t_group_sum<float>(part, G);
+                if (r == 0 && on) {
+                    float a, d;
+                    act_fd(fa, sp.c, part + bl[o], a, d);
+                    A[o] = a;
+                    D[o] = d;
                 }
             }
-            part = t_group_sum<float>(part, G);
-            if (r == 0 && o < o0 + nout) {
-                float a, d;
-                act_fd(fa, sp.c, part + bl[o], a, d);
-                A[o] = a;
-                D[o] = d;
-            }
+            wave_sync();
+            ain = aoff;
+            aoff += 2 * wp;
         }
-        __syncthreads();
-        ain = aoff;
-        aoff += 2 * wp;
-    }
-    // energy (the last layer has width 1) and the output delta
-    {
+        // energy (the last layer has width 1) and the output delta
         float* A = act + ain;
-        const int wp = (sp.width[NL] + 3) & ~3;
-        if (tid == 0) {
+        const int wpL = (sp.width[NL] + 3) & ~3;
+        if (lane == 0) {
             *e_out = (double)(sp.outb + sp.outw * A[0]) + sp.E0;
-            A[0] = sp.outw * A[wp];   // delta = outw * f'(z), f linear
+            A[0] = sp.outw * A[wpL];   // delta = outw * f'(z), f linear
         }
-        __syncthreads();
-    }
-    // backward: delta_{l-1}[k] = (sum_o W_l[k][o] delta_l[o]) f'(z_{l-1})[k]; layer 0 gives dE/dG
-    int dl = ain;   // offset of delta_l (overwrites the activations of layer l)
-    for (int l = NL - 1; l >= 0; --l) {
-        const int ni = sp.width[l], no = sp.width[l + 1], ldw = mlp_ld(no);
-        const float* Wl = W + sp.wl_off[l];
-        const int wpi = (ni + 3) & ~3;
-        const int dm = (l > 0) ? dl - 2 * wpi : 0;   // offset of layer l-1's activations
-        for (int k0 = 0; k0 < ni; k0 += BT) {
-            const int nout = min(BT, ni - k0), G = net_group(nout);
-            const int k = k0 + tid / G, r = tid % G;
-            float part = 0.f;
-            if (k < k0 + nout) {
-#pragma unroll 4
-                for (int o = r; o < no; o += G) part += Wl[k * ldw + o] * act[dl + o];
+        wave_sync();
+        // backward: delta_{l-1}[k] = (sum_o W_l[k][o] delta_l[o]) f'(z_{l-1})[k]; layer 0 gives dE/dG
+        int dl = ain;   // offset of delta_l (overwrites the activations of layer l)
+        for (int l = NL - 1; l >= 0; --l) {
+            const int ni = sp.width[l], no = sp.width[l + 1], ldw = mlp_ld(no);
+            const float* Wl = W + sp.wl_off[l];
+            const int wpi = (ni + 3) & ~3;
+            const int dm = (l > 0) ? dl - 2 * wpi : 0;   // offset of layer l-1's activations
+            for (int k0 = 0; k0 < ni; k0 += 64) {
+                const int nout = min(64, ni - k0), G = net_group(nout);
+                const int k = k0 + lane / G, r = lane % G;
+                const bool on = k < k0 + nout;
+                const int kc = on ? k : k0;
+                float part = net_dot(Wl + kc * ldw, 1, act + dl, r, G, no);
+                part = on ? part : 0.f;
+                part = t_group_sum<float>(part, G);
+                if (r == 0 && on) {
+                    if (l > 0) act[dm + k] = part * act[dm + wpi + k];
+                    else L.row[k] = (T)part * L.sc[k];
+                }
             }
-            part = t_group_sum<float>(part, G);
-            if (r == 0 && k < k0 + nout) {
-                if (l > 0) act[dm + k] = part * act[dm + wpi + k];
-                else L.row[k] = (T)part * L.sc[k];
-            }
+            wave_sync();
+            dl = dm;
         }
-        __syncthreads();
-        dl = dm;
     }
+    __syncthreads();
 }
 
 // Fixed-point accumulation of forces, energies and virials: 64-bit integer atomics in units of 2^-34
@@ -658,7 +688,7 @@ __global__ __launch_bounds__(BT) void k_basis(int natoms, const DevModel* __rest
     }
     const int I = io.type[i];
     const DevSpecies& gsp = M->sp[I];
-    const BasisLds<T> L = basis_lds<T>(basis_smem, nbm, NG, kmax, MODE);
+    const BasisLds<T> L = basis_lds<T>(basis_smem, nbm, NG, kmax, MODE, nt);
     const int nn = io.counts[i];
     const long long* krow = io.keys + (long long)i * io.cap;
     const float rcut = gsp.rc_max;

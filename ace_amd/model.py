@@ -114,6 +114,8 @@ class Model:
     angular: list           # angular[I][b] -> Angular, b = tri_index(J, K)
     nets: list              # nets[I] -> Network
     E0: np.ndarray = None
+    names: list = None      # species names (NNH::Type::name); files written without them use X0, X1, ...
+    types: list = None      # per species: the other NNH::Type properties (mass, charge, ...) as a dict
 
     def input_width(self, I: int) -> int:
         return sum(len(r.rs) for r in self.radial[I]) + sum(len(a.eta) for a in self.angular[I])

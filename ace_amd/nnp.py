@@ -43,6 +43,12 @@ class NNPDevice:
         self.N = 0
 
     # -- plumbing --------------------------------------------------------------------------------
+    @classmethod
+    def from_file(cls, path):
+        """A context for the potential in an ace potential file (NNP::read, src/nnp/nnp.cpp:233-307)."""
+        from .potential import read_potential
+        return cls(read_potential(path))
+
     def _err(self):
         if not self.ctx:
             return "no context"

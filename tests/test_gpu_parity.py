@@ -389,3 +389,25 @@ def test_c5_full_size():
             e1, f1, w1 = Ea[p]
             assert e1[q] == E[k] and np.array_equal(f1[a0:a0 + n], F[off[k]:off[k] + n]) and np.array_equal(w1[q], W[k])
         a0 += n
+
+
+def test_potential_file_on_gpu(tmp_path):
+    """A potential written in ace's text format and read back runs on the GPU and matches the oracle of
+    the read-back model (whose basis carries the format's 6-decimal rounding)."""
+    from ace_amd import potential
+    model, strucs = configs.config2()
+    scaled_model(model, strucs)
+    model.names = ["Fe", "Cr"]
+    path = tmp_path / "fecr.nn"
+    potential.write_potential(path, model)
+    dev = NNPDevice.from_file(path)
+    m2 = dev.model
+    assert m2.names == ["Fe", "Cr"]
+    dev.upload(strucs)
+    dev.build_neighbors()
+    E, F, _ = dev.compute(PREC_FP32)
+    Er, Fr, _ = O.Oracle(m2).compute(strucs[0])
+    de, df = abs(E[0] - Er) / strucs[0].natoms, np.abs(F - Fr).max()
+    log(f"potential file: dE/atom {de:.1e} dF {df:.1e}")
+    assert de < TOL[PREC_FP32]["E"] and df < TOL[PREC_FP32]["F"]
+    dev.close()

@@ -1,0 +1,72 @@
+"""Structure files (extended XYZ, VASP POSCAR) with the reference's parsing and unit rules
+(src/format/xyz_struc.cpp, src/format/vasp_struc.cpp, src/chem/units.hpp).  CPU only.  The fixtures are
+hand-written: tests/golden/mixed4.xyz (triclinic lattice, interleaved species, an atom outside the cell,
+mixed-case header), tests/golden/nacl8.poscar (direct coordinates)."""
+import numpy as np
+import pytest
+
+from ace_amd import configs
+from ace_amd import structures as S
+
+
+def test_xyz_fixture(golden_dir):
+    s, names, per_atom = S.read_xyz(f"{golden_dir}/mixed4.xyz")
+    assert names == ["Al", "Cu"] and per_atom == ["Al", "Cu", "Al", "Cu"] and list(s.type) == [0, 1, 0, 1]
+    assert s.energy == -12.5
+    # Lattice lists the lattice vectors a, b, c: the columns of R
+    assert np.array_equal(s.cell[:, 1], [0.5, 4.0, 0.0]) and np.array_equal(s.cell[:, 2], [0.2, 0.3, 4.0])
+    assert np.array_equal(s.forces[1], [-0.1, 0.2, -0.3])
+    # positions returned to the cell: the same point modulo lattice vectors, fractional in [0, 1)
+    raw = np.array([[0.1, 0.2, 0.3], [2.0, 2.0, 0.1], [-0.5, 1.0, 1.0], [1.0, 1.0, 4.5]])
+    f = np.linalg.solve(s.cell, s.pos.T).T
+    assert np.all((f >= 0) & (f < 1))
+    d = np.linalg.solve(s.cell, (s.pos - raw).T).T
+    assert np.allclose(d, np.round(d), atol=1e-12)
+    assert np.allclose(s.pos[0], raw[0]) and not np.allclose(s.pos[2], raw[2])
+
+
+def test_xyz_units_and_round_trip(golden_dir):
+    s, names, _ = S.read_xyz(f"{golden_dir}/mixed4.xyz")
+    a, _, _ = S.read_xyz(f"{golden_dir}/mixed4.xyz", units="au")
+    assert np.allclose(a.pos, s.pos * S.ANG2BOHR) and np.isclose(a.energy, -12.5 * S.EV2EH)
+    assert np.allclose(a.forces, s.forces * S.EV2EH / S.ANG2BOHR)
+    s2, names2, _ = S.read_xyz_text(S.xyz_text(s, names))
+    assert names2 == names and np.allclose(s2.pos, s.pos, atol=1e-9) and np.allclose(s2.cell, s.cell, atol=1e-6)
+    assert np.allclose(s2.forces, s.forces, atol=1e-9) and s2.energy == s.energy
+    a2, _, _ = S.read_xyz_text(S.xyz_text(a, names, units="au"), units="au")
+    assert np.allclose(a2.pos, a.pos, atol=1e-8)
+
+
+def test_poscar_fixture_and_round_trip(golden_dir):
+    s, names, per_atom = S.read_poscar(f"{golden_dir}/nacl8.poscar")
+    assert names == ["Na", "Cl"] and list(s.type) == [0] * 4 + [1] * 4 and per_atom[4] == "Cl"
+    assert np.allclose(s.pos[1], [2.82, 2.82, 0.0]) and np.allclose(s.cell, np.eye(3) * 5.64)
+    # a structure with interleaved species is written grouped by species
+    t = configs.Structure(s.cell, s.pos[[0, 4, 1, 5, 2, 6, 3, 7]], np.array([0, 1] * 4, dtype=np.int32))
+    r, names2, _ = S.read_poscar_text(S.poscar_text(t, names))
+    assert names2 == names and np.allclose(r.pos, s.pos, atol=1e-7) and list(r.type) == list(s.type)
+    r2, _, _ = S.read_poscar_text(S.poscar_text(t, names, direct=False))
+    assert np.allclose(r2.pos, s.pos, atol=1e-7)
+
+
+def test_poscar_cartesian_ignores_the_scale_factor_like_the_reference(golden_dir):
+    text = open(f"{golden_dir}/nacl8.poscar").read().replace("   1.0000", "   2.0000", 1)
+    text = text.replace("Direct", "Cartesian").replace("0.5", "1.5")
+    s, _, _ = S.read_poscar_text(text)
+    assert np.allclose(s.cell, np.eye(3) * 11.28) and np.allclose(s.pos[1], [1.5, 1.5, 0.0])
+
+
+def test_model_type_mapping(golden_dir):
+    s, names, _ = S.read_xyz(f"{golden_dir}/mixed4.xyz")
+    m = S.to_model_types(s, names, ["Cu", "Al"])
+    assert list(m.type) == [1, 0, 1, 0]
+    with pytest.raises(ValueError):
+        S.to_model_types(s, names, ["Cu"])
+
+
+@pytest.mark.parametrize("text", ["0\nProperties=species:S:1:pos:R:3\n",
+                                  "1\nProperties=species:S:2:pos:R:3\nAl 0 0 0\n",
+                                  "1\nProperties=species:S:1:pos:I:3\nAl 0 0 0\n"])
+def test_invalid_xyz_raises(text):
+    with pytest.raises(ValueError):
+        S.read_xyz_text(text)

@@ -71,24 +71,78 @@ class Adam:
 
 
 class Trainer:
-    """Full-batch training on one GPU: every uploaded structure is in the batch."""
+    """Full-batch training on one GPU: every uploaded structure is in the batch.  The update follows the
+    training loop of src/nnp/nnptefr.cpp:1031-1066: gamma <- decay(gamma), the optimiser step (any of
+    src/opt/algo.cpp, default ADAM), weight decay of the weights from the previous parameters, then the
+    "old" value, parameters and gradient."""
 
-    def __init__(self, dev, model, strucs, gamma=1e-3, w_force=0.1, loss=0, norm=0, delta=1.0):
+    def __init__(self, dev, model, strucs, gamma=1e-3, w_force=0.1, loss=0, norm=0, delta=1.0, algo="ADAM",
+                 decay=None, reg="NONE", eta_reg=0.0):
+        from .optimizers import Decay, Objective, make
         self.dev, self.model = dev, model
         self.nat = sum(s.natoms for s in strucs)
         self.nbatch = len(strucs)
-        self.opt = Adam(sum(model.param_counts()), gamma)
-        self.params = model.all_params()
+        self.obj = Objective(model.all_params(), gamma)
+        self.algo = make(algo, len(self.obj.p))
+        self.decay = decay if decay is not None else Decay()
+        self.reg, self.eta_reg = reg, eta_reg
+        # weight mask: per species the biases (0) then the weights (1), the parameter vector order
+        self.mask = np.concatenate([np.r_[np.zeros(sum(n.widths[1:])), np.ones(n.nparams() - sum(n.widths[1:]))]
+                                    for n in model.nets])
+        self.it = 0
         self.w_force, self.loss, self.norm, self.delta = w_force, loss, norm, delta
-        have_f = all(s.forces is not None for s in strucs)
+        have_f = bool(strucs) and all(s.forces is not None for s in strucs)
         dev.set_targets([s.energy for s in strucs],
                         np.concatenate([s.forces for s in strucs]) if have_f else None)
 
+    @property
+    def params(self):
+        return self.obj.p
+
+    def update(self, loss, g):
+        """One parameter update from the loss value and its gradient (host, FP64)."""
+        from .optimizers import weight_decay
+        o = self.obj
+        o.g = np.array(g, dtype=np.float64)
+        o.val = float(loss)
+        o.gamma = self.decay.step(o.gamma, self.it)
+        self.algo.step(o)
+        weight_decay(o, self.mask, self.eta_reg, self.reg)
+        o.val_old, o.p_old, o.g_old = o.val, o.p.copy(), o.g.copy()
+        self.it += 1
+        return o.p
+
+    def save_restart(self, path):
+        """Training state for a restart (the content of NNPTEFR::write_restart, src/nnp/nnptefr.cpp:509-537:
+        parameters, optimiser state, gamma, iteration, old value/parameters/gradient) as a portable .npz
+        rather than the reference's memory image of the trainer object."""
+        o = self.obj
+        state = {"p": o.p, "p_old": o.p_old, "g_old": o.g_old, "val_old": o.val_old, "gamma": o.gamma,
+                 "it": self.it, "algo": type(self.algo).__name__}
+        for k, v in vars(self.algo).items():
+            state["algo_" + k] = np.asarray(v)
+        np.savez(path, **state)
+
+    def load_restart(self, path):
+        """Continues from a state written by save_restart (NNPTEFR::read_restart, nnptefr.cpp:539-574)."""
+        from .optimizers import make
+        z = np.load(path, allow_pickle=False)
+        o = self.obj
+        o.p, o.p_old, o.g_old = z["p"].copy(), z["p_old"].copy(), z["g_old"].copy()
+        o.val_old, o.gamma, self.it = float(z["val_old"]), float(z["gamma"]), int(z["it"])
+        self.algo = make(str(z["algo"]), len(o.p))
+        for k in vars(self.algo):
+            v = z["algo_" + k]
+            setattr(self.algo, k, v.copy() if v.ndim else v.item())
+        self.model.set_all_params(o.p)
+        if hasattr(self.dev, "set_all_params"):
+            self.dev.set_all_params(o.p)
+
     def step(self):
-        """Loss terms and the updated parameter vector of one iteration."""
+        """Loss terms of one iteration; the parameters are updated on the host and uploaded."""
         wf = self.w_force / (3 * self.nat) if self.w_force else 0.0
         out, g = self.dev.train_step(self.loss, self.norm, self.delta, 1.0 / self.nbatch, wf)
-        self.params = self.opt.step(self.params, g)
-        self.dev.set_all_params(self.params)
-        self.model.set_all_params(self.params)
+        p = self.update(out[0], g)
+        self.dev.set_all_params(p)
+        self.model.set_all_params(p)
         return out

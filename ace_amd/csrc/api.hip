@@ -717,7 +717,18 @@ int basis_bytes(ace_ctx* ctx, int mode) {
 }
 
 // the fused symmetry-function + network + adjoint kernel is used when its LDS keeps two workgroups per CU
-bool use_fused(ace_ctx* ctx) { return basis_bytes(ctx, 3) <= 64 * 1024; }
+// The fused kernel (symmetry functions, network, adjoint of one atom per workgroup) saves two launches and
+// the G / dE/dG round trips, which wins while the batch is a few rounds of resident workgroups; its LDS
+// footprint limits residency, so larger batches run faster as symm -> network -> adjoint (measured on
+// MI355X, 80-32-32-1 bcc: FP32 fused 0.178 / split 0.186 ms at 4000 atoms, 0.323 / 0.319 at 8000; FP64
+// basis 0.128 / 0.132 at 2000, 0.230 / 0.223 at 4000).  Fused when the network fits 64 KB of LDS and the
+// grid is at most four rounds of 256 CUs x the workgroups that fit in 160 KB.
+bool use_fused(ace_ctx* ctx) {
+    const int bytes = basis_bytes(ctx, 3);
+    if (bytes > 64 * 1024) return false;
+    const long long per_round = 256LL * std::max(1, (160 * 1024) / bytes);
+    return ctx->N <= 4 * per_round;
+}
 
 // mode 0: G -> out; mode 1: per-pair gradients from dE/dG; mode 2: tangent rows along eps -> out;
 // mode 3: per-atom energies and per-pair gradients in one pass

@@ -19,7 +19,7 @@ struct TrainState {
     std::vector<int4> tiles;
     std::vector<int> tile_first;     // first tile of each species (ntypes + 1)
     DBuf<int4> d_tiles;
-    DBuf<double> d_Eref, d_Fref, d_gfac, d_eps, d_out, d_grad;
+    DBuf<double> d_Eref, d_Fref, d_gfac, d_eps, d_out, d_grad, d_gsum;
     DBuf<float> d_V, d_part;
     int ldpart = 0;
 };
@@ -243,12 +243,40 @@ __global__ __launch_bounds__(256) void k_mlp_train(
     }
 }
 
-// thread per parameter of species I: FP64 sum of the per-tile partials in tile order
-__global__ void k_grad_reduce(int np, int t0, int t1, int ldpart, const float* __restrict__ part,
-                              double* __restrict__ grad) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= np) return;
+// Gradient of the batch from the per-tile partials (tile rows of ldpart floats), in two passes whose sums
+// run in a fixed order: pass 1 adds the partials of GRED_GROUP consecutive tiles of one species (FP64),
+// workgroups spread over (parameter chunk, tile group, species); pass 2 adds the groups of each parameter.
+#define GRED_GROUP 16
+struct GradRed {
+    int nt;
+    int np[ACE_MAXT];      // parameters of each species
+    int off[ACE_MAXT];     // offset of the species in the gradient
+    int t0[ACE_MAXT];      // its tiles [t0, t1)
+    int t1[ACE_MAXT];
+    long long gofs[ACE_MAXT];   // offset of its group sums [groups][np] in the pass-1 buffer
+};
+
+__global__ __launch_bounds__(256) void k_grad_reduce1(GradRed R, int ldpart, const float* __restrict__ part,
+                                                      double* __restrict__ gsum) {
+    const int I = blockIdx.z, q = blockIdx.x * 256 + threadIdx.x, gi = blockIdx.y;
+    const int ta = R.t0[I] + gi * GRED_GROUP;
+    if (q >= R.np[I] || ta >= R.t1[I]) return;
+    const int n = min(GRED_GROUP, R.t1[I] - ta);
+    float v[GRED_GROUP];
+#pragma unroll
+    for (int u = 0; u < GRED_GROUP; ++u) v[u] = (u < n) ? part[(long long)(ta + u) * ldpart + q] : 0.f;
     double s = 0;
-    for (int t = t0; t < t1; ++t) s += (double)part[(long long)t * ldpart + q];
-    grad[q] = s;
+#pragma unroll
+    for (int u = 0; u < GRED_GROUP; ++u) s += (double)v[u];
+    gsum[R.gofs[I] + (long long)gi * R.np[I] + q] = s;
+}
+
+__global__ __launch_bounds__(256) void k_grad_reduce2(GradRed R, const double* __restrict__ gsum,
+                                                      double* __restrict__ grad) {
+    const int I = blockIdx.z, q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= R.np[I]) return;
+    const int ng = (R.t1[I] - R.t0[I] + GRED_GROUP - 1) / GRED_GROUP;
+    double s = 0;
+    for (int g = 0; g < ng; ++g) s += gsum[R.gofs[I] + (long long)g * R.np[I] + q];
+    grad[R.off[I] + q] = s;
 }

@@ -4,7 +4,7 @@ namespace {
 void train_release(ace_ctx* ctx) {
     TrainState& T = ctx->tr;
     release(T.d_tiles); release(T.d_Eref); release(T.d_Fref); release(T.d_gfac); release(T.d_eps);
-    release(T.d_out); release(T.d_grad); release(T.d_V); release(T.d_part);
+    release(T.d_out); release(T.d_grad); release(T.d_gsum); release(T.d_V); release(T.d_part);
 }
 
 void train_on_upload(ace_ctx* ctx) {
@@ -120,21 +120,30 @@ int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* 
         size_t ntot = 0;
         for (int I = 0; I < ctx->nt; ++I) ntot += ctx->sp[I].nparams();
         if (ensure(ctx, T.d_grad, ntot)) return -1;
-        size_t off = 0;
+        GradRed R;
+        memset(&R, 0, sizeof(R));
+        R.nt = ctx->nt;
+        int off = 0, npmax = 1, ngmax = 1;
+        long long gofs = 0;
         for (int I = 0; I < ctx->nt; ++I) {
-            const int np = ctx->sp[I].nparams();
-            const int t0 = T.tile_first[I], t1 = T.tile_first[I + 1];
-            double* gout = T.d_grad.p + off;
-            if (t1 > t0) {
-                if (timed(ctx, "grad_reduce", [&] {
-                        k_grad_reduce<<<cdiv(np, 256), 256, 0, ctx->stream>>>(np, t0, t1, T.ldpart, T.d_part.p, gout);
-                    }))
-                    return -1;
-            } else {
-                HC(hipMemsetAsync(gout, 0, np * sizeof(double), ctx->stream));
-            }
-            off += np;
+            R.np[I] = ctx->sp[I].nparams();
+            R.off[I] = off;
+            R.t0[I] = T.tile_first[I];
+            R.t1[I] = T.tile_first[I + 1];
+            R.gofs[I] = gofs;
+            const int ng = (R.t1[I] - R.t0[I] + GRED_GROUP - 1) / GRED_GROUP;
+            gofs += (long long)ng * R.np[I];
+            off += R.np[I];
+            npmax = std::max(npmax, R.np[I]);
+            ngmax = std::max(ngmax, ng);
         }
+        if (ensure(ctx, T.d_gsum, (size_t)std::max(gofs, 1LL))) return -1;
+        const dim3 g1(cdiv(npmax, 256), ngmax, ctx->nt), g2(cdiv(npmax, 256), 1, ctx->nt);
+        if (timed(ctx, "grad_reduce", [&] {
+                k_grad_reduce1<<<g1, 256, 0, ctx->stream>>>(R, T.ldpart, T.d_part.p, T.d_gsum.p);
+            }) ||
+            timed(ctx, "grad_reduce", [&] { k_grad_reduce2<<<g2, 256, 0, ctx->stream>>>(R, T.d_gsum.p, T.d_grad.p); }))
+            return -1;
         HC(hipMemcpyAsync(grad, T.d_grad.p, ntot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     }
     std::vector<double> out(3 * (size_t)S);

@@ -122,122 +122,201 @@ __global__ __launch_bounds__(LOSS_THREADS) void k_loss(
     }
 }
 
-// dynamic LDS per tile: X, Xt [tm][K]; per layer A, D, D2, At, Zt [tm][w_{l+1}]
-__host__ __device__ inline int train_lds_floats(const DevSpecies& sp, int tm) {
-    int n = 2 * tm * sp.K;
-    for (int l = 0; l < sp.L; ++l) n += 5 * tm * sp.width[l + 1];
+// ---- matrix-core training pass -------------------------------------------------------------------
+// One workgroup of four waves per tile of 16 atoms of one species (the v_mfma_f32_16x16x4f32 helpers and the
+// zero-padded weight layout of the inference network kernel); the 16-column tiles of every product are spread
+// over the waves, with one barrier per phase.  Per layer l (z = W a + b, a' = f(z)):
+//   forward, primal and tangent:  Z = A_{l-1} W^T, Zt = At_{l-1} W^T; A = f(Z), D = f'(Z), D2 = f''(Z),
+//                                  At = D Zt
+//   backward from the seeds (primal adjoint gfac outw, tangent adjoint -outw at the linear output):
+//     zb = D2 Zt atb + D ab,  ztb = D atb                        (elementwise, into D2 and Zt)
+//     d bias = sum_rows zb,  d W = zb^T A_{l-1} + ztb^T At_{l-1}  (16 x 16 tiles, K = the 16 rows)
+//     ab_{l-1} = zb W,  atb_{l-1} = ztb W                         (into A_{l-1}, At_{l-1})
+// LDS rows have odd strides (mlp_ld) and start zeroed, so padding rows and columns contribute nothing; rows
+// past the tile's atoms get zero seeds.  The tile's gradient goes to part (reference parameter order).
+__host__ __device__ inline int train_mc_lds_floats(const DevSpecies& sp) {
+    int n = 2 * MLP_ROWS * mlp_ld(sp.K);
+    for (int l = 0; l < sp.L; ++l) n += 5 * MLP_ROWS * mlp_ld(sp.width[l + 1]);
     return n;
 }
 
-// tile of tm atoms of one species: per-tile partial gradient of sum_i (gfac E_i - dE_i/dG . v_i)
-__global__ __launch_bounds__(256) void k_mlp_train(
-    const DevModel* __restrict__ M, const float* __restrict__ params, const float* __restrict__ inpw,
-    const float* __restrict__ inpb, const int4* __restrict__ tiles, int ldg, const float* __restrict__ G,
-    const float* __restrict__ V, const int* __restrict__ atom_of_row, const int* __restrict__ struc_of,
-    const double* __restrict__ gfac, int tm, int ldpart, float* __restrict__ part) {
-    extern __shared__ float smem[];
+// D[o][k] (16 x 16 tile at (o0, k0)) = sum_{r < 16} P[r][o] Q[r][k] + Pt[r][o] Qt[r][k]; P, Pt rows of stride
+// ldo, Q, Qt rows of stride ldi, all in LDS.  A operand = P^T: lane l supplies A[o0 + l % 16][4 u + l / 16].
+__device__ inline mfma_f4 mfma_wgrad(int P, int Pt, int ldo, int Q, int Qt, int ldi, int o0, int k0, int lane) {
+    const int c = lane & 15, q = lane >> 4;
+    mfma_f4 a0 = mfma_f4{0.f, 0.f, 0.f, 0.f}, a1 = mfma_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int r = 4 * u + q;
+        a0 = __builtin_amdgcn_mfma_f32_16x16x4f32(mlp_smem[P + r * ldo + o0 + c], mlp_smem[Q + r * ldi + k0 + c], a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(mlp_smem[Pt + r * ldo + o0 + c], mlp_smem[Qt + r * ldi + k0 + c], a1, 0, 0, 0);
+    }
+    return a0 + a1;
+}
+
+#define TRAIN_WAVES 4
+
+// the 16-column output tiles of a product, tile t on wave t % TRAIN_WAVES (one matrix-core chain per tile)
+template <class EPI>
+__device__ inline void mfma_tiles(int a, int lda, int k, const float* __restrict__ Bg, int so, int sk, int n, int lane,
+                                  int wv, EPI epi) {
+    for (int o0 = 16 * wv; o0 < n; o0 += 16 * TRAIN_WAVES) {
+        mfma_f4 acc[1];
+        mfma_group<false, 1>(a, lda, k, 0, Bg, so, sk, n, o0, lane, acc);
+        epi(acc[0], o0);
+    }
+}
+
+__global__ __launch_bounds__(64 * TRAIN_WAVES) void k_mlp_train_mc(
+    const DevModel* __restrict__ M, const float* __restrict__ wpad, const float* __restrict__ inpw,
+    const int4* __restrict__ tiles, int ldg, const float* __restrict__ G, const float* __restrict__ V,
+    const int* __restrict__ atom_of_row, const int* __restrict__ struc_of, const double* __restrict__ gfac,
+    int ldpart, float* __restrict__ part) {
+    constexpr int NTH = 64 * TRAIN_WAVES;
+    __shared__ DevSpecies s_sp;
+    __shared__ int s_off[5 * ACE_MAXL + 2];
     const int4 tile = tiles[blockIdx.x];
     const int I = tile.x, r0 = tile.y, nr = tile.z;
-    const DevSpecies& sp = M->sp[I];
-    const int K = sp.K, L = sp.L;
-    const bool tan = (V != nullptr);
-    float* X = smem;
-    float* Xt = X + tm * K;
-    float *A[ACE_MAXL], *D[ACE_MAXL], *D2[ACE_MAXL], *At[ACE_MAXL], *Zt[ACE_MAXL];
-    {
-        float* q = Xt + tm * K;
-        for (int l = 0; l < L; ++l) {
-            const int w = tm * sp.width[l + 1];
-            A[l] = q; q += w;
-            D[l] = q; q += w;
-            D2[l] = q; q += w;
-            At[l] = q; q += w;
-            Zt[l] = q; q += w;
-        }
-    }
-    const float* p = params + sp.poff;
-    const float* iw = inpw + sp.in_off;
-    const float* ib = inpb + sp.in_off;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < tm * K; idx += 256) {
-        const int r = idx / K, k = idx - r * K;
-        const long long gi = (long long)(r0 + r) * ldg + k;
-        X[idx] = (r < nr) ? G[gi] : 0.f;          // rows already hold inpw*(G + inpb)
-        Xt[idx] = (r < nr && tan) ? iw[k] * V[gi] : 0.f;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid < 64) {
+        constexpr int n16 = (int)(sizeof(DevSpecies) / 16);
+        static_assert(n16 >= 64 && n16 <= 128, "DevSpecies copy covers 64 to 128 words");
+        const int4* src = (const int4*)&M->sp[I];
+        int4* dst = (int4*)&s_sp;
+        const int4 v0 = src[lane];
+        const int4 v1 = (64 + lane < n16) ? src[64 + lane] : v0;
+        dst[lane] = v0;
+        if (64 + lane < n16) dst[64 + lane] = v1;
     }
     __syncthreads();
-    // forward with tangent
-    for (int l = 0; l < L; ++l) {
-        const int no = sp.width[l + 1], ni = sp.width[l];
-        const float* W = p + sp.woff[l];
-        const float* b = p + sp.boff[l];
-        const float* in = (l == 0) ? X : A[l - 1];
-        const float* int_ = (l == 0) ? Xt : At[l - 1];
-        const int act = (l == L - 1) ? ACE_ACT_LINEAR : sp.act;
-        for (int idx = tid; idx < tm * no; idx += 256) {
-            const int r = idx / no, o = idx - r * no;
-            float z = b[o], zt = 0.f;
-            for (int k = 0; k < ni; ++k) {
-                const float w = W[o + k * no];
-                z += w * in[r * ni + k];
-                zt += w * int_[r * ni + k];
-            }
-            float a, d, d2;
-            act_fd2(act, sp.c, z, a, d, d2);
-            A[l][idx] = a;
-            D[l][idx] = d;
-            D2[l][idx] = d2;
-            Zt[l][idx] = zt;
-            At[l][idx] = d * zt;
+    const DevSpecies& sp = s_sp;
+    const int K = sp.K, L = sp.L, ldx = mlp_ld(K);
+    const bool tan = (V != nullptr);
+    if (tid == 0) {
+        int q = 2 * MLP_ROWS * ldx;
+        for (int l = 0; l < L; ++l) {
+            const int w = MLP_ROWS * mlp_ld(sp.width[l + 1]);
+            for (int j = 0; j < 5; ++j) s_off[5 * l + j] = q + j * w;   // A, D, D2, At, Zt
+            q += 5 * w;
         }
-        __syncthreads();
+        s_off[5 * ACE_MAXL] = q;
     }
-    // seeds at the linear output layer: primal adjoint gfac*outw, tangent adjoint -outw
-    for (int r = tid; r < tm; r += 256) {
-        float g = 0.f;
-        if (r < nr) g = (float)gfac[struc_of[atom_of_row[r0 + r]]];
-        A[L - 1][r] = g * sp.outw;
-        At[L - 1][r] = (tan && r < nr) ? -sp.outw : 0.f;
+    __syncthreads();
+    const int total = s_off[5 * ACE_MAXL];
+    for (int e = tid; e < total; e += NTH) mlp_smem[e] = 0.f;
+    __syncthreads();
+    const int X = 0, Xt = MLP_ROWS * ldx;
+    const float* iw = inpw + sp.in_off;
+    for (int e = tid; e < MLP_ROWS * K; e += NTH) {
+        const int r = e / K, k = e - r * K;
+        if (r < nr) {
+            const long long gi = (long long)(r0 + r) * ldg + k;
+            mlp_smem[X + r * ldx + k] = G[gi];                    // rows hold inpw (G + inpb)
+            if (tan) mlp_smem[Xt + r * ldx + k] = iw[k] * V[gi];
+        }
+    }
+    __syncthreads();
+    const float* Wg = wpad + sp.wp_off;
+    const int cr = lane & 15, rq = 4 * (lane >> 4);
+    // forward, primal and tangent (the primal epilogue of a tile reads back the tangent values the same
+    // lanes wrote for that tile)
+    int ain = X, atin = Xt, lda = ldx;
+    for (int l = 0; l < L; ++l) {
+        const int no = sp.width[l + 1], ni = sp.width[l], ldo = mlp_ld(no);
+        const int Ao = s_off[5 * l], Do = s_off[5 * l + 1], D2o = s_off[5 * l + 2], Ato = s_off[5 * l + 3],
+                  Zto = s_off[5 * l + 4];
+        const int act = (l == L - 1) ? ACE_ACT_LINEAR : sp.act;
+        const float* bl = Wg + sp.bl_off[l];
+        mfma_tiles(atin, lda, ni, Wg + sp.wl_off[l], 1, ldo, no, lane, wv, [&](const mfma_f4& acc, int o0) {
+            const int o = o0 + cr;
+            if (o < no)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) mlp_smem[Zto + (rq + r) * ldo + o] = acc[r];
+        });
+        mfma_tiles(ain, lda, ni, Wg + sp.wl_off[l], 1, ldo, no, lane, wv, [&](const mfma_f4& acc, int o0) {
+            const int o = o0 + cr;
+            if (o < no) {
+                const float bo = bl[o];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int e = (rq + r) * ldo + o;
+                    float a, d, d2;
+                    act_fd2(act, sp.c, acc[r] + bo, a, d, d2);
+                    mlp_smem[Ao + e] = a;
+                    mlp_smem[Do + e] = d;
+                    mlp_smem[D2o + e] = d2;
+                    mlp_smem[Ato + e] = d * mlp_smem[Zto + e];
+                }
+            }
+        });
+        __syncthreads();
+        ain = Ao;
+        atin = Ato;
+        lda = ldo;
+    }
+    // seeds at the linear output layer: primal adjoint gfac outw, tangent adjoint -outw
+    {
+        const int ldL = mlp_ld(1), AL = s_off[5 * (L - 1)], AtL = s_off[5 * (L - 1) + 3];
+        if (tid < MLP_ROWS) {
+            const bool on = tid < nr;
+            const float g = on ? (float)gfac[struc_of[atom_of_row[r0 + tid]]] : 0.f;
+            mlp_smem[AL + tid * ldL] = g * sp.outw;
+            mlp_smem[AtL + tid * ldL] = (tan && on) ? -sp.outw : 0.f;
+        }
     }
     __syncthreads();
     float* out = part + (long long)blockIdx.x * ldpart;
     for (int l = L - 1; l >= 0; --l) {
-        const int no = sp.width[l + 1], ni = sp.width[l];
-        const float* W = p + sp.woff[l];
-        // zb (into D2) and ztb (into Zt), elementwise
-        for (int idx = tid; idx < tm * no; idx += 256) {
-            const float ab = A[l][idx], atb = At[l][idx], d = D[l][idx];
-            const float zb = D2[l][idx] * Zt[l][idx] * atb + d * ab;
-            D2[l][idx] = zb;
-            Zt[l][idx] = d * atb;
+        const int no = sp.width[l + 1], ni = sp.width[l], ldo = mlp_ld(no);
+        const int Ao = s_off[5 * l], Do = s_off[5 * l + 1], D2o = s_off[5 * l + 2], Ato = s_off[5 * l + 3],
+                  Zto = s_off[5 * l + 4];
+        // zb (into D2), ztb (into Zt)
+        for (int e = tid; e < MLP_ROWS * no; e += NTH) {
+            const int r = e / no, o = e - r * no, q = r * ldo + o;
+            const float ab = mlp_smem[Ao + q], atb = mlp_smem[Ato + q], d = mlp_smem[Do + q];
+            mlp_smem[D2o + q] = mlp_smem[D2o + q] * mlp_smem[Zto + q] * atb + d * ab;
+            mlp_smem[Zto + q] = d * atb;
         }
         __syncthreads();
-        const float* in = (l == 0) ? X : A[l - 1];
-        const float* int_ = (l == 0) ? Xt : At[l - 1];
-        for (int o = tid; o < no; o += 256) {
-            float s = 0.f;
-            for (int r = 0; r < tm; ++r) s += D2[l][r * no + o];
-            out[sp.boff[l] + o] = s;
+        // bias gradient: column sums of zb
+        for (int o = tid; o < no; o += NTH) {
+            float sb = 0.f;
+#pragma unroll
+            for (int r = 0; r < MLP_ROWS; ++r) sb += mlp_smem[D2o + r * ldo + o];
+            out[sp.boff[l] + o] = sb;
         }
-        for (int idx = tid; idx < no * ni; idx += 256) {
-            const int o = idx % no, k = idx / no;
-            float s = 0.f;
-            for (int r = 0; r < tm; ++r) s += D2[l][r * no + o] * in[r * ni + k] + Zt[l][r * no + o] * int_[r * ni + k];
-            out[sp.woff[l] + idx] = s;
+        // weight gradient zb^T A_{l-1} + ztb^T At_{l-1}, written as W (no x ni) column-major; 16 x 16 tiles
+        // over the waves
+        const int in = (l == 0) ? X : s_off[5 * (l - 1)], int_ = (l == 0) ? Xt : s_off[5 * (l - 1) + 3];
+        const int ldi = (l == 0) ? ldx : mlp_ld(ni);
+        const int nob = (no + 15) >> 4, nkb = (ni + 15) >> 4;
+        for (int t = wv; t < nob * nkb; t += TRAIN_WAVES) {
+            const int o0 = 16 * (t % nob), k0 = 16 * (t / nob);
+            const mfma_f4 acc = mfma_wgrad(D2o, Zto, ldo, in, int_, ldi, o0, k0, lane);
+            const int k = k0 + cr;
+            if (k < ni)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int o = o0 + rq + r;
+                    if (o < no) out[sp.woff[l] + o + k * no] = acc[r];
+                }
         }
         __syncthreads();
         if (l > 0) {
-            for (int idx = tid; idx < tm * ni; idx += 256) {
-                const int r = idx / ni, k = idx - r * ni;
-                float s = 0.f, st = 0.f;
-                for (int o = 0; o < no; ++o) {
-                    const float w = W[o + k * no];
-                    s += w * D2[l][r * no + o];
-                    st += w * Zt[l][r * no + o];
-                }
-                A[l - 1][idx] = s;
-                At[l - 1][idx] = st;
-            }
+            // ab_{l-1} = zb W, atb_{l-1} = ztb W: B[o][k] = W_l[o][k] (wpad rows k, stride ldo)
+            const int ldm = mlp_ld(ni), Am = s_off[5 * (l - 1)], Atm = s_off[5 * (l - 1) + 3];
+            mfma_tiles(D2o, ldo, no, Wg + sp.wl_off[l], ldo, 1, ni, lane, wv, [&](const mfma_f4& acc, int k0) {
+                const int k = k0 + cr;
+                if (k < ni)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) mlp_smem[Am + (rq + r) * ldm + k] = acc[r];
+            });
+            mfma_tiles(Zto, ldo, no, Wg + sp.wl_off[l], ldo, 1, ni, lane, wv, [&](const mfma_f4& acc, int k0) {
+                const int k = k0 + cr;
+                if (k < ni)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) mlp_smem[Atm + (rq + r) * ldm + k] = acc[r];
+            });
             __syncthreads();
         }
     }

@@ -14,16 +14,11 @@ void train_on_upload(ace_ctx* ctx) {
     ctx->tr.tiles.clear();
 }
 
-// training tiles: the largest row count whose LDS fits 64 KB (at least 8 rows)
+// training tiles: 16 atoms of one species (the matrix-core tile height)
 int train_tiles(ace_ctx* ctx) {
     TrainState& T = ctx->tr;
     if (!T.tiles.empty()) return 0;
-    int tm = 32;
-    for (; tm > 8; tm >>= 1) {
-        int lds = 0;
-        for (int I = 0; I < ctx->nt; ++I) lds = std::max(lds, train_lds_floats(ctx->hdm.sp[I], tm) * 4);
-        if (lds <= 64 * 1024) break;
-    }
+    const int tm = MLP_ROWS;
     T.tm = tm;
     std::vector<int> cnt(ctx->nt, 0);
     for (int t : ctx->type_h) cnt[t]++;
@@ -109,12 +104,14 @@ int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* 
     }
     if (grad) {
         int lds = 0;
-        for (int I = 0; I < ctx->nt; ++I) lds = std::max(lds, train_lds_floats(ctx->hdm.sp[I], T.tm) * 4);
+        for (int I = 0; I < ctx->nt; ++I) lds = std::max(lds, train_mc_lds_floats(ctx->hdm.sp[I]) * 4);
+        if (lds > 160 * 1024) return fail(ctx, "ace_train_step: network too wide for the training kernel's LDS");
+        if (lds > 64 * 1024)
+            HC(hipFuncSetAttribute((const void*)k_mlp_train_mc, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
         if (timed(ctx, "mlp_train", [&] {
-                k_mlp_train<<<(int)T.tiles.size(), 256, lds, ctx->stream>>>(
-                    ctx->d_model.p, ctx->d_params.p, ctx->d_inpw.p, ctx->d_inpb.p, T.d_tiles.p, ctx->ldg, ctx->d_G.p,
-                    wf ? T.d_V.p : nullptr, ctx->d_atom_of_row.p, ctx->d_struc_of.p, T.d_gfac.p, T.tm, T.ldpart,
-                    T.d_part.p);
+                k_mlp_train_mc<<<(int)T.tiles.size(), 64 * TRAIN_WAVES, lds, ctx->stream>>>(
+                    ctx->d_model.p, ctx->d_wpad.p, ctx->d_inpw.p, T.d_tiles.p, ctx->ldg, ctx->d_G.p,
+                    wf ? T.d_V.p : nullptr, ctx->d_atom_of_row.p, ctx->d_struc_of.p, T.d_gfac.p, T.ldpart, T.d_part.p);
             }))
             return -1;
         size_t ntot = 0;

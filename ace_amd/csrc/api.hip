@@ -58,8 +58,12 @@ struct ace_ctx {
     bool all_canon = true;
     DevModel hdm;
     DBuf<DevModel> d_model;
-    DBuf<float> d_r_rs, d_r_eta, d_a_lamf, d_a_zeta, d_params, d_inpw, d_inpb, d_wpad;
+    DBuf<float> d_r_rs, d_r_eta, d_a_lamf, d_a_zeta, d_inpw, d_inpb, d_wpad;
     DBuf<uint16_t> d_w0bf;           // BF16 split copies of every species' first-layer weights
+    // pinned staging of the layouts above (reused; the copy out of it is fenced by stage_done)
+    char* h_stage = nullptr;
+    size_t h_stage_bytes = 0;
+    hipEvent_t stage_done = nullptr;
     int wp_total = 0, bf_total = 0;
     DBuf<int> d_a_zint, d_a_grp, d_a_sel, d_a_cidx;
     bool plain_ok = false;
@@ -67,7 +71,9 @@ struct ace_ctx {
     PlainModel hpm;
     DBuf<PlainModel> d_pmodel;
     DBuf<double> d_r_rs64, d_r_eta64, d_a_eta64, d_a_zeta64, d_a_lamf64, d_nn64, d_inpw64, d_inpb64;
-    bool params_dirty = true;
+    bool params_dirty = true;        // network parameters changed: matrix-core layouts to upload
+    bool scale_dirty = true;         // input scaling changed
+    bool nn64_dirty = true;          // FP64 parameter copy of the plain port is stale
     int precision = ACE_PREC_FP32;   // default of ace_symm / ace_train_step
     bool basis_f64 = false;          // precision of the basis kernels of the current call
     // structures
@@ -301,8 +307,25 @@ float bf16_value(uint16_t h) {
     return x;
 }
 
+// The matrix-core layouts of every network (DevSpecies::wp_off / wl_off / bl_off, and the BF16 split copies
+// of W_0 at bf_off) built in pinned host memory and copied asynchronously; the buffer is reused once the
+// previous copy out of it has completed.  Parameters change every training step, so this is on that path.
 int upload_wpad(ace_ctx* ctx) {
-    std::vector<float> w(std::max(ctx->wp_total, 4), 0.f);
+    const size_t nw = std::max(ctx->wp_total, 4), nb = std::max(ctx->bf_total, 8);
+    const size_t bytes = nw * sizeof(float) + nb * sizeof(uint16_t);
+    if (ctx->stage_done) HC(hipEventSynchronize(ctx->stage_done));
+    else HC(hipEventCreateWithFlags(&ctx->stage_done, hipEventDisableTiming));
+    if (ctx->h_stage_bytes < bytes) {
+        if (ctx->h_stage) HC(hipHostFree(ctx->h_stage));
+        ctx->h_stage = nullptr;
+        ctx->h_stage_bytes = 0;
+        HC(hipHostMalloc((void**)&ctx->h_stage, bytes, hipHostMallocDefault));
+        ctx->h_stage_bytes = bytes;
+    }
+    if (ensure(ctx, ctx->d_wpad, nw) || ensure(ctx, ctx->d_w0bf, nb)) return -1;
+    float* w = (float*)ctx->h_stage;
+    uint16_t* b = (uint16_t*)(ctx->h_stage + nw * sizeof(float));
+    memset(ctx->h_stage, 0, bytes);
     for (int I = 0; I < ctx->nt; ++I) {
         const HSpecies& s = ctx->sp[I];
         const DevSpecies& d = ctx->hdm.sp[I];
@@ -313,26 +336,23 @@ int upload_wpad(ace_ctx* ctx) {
                     w[d.wp_off + d.wl_off[l] + k * mlp_ld(no) + o] = (float)s.params[d.woff[l] + o + k * no];
             for (int o = 0; o < no; ++o) w[d.wp_off + d.bl_off[l] + o] = (float)s.params[d.boff[l] + o];
         }
-    }
-    if (upload(ctx, ctx->d_wpad, w.data(), w.size())) return -1;
-    // BF16 high/low parts of W_0 in the two layouts the BF16 first layer reads (DevSpecies::bf_off)
-    std::vector<uint16_t> b(std::max(ctx->bf_total, 8), 0);
-    for (int I = 0; I < ctx->nt; ++I) {
-        const HSpecies& s = ctx->sp[I];
+        // BF16 high/low parts of W_0 in the two layouts the BF16 first layer reads
         const int K = s.width[0], n1 = s.width[1];
         const int fwd = ((n1 + 15) & ~15) * ((K + 31) & ~31), bwd = ((K + 15) & ~15) * ((n1 + 31) & ~31);
-        uint16_t* f = b.data() + ctx->hdm.sp[I].bf_off;
+        uint16_t* f = b + d.bf_off;
         uint16_t* g = f + 2 * fwd;
         for (int o = 0; o < n1; ++o)
             for (int k = 0; k < K; ++k) {
-                const float x = (float)s.params[ctx->hdm.sp[I].woff[0] + o + k * n1];
+                const float x = (float)s.params[d.woff[0] + o + k * n1];
                 const uint16_t hi = bf16_rne(x), lo = bf16_rne(x - bf16_value(hi));
                 const int ef = o * ((K + 31) & ~31) + k, eb = k * ((n1 + 31) & ~31) + o;
                 f[ef] = hi; f[fwd + ef] = lo;
                 g[eb] = hi; g[bwd + eb] = lo;
             }
     }
-    if (upload(ctx, ctx->d_w0bf, b.data(), b.size())) return -1;
+    HC(hipMemcpyAsync(ctx->d_wpad.p, w, nw * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    HC(hipMemcpyAsync(ctx->d_w0bf.p, b, nb * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream));
+    HC(hipEventRecord(ctx->stage_done, ctx->stream));
     return 0;
 }
 
@@ -510,7 +530,6 @@ int build_tables(ace_ctx* ctx) {
     if (upload(ctx, ctx->d_a_grp, a_grp.data(), a_grp.size())) return -1;
     if (upload(ctx, ctx->d_a_sel, a_sel.data(), a_sel.size())) return -1;
     if (upload(ctx, ctx->d_a_cidx, a_cidx.data(), a_cidx.size())) return -1;
-    if (upload(ctx, ctx->d_params, prm.data(), prm.size())) return -1;
     if (upload(ctx, ctx->d_inpw, iw.data(), iw.size())) return -1;
     if (upload(ctx, ctx->d_inpb, ib.data(), ib.size())) return -1;
     ctx->wp_total = wp_total;
@@ -571,27 +590,38 @@ int build_tables(ace_ctx* ctx) {
     return 0;
 }
 
+// device copies of what changed since the last launch: the matrix-core layouts of the networks, the input
+// scaling (FP32 and FP64); the plain port's FP64 parameters are refreshed by the port itself
 int refresh_params(ace_ctx* ctx) {
-    if (!ctx->params_dirty) return 0;
-    std::vector<float> prm, iw, ib;
-    std::vector<double> nn, iw64, ib64;
-    for (int I = 0; I < ctx->nt; ++I) {
-        const HSpecies& s = ctx->sp[I];
-        for (double v : s.params) { prm.push_back((float)v); nn.push_back(v); }
-        for (int k = 0; k < s.K; ++k) {
-            iw.push_back((float)s.inpw[k]); ib.push_back((float)s.inpb[k]);
-            iw64.push_back(s.inpw[k]); ib64.push_back(s.inpb[k]);
-        }
-    }
-    if (upload(ctx, ctx->d_params, prm.data(), prm.size())) return -1;
-    if (upload_wpad(ctx)) return -1;
-    if (upload(ctx, ctx->d_inpw, iw.data(), iw.size())) return -1;
-    if (upload(ctx, ctx->d_inpb, ib.data(), ib.size())) return -1;
-    if (upload(ctx, ctx->d_nn64, nn.data(), nn.size())) return -1;
-    if (upload(ctx, ctx->d_inpw64, iw64.data(), iw64.size())) return -1;
-    if (upload(ctx, ctx->d_inpb64, ib64.data(), ib64.size())) return -1;
-    HC(hipStreamSynchronize(ctx->stream));
+    if (!ctx->params_dirty && !ctx->scale_dirty) return 0;
+    std::vector<float> iw, ib;
+    std::vector<double> iw64, ib64;
+    if (ctx->params_dirty && upload_wpad(ctx)) return -1;
     ctx->params_dirty = false;
+    if (ctx->scale_dirty) {
+        for (int I = 0; I < ctx->nt; ++I) {
+            const HSpecies& s = ctx->sp[I];
+            for (int k = 0; k < s.K; ++k) {
+                iw.push_back((float)s.inpw[k]); ib.push_back((float)s.inpb[k]);
+                iw64.push_back(s.inpw[k]); ib64.push_back(s.inpb[k]);
+            }
+        }
+        if (upload(ctx, ctx->d_inpw, iw.data(), iw.size()) || upload(ctx, ctx->d_inpb, ib.data(), ib.size()) ||
+            upload(ctx, ctx->d_inpw64, iw64.data(), iw64.size()) || upload(ctx, ctx->d_inpb64, ib64.data(), ib64.size()))
+            return -1;
+        HC(hipStreamSynchronize(ctx->stream));   // the host vectors end here
+        ctx->scale_dirty = false;
+    }
+    return 0;
+}
+
+int refresh_nn64(ace_ctx* ctx) {
+    if (!ctx->nn64_dirty) return 0;
+    std::vector<double> nn;
+    for (int I = 0; I < ctx->nt; ++I) nn.insert(nn.end(), ctx->sp[I].params.begin(), ctx->sp[I].params.end());
+    if (upload(ctx, ctx->d_nn64, nn.data(), nn.size())) return -1;
+    HC(hipStreamSynchronize(ctx->stream));
+    ctx->nn64_dirty = false;
     return 0;
 }
 
@@ -922,7 +952,7 @@ void ace_destroy(ace_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& r : ctx->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     release(ctx->d_model); release(ctx->d_r_rs); release(ctx->d_r_eta); release(ctx->d_a_lamf);
-    release(ctx->d_a_zeta); release(ctx->d_params); release(ctx->d_wpad); release(ctx->d_w0bf); release(ctx->d_inpw); release(ctx->d_inpb);
+    release(ctx->d_a_zeta); release(ctx->d_wpad); release(ctx->d_w0bf); release(ctx->d_inpw); release(ctx->d_inpb);
     release(ctx->d_a_zint); release(ctx->d_a_grp); release(ctx->d_a_sel); release(ctx->d_a_cidx); release(ctx->d_pmodel); release(ctx->d_r_rs64);
     release(ctx->d_r_eta64); release(ctx->d_a_eta64); release(ctx->d_a_zeta64); release(ctx->d_a_lamf64);
     release(ctx->d_nn64); release(ctx->d_inpw64); release(ctx->d_inpb64);
@@ -937,6 +967,8 @@ void ace_destroy(ace_ctx* ctx) {
     release(ctx->d_nb_off); release(ctx->d_scan_blk); release(ctx->d_scal); release(ctx->d_nb_j);
     release(ctx->d_nb_img); release(ctx->d_G64);
     release(ctx->d_vel); release(ctx->d_minv); release(ctx->d_md_rec); release(ctx->d_md_ctr);
+    if (ctx->stage_done) (void)hipEventDestroy(ctx->stage_done);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
     for (int q = 0; q < 2; ++q)
         if (ctx->md_exec[q]) (void)hipGraphExecDestroy(ctx->md_exec[q]);
     train_release(ctx);
@@ -962,6 +994,7 @@ int ace_set_params(ace_ctx* ctx, int type, const double* params, int n) {
     if (n != ctx->sp[type].nparams()) return fail(ctx, "ace_set_params: invalid size: vector and network mismatch");
     ctx->sp[type].params.assign(params, params + n);
     ctx->params_dirty = true;
+    ctx->nn64_dirty = true;
     return 0;
 }
 
@@ -978,7 +1011,7 @@ int ace_set_input_scaling(ace_ctx* ctx, int type, const double* inpw, const doub
     if (n != ctx->sp[type].K) return fail(ctx, "ace_set_input_scaling: size differs from the input width");
     ctx->sp[type].inpw.assign(inpw, inpw + n);
     ctx->sp[type].inpb.assign(inpb, inpb + n);
-    ctx->params_dirty = true;
+    ctx->scale_dirty = true;
     ctx->tr.g_valid = false;     // cached network-input rows include the scaling
     return 0;
 }
@@ -1388,7 +1421,7 @@ int ace_md_state(ace_ctx* ctx, double* pos, double* vel) {
 int ace_plain_energy_forces(ace_ctx* ctx, double* E_struc, double* F) {
     if (!ctx) return -1;
     if (!ctx->plain_ok) return fail(ctx, "ace_plain_energy_forces: model outside the port's forms (" + ctx->plain_why + ")");
-    if (require_nb(ctx)) return -1;
+    if (require_nb(ctx) || refresh_nn64(ctx)) return -1;
     if (ensure_csr(ctx)) return -1;
     int nmax = 0;
     for (int s = 0; s < ctx->S; ++s) nmax = std::max(nmax, ctx->natoms[s]);

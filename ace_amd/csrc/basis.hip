@@ -171,10 +171,27 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
 #pragma unroll
                 for (int g = 0; g < NG; ++g) wgt[s][k][g] = wtab[(s * 7 + k) * NG + g];
     }
-    const float rn = 1.f / (float)(same ? nJ : min(nJ, nK));
+    // (row d, column x) of triplet t = d m + x, advanced by BT per iteration from one exact decode
+    const int m = same ? nJ : min(nJ, nK), Mx = same ? nJ : max(nJ, nK);
+    const bool sw = !same && nJ > nK;
+    const int qB = BT / m, rB = BT - qB * m;
+    int x, d;
+    {
+        int a0, c0;
+        tri_decode(tid, same, nJ, nK, 1.f / (float)m, a0, c0);
+        x = sw ? c0 : a0;
+        d = (tid - x) / m;
+    }
     for (int t = tid; t < T3; t += BT) {
-        int a, c;
-        tri_decode(t, same, nJ, nK, rn, a, c);
+        int y = x + d + (same ? 1 : 0);
+        y -= (y >= Mx) ? Mx : 0;
+        const int a = sw ? y : x, c = sw ? x : y;
+        x += rB;
+        d += qB;
+        if (x >= m) {
+            x -= m;
+            ++d;
+        }
         const int sa = bJ + a, sc = bK + c;
         const T uax = L.u[4 * sa], uay = L.u[4 * sa + 1], uaz = L.u[4 * sa + 2];
         const T ucx = L.u[4 * sc], ucy = L.u[4 * sc + 1], ucz = L.u[4 * sc + 2];

@@ -71,6 +71,11 @@ struct ace_ctx {
     PlainModel hpm;
     DBuf<PlainModel> d_pmodel;
     DBuf<double> d_r_rs64, d_r_eta64, d_a_eta64, d_a_zeta64, d_a_lamf64, d_nn64, d_inpw64, d_inpb64;
+    // ace_autotune's choices (-1: the size rule), valid for the batch size and row capacity measured on
+    int tune_fused[3] = {-1, -1, -1};   // per precision: 1 fused kernel, 0 symm -> network -> adjoint
+    int tune_stage[3] = {-1, -1, -1};   // per precision: network weights staged in LDS (1) or read from memory
+    long long tune_N = -1;
+    int tune_slots = -1;
     bool params_dirty = true;        // network parameters changed: matrix-core layouts to upload
     bool scale_dirty = true;         // input scaling changed
     bool nn64_dirty = true;          // FP64 parameter copy of the plain port is stale
@@ -753,8 +758,12 @@ int basis_bytes(ace_ctx* ctx, int mode) {
 // MI355X, 80-32-32-1 bcc: FP32 fused 0.178 / split 0.186 ms at 4000 atoms, 0.323 / 0.319 at 8000; FP64
 // basis 0.128 / 0.132 at 2000, 0.230 / 0.223 at 4000).  Fused when the network fits 64 KB of LDS and the
 // grid is at most four rounds of 256 CUs x the workgroups that fit in 160 KB.
-bool use_fused(ace_ctx* ctx) {
+bool tuned(const ace_ctx* ctx) { return ctx->tune_N == ctx->N && ctx->tune_slots == ctx->slots_cap; }
+
+bool use_fused(ace_ctx* ctx, int precision) {
     const int bytes = basis_bytes(ctx, 3);
+    if (precision == ACE_PREC_BF16L1 || bytes > 160 * 1024) return false;
+    if (tuned(ctx) && ctx->tune_fused[precision] >= 0) return ctx->tune_fused[precision] == 1;
     if (bytes > 64 * 1024) return false;
     const long long per_round = 256LL * std::max(1, (160 * 1024) / bytes);
     return ctx->N <= 4 * per_round;
@@ -814,13 +823,14 @@ int run_finalize(ace_ctx* ctx) {
     });
 }
 
-int run_mlp(ace_ctx* ctx, bool bf16) {
+int run_mlp(ace_ctx* ctx, bool bf16, int precision = ACE_PREC_FP32) {
     int lds = 0, lds_w = 0;
     for (int I = 0; I < ctx->nt; ++I) {
         lds = std::max(lds, mlp_lds_floats(ctx->hdm.sp[I]) * 4);
         lds_w = std::max(lds_w, (mlp_lds_floats(ctx->hdm.sp[I]) + ctx->hdm.sp[I].wp_n) * 4);
     }
-    const bool stage = lds_w <= 64 * 1024;
+    bool stage = lds_w <= 64 * 1024;
+    if (tuned(ctx) && ctx->tune_stage[precision] >= 0 && lds_w <= 160 * 1024) stage = ctx->tune_stage[precision] == 1;
     if (stage) lds = lds_w;
     if (lds > 160 * 1024) return fail(ctx, "mlp: network too wide for LDS");
     auto f = stage ? (bf16 ? k_mlp_energy_grad<true, true> : k_mlp_energy_grad<true, false>)
@@ -838,9 +848,9 @@ int run_mlp(ace_ctx* ctx, bool bf16) {
 // the fixed-point accumulators.  The BF16 first layer runs on the matrix cores of the network kernel, so
 // that mode takes the split path.
 int run_ef(ace_ctx* ctx, int precision) {
-    if (precision != ACE_PREC_BF16L1 && use_fused(ctx)) return run_basis(ctx, 3);
+    if (use_fused(ctx, precision)) return run_basis(ctx, 3);
     if (run_basis(ctx, 0)) return -1;
-    if (run_mlp(ctx, precision == ACE_PREC_BF16L1)) return -1;
+    if (run_mlp(ctx, precision == ACE_PREC_BF16L1, precision)) return -1;
     return run_basis(ctx, 1);
 }
 
@@ -1415,6 +1425,62 @@ int ace_md_state(ace_ctx* ctx, double* pos, double* vel) {
     if (pos) HC(hipMemcpyAsync(pos, ctx->d_pos.p, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (vel) HC(hipMemcpyAsync(vel, ctx->d_vel.p, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HC(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int ace_autotune(ace_ctx* ctx, int precision, double* ms) {
+    if (!ctx) return -1;
+    if (precision != ACE_PREC_FP32 && precision != ACE_PREC_FP64_BASIS && precision != ACE_PREC_BF16L1)
+        return fail(ctx, "ace_autotune: precision mode not available");
+    ctx->basis_f64 = precision == ACE_PREC_FP64_BASIS;
+    if (require_nb(ctx)) return -1;
+    if (ctx->nb_pending && nb_verify(ctx)) return -1;
+    int lds_w = 0;
+    for (int I = 0; I < ctx->nt; ++I) lds_w = std::max(lds_w, (mlp_lds_floats(ctx->hdm.sp[I]) + ctx->hdm.sp[I].wp_n) * 4);
+    // candidates: fused; split with staged weights; split with weights read from memory
+    const bool can_fused = precision != ACE_PREC_BF16L1 && basis_bytes(ctx, 3) <= 160 * 1024;
+    const bool can_stage = lds_w <= 160 * 1024;
+    const int cf[3] = {1, 0, 0}, cs[3] = {-1, 1, 0};
+    const bool ok[3] = {can_fused, can_stage, true};
+    double t[3] = {-1, -1, -1};
+    const bool prof = ctx->prof;
+    ctx->prof = false;
+    ctx->tune_N = ctx->N;
+    ctx->tune_slots = ctx->slots_cap;
+    hipEvent_t e0, e1;
+    HC(hipEventCreate(&e0));
+    HC(hipEventCreate(&e1));
+    int best = -1;
+    for (int c = 0; c < 3; ++c) {
+        if (!ok[c]) continue;
+        ctx->tune_fused[precision] = cf[c];
+        ctx->tune_stage[precision] = cs[c];
+        if (run_ef(ctx, precision)) return -1;   // warm-up (kernel loads, LDS attributes)
+        float el = 0;
+        int reps = 3;
+        for (int pass = 0; pass < 2; ++pass) {
+            HC(hipEventRecord(e0, ctx->stream));
+            for (int r = 0; r < reps; ++r)
+                if (run_ef(ctx, precision)) return -1;
+            HC(hipEventRecord(e1, ctx->stream));
+            HC(hipEventSynchronize(e1));
+            HC(hipEventElapsedTime(&el, e0, e1));
+            if (el / reps > 0.5f) break;
+            reps = 20;                               // short launches: time more of them
+        }
+        t[c] = (double)el / reps;
+        if (best < 0 || t[c] < t[best]) best = c;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->prof = prof;
+    ctx->tune_fused[precision] = cf[best];
+    ctx->tune_stage[precision] = cs[best];
+    if (check_err_flag(ctx, "ace_autotune")) return -1;
+    if (ms) {
+        for (int c = 0; c < 3; ++c) ms[c] = t[c];
+        ms[3] = best;
+    }
     return 0;
 }
 

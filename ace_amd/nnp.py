@@ -178,6 +178,14 @@ class NNPDevice:
         forces, in one call; results stay in HBM."""
         self._check(self.lib.ace_compute(self.ctx, precision, float(skin), None, None, None), "ace_compute")
 
+    def autotune(self, precision=PREC_FP32):
+        """Measures the energy/force paths on the uploaded batch and keeps the fastest; returns the
+        milliseconds of (fused, split with staged weights, split with weights in memory) and the choice."""
+        ms = np.zeros(4)
+        self._check(self.lib.ace_autotune(self.ctx, precision, _dp(ms)), "ace_autotune")
+        names = ("fused", "split_staged", "split")
+        return {"ms": {names[i]: float(ms[i]) for i in range(3) if ms[i] >= 0}, "choice": names[int(ms[3])]}
+
     def md_init(self, masses, vel=None):
         """Per-species masses (amu) and velocities (N x 3, A/ps; None = at rest) for md_run."""
         m = np.ascontiguousarray(masses, dtype=np.float64)

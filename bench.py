@@ -241,6 +241,7 @@ def main():
     G = dev.compute_symm()
     configs.set_input_scaling(model, [s], G)       # PreBias MEAN / PreScale DEV from the GPU's G
     dev.sync_model_scaling()
+    tune = {"fp32": dev.autotune(PREC_FP32), "fp64_basis": dev.autotune(PREC_FP64_BASIS)}
 
     def run(prec, steps, events):
         """K steps of NNP::compute (neighbour build + energy/forces, one ace_compute call each); returns ms per
@@ -332,6 +333,7 @@ def main():
                      "flops_per_launch": int(flops[dom]), "avg_launch_ms": avg[dom],
                      "note": "FP32 vector-ALU kernel priced against the FP32 dense peak"},
         "kernels_avg_us": {k: round(v * 1e3, 2) for k, v in avg.items()},
+        "autotune": tune,
         "ms_per_step_with_kernel_events": ms_ev,
         "fp64_basis": {"value": s.natoms / (ms64 * 1e-3), "unit": "atoms/s", "ms_per_step": ms64,
                        "kernels_avg_us": {k: round(v * 1e3, 2) for k, v in avg64.items()}},

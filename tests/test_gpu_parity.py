@@ -411,3 +411,20 @@ def test_potential_file_on_gpu(tmp_path):
     log(f"potential file: dE/atom {de:.1e} dF {df:.1e}")
     assert de < TOL[PREC_FP32]["E"] and df < TOL[PREC_FP32]["F"]
     dev.close()
+
+
+def test_autotune_keeps_results_within_tolerance():
+    """Every autotuner candidate computes the same quantities: after tuning, energies and forces still
+    match the oracle to the FP32 bounds, and each candidate that was timed is reported."""
+    m, strucs = configs.config2()
+    scaled_model(m, strucs)
+    dev = NNPDevice(m)
+    dev.upload(strucs)
+    dev.build_neighbors()
+    r = dev.autotune(PREC_FP32)
+    log(f"autotune c2: {r}")
+    assert r["choice"] in r["ms"] and all(v > 0 for v in r["ms"].values()) and len(r["ms"]) >= 2
+    E, F, _ = dev.compute(PREC_FP32)
+    Er, Fr, _ = O.Oracle(m).compute(strucs[0])
+    assert abs(E[0] - Er) / strucs[0].natoms < TOL[PREC_FP32]["E"] and np.abs(F - Fr).max() < TOL[PREC_FP32]["F"]
+    dev.close()

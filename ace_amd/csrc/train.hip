@@ -22,6 +22,9 @@ struct TrainState {
     DBuf<double> d_Eref, d_Fref, d_gfac, d_eps, d_out, d_grad, d_gsum;
     DBuf<float> d_V, d_part;
     int ldpart = 0;
+    bool use_graph = false;           // replay the device part of a step from a captured HIP graph
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    long long graph_sig[2][12] = {};
 };
 
 __device__ inline void act_fd2(int act, float c, float z, float& a, float& d, float& d2) {

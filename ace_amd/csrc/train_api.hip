@@ -5,6 +5,11 @@ void train_release(ace_ctx* ctx) {
     TrainState& T = ctx->tr;
     release(T.d_tiles); release(T.d_Eref); release(T.d_Fref); release(T.d_gfac); release(T.d_eps);
     release(T.d_out); release(T.d_grad); release(T.d_gsum); release(T.d_V); release(T.d_part);
+    for (int q = 0; q < 2; ++q)
+        if (ctx->tr.graph[q]) {
+            (void)hipGraphExecDestroy(ctx->tr.graph[q]);
+            ctx->tr.graph[q] = nullptr;
+        }
 }
 
 void train_on_upload(ace_ctx* ctx) {
@@ -38,6 +43,123 @@ int train_tiles(ace_ctx* ctx) {
     return 0;
 }
 
+// The device part of a training step (network, forces, loss, tangent, gradient), after the G rows and the
+// buffers are in place: a fixed sequence of launches, captured into a HIP graph when the caller asks.
+int train_launch(ace_ctx* ctx, const ace_loss_desc& L, bool wf, bool want_grad) {
+    TrainState& T = ctx->tr;
+    const int S = ctx->S;
+    if (run_mlp(ctx, false)) return -1;
+    if (wf) {
+        if (run_basis(ctx, 1)) return -1;
+        if (run_finalize(ctx)) return -1;
+    } else if (timed(ctx, "struct_reduce", [&] {
+                   k_struct_reduce<<<S, RED_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p,
+                                                                       ctx->d_e_atom.p, ctx->d_E.p);
+               })) {
+        return -1;
+    }
+    if (timed(ctx, "loss", [&] {
+            k_loss<<<S, LOSS_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p, ctx->d_E.p, T.d_Eref.p,
+                                                        ctx->d_F.p, wf ? T.d_Fref.p : nullptr, L, T.d_gfac.p,
+                                                        T.d_eps.p, T.d_out.p);
+        }))
+        return -1;
+    if (wf) {
+        if (run_basis(ctx, 2, T.d_V.p, T.d_eps.p)) return -1;
+    }
+    if (want_grad) {
+        int lds = 0;
+        for (int I = 0; I < ctx->nt; ++I) lds = std::max(lds, train_mc_lds_floats(ctx->hdm.sp[I]) * 4);
+        if (lds > 160 * 1024) return fail(ctx, "ace_train_step: network too wide for the training kernel's LDS");
+        if (lds > 64 * 1024)
+            HC(hipFuncSetAttribute((const void*)k_mlp_train_mc, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+        if (timed(ctx, "mlp_train", [&] {
+                k_mlp_train_mc<<<(int)T.tiles.size(), 64 * TRAIN_WAVES, lds, ctx->stream>>>(
+                    ctx->d_model.p, ctx->d_wpad.p, ctx->d_inpw.p, T.d_tiles.p, ctx->ldg, ctx->d_G.p,
+                    wf ? T.d_V.p : nullptr, ctx->d_atom_of_row.p, ctx->d_struc_of.p, T.d_gfac.p, T.ldpart, T.d_part.p);
+            }))
+            return -1;
+        GradRed R;
+        memset(&R, 0, sizeof(R));
+        R.nt = ctx->nt;
+        int off = 0, npmax = 1, ngmax = 1;
+        long long gofs = 0;
+        for (int I = 0; I < ctx->nt; ++I) {
+            R.np[I] = ctx->sp[I].nparams();
+            R.off[I] = off;
+            R.t0[I] = T.tile_first[I];
+            R.t1[I] = T.tile_first[I + 1];
+            R.gofs[I] = gofs;
+            const int ng = (R.t1[I] - R.t0[I] + GRED_GROUP - 1) / GRED_GROUP;
+            gofs += (long long)ng * R.np[I];
+            off += R.np[I];
+            npmax = std::max(npmax, R.np[I]);
+            ngmax = std::max(ngmax, ng);
+        }
+        const dim3 g1(cdiv(npmax, 256), ngmax, ctx->nt), g2(cdiv(npmax, 256), 1, ctx->nt);
+        if (timed(ctx, "grad_reduce", [&] {
+                k_grad_reduce1<<<g1, 256, 0, ctx->stream>>>(R, T.ldpart, T.d_part.p, T.d_gsum.p);
+            }) ||
+            timed(ctx, "grad_reduce", [&] { k_grad_reduce2<<<g2, 256, 0, ctx->stream>>>(R, T.d_gsum.p, T.d_grad.p); }))
+            return -1;
+    }
+    return 0;
+}
+
+// grad-reduction layout of the current tiles: sizes of the pass-1 buffer and the gradient
+void train_sizes(ace_ctx* ctx, size_t& ntot, long long& gofs) {
+    TrainState& T = ctx->tr;
+    ntot = 0;
+    gofs = 0;
+    for (int I = 0; I < ctx->nt; ++I) {
+        const int np = ctx->sp[I].nparams();
+        ntot += np;
+        gofs += (long long)((T.tile_first[I + 1] - T.tile_first[I] + GRED_GROUP - 1) / GRED_GROUP) * np;
+    }
+}
+
+// one graph per accumulator buffer the step's force launch adds into (the step toggles it once); captured
+// again when a buffer was reallocated or a value baked into the launches changed
+int train_graph_launch(ace_ctx* ctx, const ace_loss_desc& L, bool wf, bool want_grad) {
+    TrainState& T = ctx->tr;
+    const int p = ctx->fbuf;
+    long long sig[12];
+    memcpy(&sig[0], &L.delta, 8);
+    memcpy(&sig[1], &L.w_energy, 8);
+    memcpy(&sig[2], &L.w_force, 8);
+    sig[3] = (long long)ctx->gen; sig[4] = L.loss; sig[5] = L.norm; sig[6] = wf; sig[7] = want_grad;
+    sig[8] = ctx->N; sig[9] = ctx->S; sig[10] = (long long)T.tiles.size() * 64 + ctx->slots_cap;
+    sig[11] = ctx->basis_f64;
+    if (!T.graph[p] || memcmp(sig, T.graph_sig[p], sizeof(sig)) != 0) {
+        if (T.graph[p]) {
+            HC(hipGraphExecDestroy(T.graph[p]));
+            T.graph[p] = nullptr;
+        }
+        const bool prof = ctx->prof;
+        ctx->prof = false;
+        HC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        const int rc = train_launch(ctx, L, wf, want_grad);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        ctx->prof = prof;
+        ctx->fbuf = p;
+        if (rc || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? -1 : fail_hip(ctx, "train: graph capture", e);
+        }
+        const hipError_t ei = hipGraphInstantiate(&T.graph[p], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            T.graph[p] = nullptr;
+            return fail_hip(ctx, "train: graph instantiate", ei);
+        }
+        memcpy(T.graph_sig[p], sig, sizeof(sig));
+    }
+    HC(hipGraphLaunch(T.graph[p], ctx->stream));
+    ctx->fbuf = 1 - p;   // as the launches inside did
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -55,6 +177,12 @@ int ace_set_targets(ace_ctx* ctx, const double* E_ref, const double* F_ref) {
     }
     HC(hipStreamSynchronize(ctx->stream));
     T.have_targets = true;
+    return 0;
+}
+
+int ace_train_graph(ace_ctx* ctx, int on) {
+    if (!ctx) return -1;
+    ctx->tr.use_graph = on != 0;
     return 0;
 }
 
@@ -81,68 +209,19 @@ int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* 
         if (run_basis(ctx, 0)) return -1;
         T.g_valid = true;
     }
-    if (run_mlp(ctx, false)) return -1;
-    if (wf) {
-        if (run_basis(ctx, 1)) return -1;
-        if (run_finalize(ctx)) return -1;
-    } else if (timed(ctx, "struct_reduce", [&] {
-                   k_struct_reduce<<<S, RED_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p,
-                                                                       ctx->d_e_atom.p, ctx->d_E.p);
-               })) {
-        return -1;
-    }
     const ace_loss_desc L = *Ld;
-    if (timed(ctx, "loss", [&] {
-            k_loss<<<S, LOSS_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p, ctx->d_E.p, T.d_Eref.p,
-                                                        ctx->d_F.p, wf ? T.d_Fref.p : nullptr, L, T.d_gfac.p,
-                                                        T.d_eps.p, T.d_out.p);
-        }))
+    size_t ntot = 0;
+    long long gofs = 0;
+    train_sizes(ctx, ntot, gofs);
+    if ((wf && ensure(ctx, T.d_V, (size_t)N * ctx->ldg)) ||
+        (grad && (ensure(ctx, T.d_grad, ntot) || ensure(ctx, T.d_gsum, (size_t)std::max(gofs, 1LL)))))
         return -1;
-    if (wf) {
-        if (ensure(ctx, T.d_V, (size_t)N * ctx->ldg)) return -1;
-        if (run_basis(ctx, 2, T.d_V.p, T.d_eps.p)) return -1;
+    if (T.use_graph) {
+        if (train_graph_launch(ctx, L, wf, grad != nullptr)) return -1;
+    } else if (train_launch(ctx, L, wf, grad != nullptr)) {
+        return -1;
     }
-    if (grad) {
-        int lds = 0;
-        for (int I = 0; I < ctx->nt; ++I) lds = std::max(lds, train_mc_lds_floats(ctx->hdm.sp[I]) * 4);
-        if (lds > 160 * 1024) return fail(ctx, "ace_train_step: network too wide for the training kernel's LDS");
-        if (lds > 64 * 1024)
-            HC(hipFuncSetAttribute((const void*)k_mlp_train_mc, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-        if (timed(ctx, "mlp_train", [&] {
-                k_mlp_train_mc<<<(int)T.tiles.size(), 64 * TRAIN_WAVES, lds, ctx->stream>>>(
-                    ctx->d_model.p, ctx->d_wpad.p, ctx->d_inpw.p, T.d_tiles.p, ctx->ldg, ctx->d_G.p,
-                    wf ? T.d_V.p : nullptr, ctx->d_atom_of_row.p, ctx->d_struc_of.p, T.d_gfac.p, T.ldpart, T.d_part.p);
-            }))
-            return -1;
-        size_t ntot = 0;
-        for (int I = 0; I < ctx->nt; ++I) ntot += ctx->sp[I].nparams();
-        if (ensure(ctx, T.d_grad, ntot)) return -1;
-        GradRed R;
-        memset(&R, 0, sizeof(R));
-        R.nt = ctx->nt;
-        int off = 0, npmax = 1, ngmax = 1;
-        long long gofs = 0;
-        for (int I = 0; I < ctx->nt; ++I) {
-            R.np[I] = ctx->sp[I].nparams();
-            R.off[I] = off;
-            R.t0[I] = T.tile_first[I];
-            R.t1[I] = T.tile_first[I + 1];
-            R.gofs[I] = gofs;
-            const int ng = (R.t1[I] - R.t0[I] + GRED_GROUP - 1) / GRED_GROUP;
-            gofs += (long long)ng * R.np[I];
-            off += R.np[I];
-            npmax = std::max(npmax, R.np[I]);
-            ngmax = std::max(ngmax, ng);
-        }
-        if (ensure(ctx, T.d_gsum, (size_t)std::max(gofs, 1LL))) return -1;
-        const dim3 g1(cdiv(npmax, 256), ngmax, ctx->nt), g2(cdiv(npmax, 256), 1, ctx->nt);
-        if (timed(ctx, "grad_reduce", [&] {
-                k_grad_reduce1<<<g1, 256, 0, ctx->stream>>>(R, T.ldpart, T.d_part.p, T.d_gsum.p);
-            }) ||
-            timed(ctx, "grad_reduce", [&] { k_grad_reduce2<<<g2, 256, 0, ctx->stream>>>(R, T.d_gsum.p, T.d_grad.p); }))
-            return -1;
-        HC(hipMemcpyAsync(grad, T.d_grad.p, ntot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    }
+    if (grad) HC(hipMemcpyAsync(grad, T.d_grad.p, ntot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     std::vector<double> out(3 * (size_t)S);
     HC(hipMemcpyAsync(out.data(), T.d_out.p, 3 * (size_t)S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HC(hipStreamSynchronize(ctx->stream));

@@ -220,6 +220,10 @@ class NNPDevice:
         f = None if F_ref is None else np.ascontiguousarray(F_ref, dtype=np.float64).reshape(-1)
         self._check(self.lib.ace_set_targets(self.ctx, _dp(e), _dp(f)), "ace_set_targets")
 
+    def train_graph(self, on=True):
+        """Replay the device part of train_step from a captured HIP graph."""
+        self._check(self.lib.ace_train_graph(self.ctx, 1 if on else 0), "ace_train_graph")
+
     def train_step(self, loss=0, norm=0, delta=1.0, w_energy=1.0, w_force=0.0, want_grad=True):
         d = LossDesc(loss, norm, delta, w_energy, w_force)
         out = np.zeros(4)

@@ -77,7 +77,7 @@ class Trainer:
     "old" value, parameters and gradient."""
 
     def __init__(self, dev, model, strucs, gamma=1e-3, w_force=0.1, loss=0, norm=0, delta=1.0, algo="ADAM",
-                 decay=None, reg="NONE", eta_reg=0.0):
+                 decay=None, reg="NONE", eta_reg=0.0, graph=False):
         from .optimizers import Decay, Objective, make
         self.dev, self.model = dev, model
         self.nat = sum(s.natoms for s in strucs)
@@ -91,6 +91,8 @@ class Trainer:
                                     for n in model.nets])
         self.it = 0
         self.w_force, self.loss, self.norm, self.delta = w_force, loss, norm, delta
+        if graph:
+            dev.train_graph(True)
         have_f = bool(strucs) and all(s.forces is not None for s in strucs)
         dev.set_targets([s.energy for s in strucs],
                         np.concatenate([s.forces for s in strucs]) if have_f else None)

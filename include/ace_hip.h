@@ -205,6 +205,9 @@ int  ace_plain_energy_forces(ace_ctx* ctx, double* E_struc, double* F);
  * loss[3] = sum_s (dE/nAtoms)^2.  grad may be NULL.
  */
 int  ace_set_targets(ace_ctx* ctx, const double* E_ref, const double* F_ref);
+/* on != 0: ace_train_step replays its device launches (networks, forces, loss, tangent, gradient) from a
+ * captured HIP graph (no counterpart in ace; same results as the plain launches). */
+int  ace_train_graph(ace_ctx* ctx, int on);
 int  ace_train_step(ace_ctx* ctx, const ace_loss_desc* loss_desc, double* loss, double* grad);
 
 /* Timing of the last calls: per-kernel HIP-event durations accumulated while enabled. */

@@ -40,6 +40,7 @@ SIGNATURES = {
     "ace_md_run": (C.c_int, [P, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, dp]),
     "ace_md_state": (C.c_int, [P, dp, dp]),
     "ace_autotune": (C.c_int, [P, C.c_int, dp]),
+    "ace_set_path": (C.c_int, [P, C.c_int, C.c_int]),
     "ace_train_graph": (C.c_int, [P, C.c_int]),
     "ace_plain_energy_forces": (C.c_int, [P, dp, dp]),
     "ace_set_targets": (C.c_int, [P, dp, dp]),

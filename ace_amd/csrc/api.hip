@@ -72,10 +72,12 @@ struct ace_ctx {
     DBuf<PlainModel> d_pmodel;
     DBuf<double> d_r_rs64, d_r_eta64, d_a_eta64, d_a_zeta64, d_a_lamf64, d_nn64, d_inpw64, d_inpb64;
     // ace_autotune's choices (-1: the size rule), valid for the batch size and row capacity measured on
-    int tune_fused[3] = {-1, -1, -1};   // per precision: 1 fused kernel, 0 symm -> network -> adjoint
+    int tune_fused[3] = {-1, -1, -1};   // per precision: fused kernel with 4 (1) or 2 (2) waves per atom, or
+                                        // symm -> network -> adjoint (0)
     int tune_stage[3] = {-1, -1, -1};   // per precision: network weights staged in LDS (1) or read from memory
     long long tune_N = -1;
     int tune_slots = -1;
+    bool pinned[3] = {false, false, false};   // ace_set_path: the choice holds for every batch
     bool params_dirty = true;        // network parameters changed: matrix-core layouts to upload
     bool scale_dirty = true;         // input scaling changed
     bool nn64_dirty = true;          // FP64 parameter copy of the plain port is stale
@@ -655,6 +657,7 @@ DevBasisParams basis_params(ace_ctx* ctx) {
 
 typedef void (*BasisFn)(int, const DevModel*, DevBasisParams, BasisIO, int, int, int);
 
+// f[0..3]: modes 0-3 with four waves per atom; f[4]: the fused kernel (mode 3) with two
 template <class T, int NG, int NZ>
 void pick_basis(BasisFn* f, bool gen) {
     if (gen) {
@@ -662,11 +665,13 @@ void pick_basis(BasisFn* f, bool gen) {
         f[1] = k_basis<T, NG, NZ, 1, 1>;
         f[2] = k_basis<T, NG, NZ, 2, 1>;
         f[3] = k_basis<T, NG, NZ, 3, 1>;
+        f[4] = k_basis<T, NG, NZ, 3, 1, 2>;
     } else {
         f[0] = k_basis<T, NG, NZ, 0, 0>;
         f[1] = k_basis<T, NG, NZ, 1, 0>;
         f[2] = k_basis<T, NG, NZ, 2, 0>;
         f[3] = k_basis<T, NG, NZ, 3, 0>;
+        f[4] = k_basis<T, NG, NZ, 3, 0, 2>;
     }
 }
 
@@ -738,47 +743,67 @@ int net_lds_floats(const ace_ctx* ctx) {
     int m = 0;
     for (int I = 0; I < ctx->nt; ++I) {
         const DevSpecies& d = ctx->hdm.sp[I];
-        int n = d.wp_n + BT;
+        int n = d.wp_n;
         for (int l = 0; l < d.L; ++l) n += 2 * ((d.width[l + 1] + 3) & ~3);
         m = std::max(m, n);
     }
     return m;
 }
 
-int basis_bytes(ace_ctx* ctx, int mode) {
+// dynamic LDS of one basis workgroup; waves: 4, or 2 (fused kernel only)
+int basis_bytes(ace_ctx* ctx, int mode, int waves = 4) {
     const int nbm = nb_slots(ctx), netf = mode == 3 ? net_lds_floats(ctx) : 0;
-    return ctx->basis_f64 ? basis_lds_bytes<double>(nbm, ctx->ngk, ctx->kmax, mode, netf, ctx->nt)
-                          : basis_lds_bytes<float>(nbm, ctx->ngk, ctx->kmax, mode, netf, ctx->nt);
+    if (waves == 2)
+        return ctx->basis_f64 ? basis_lds_bytes<double, 2>(nbm, ctx->ngk, ctx->kmax, mode, netf, ctx->nt)
+                              : basis_lds_bytes<float, 2>(nbm, ctx->ngk, ctx->kmax, mode, netf, ctx->nt);
+    return ctx->basis_f64 ? basis_lds_bytes<double, 4>(nbm, ctx->ngk, ctx->kmax, mode, netf, ctx->nt)
+                          : basis_lds_bytes<float, 4>(nbm, ctx->ngk, ctx->kmax, mode, netf, ctx->nt);
 }
 
-// the fused symmetry-function + network + adjoint kernel is used when its LDS keeps two workgroups per CU
 // The fused kernel (symmetry functions, network, adjoint of one atom per workgroup) saves two launches and
 // the G / dE/dG round trips, which wins while the batch is a few rounds of resident workgroups; its LDS
 // footprint limits residency, so larger batches run faster as symm -> network -> adjoint (measured on
 // MI355X, 80-32-32-1 bcc: FP32 fused 0.178 / split 0.186 ms at 4000 atoms, 0.323 / 0.319 at 8000; FP64
 // basis 0.128 / 0.132 at 2000, 0.230 / 0.223 at 4000).  Fused when the network fits 64 KB of LDS and the
-// grid is at most four rounds of 256 CUs x the workgroups that fit in 160 KB.
-bool tuned(const ace_ctx* ctx) { return ctx->tune_N == ctx->N && ctx->tune_slots == ctx->slots_cap; }
+// grid is at most four rounds of 256 CUs x the workgroups that fit in 160 KB.  Two waves per atom halve
+// the per-wave gradient accumulators; where that lets more atoms share a CU the grid takes fewer rounds,
+// which outweighs the longer time per atom (MI355X, 1000-atom bcc, FP64 basis: 54.6 us with two waves
+// in one round against 70.3 us with four in two; FP32, one round either way: 48.6 against 43.8 us).
+bool tuned(const ace_ctx* ctx, int precision) {
+    return ctx->pinned[precision] || (ctx->tune_N == ctx->N && ctx->tune_slots == ctx->slots_cap);
+}
 
-bool use_fused(ace_ctx* ctx, int precision) {
-    const int bytes = basis_bytes(ctx, 3);
-    if (precision == ACE_PREC_BF16L1 || bytes > 160 * 1024) return false;
-    if (tuned(ctx) && ctx->tune_fused[precision] >= 0) return ctx->tune_fused[precision] == 1;
-    if (bytes > 64 * 1024) return false;
-    const long long per_round = 256LL * std::max(1, (160 * 1024) / bytes);
-    return ctx->N <= 4 * per_round;
+long long fused_rounds(ace_ctx* ctx, int waves) {
+    const long long per_round = 256LL * std::max(1, (160 * 1024) / basis_bytes(ctx, 3, waves));
+    return (ctx->N + per_round - 1) / per_round;
+}
+
+// waves per atom of the fused kernel, or 0 for symm -> network -> adjoint
+int fused_waves(ace_ctx* ctx, int precision) {
+    if (precision == ACE_PREC_BF16L1) return 0;
+    const bool fit4 = basis_bytes(ctx, 3, 4) <= 160 * 1024, fit2 = basis_bytes(ctx, 3, 2) <= 160 * 1024;
+    if (tuned(ctx, precision) && ctx->tune_fused[precision] >= 0) {
+        const int c = ctx->tune_fused[precision];
+        return (c == 1 && fit4) ? 4 : (c == 2 && fit2) ? 2 : 0;
+    }
+    if (basis_bytes(ctx, 3, 4) > 64 * 1024) return 0;
+    const long long r4 = fused_rounds(ctx, 4), r2 = fused_rounds(ctx, 2);
+    if (std::min(r4, r2) > 4) return 0;
+    return r2 < r4 ? 2 : 4;
 }
 
 // mode 0: G -> out; mode 1: per-pair gradients from dE/dG; mode 2: tangent rows along eps -> out;
 // mode 3: per-atom energies and per-pair gradients in one pass
-int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = nullptr) {
-    BasisFn fns[4];
+// (waves: 4, or 2 for the fused kernel)
+int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = nullptr, int waves = 4) {
+    BasisFn fns[5];
     const bool f64 = ctx->basis_f64;
     basis_fns(ctx, f64, fns);
     const int nbm = nb_slots(ctx);
-    const int bytes = basis_bytes(ctx, mode);
+    if (waves != 4 && (waves != 2 || mode != 3)) return fail(ctx, "basis: no kernel for this wave count");
+    const int bytes = basis_bytes(ctx, mode, waves);
     if (bytes > 160 * 1024) return fail(ctx, "basis: neighbour list too long for LDS");
-    BasisFn f = fns[mode];
+    BasisFn f = waves == 2 ? fns[4] : fns[mode];
     if (bytes > 64 * 1024) HC(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     DevBasisParams P = basis_params(ctx);
     int* err = ctx->d_scal.p + 3;
@@ -806,7 +831,7 @@ int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = 
     io.F_fix = ctx->d_Ffix[b].p; io.E_fix = ctx->d_Efix[b].p; io.W_fix = ctx->d_Wfix[b].p;
     io.F_nxt = ctx->d_Ffix[1 - b].p; io.E_nxt = ctx->d_Efix[1 - b].p; io.W_nxt = ctx->d_Wfix[1 - b].p;
     if (timed(ctx, names[f64][mode], [&] {
-            f<<<ctx->N, BT, bytes, ctx->stream>>>(ctx->N, ctx->d_model.p, P, io, nbm, ctx->nt, ctx->kmax);
+            f<<<ctx->N, 64 * waves, bytes, ctx->stream>>>(ctx->N, ctx->d_model.p, P, io, nbm, ctx->nt, ctx->kmax);
         }))
         return -1;
     if (mode == 1 || mode == 3) ctx->fbuf = 1 - b;   // the next gradient launch adds into the zeroed buffer
@@ -830,7 +855,7 @@ int run_mlp(ace_ctx* ctx, bool bf16, int precision = ACE_PREC_FP32) {
         lds_w = std::max(lds_w, (mlp_lds_floats(ctx->hdm.sp[I]) + ctx->hdm.sp[I].wp_n) * 4);
     }
     bool stage = lds_w <= 64 * 1024;
-    if (tuned(ctx) && ctx->tune_stage[precision] >= 0 && lds_w <= 160 * 1024) stage = ctx->tune_stage[precision] == 1;
+    if (tuned(ctx, precision) && ctx->tune_stage[precision] >= 0 && lds_w <= 160 * 1024) stage = ctx->tune_stage[precision] == 1;
     if (stage) lds = lds_w;
     if (lds > 160 * 1024) return fail(ctx, "mlp: network too wide for LDS");
     auto f = stage ? (bf16 ? k_mlp_energy_grad<true, true> : k_mlp_energy_grad<true, false>)
@@ -848,7 +873,8 @@ int run_mlp(ace_ctx* ctx, bool bf16, int precision = ACE_PREC_FP32) {
 // the fixed-point accumulators.  The BF16 first layer runs on the matrix cores of the network kernel, so
 // that mode takes the split path.
 int run_ef(ace_ctx* ctx, int precision) {
-    if (use_fused(ctx, precision)) return run_basis(ctx, 3);
+    const int fw = fused_waves(ctx, precision);
+    if (fw) return run_basis(ctx, 3, nullptr, nullptr, fw);
     if (run_basis(ctx, 0)) return -1;
     if (run_mlp(ctx, precision == ACE_PREC_BF16L1, precision)) return -1;
     return run_basis(ctx, 1);
@@ -1437,12 +1463,15 @@ int ace_autotune(ace_ctx* ctx, int precision, double* ms) {
     if (ctx->nb_pending && nb_verify(ctx)) return -1;
     int lds_w = 0;
     for (int I = 0; I < ctx->nt; ++I) lds_w = std::max(lds_w, (mlp_lds_floats(ctx->hdm.sp[I]) + ctx->hdm.sp[I].wp_n) * 4);
-    // candidates: fused; split with staged weights; split with weights read from memory
-    const bool can_fused = precision != ACE_PREC_BF16L1 && basis_bytes(ctx, 3) <= 160 * 1024;
+    // candidates: fused (four waves per atom); split with staged weights; split with weights read from
+    // memory; fused with two waves per atom
+    const bool fu = precision != ACE_PREC_BF16L1;
+    const bool can_fused = fu && basis_bytes(ctx, 3, 4) <= 160 * 1024;
+    const bool can_fused2 = fu && basis_bytes(ctx, 3, 2) <= 160 * 1024;
     const bool can_stage = lds_w <= 160 * 1024;
-    const int cf[3] = {1, 0, 0}, cs[3] = {-1, 1, 0};
-    const bool ok[3] = {can_fused, can_stage, true};
-    double t[3] = {-1, -1, -1};
+    const int cf[4] = {1, 0, 0, 2}, cs[4] = {-1, 1, 0, -1};
+    const bool ok[4] = {can_fused, can_stage, true, can_fused2};
+    double t[4] = {-1, -1, -1, -1};
     const bool prof = ctx->prof;
     ctx->prof = false;
     ctx->tune_N = ctx->N;
@@ -1451,7 +1480,7 @@ int ace_autotune(ace_ctx* ctx, int precision, double* ms) {
     HC(hipEventCreate(&e0));
     HC(hipEventCreate(&e1));
     int best = -1;
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < 4; ++c) {
         if (!ok[c]) continue;
         ctx->tune_fused[precision] = cf[c];
         ctx->tune_stage[precision] = cs[c];
@@ -1476,11 +1505,26 @@ int ace_autotune(ace_ctx* ctx, int precision, double* ms) {
     ctx->prof = prof;
     ctx->tune_fused[precision] = cf[best];
     ctx->tune_stage[precision] = cs[best];
+    ctx->pinned[precision] = false;
     if (check_err_flag(ctx, "ace_autotune")) return -1;
     if (ms) {
-        for (int c = 0; c < 3; ++c) ms[c] = t[c];
-        ms[3] = best;
+        for (int c = 0; c < 4; ++c) ms[c] = t[c];
+        ms[4] = best;
     }
+    return 0;
+}
+
+int ace_set_path(ace_ctx* ctx, int precision, int path) {
+    if (!ctx) return -1;
+    if (precision != ACE_PREC_FP32 && precision != ACE_PREC_FP64_BASIS && precision != ACE_PREC_BF16L1)
+        return fail(ctx, "ace_set_path: precision mode not available");
+    if (path < -1 || path > 3) return fail(ctx, "ace_set_path: path must be -1 (size rule) or 0..3");
+    if (precision == ACE_PREC_BF16L1 && (path == 0 || path == 3))
+        return fail(ctx, "ace_set_path: the BF16 first layer runs on the split path only");
+    static const int cf[4] = {1, 0, 0, 2}, cs[4] = {-1, 1, 0, -1};
+    ctx->tune_fused[precision] = path < 0 ? -1 : cf[path];
+    ctx->tune_stage[precision] = path < 0 ? -1 : cs[path];
+    ctx->pinned[precision] = path >= 0;
     return 0;
 }
 

@@ -2,7 +2,7 @@
 // for every neighbour pair (the geometric half of NNP::compute_force, src/nnp/nnp.cpp:412-476), and
 // their tangent along per-atom displacements (dG/dr recomputed for the force-matching gradient).
 //
-// One workgroup (4 wavefronts) per centre atom; the real type T (float or double) is a template
+// One workgroup (BW = 4 or 2 wavefronts) per centre atom; the real type T (float or double) is a template
 // parameter: on gfx950 without packed-FP32 code the vector ALU runs FP64 FMAs at the FP32 rate, and the
 // triplet loop below has no transcendental, so T = double costs registers and LDS, not ALU rate.
 //
@@ -13,15 +13,16 @@
 //   fc(rij) fc(rik) |1/2 + lambda/2 cos|^zeta exp(-(rij^2+rik^2)/eta^2)      (basis_angular.cpp:371-382)
 // costs one product E_j E_k.  Window powers come from a per-triplet cache of (1 +- cos)/2 raised to
 // 2^k and 2^k - 1, so a power-of-two zeta is one register pick.  Triplets of an angular block (J,K) --
-// the rectangle J x K, or the triangle j<k of J x J -- are spread over the 256 threads; functions are
-// kept in registers and reduced wave by wave, the four wave partials added in a fixed order.  Radial
+// the rectangle J x K, or the triangle j<k of J x J -- are spread over the workgroup's threads; functions are
+// kept in registers and reduced wave by wave, the wave partials added in a fixed order.  Radial
 // functions map one function to one lane and neighbour groups to lane groups.  The adjoint accumulates
-// dE/dd per slot in one LDS array per wave and sums the four in a fixed order: results are
+// dE/dd per slot in one LDS array per wave and sums them in a fixed order: results are
 // bit-reproducible.
 #include "basis_math.hpp"
 
-#define BW 4            // waves per workgroup (one atom)
-#define BT (BW * 64)
+// BW (a template parameter of the kernel and its helpers) is the number of wavefronts per workgroup, i.e.
+// per atom, and BT = 64 BW its thread count: 4 by default; 2 halves the per-wave accumulators of the
+// gradient modes, which lets more atoms share a CU where LDS limits residency.
 
 // Modes: 0 = network input rows inpw*(G + inpb); 1 = adjoint (per-pair dE/dd from dE/dG rows);
 // 2 = tangent rows v = sum_a eps_a . dG/dr_a.
@@ -32,12 +33,13 @@
 //   network area (netf floats: staged weights, activations, partial sums), 16-byte aligned.
 // red holds the per-wave partial sums (BW 64) and, in the adjoint, the dE/dG weights of every canonical
 // angular block in (sign, level, group) order (14 ng per TensorTri block).
+template <int BW>
 __host__ __device__ inline int basis_nred(int nt, int ng) { return max(BW * 64, nt * (nt + 1) / 2 * 14 * ng); }
 
-template <class T>
+template <class T, int BW>
 __host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int mode, int netf, int nt) {
     const int kp = (kmax + 3) & ~3;
-    int n = 4 * nbm + ng * nbm + 3 * nbm + kp + basis_nred(nt, ng) + 2 * kp;
+    int n = 4 * nbm + ng * nbm + 3 * nbm + kp + basis_nred<BW>(nt, ng) + 2 * kp;
     if (mode == 0 || mode == 3) n += 2 * kp;
     if (mode == 2) n += 4 * nbm;
     if (mode >= 1) n += ng * nbm;
@@ -67,7 +69,7 @@ struct BasisLds {
     float* nw; // mode 3: the species' network parameters (matrix-core layout), then activations
 };
 
-template <class T>
+template <class T, int BW>
 __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, int mode, int nt) {
     BasisLds<T> L;
     L.g = (double*)base;
@@ -81,7 +83,7 @@ __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, i
     L.ri = c; c += nbm;
     L.fr = c; if (mode == 1 || mode == 3) c += nbm;
     L.row = c; c += (kmax + 3) & ~3;
-    L.red = c; c += basis_nred(nt, ng);
+    L.red = c; c += basis_nred<BW>(nt, ng);
     const int kp = (kmax + 3) & ~3;
     L.rp = c; c += 2 * kp;
     L.sc = c; if (mode == 0 || mode == 3) c += 2 * kp;
@@ -146,12 +148,13 @@ __device__ inline T prm(const float* f, const double* d, int i) {
 // held in registers with static indices; cidx maps each function of the block to (s*7 + k)*NG + g.
 // NZ (a compile-time bound on the levels in use) keeps the loop free of branches; unused levels are
 // computed and discarded.  E / Ed are interleaved per slot: E[slot * NG + g], zero for g >= ng.
-template <class T, int NG, int NZ, int MODE>
+template <class T, int NG, int NZ, int MODE, int BW>
 __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int nJ, int nK, bool same, int T3,
                                      const int* __restrict__ cidx, const T* wtab, int nA, int goff, double* gw,
                                      int tid, int wv, int lane) {
     constexpr int ADJ = (MODE == 1);
     constexpr int TAN = (MODE == 2);
+    constexpr int BT = BW * 64;
     T acc[2][NZ][NG], wgt[2][NZ][NG];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
@@ -309,10 +312,11 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
 // Radial and angular sums of one atom (slots, tables and row already in LDS).  PM 0: G into row;
 // PM 1: per-slot dE/dd into the per-wave accumulators g from the dE/dG weights in row; PM 2: tangent
 // rows into row.  Ends with a workgroup barrier.
-template <class T, int NG, int NZ, int PM, int GEN>
+template <class T, int NG, int NZ, int PM, int GEN, int BW>
 __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, const DevBasisParams& P,
                                   const int* s_cnt, int nt, int nbm, int tid, int wv, int lane) {
     constexpr int MODE = PM;
+    constexpr int BT = BW * 64;
     constexpr int ADJ = (PM == 1);
     constexpr int TAN = (PM == 2);
     constexpr int MAXA = 16;   // functions per register chunk of the generic path
@@ -394,7 +398,7 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
             const int T3 = same ? nJ * (nJ - 1) / 2 : nJ * nK;
             if (nA == 0 || T3 == 0) continue;
             if (!GEN || sp.a_canon[b]) {
-                angular_canon<T, NG, NZ, MODE>(L, bJ, bK, nJ, nK, same, T3, L.ci + (sp.a_poff[b] - sp.a_poff[0]),
+                angular_canon<T, NG, NZ, MODE, BW>(L, bJ, bK, nJ, nK, same, T3, L.ci + (sp.a_poff[b] - sp.a_poff[0]),
                                                L.red + 14 * NG * b, nA, sp.a_goff[b], gw, tid, wv, lane);
                 continue;
             }
@@ -679,14 +683,18 @@ struct BasisIO {
 // NG: eta groups per species (1, 2 or 4, at least the model's largest count); NZ: canonical power levels
 // (4 or 7, at least the model's largest log2(zeta) + 1).
 // MODE 0: network input rows; 1: gradients from dE/dG rows; 2: tangent rows; 3: fused energy and forces.
-template <class T, int NG, int NZ, int MODE, int GEN>
-__global__ __launch_bounds__(BT) void k_basis(int natoms, const DevModel* __restrict__ M, DevBasisParams P, BasisIO io,
+template <class T, int NG, int NZ, int MODE, int GEN, int BW = 4>
+__global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* __restrict__ M, DevBasisParams P, BasisIO io,
                                               int nbm, int nt, int kmax) {
     constexpr int ADJ = (MODE == 1);
     constexpr int TAN = (MODE == 2);
     constexpr int FUSED = (MODE == 3);        // symmetry functions, network and adjoint in one pass
     constexpr int GACC = (ADJ || FUSED);      // per-slot gradient accumulators
     constexpr bool F64 = sizeof(T) == 8;
+    constexpr int BT = BW * 64;
+    // the waves that stage the radial and the angular tables (wave 1 copies the species record)
+    constexpr int W_RAD = BW > 2 ? 2 : 1, W_ANG = BW > 3 ? 3 : BW - 1;
+    static_assert(BW >= 2 && BW <= 4, "two to four waves per atom");
     extern __shared__ double4 basis_smem[];
     __shared__ int s_cnt[ACE_MAXT + 1];
     __shared__ DevSpecies s_sp;
@@ -705,13 +713,13 @@ __global__ __launch_bounds__(BT) void k_basis(int natoms, const DevModel* __rest
     }
     const int I = io.type[i];
     const DevSpecies& gsp = M->sp[I];
-    const BasisLds<T> L = basis_lds<T>(basis_smem, nbm, NG, kmax, MODE, nt);
+    const BasisLds<T> L = basis_lds<T, BW>(basis_smem, nbm, NG, kmax, MODE, nt);
     const int nn = io.counts[i];
     const long long* krow = io.keys + (long long)i * io.cap;
     const float rcut = gsp.rc_max;
     const int row = io.row_of[i];
 
-    // --- waves 1-3 stage the species tables in LDS while wave 0 buckets the neighbours --------------
+    // --- waves 1.. stage the species tables in LDS while wave 0 buckets the neighbours --------------
     if (wv == 1) {
         constexpr int n16 = (int)(sizeof(DevSpecies) / 16);
         static_assert(sizeof(DevSpecies) % 16 == 0, "DevSpecies is copied in 16-byte words");
@@ -722,7 +730,8 @@ __global__ __launch_bounds__(BT) void k_basis(int natoms, const DevModel* __rest
         dst[lane] = v0;
         if (64 + lane < n16) dst[64 + lane] = v1;
         static_assert(n16 >= 64 && n16 <= 128, "DevSpecies copy covers 64 to 128 words");
-    } else if (wv == 2) {
+    }
+    if (wv == W_RAD) {
         const int nR = gsp.nR, r0 = gsp.r_poff[0];
         for (int q = lane; q < nR; q += 64) {
             L.rp[q] = prm<T>(P.r_eta, P.r_eta64, r0 + q);
@@ -730,7 +739,8 @@ __global__ __launch_bounds__(BT) void k_basis(int natoms, const DevModel* __rest
         }
         if (ADJ)
             for (int k = lane; k < gsp.K; k += 64) L.row[k] = (T)io.dEdG[(long long)row * io.ldg + k];
-    } else if (wv == 3) {
+    }
+    if (wv == W_ANG) {
         const int nA = gsp.K - gsp.nR, a0 = gsp.a_poff[0];
         for (int q = lane; q < nA; q += 64) L.ci[q] = P.a_cidx[a0 + q];
         if (MODE == 0 || FUSED)
@@ -740,7 +750,7 @@ __global__ __launch_bounds__(BT) void k_basis(int natoms, const DevModel* __rest
             }
     }
     if (FUSED && wv > 0) {
-        // the network parameters of the species, staged by waves 1-3 (16-byte copies)
+        // the network parameters of the species, staged by waves 1.. (16-byte copies)
         const float4* src = (const float4*)(io.wpad + gsp.wp_off);
         float4* dst = (float4*)L.nw;
         const int n4 = gsp.wp_n >> 2;
@@ -873,11 +883,11 @@ __global__ __launch_bounds__(BT) void k_basis(int natoms, const DevModel* __rest
     __syncthreads();
 
     if (FUSED) {
-        basis_sums<T, NG, NZ, 0, GEN>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
+        basis_sums<T, NG, NZ, 0, GEN, BW>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
         block_network<T>(L, sp, tid, &io.e_atom[i]);
-        basis_sums<T, NG, NZ, 1, GEN>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
+        basis_sums<T, NG, NZ, 1, GEN, BW>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
     } else {
-        basis_sums<T, NG, NZ, MODE, GEN>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
+        basis_sums<T, NG, NZ, MODE, GEN, BW>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
     }
     if (MODE == 0) {
         // network input row: the ANN's input scaling inpw*(G + inpb) (nn.cpp:1406) applied here, in T,

@@ -180,11 +180,19 @@ class NNPDevice:
 
     def autotune(self, precision=PREC_FP32):
         """Measures the energy/force paths on the uploaded batch and keeps the fastest; returns the
-        milliseconds of (fused, split with staged weights, split with weights in memory) and the choice."""
-        ms = np.zeros(4)
+        milliseconds of (fused with four waves per atom, split with staged weights, split with weights in
+        memory, fused with two waves per atom) and the choice."""
+        ms = np.zeros(5)
         self._check(self.lib.ace_autotune(self.ctx, precision, _dp(ms)), "ace_autotune")
-        names = ("fused", "split_staged", "split")
-        return {"ms": {names[i]: float(ms[i]) for i in range(3) if ms[i] >= 0}, "choice": names[int(ms[3])]}
+        names = self.PATHS
+        return {"ms": {names[i]: float(ms[i]) for i in range(4) if ms[i] >= 0}, "choice": names[int(ms[4])]}
+
+    PATHS = ("fused", "split_staged", "split", "fused_2w")
+
+    def set_path(self, path, precision=PREC_FP32):
+        """Fixes the energy/force path (a name of PATHS, or None for the size rule) for every batch."""
+        k = -1 if path is None else self.PATHS.index(path)
+        self._check(self.lib.ace_set_path(self.ctx, precision, k), "ace_set_path")
 
     def md_init(self, masses, vel=None):
         """Per-species masses (amu) and velocities (N x 3, A/ps; None = at rest) for md_run."""

@@ -23,6 +23,7 @@
  *                                   (network factor NN::D2ODZDI, src/ml/nn.cpp:2047-2140)
  *   ace_md_init / ace_md_run / ace_md_state  velocity-Verlet steps (src/sim/integrator.cpp:548-571), HIP graphs
  *   ace_autotune ................... (no counterpart) measured choice of the energy/force path
+ *   ace_set_path ................... (no counterpart) fixed choice of the energy/force path
  *   ace_plain_energy_forces ........ same as ace_energy_forces, unbatched FP64 one-thread-per-atom
  *                                   GPU port (the baseline the batched path is measured against)
  */
@@ -188,11 +189,16 @@ int  ace_md_run(ace_ctx* ctx, int precision, int nsteps, double dt, int rebuild_
 int  ace_md_state(ace_ctx* ctx, double* pos, double* vel);
 
 /* Autotuner (no counterpart in ace): times the energy/force paths on the uploaded batch (neighbour list
- * built) and keeps the fastest for this precision -- the fused per-atom kernel, or symmetry functions ->
- * network -> adjoint with the network weights staged in LDS or read from memory.  ms: NULL or 4 entries,
- * the milliseconds of the three candidates (-1 where not feasible) and the index chosen.  The choice holds
- * while the batch size and neighbour-row capacity stay as measured; otherwise a size rule decides. */
+ * built) and keeps the fastest for this precision -- the fused per-atom kernel with four waves per atom,
+ * symmetry functions -> network -> adjoint with the network weights staged in LDS or read from memory,
+ * or the fused kernel with two waves per atom.  ms: NULL or 5 entries, the milliseconds of the four
+ * candidates in that order (-1 where not feasible) and the index chosen.  The choice holds while the
+ * batch size and neighbour-row capacity stay as measured; otherwise a size rule decides. */
 int  ace_autotune(ace_ctx* ctx, int precision, double* ms);
+/* Fixes the energy/force path of a precision for every batch: path 0..3 as ace_autotune's candidates
+ * (a fused path whose workgroup does not fit in LDS runs split), -1 returns to the size rule.  The BF16
+ * first-layer mode has the split paths (1, 2) only. */
+int  ace_set_path(ace_ctx* ctx, int precision, int path);
 
 /* Same quantities from the unbatched FP64 port: one structure per launch, one thread per atom. */
 int  ace_plain_energy_forces(ace_ctx* ctx, double* E_struc, double* F);

@@ -32,18 +32,22 @@ def structure(cell, pos, typ):
     return configs.Structure(np.asarray(cell, float), np.asarray(pos, float), np.asarray(typ, dtype=np.int32))
 
 
-def check_against_oracle(model, strucs, name, check_plain=True):
+def check_against_oracle(model, strucs, name, check_plain=True, paths=(None,)):
+    """Neighbour lists bit-exact, G, energies, forces and virials within TOL, for each energy/force path
+    in `paths` (None: the size rule)."""
     orc = O.Oracle(model)
     dev = NNPDevice(model)
     dev.upload(strucs)
     dev.build_neighbors()
     off, j, img = dev.neighbors()
     res = {}
-    for prec in PRECS:
-        dev.set_precision(prec)
-        Gd = dev.compute_symm()
-        E, F, W = dev.compute(prec)
-        res[prec] = (Gd, E, F, W)
+    for path in paths:
+        for prec in PRECS:
+            dev.set_path(path, prec)
+            dev.set_precision(prec)
+            Gd = dev.compute_symm()
+            E, F, W = dev.compute(prec)
+            res[(path, prec)] = (Gd, E, F, W)
     if check_plain:
         Ep, Fp = dev.plain_compute()
     a0 = 0
@@ -60,15 +64,15 @@ def check_against_oracle(model, strucs, name, check_plain=True):
         kk = G.shape[1]
         pe, Fr, Wr = orc.force(s, G, nl)
         Er = pe + np.asarray(model.E0)[s.type].sum()
-        for prec in PRECS:
-            Gd, E, F, W = res[prec]
+        for (path, prec), (Gd, E, F, W) in res.items():
             t = TOL[prec]
             gerr = np.max(np.abs(Gd[a0:a0 + n, :kk] - G) / np.maximum(np.abs(G), 1.0))
             de = abs(E[k] - Er) / n
             df = np.max(np.abs(F[a0:a0 + n] - Fr))
             dw = np.max(np.abs(W[k] - Wr)) / max(1.0, np.max(np.abs(Wr)))
-            log(f"{name}[{k}] {NAME[prec]} n={n} pairs={len(nl.j)} G {gerr:.1e} dE/atom {de:.1e} dF {df:.1e} dW {dw:.1e}")
-            assert gerr < t["G"] and de < t["E"] and df < t["F"] and dw < t["W"], (name, k, NAME[prec])
+            tag = NAME[prec] + ("" if path is None else " " + path)
+            log(f"{name}[{k}] {tag} n={n} pairs={len(nl.j)} G {gerr:.1e} dE/atom {de:.1e} dF {df:.1e} dW {dw:.1e}")
+            assert gerr < t["G"] and de < t["E"] and df < t["F"] and dw < t["W"], (name, k, tag)
         if check_plain:
             assert abs(Ep[k] - Er) < 1e-9 * max(1, abs(Er)) and np.max(np.abs(Fp[a0:a0 + n] - Fr)) < 1e-9
         a0 += n
@@ -136,7 +140,7 @@ def test_isolated_and_single_atom():
 def test_c2_full_size():
     m, strucs = configs.config2()
     scaled_model(m, strucs)
-    check_against_oracle(m, strucs, "c2")
+    check_against_oracle(m, strucs, "c2", paths=(None,) + NNPDevice.PATHS)
 
 
 def test_every_radial_and_cutoff_form_and_generic_angular():
@@ -167,7 +171,7 @@ def test_every_radial_and_cutoff_form_and_generic_angular():
         cell, pos = configs.random_packed(n, 0.08, 1.6, rng)
         strucs.append(structure(cell, pos, rng.integers(0, 3, n)))
     scaled_model(m, strucs)
-    check_against_oracle(m, strucs, "forms", check_plain=False)
+    check_against_oracle(m, strucs, "forms", check_plain=False, paths=NNPDevice.PATHS)
 
 
 @pytest.mark.parametrize("hidden,act", [((128, 128), 3), ((40, 24, 17), 2), ((8,), 10), ((33, 31), 15)])

@@ -41,6 +41,8 @@ SIGNATURES = {
     "ace_md_state": (C.c_int, [P, dp, dp]),
     "ace_autotune": (C.c_int, [P, C.c_int, dp]),
     "ace_set_path": (C.c_int, [P, C.c_int, C.c_int]),
+    "ace_train_optimizer": (C.c_int, [P, C.c_int, C.c_double, C.c_double]),
+    "ace_train_opt_state": (C.c_int, [P, C.c_int, dp, dp, dp, dp]),
     "ace_train_graph": (C.c_int, [P, C.c_int]),
     "ace_plain_energy_forces": (C.c_int, [P, dp, dp]),
     "ace_set_targets": (C.c_int, [P, dp, dp]),

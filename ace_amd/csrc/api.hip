@@ -65,6 +65,7 @@ struct ace_ctx {
     size_t h_stage_bytes = 0;
     hipEvent_t stage_done = nullptr;
     int wp_total = 0, bf_total = 0;
+    std::vector<int> pmap;           // device-layout indices of every parameter (param_map)
     DBuf<int> d_a_zint, d_a_grp, d_a_sel, d_a_cidx;
     bool plain_ok = false;
     std::string plain_why;
@@ -301,17 +302,46 @@ int parse_model(ace_ctx* ctx, const ace_model_desc* m) {
 
 // matrix-core copy of the network parameters (layout in DevSpecies::wl_off / bl_off)
 // FP32 -> BF16 bits, round to nearest even (finite inputs)
-uint16_t bf16_rne(float x) {
-    uint32_t u;
-    memcpy(&u, &x, 4);
+__host__ __device__ inline uint16_t bf16_rne(float x) {
+    uint32_t u = __builtin_bit_cast(uint32_t, x);
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
 }
-float bf16_value(uint16_t h) {
-    const uint32_t u = (uint32_t)h << 16;
-    float x;
-    memcpy(&x, &u, 4);
-    return x;
+__host__ __device__ inline float bf16_value(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+
+// Where each parameter lives in the device layouts, 5 ints per parameter (species concatenated in type
+// order, each species' parameters in the ANN order [biases][weights] of its layers): the FP32 matrix-core
+// index in wpad, then for first-layer weights the BF16 high / low indices of the forward [o][k] and the
+// backward [k][o] copies in w0bf (-1 elsewhere).
+const std::vector<int>& param_map(ace_ctx* ctx) {
+    if (!ctx->pmap.empty()) return ctx->pmap;
+    std::vector<int>& map = ctx->pmap;
+    int ntot = 0;
+    for (int I = 0; I < ctx->nt; ++I) ntot += ctx->sp[I].nparams();
+    map.assign(5 * (size_t)ntot, -1);
+    int q0 = 0;
+    for (int I = 0; I < ctx->nt; ++I) {
+        const HSpecies& s = ctx->sp[I];
+        const DevSpecies& d = ctx->hdm.sp[I];
+        const int K = s.width[0], n1 = s.width[1];
+        const int fwd = ((n1 + 15) & ~15) * ((K + 31) & ~31), bwd = ((K + 15) & ~15) * ((n1 + 31) & ~31);
+        for (int l = 0; l < s.L; ++l) {
+            const int ni = s.width[l], no = s.width[l + 1];
+            for (int k = 0; k < ni; ++k)
+                for (int o = 0; o < no; ++o) {
+                    int* e = &map[5 * (size_t)(q0 + d.woff[l] + o + k * no)];
+                    e[0] = d.wp_off + d.wl_off[l] + k * mlp_ld(no) + o;
+                    if (l == 0) {
+                        const int ef = d.bf_off + o * ((K + 31) & ~31) + k;
+                        const int eb = d.bf_off + 2 * fwd + k * ((n1 + 31) & ~31) + o;
+                        e[1] = ef; e[2] = ef + fwd; e[3] = eb; e[4] = eb + bwd;
+                    }
+                }
+            for (int o = 0; o < no; ++o) map[5 * (size_t)(q0 + d.boff[l] + o)] = d.wp_off + d.bl_off[l] + o;
+        }
+        q0 += s.nparams();
+    }
+    return map;
 }
 
 // The matrix-core layouts of every network (DevSpecies::wp_off / wl_off / bl_off, and the BF16 split copies
@@ -333,30 +363,19 @@ int upload_wpad(ace_ctx* ctx) {
     float* w = (float*)ctx->h_stage;
     uint16_t* b = (uint16_t*)(ctx->h_stage + nw * sizeof(float));
     memset(ctx->h_stage, 0, bytes);
-    for (int I = 0; I < ctx->nt; ++I) {
-        const HSpecies& s = ctx->sp[I];
-        const DevSpecies& d = ctx->hdm.sp[I];
-        for (int l = 0; l < s.L; ++l) {
-            const int ni = s.width[l], no = s.width[l + 1];
-            for (int k = 0; k < ni; ++k)
-                for (int o = 0; o < no; ++o)
-                    w[d.wp_off + d.wl_off[l] + k * mlp_ld(no) + o] = (float)s.params[d.woff[l] + o + k * no];
-            for (int o = 0; o < no; ++o) w[d.wp_off + d.bl_off[l] + o] = (float)s.params[d.boff[l] + o];
-        }
-        // BF16 high/low parts of W_0 in the two layouts the BF16 first layer reads
-        const int K = s.width[0], n1 = s.width[1];
-        const int fwd = ((n1 + 15) & ~15) * ((K + 31) & ~31), bwd = ((K + 15) & ~15) * ((n1 + 31) & ~31);
-        uint16_t* f = b + d.bf_off;
-        uint16_t* g = f + 2 * fwd;
-        for (int o = 0; o < n1; ++o)
-            for (int k = 0; k < K; ++k) {
-                const float x = (float)s.params[d.woff[0] + o + k * n1];
+    const std::vector<int>& map = param_map(ctx);
+    size_t q = 0;
+    for (int I = 0; I < ctx->nt; ++I)
+        for (const double v : ctx->sp[I].params) {
+            const int* e = &map[5 * q++];
+            const float x = (float)v;
+            w[e[0]] = x;
+            if (e[1] >= 0) {
+                // BF16 high/low parts of W_0 in the two layouts the BF16 first layer reads
                 const uint16_t hi = bf16_rne(x), lo = bf16_rne(x - bf16_value(hi));
-                const int ef = o * ((K + 31) & ~31) + k, eb = k * ((n1 + 31) & ~31) + o;
-                f[ef] = hi; f[fwd + ef] = lo;
-                g[eb] = hi; g[bwd + eb] = lo;
+                b[e[1]] = hi; b[e[2]] = lo; b[e[3]] = hi; b[e[4]] = lo;
             }
-    }
+        }
     HC(hipMemcpyAsync(ctx->d_wpad.p, w, nw * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     HC(hipMemcpyAsync(ctx->d_w0bf.p, b, nb * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream));
     HC(hipEventRecord(ctx->stage_done, ctx->stream));
@@ -599,6 +618,20 @@ int build_tables(ace_ctx* ctx) {
 
 // device copies of what changed since the last launch: the matrix-core layouts of the networks, the input
 // scaling (FP32 and FP64); the plain port's FP64 parameters are refreshed by the port itself
+// parameters updated by the device optimiser, copied back into the host copies (one round trip)
+int sync_host_params(ace_ctx* ctx) {
+    TrainState& T = ctx->tr;
+    if (!T.opt_on || !T.opt_host_stale) return 0;
+    std::vector<double> p(T.opt_n);
+    HC(hipMemcpyAsync(p.data(), T.d_p.p, T.opt_n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    size_t q = 0;
+    for (int I = 0; I < ctx->nt; ++I)
+        for (double& v : ctx->sp[I].params) v = p[q++];
+    T.opt_host_stale = false;
+    return 0;
+}
+
 int refresh_params(ace_ctx* ctx) {
     if (!ctx->params_dirty && !ctx->scale_dirty) return 0;
     std::vector<float> iw, ib;
@@ -624,6 +657,7 @@ int refresh_params(ace_ctx* ctx) {
 
 int refresh_nn64(ace_ctx* ctx) {
     if (!ctx->nn64_dirty) return 0;
+    if (sync_host_params(ctx)) return -1;
     std::vector<double> nn;
     for (int I = 0; I < ctx->nt; ++I) nn.insert(nn.end(), ctx->sp[I].params.begin(), ctx->sp[I].params.end());
     if (upload(ctx, ctx->d_nn64, nn.data(), nn.size())) return -1;
@@ -1028,15 +1062,19 @@ int ace_set_params(ace_ctx* ctx, int type, const double* params, int n) {
     if (!ctx) return -1;
     if (type < 0 || type >= ctx->nt) return fail(ctx, "ace_set_params: invalid type");
     if (n != ctx->sp[type].nparams()) return fail(ctx, "ace_set_params: invalid size: vector and network mismatch");
+    if (sync_host_params(ctx)) return -1;
     ctx->sp[type].params.assign(params, params + n);
     ctx->params_dirty = true;
     ctx->nn64_dirty = true;
+    ctx->tr.opt_dev_stale = ctx->tr.opt_on;
     return 0;
 }
 
-int ace_get_params(const ace_ctx* ctx, int type, double* params, int n) {
+int ace_get_params(const ace_ctx* cctx, int type, double* params, int n) {
+    ace_ctx* ctx = const_cast<ace_ctx*>(cctx);
     if (!ctx || type < 0 || type >= ctx->nt) return -1;
     if (n != ctx->sp[type].nparams()) return -1;
+    if (sync_host_params(ctx)) return -1;
     std::copy(ctx->sp[type].params.begin(), ctx->sp[type].params.end(), params);
     return 0;
 }

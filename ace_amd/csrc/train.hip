@@ -25,6 +25,14 @@ struct TrainState {
     bool use_graph = false;           // replay the device part of a step from a captured HIP graph
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     long long graph_sig[2][12] = {};
+    // device optimiser: FP64 master parameters (species concatenated), ADAM moments, (beta1^t, beta2^t)
+    bool opt_on = false;
+    bool opt_host_stale = false;      // the device parameters are newer than the host copies
+    bool opt_dev_stale = false;       // the host parameters were set after the device ones
+    double opt_gamma = 0, opt_eps = 1e-16;
+    int opt_n = 0;
+    DBuf<double> d_p, d_m, d_v, d_bt;
+    DBuf<int> d_pmap;
 };
 
 __device__ inline void act_fd2(int act, float c, float z, float& a, float& d, float& d2) {

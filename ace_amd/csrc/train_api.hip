@@ -1,10 +1,53 @@
 // train_api.hip -- host side of the batched training step (included by api.hip after the helpers).
+
+// One ADAM step (ADAM::step, src/opt/algo.cpp:280-296, with the bias corrections of its moments) on the
+// FP64 master parameters, in the operation order of the host restatement and without contraction, so
+// device and host trajectories agree bit for bit; each thread also writes its parameter into the FP32
+// matrix-core layout and, for first-layer weights, the BF16 high/low copies (the indices of param_map).
+__global__ void k_adam(int n, double* __restrict__ p, double* __restrict__ m, double* __restrict__ v,
+                       const double* __restrict__ g, const double* __restrict__ bt, double gamma, double b1, double b2,
+                       double eps, const int* __restrict__ map, float* __restrict__ wpad, uint16_t* __restrict__ w0bf) {
+#pragma clang fp contract(off)
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double c1 = 1.0 / (1.0 - bt[0]), c2 = 1.0 / (1.0 - bt[1]);
+    const double gq = g[q];
+    const double mq = m[q] * b1 + (1.0 - b1) * gq;
+    const double vq = v[q] * b2 + (1.0 - b2) * gq * gq;
+    const double pq = p[q] - gamma * c1 * mq / (sqrt(c2 * vq) + eps);
+    m[q] = mq;
+    v[q] = vq;
+    p[q] = pq;
+    const int* e = map + 5 * (size_t)q;
+    const float x = (float)pq;
+    wpad[e[0]] = x;
+    if (e[1] >= 0) {
+        const uint16_t hi = bf16_rne(x), lo = bf16_rne(x - bf16_value(hi));
+        w0bf[e[1]] = hi;
+        w0bf[e[2]] = lo;
+        w0bf[e[3]] = hi;
+        w0bf[e[4]] = lo;
+    }
+}
+
+// (beta1^t, beta2^t) -> (beta1^(t+1), beta2^(t+1)) after the step that read them
+__global__ void k_adam_advance(double* bt, double b1, double b2) {
+    if (threadIdx.x == 0) {
+        bt[0] *= b1;
+        bt[1] *= b2;
+    }
+}
+
+#define ADAM_B1 0.9
+#define ADAM_B2 0.999
+
 namespace {
 
 void train_release(ace_ctx* ctx) {
     TrainState& T = ctx->tr;
     release(T.d_tiles); release(T.d_Eref); release(T.d_Fref); release(T.d_gfac); release(T.d_eps);
     release(T.d_out); release(T.d_grad); release(T.d_gsum); release(T.d_V); release(T.d_part);
+    release(T.d_p); release(T.d_m); release(T.d_v); release(T.d_bt); release(T.d_pmap);
     for (int q = 0; q < 2; ++q)
         if (ctx->tr.graph[q]) {
             (void)hipGraphExecDestroy(ctx->tr.graph[q]);
@@ -180,6 +223,66 @@ int ace_set_targets(ace_ctx* ctx, const double* E_ref, const double* F_ref) {
     return 0;
 }
 
+int ace_train_optimizer(ace_ctx* ctx, int algo, double gamma, double eps) {
+    if (!ctx) return -1;
+    TrainState& T = ctx->tr;
+    if (sync_host_params(ctx)) return -1;
+    if (algo == ACE_OPT_HOST) {
+        T.opt_on = false;
+        return 0;
+    }
+    if (algo != ACE_OPT_ADAM) return fail(ctx, "ace_train_optimizer: only ADAM runs on the device");
+    if (!(gamma > 0) || !(eps >= 0)) return fail(ctx, "ace_train_optimizer: invalid gamma or eps");
+    if (refresh_params(ctx)) return -1;    // device layouts current before the optimiser writes into them
+    const std::vector<int>& map = param_map(ctx);
+    std::vector<double> p;
+    for (int I = 0; I < ctx->nt; ++I) p.insert(p.end(), ctx->sp[I].params.begin(), ctx->sp[I].params.end());
+    const int n = (int)p.size();
+    const double bt[2] = {ADAM_B1, ADAM_B2};
+    if (upload(ctx, T.d_p, p.data(), n) || upload(ctx, T.d_pmap, map.data(), map.size()) ||
+        upload(ctx, T.d_bt, bt, 2) || ensure(ctx, T.d_m, n) || ensure(ctx, T.d_v, n))
+        return -1;
+    HC(hipMemsetAsync(T.d_m.p, 0, n * sizeof(double), ctx->stream));
+    HC(hipMemsetAsync(T.d_v.p, 0, n * sizeof(double), ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    T.opt_n = n;
+    T.opt_gamma = gamma;
+    T.opt_eps = eps;
+    T.opt_on = true;
+    T.opt_host_stale = false;
+    T.opt_dev_stale = false;
+    return 0;
+}
+
+int ace_train_opt_state(ace_ctx* ctx, int set, double* p, double* m, double* v, double* bt) {
+    if (!ctx) return -1;
+    TrainState& T = ctx->tr;
+    if (!T.opt_on) return fail(ctx, "ace_train_opt_state: no device optimiser");
+    if (!p || !m || !v || !bt) return fail(ctx, "ace_train_opt_state: null array");
+    const size_t n = T.opt_n;
+    if (set) {
+        if (upload(ctx, T.d_p, p, n) || upload(ctx, T.d_m, m, n) || upload(ctx, T.d_v, v, n) ||
+            upload(ctx, T.d_bt, bt, 2))
+            return -1;
+        HC(hipStreamSynchronize(ctx->stream));
+        size_t q = 0;
+        for (int I = 0; I < ctx->nt; ++I)
+            for (double& x : ctx->sp[I].params) x = p[q++];
+        ctx->params_dirty = true;
+        ctx->nn64_dirty = true;
+        T.opt_host_stale = false;
+        T.opt_dev_stale = false;
+        return 0;
+    }
+    if (sync_host_params(ctx)) return -1;
+    HC(hipMemcpyAsync(p, T.d_p.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipMemcpyAsync(m, T.d_m.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipMemcpyAsync(v, T.d_v.p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipMemcpyAsync(bt, T.d_bt.p, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 int ace_train_graph(ace_ctx* ctx, int on) {
     if (!ctx) return -1;
     ctx->tr.use_graph = on != 0;
@@ -213,13 +316,34 @@ int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* 
     size_t ntot = 0;
     long long gofs = 0;
     train_sizes(ctx, ntot, gofs);
+    const bool want_grad = grad != nullptr || T.opt_on;
     if ((wf && ensure(ctx, T.d_V, (size_t)N * ctx->ldg)) ||
-        (grad && (ensure(ctx, T.d_grad, ntot) || ensure(ctx, T.d_gsum, (size_t)std::max(gofs, 1LL)))))
+        (want_grad && (ensure(ctx, T.d_grad, ntot) || ensure(ctx, T.d_gsum, (size_t)std::max(gofs, 1LL)))))
         return -1;
+    if (T.opt_on && T.opt_dev_stale) {
+        // parameters set on the host since the device optimiser last ran: they become the master copy
+        std::vector<double> p;
+        for (int I = 0; I < ctx->nt; ++I) p.insert(p.end(), ctx->sp[I].params.begin(), ctx->sp[I].params.end());
+        if (upload(ctx, T.d_p, p.data(), p.size())) return -1;
+        HC(hipStreamSynchronize(ctx->stream));
+        T.opt_dev_stale = false;
+    }
     if (T.use_graph) {
-        if (train_graph_launch(ctx, L, wf, grad != nullptr)) return -1;
-    } else if (train_launch(ctx, L, wf, grad != nullptr)) {
+        if (train_graph_launch(ctx, L, wf, want_grad)) return -1;
+    } else if (train_launch(ctx, L, wf, want_grad)) {
         return -1;
+    }
+    if (T.opt_on) {
+        const int n = T.opt_n;
+        if (timed(ctx, "adam", [&] {
+                k_adam<<<cdiv(n, 256), 256, 0, ctx->stream>>>(n, T.d_p.p, T.d_m.p, T.d_v.p, T.d_grad.p, T.d_bt.p,
+                                                              T.opt_gamma, ADAM_B1, ADAM_B2, T.opt_eps, T.d_pmap.p,
+                                                              ctx->d_wpad.p, ctx->d_w0bf.p);
+                k_adam_advance<<<1, 64, 0, ctx->stream>>>(T.d_bt.p, ADAM_B1, ADAM_B2);
+            }))
+            return -1;
+        T.opt_host_stale = true;
+        ctx->nn64_dirty = true;
     }
     if (grad) HC(hipMemcpyAsync(grad, T.d_grad.p, ntot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     std::vector<double> out(3 * (size_t)S);

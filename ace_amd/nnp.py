@@ -240,6 +240,29 @@ class NNPDevice:
                     "ace_train_step")
         return out, g
 
+    def train_optimizer(self, algo=None, gamma=1e-3, eps=1e-16):
+        """algo "ADAM": every later train_step also applies one ADAM step on the GPU (FP64 master
+        parameters, device layouts refreshed in place); None: the caller updates the parameters."""
+        k = {None: 0, "ADAM": 1}.get(None if algo is None else algo.upper(), -1)
+        if k < 0:
+            raise ValueError("only ADAM runs on the device")
+        self._check(self.lib.ace_train_optimizer(self.ctx, k, gamma, eps), "ace_train_optimizer")
+
+    def train_opt_state(self, state=None):
+        """The device ADAM's state as a dict (p, m, v, bt), or sets it from one."""
+        n = sum(self.model.param_counts())
+        if state is None:
+            p, m, v, bt = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(2)
+            self._check(self.lib.ace_train_opt_state(self.ctx, 0, _dp(p), _dp(m), _dp(v), _dp(bt)),
+                        "ace_train_opt_state")
+            return {"p": p, "m": m, "v": v, "bt": bt}
+        arr = {k: np.ascontiguousarray(state[k], dtype=np.float64) for k in ("p", "m", "v", "bt")}
+        self._check(self.lib.ace_train_opt_state(self.ctx, 1, _dp(arr["p"]), _dp(arr["m"]), _dp(arr["v"]),
+                                                 _dp(arr["bt"])), "ace_train_opt_state")
+
+    def get_all_params(self):
+        return np.concatenate([self.get_params(I) for I in range(self.model.ntypes)])
+
     # -- timing ----------------------------------------------------------------------------------
     def profile(self, on=True):
         self._check(self.lib.ace_profile_enable(self.ctx, 1 if on else 0), "ace_profile_enable")

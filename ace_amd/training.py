@@ -74,10 +74,14 @@ class Trainer:
     """Full-batch training on one GPU: every uploaded structure is in the batch.  The update follows the
     training loop of src/nnp/nnptefr.cpp:1031-1066: gamma <- decay(gamma), the optimiser step (any of
     src/opt/algo.cpp, default ADAM), weight decay of the weights from the previous parameters, then the
-    "old" value, parameters and gradient."""
+    "old" value, parameters and gradient.
+
+    device_opt=True runs the ADAM update on the GPU after each gradient (constant gamma, no weight decay:
+    the configuration it restates exactly); the parameters then stay on the device between steps and
+    `params`, `save_restart` read them back."""
 
     def __init__(self, dev, model, strucs, gamma=1e-3, w_force=0.1, loss=0, norm=0, delta=1.0, algo="ADAM",
-                 decay=None, reg="NONE", eta_reg=0.0, graph=False):
+                 decay=None, reg="NONE", eta_reg=0.0, graph=False, device_opt=False):
         from .optimizers import Decay, Objective, make
         self.dev, self.model = dev, model
         self.nat = sum(s.natoms for s in strucs)
@@ -93,13 +97,28 @@ class Trainer:
         self.w_force, self.loss, self.norm, self.delta = w_force, loss, norm, delta
         if graph:
             dev.train_graph(True)
+        self.device_opt = device_opt
+        if device_opt:
+            if algo.upper() != "ADAM" or self.decay.name != "CONST" or reg.upper() != "NONE":
+                raise ValueError("the device optimiser runs ADAM with constant gamma and no weight decay")
+            dev.train_optimizer("ADAM", gamma, self.algo.eps)
         have_f = bool(strucs) and all(s.forces is not None for s in strucs)
         dev.set_targets([s.energy for s in strucs],
                         np.concatenate([s.forces for s in strucs]) if have_f else None)
 
     @property
     def params(self):
+        if self.device_opt:
+            self.obj.p = self.dev.get_all_params()
+            self.model.set_all_params(self.obj.p)
         return self.obj.p
+
+    def _pull_device_state(self):
+        st = self.dev.train_opt_state()
+        self.obj.p = st["p"]
+        self.algo.m, self.algo.v = st["m"], st["v"]
+        self.algo.b1t, self.algo.b2t = float(st["bt"][0]), float(st["bt"][1])
+        self.model.set_all_params(self.obj.p)
 
     def update(self, loss, g):
         """One parameter update from the loss value and its gradient (host, FP64)."""
@@ -118,6 +137,8 @@ class Trainer:
         """Training state for a restart (the content of NNPTEFR::write_restart, src/nnp/nnptefr.cpp:509-537:
         parameters, optimiser state, gamma, iteration, old value/parameters/gradient) as a portable .npz
         rather than the reference's memory image of the trainer object."""
+        if self.device_opt:
+            self._pull_device_state()
         o = self.obj
         state = {"p": o.p, "p_old": o.p_old, "g_old": o.g_old, "val_old": o.val_old, "gamma": o.gamma,
                  "it": self.it, "algo": type(self.algo).__name__}
@@ -137,12 +158,21 @@ class Trainer:
             v = z["algo_" + k]
             setattr(self.algo, k, v.copy() if v.ndim else v.item())
         self.model.set_all_params(o.p)
-        if hasattr(self.dev, "set_all_params"):
+        if self.device_opt:
+            self.dev.train_opt_state({"p": o.p, "m": self.algo.m, "v": self.algo.v,
+                                      "bt": np.array([self.algo.b1t, self.algo.b2t])})
+        elif hasattr(self.dev, "set_all_params"):
             self.dev.set_all_params(o.p)
 
     def step(self):
-        """Loss terms of one iteration; the parameters are updated on the host and uploaded."""
+        """Loss terms of one iteration; the parameters are updated on the host and uploaded, or on the
+        device (device_opt)."""
         wf = self.w_force / (3 * self.nat) if self.w_force else 0.0
+        if self.device_opt:
+            out, _ = self.dev.train_step(self.loss, self.norm, self.delta, 1.0 / self.nbatch, wf, want_grad=False)
+            self.obj.val_old = self.obj.val = float(out[0])
+            self.it += 1
+            return out
         out, g = self.dev.train_step(self.loss, self.norm, self.delta, 1.0 / self.nbatch, wf)
         p = self.update(out[0], g)
         self.dev.set_all_params(p)

@@ -19,6 +19,9 @@
  *                                   (src/nnp/nnp.cpp:329-338, 412-494)
  *   ace_compute .................... NNP::compute(Structure&) (src/nnp/nnp.cpp:489-494): build + energy
  *   ace_train_step ................. NNPTEFR::error_cost_o1 (src/nnp/nnptefr.cpp:1292-1425)
+ *   ace_train_optimizer ............ (no counterpart) the ADAM update of the training loop
+ *                                    (nnptefr.cpp:1031-1066, opt::ADAM algo.cpp:280-296) on the GPU
+ *   ace_train_opt_state ............ (no counterpart) its state, for restart files
  *                                   extended with the force-matching term d(F)/dW
  *                                   (network factor NN::D2ODZDI, src/ml/nn.cpp:2047-2140)
  *   ace_md_init / ace_md_run / ace_md_state  velocity-Verlet steps (src/sim/integrator.cpp:548-571), HIP graphs
@@ -215,6 +218,19 @@ int  ace_set_targets(ace_ctx* ctx, const double* E_ref, const double* F_ref);
  * captured HIP graph (no counterpart in ace; same results as the plain launches). */
 int  ace_train_graph(ace_ctx* ctx, int on);
 int  ace_train_step(ace_ctx* ctx, const ace_loss_desc* loss_desc, double* loss, double* grad);
+
+/* Device optimiser (no counterpart in ace, whose training loop updates the parameters on the host,
+ * nnptefr.cpp:1031-1066).  algo ACE_OPT_ADAM: every later ace_train_step also applies one ADAM step
+ * (opt::ADAM, algo.cpp:280-296; beta1 0.9, beta2 0.999, constant gamma, eps) to FP64 master parameters on
+ * the GPU and refreshes the networks' device layouts in the same stream, so a training loop moves no
+ * parameters between host and device; ace_get_params reads the current parameters back, and parameters
+ * set with ace_set_params become the master copy.  Starts from the current parameters with zero moments.
+ * ACE_OPT_HOST: back to host updates (the caller's ace_set_params after each step). */
+enum { ACE_OPT_HOST = 0, ACE_OPT_ADAM = 1 };
+int  ace_train_optimizer(ace_ctx* ctx, int algo, double gamma, double eps);
+/* The device optimiser's state: set == 0 reads, set != 0 writes p, m, v (the parameter count each,
+ * species concatenated) and bt[2] = (beta1^t, beta2^t) of the next step. */
+int  ace_train_opt_state(ace_ctx* ctx, int set, double* p, double* m, double* v, double* bt);
 
 /* Timing of the last calls: per-kernel HIP-event durations accumulated while enabled. */
 int  ace_profile_enable(ace_ctx* ctx, int on);

@@ -21,6 +21,7 @@
 
 #define NB_BIN_THREADS 1024
 #define NB_MAXC 8192   // cells per structure (the host coarsens the grid to stay within it)
+#define NB_BIN_REG 4   // atoms per thread held in registers between the count and the scatter
 
 // cell_start[cell_off + c] (global index into the cell-ordered arrays, per structure, with the end entry
 // cell_start[cell_off + ncl] written by the structure's own workgroup at index cell_off + ncl; the
@@ -42,9 +43,15 @@ __global__ __launch_bounds__(NB_BIN_THREADS) void k_nb_bin(const double* __restr
     if (blockIdx.x == 0 && t < 16) flags[t] = 0;      // build flags and counters (maximum, totals, errors)
     for (int c = t; c < ncl; c += NB_BIN_THREADS) cnt[c] = 0;
     __syncthreads();
-    for (int k = t; k < na; k += NB_BIN_THREADS) {
+    // the first NB_BIN_REG atoms of each thread keep position, cell, slot and wrap in registers for the
+    // scatter below; atoms past them go through slot_of
+    double rx[NB_BIN_REG], ry[NB_BIN_REG], rz[NB_BIN_REG];
+    int rcell[NB_BIN_REG], rslot[NB_BIN_REG], rwrap[NB_BIN_REG];
+    auto bin_one = [&](int k, double& x, double& y, double& z, int& cell, int& slot, int& wrap) {
         const int i = a0 + k;
-        const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        x = pos[3 * i];
+        y = pos[3 * i + 1];
+        z = pos[3 * i + 2];
         int c[3], w[3];
         for (int d = 0; d < 3; ++d) {
             const double f = S.Rinv[d] * x + S.Rinv[d + 3] * y + S.Rinv[d + 6] * z;
@@ -54,10 +61,22 @@ __global__ __launch_bounds__(NB_BIN_THREADS) void k_nb_bin(const double* __restr
             c[d] = cc;
             w[d] = (int)fl;
         }
-        const int cell = (c[2] * S.nc[1] + c[1]) * S.nc[0] + c[0];
+        cell = (c[2] * S.nc[1] + c[1]) * S.nc[0] + c[0];
+        wrap = img_pack(w[0], w[1], w[2]);
         cell_of[i] = cell;
-        wrap_of[i] = img_pack(w[0], w[1], w[2]);
-        slot_of[i] = atomicAdd(&cnt[cell], 1);
+        wrap_of[i] = wrap;
+        slot = atomicAdd(&cnt[cell], 1);
+    };
+#pragma unroll
+    for (int u = 0; u < NB_BIN_REG; ++u) {
+        const int k = t + u * NB_BIN_THREADS;
+        if (k < na) bin_one(k, rx[u], ry[u], rz[u], rcell[u], rslot[u], rwrap[u]);
+    }
+    for (int k = t + NB_BIN_REG * NB_BIN_THREADS; k < na; k += NB_BIN_THREADS) {
+        double x, y, z;
+        int cell, slot, wrap;
+        bin_one(k, x, y, z, cell, slot, wrap);
+        slot_of[a0 + k] = slot;
     }
     __syncthreads();
     // exclusive scan of cnt: each thread sums a run of ceil(ncl / threads) cells, then a wave/block scan
@@ -85,7 +104,16 @@ __global__ __launch_bounds__(NB_BIN_THREADS) void k_nb_bin(const double* __restr
     }
     if (t == 0) cell_start[cs0 + ncl] = a0 + na;
     __syncthreads();
-    for (int k = t; k < na; k += NB_BIN_THREADS) {
+#pragma unroll
+    for (int u = 0; u < NB_BIN_REG; ++u) {
+        const int k = t + u * NB_BIN_THREADS;
+        if (k < na) {
+            const int dst = cnt[rcell[u]] + rslot[u];
+            cpos[dst] = make_double4(rx[u], ry[u], rz[u], 0.0);
+            cinfo[dst] = make_int2(a0 + k, rwrap[u]);
+        }
+    }
+    for (int k = t + NB_BIN_REG * NB_BIN_THREADS; k < na; k += NB_BIN_THREADS) {
         const int i = a0 + k;
         const int dst = cnt[cell_of[i]] + slot_of[i];
         cpos[dst] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.0);

@@ -76,6 +76,7 @@ struct ace_ctx {
     int tune_fused[3] = {-1, -1, -1};   // per precision: fused kernel with 4 (1) or 2 (2) waves per atom, or
                                         // symm -> network -> adjoint (0)
     int tune_stage[3] = {-1, -1, -1};   // per precision: network weights staged in LDS (1) or read from memory
+    int tune_waves[3] = {-1, -1, -1};   // per precision: waves per atom of the split path's kernels (4 or 2)
     long long tune_N = -1;
     int tune_slots = -1;
     bool pinned[3] = {false, false, false};   // ace_set_path: the choice holds for every batch
@@ -691,22 +692,23 @@ DevBasisParams basis_params(ace_ctx* ctx) {
 
 typedef void (*BasisFn)(int, const DevModel*, DevBasisParams, BasisIO, int, int, int);
 
-// f[0..3]: modes 0-3 with four waves per atom; f[4]: the fused kernel (mode 3) with two
+// f[0..3]: modes 0-3 with four waves per atom; f[4..7]: modes 0-3 with two
+template <class T, int NG, int NZ, int GEN>
+void pick_basis_gen(BasisFn* f) {
+    f[0] = k_basis<T, NG, NZ, 0, GEN>;
+    f[1] = k_basis<T, NG, NZ, 1, GEN>;
+    f[2] = k_basis<T, NG, NZ, 2, GEN>;
+    f[3] = k_basis<T, NG, NZ, 3, GEN>;
+    f[4] = k_basis<T, NG, NZ, 0, GEN, 2>;
+    f[5] = k_basis<T, NG, NZ, 1, GEN, 2>;
+    f[6] = k_basis<T, NG, NZ, 2, GEN, 2>;
+    f[7] = k_basis<T, NG, NZ, 3, GEN, 2>;
+}
+
 template <class T, int NG, int NZ>
 void pick_basis(BasisFn* f, bool gen) {
-    if (gen) {
-        f[0] = k_basis<T, NG, NZ, 0, 1>;
-        f[1] = k_basis<T, NG, NZ, 1, 1>;
-        f[2] = k_basis<T, NG, NZ, 2, 1>;
-        f[3] = k_basis<T, NG, NZ, 3, 1>;
-        f[4] = k_basis<T, NG, NZ, 3, 1, 2>;
-    } else {
-        f[0] = k_basis<T, NG, NZ, 0, 0>;
-        f[1] = k_basis<T, NG, NZ, 1, 0>;
-        f[2] = k_basis<T, NG, NZ, 2, 0>;
-        f[3] = k_basis<T, NG, NZ, 3, 0>;
-        f[4] = k_basis<T, NG, NZ, 3, 0, 2>;
-    }
+    if (gen) pick_basis_gen<T, NG, NZ, 1>(f);
+    else pick_basis_gen<T, NG, NZ, 0>(f);
 }
 
 template <class T, int NG>
@@ -784,7 +786,7 @@ int net_lds_floats(const ace_ctx* ctx) {
     return m;
 }
 
-// dynamic LDS of one basis workgroup; waves: 4, or 2 (fused kernel only)
+// dynamic LDS of one basis workgroup of 4 or 2 waves
 int basis_bytes(ace_ctx* ctx, int mode, int waves = 4) {
     const int nbm = nb_slots(ctx), netf = mode == 3 ? net_lds_floats(ctx) : 0;
     if (waves == 2)
@@ -803,6 +805,15 @@ int basis_bytes(ace_ctx* ctx, int mode, int waves = 4) {
 // the per-wave gradient accumulators; where that lets more atoms share a CU the grid takes fewer rounds,
 // which outweighs the longer time per atom (MI355X, 1000-atom bcc, FP64 basis: 54.6 us with two waves
 // in one round against 70.3 us with four in two; FP32, one round either way: 48.6 against 43.8 us).
+// the kernel configuration of path c (ACE_PATH_*; -1: the size rules) as the tuned choice of a precision
+void set_path(ace_ctx* ctx, int precision, int c) {
+    static const int cf[ACE_NPATHS] = {1, 0, 0, 2, 0, 0}, cs[ACE_NPATHS] = {-1, 1, 0, -1, 1, 0},
+                     cw[ACE_NPATHS] = {-1, 4, 4, -1, 2, 2};
+    ctx->tune_fused[precision] = c < 0 ? -1 : cf[c];
+    ctx->tune_stage[precision] = c < 0 ? -1 : cs[c];
+    ctx->tune_waves[precision] = c < 0 ? -1 : cw[c];
+}
+
 bool tuned(const ace_ctx* ctx, int precision) {
     return ctx->pinned[precision] || (ctx->tune_N == ctx->N && ctx->tune_slots == ctx->slots_cap);
 }
@@ -826,18 +837,33 @@ int fused_waves(ace_ctx* ctx, int precision) {
     return r2 < r4 ? 2 : 4;
 }
 
+// Waves per atom of the symmetry-function, adjoint and tangent kernels.  Once the batch takes several
+// rounds of resident workgroups, throughput follows the atoms in flight per CU: LDS allows 160 KB / bytes
+// workgroups and the wave slots (8 per SIMD) 32 / waves.  Two waves per atom keep more atoms in flight
+// where LDS allows, at ~15% more time per atom (the fused kernel's measured ratio).
+int split_waves(ace_ctx* ctx, int mode, int precision) {
+    if (tuned(ctx, precision) && ctx->tune_waves[precision] > 0) return ctx->tune_waves[precision];
+    const int b4 = basis_bytes(ctx, mode, 4), b2 = basis_bytes(ctx, mode, 2);
+    const long long a4 = std::min(160 * 1024 / b4, 8), a2 = std::min(160 * 1024 / b2, 16);
+    if (a4 < 1) return a2 >= 1 ? 2 : 4;
+    const long long r4 = (ctx->N + 256 * a4 - 1) / (256 * a4), r2 = (ctx->N + 256 * a2 - 1) / (256 * a2);
+    return (r2 * 115 < r4 * 100) ? 2 : 4;
+}
+
 // mode 0: G -> out; mode 1: per-pair gradients from dE/dG; mode 2: tangent rows along eps -> out;
-// mode 3: per-atom energies and per-pair gradients in one pass
-// (waves: 4, or 2 for the fused kernel)
-int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = nullptr, int waves = 4) {
-    BasisFn fns[5];
+// mode 3: per-atom energies and per-pair gradients in one pass.  waves: 4 or 2 per atom, -1 for
+// split_waves of the precision (modes 0-2; -1: the context's precision) or four (mode 3)
+int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = nullptr, int waves = -1,
+              int precision = -1) {
+    BasisFn fns[8];
     const bool f64 = ctx->basis_f64;
     basis_fns(ctx, f64, fns);
     const int nbm = nb_slots(ctx);
-    if (waves != 4 && (waves != 2 || mode != 3)) return fail(ctx, "basis: no kernel for this wave count");
+    if (waves < 0) waves = mode == 3 ? 4 : split_waves(ctx, mode, precision < 0 ? ctx->precision : precision);
+    if (waves != 4 && waves != 2) return fail(ctx, "basis: no kernel for this wave count");
     const int bytes = basis_bytes(ctx, mode, waves);
     if (bytes > 160 * 1024) return fail(ctx, "basis: neighbour list too long for LDS");
-    BasisFn f = waves == 2 ? fns[4] : fns[mode];
+    BasisFn f = fns[mode + (waves == 2 ? 4 : 0)];
     if (bytes > 64 * 1024) HC(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     DevBasisParams P = basis_params(ctx);
     int* err = ctx->d_scal.p + 3;
@@ -909,9 +935,9 @@ int run_mlp(ace_ctx* ctx, bool bf16, int precision = ACE_PREC_FP32) {
 int run_ef(ace_ctx* ctx, int precision) {
     const int fw = fused_waves(ctx, precision);
     if (fw) return run_basis(ctx, 3, nullptr, nullptr, fw);
-    if (run_basis(ctx, 0)) return -1;
+    if (run_basis(ctx, 0, nullptr, nullptr, -1, precision)) return -1;
     if (run_mlp(ctx, precision == ACE_PREC_BF16L1, precision)) return -1;
-    return run_basis(ctx, 1);
+    return run_basis(ctx, 1, nullptr, nullptr, -1, precision);
 }
 
 // one velocity-Verlet step: half kick with the forces of the last gradient launch, drift, energy/force
@@ -1501,15 +1527,14 @@ int ace_autotune(ace_ctx* ctx, int precision, double* ms) {
     if (ctx->nb_pending && nb_verify(ctx)) return -1;
     int lds_w = 0;
     for (int I = 0; I < ctx->nt; ++I) lds_w = std::max(lds_w, (mlp_lds_floats(ctx->hdm.sp[I]) + ctx->hdm.sp[I].wp_n) * 4);
-    // candidates: fused (four waves per atom); split with staged weights; split with weights read from
-    // memory; fused with two waves per atom
+    // candidates (ACE_PATH_*): fused with four waves per atom; split with staged weights; split with weights
+    // read from memory; fused with two waves per atom; the two split forms with two waves per atom
     const bool fu = precision != ACE_PREC_BF16L1;
     const bool can_fused = fu && basis_bytes(ctx, 3, 4) <= 160 * 1024;
     const bool can_fused2 = fu && basis_bytes(ctx, 3, 2) <= 160 * 1024;
     const bool can_stage = lds_w <= 160 * 1024;
-    const int cf[4] = {1, 0, 0, 2}, cs[4] = {-1, 1, 0, -1};
-    const bool ok[4] = {can_fused, can_stage, true, can_fused2};
-    double t[4] = {-1, -1, -1, -1};
+    const bool ok[ACE_NPATHS] = {can_fused, can_stage, true, can_fused2, can_stage, true};
+    double t[ACE_NPATHS] = {-1, -1, -1, -1, -1, -1};
     const bool prof = ctx->prof;
     ctx->prof = false;
     ctx->tune_N = ctx->N;
@@ -1518,10 +1543,9 @@ int ace_autotune(ace_ctx* ctx, int precision, double* ms) {
     HC(hipEventCreate(&e0));
     HC(hipEventCreate(&e1));
     int best = -1;
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < ACE_NPATHS; ++c) {
         if (!ok[c]) continue;
-        ctx->tune_fused[precision] = cf[c];
-        ctx->tune_stage[precision] = cs[c];
+        set_path(ctx, precision, c);
         if (run_ef(ctx, precision)) return -1;   // warm-up (kernel loads, LDS attributes)
         float el = 0;
         int reps = 3;
@@ -1541,13 +1565,12 @@ int ace_autotune(ace_ctx* ctx, int precision, double* ms) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     ctx->prof = prof;
-    ctx->tune_fused[precision] = cf[best];
-    ctx->tune_stage[precision] = cs[best];
+    set_path(ctx, precision, best);
     ctx->pinned[precision] = false;
     if (check_err_flag(ctx, "ace_autotune")) return -1;
     if (ms) {
-        for (int c = 0; c < 4; ++c) ms[c] = t[c];
-        ms[4] = best;
+        for (int c = 0; c < ACE_NPATHS; ++c) ms[c] = t[c];
+        ms[ACE_NPATHS] = best;
     }
     return 0;
 }
@@ -1556,12 +1579,10 @@ int ace_set_path(ace_ctx* ctx, int precision, int path) {
     if (!ctx) return -1;
     if (precision != ACE_PREC_FP32 && precision != ACE_PREC_FP64_BASIS && precision != ACE_PREC_BF16L1)
         return fail(ctx, "ace_set_path: precision mode not available");
-    if (path < -1 || path > 3) return fail(ctx, "ace_set_path: path must be -1 (size rule) or 0..3");
-    if (precision == ACE_PREC_BF16L1 && (path == 0 || path == 3))
+    if (path < -1 || path >= ACE_NPATHS) return fail(ctx, "ace_set_path: path must be -1 (size rule) or an ACE_PATH_*");
+    if (precision == ACE_PREC_BF16L1 && (path == ACE_PATH_FUSED || path == ACE_PATH_FUSED_2W))
         return fail(ctx, "ace_set_path: the BF16 first layer runs on the split path only");
-    static const int cf[4] = {1, 0, 0, 2}, cs[4] = {-1, 1, 0, -1};
-    ctx->tune_fused[precision] = path < 0 ? -1 : cf[path];
-    ctx->tune_stage[precision] = path < 0 ? -1 : cs[path];
+    set_path(ctx, precision, path);
     ctx->pinned[precision] = path >= 0;
     return 0;
 }

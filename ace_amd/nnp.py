@@ -180,14 +180,15 @@ class NNPDevice:
 
     def autotune(self, precision=PREC_FP32):
         """Measures the energy/force paths on the uploaded batch and keeps the fastest; returns the
-        milliseconds of (fused with four waves per atom, split with staged weights, split with weights in
-        memory, fused with two waves per atom) and the choice."""
-        ms = np.zeros(5)
+        milliseconds of every path of PATHS (fused, split with staged weights, split with weights in memory;
+        four or two waves per atom) and the choice."""
+        n = len(self.PATHS)
+        ms = np.zeros(n + 1)
         self._check(self.lib.ace_autotune(self.ctx, precision, _dp(ms)), "ace_autotune")
         names = self.PATHS
-        return {"ms": {names[i]: float(ms[i]) for i in range(4) if ms[i] >= 0}, "choice": names[int(ms[4])]}
+        return {"ms": {names[i]: float(ms[i]) for i in range(n) if ms[i] >= 0}, "choice": names[int(ms[n])]}
 
-    PATHS = ("fused", "split_staged", "split", "fused_2w")
+    PATHS = ("fused", "split_staged", "split", "fused_2w", "split_staged_2w", "split_2w")
 
     def set_path(self, path, precision=PREC_FP32):
         """Fixes the energy/force path (a name of PATHS, or None for the size rule) for every batch."""

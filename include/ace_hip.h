@@ -191,16 +191,20 @@ int  ace_md_run(ace_ctx* ctx, int precision, int nsteps, double dt, int rebuild_
                 double* energy);
 int  ace_md_state(ace_ctx* ctx, double* pos, double* vel);
 
-/* Autotuner (no counterpart in ace): times the energy/force paths on the uploaded batch (neighbour list
- * built) and keeps the fastest for this precision -- the fused per-atom kernel with four waves per atom,
- * symmetry functions -> network -> adjoint with the network weights staged in LDS or read from memory,
- * or the fused kernel with two waves per atom.  ms: NULL or 5 entries, the milliseconds of the four
- * candidates in that order (-1 where not feasible) and the index chosen.  The choice holds while the
- * batch size and neighbour-row capacity stay as measured; otherwise a size rule decides. */
+/* Energy/force paths: the fused per-atom kernel (symmetry functions, network, adjoint) with four or two
+ * waves per atom, or symmetry functions -> network -> adjoint with the network weights staged in LDS or
+ * read from memory, its per-atom kernels with four or two waves per atom. */
+enum { ACE_PATH_FUSED = 0, ACE_PATH_SPLIT_STAGED = 1, ACE_PATH_SPLIT = 2, ACE_PATH_FUSED_2W = 3,
+       ACE_PATH_SPLIT_STAGED_2W = 4, ACE_PATH_SPLIT_2W = 5, ACE_NPATHS = 6 };
+/* Autotuner (no counterpart in ace): times the paths on the uploaded batch (neighbour list built) and
+ * keeps the fastest for this precision.  ms: NULL or ACE_NPATHS + 1 entries, the milliseconds of each
+ * path (-1 where not feasible) and the index chosen.  The choice holds while the batch size and
+ * neighbour-row capacity stay as measured; otherwise size rules decide. */
 int  ace_autotune(ace_ctx* ctx, int precision, double* ms);
-/* Fixes the energy/force path of a precision for every batch: path 0..3 as ace_autotune's candidates
- * (a fused path whose workgroup does not fit in LDS runs split), -1 returns to the size rule.  The BF16
- * first-layer mode has the split paths (1, 2) only. */
+/* Fixes the energy/force path of a precision (ACE_PATH_*) for every batch, the wave count of the
+ * symmetry-function, adjoint and tangent kernels of training included (a fused path whose workgroup does
+ * not fit in LDS runs split); -1 returns to the size rules.  The BF16 first-layer mode has the split
+ * paths only. */
 int  ace_set_path(ace_ctx* ctx, int precision, int path);
 
 /* Same quantities from the unbatched FP64 port: one structure per launch, one thread per atom. */

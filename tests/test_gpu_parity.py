@@ -357,9 +357,11 @@ def test_bf16_first_layer():
 def test_c5_full_size():
     """The 256-structure, 568k-atom mixed-precision inference batch: FP32 and BF16L1 energies and forces
     of the two smallest structures vs the oracle, and those two structures uploaded alone give the same
-    bits as inside the batch (every per-atom result is independent of the batch; sums are exact)."""
+    bits as inside the batch on the same path (every per-atom result is independent of the batch; sums
+    are exact).  The path is the one the size rules take for the full batch: split, two waves per atom."""
     model, strucs = configs.config5()
     N = sum(s.natoms for s in strucs)
+    path = "split_2w"
     dev = NNPDevice(model)
     dev.upload(strucs)
     dev.build_neighbors()
@@ -369,6 +371,8 @@ def test_c5_full_size():
     del G
     dev.sync_model_scaling()
     dev.set_precision(PREC_FP32)
+    for p in (PREC_FP32, PREC_BF16L1):
+        dev.set_path(path, p)
     res = {p: dev.compute(p) for p in (PREC_FP32, PREC_BF16L1)}
     dev.close()
     off = np.concatenate([[0], np.cumsum([s.natoms for s in strucs])])
@@ -377,6 +381,8 @@ def test_c5_full_size():
     alone = NNPDevice(model)
     alone.upload([strucs[k] for k in small])
     alone.build_neighbors()
+    for p in (PREC_FP32, PREC_BF16L1):
+        alone.set_path(path, p)
     Ea = {p: alone.compute(p) for p in (PREC_FP32, PREC_BF16L1)}
     alone.close()
     a0 = 0

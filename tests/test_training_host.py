@@ -61,3 +61,99 @@ def test_output_scaling_scales_energies_and_forces():
     assert np.allclose(F2, 0.37 * F1, rtol=1e-10, atol=1e-14)
     f = configs.calibrate_forces(model, F2, rms=0.1)
     assert np.isclose(np.sqrt(np.mean((f * F2) ** 2)), 0.1, rtol=1e-12)
+
+
+class _DeviceStub:
+    """The device calls the trainer makes, recorded (no GPU): a device ADAM restated with the host ADAM."""
+
+    def __init__(self, model):
+        from ace_amd.optimizers import ADAM, Objective
+        self.model = model
+        self.calls = []
+        self.obj = Objective(model.all_params(), 0.0)
+        self.adam = None
+
+    def train_graph(self, on):
+        self.calls.append(("graph", on))
+
+    def set_targets(self, E, F):
+        self.calls.append(("targets",))
+
+    def train_optimizer(self, algo, gamma, eps):
+        from ace_amd.optimizers import ADAM
+        self.calls.append(("optimizer", algo, gamma, eps))
+        self.obj.gamma = gamma
+        self.adam = ADAM(len(self.obj.p), eps)
+
+    def train_step(self, loss, norm, delta, w_energy, w_force, want_grad=True):
+        self.calls.append(("step", want_grad))
+        g = np.linspace(-1.0, 1.0, len(self.obj.p))
+        out = np.array([1.0, 0.5, 0.5, 0.0])
+        if self.adam is not None:
+            self.obj.g = g
+            self.adam.step(self.obj)
+        return out, (g if want_grad else None)
+
+    def get_all_params(self):
+        return self.obj.p.copy()
+
+    def set_all_params(self, p):
+        self.obj.p = np.array(p, dtype=np.float64)
+
+    def train_opt_state(self, state=None):
+        if state is None:
+            return {"p": self.obj.p.copy(), "m": self.adam.m.copy(), "v": self.adam.v.copy(),
+                    "bt": np.array([self.adam.b1t, self.adam.b2t])}
+        self.obj.p = np.array(state["p"])
+        self.adam.m, self.adam.v = np.array(state["m"]), np.array(state["v"])
+        self.adam.b1t, self.adam.b2t = float(state["bt"][0]), float(state["bt"][1])
+
+
+def _small():
+    model, [s] = configs.config1()
+    s.energy, s.forces = -1.0, np.zeros((s.natoms, 3))
+    return model, [s]
+
+
+def test_device_optimizer_refuses_what_it_does_not_restate():
+    import pytest
+    from ace_amd.optimizers import Decay
+    from ace_amd.training import Trainer
+    for kw in ({"algo": "SGD"}, {"decay": Decay("EXP", 0.1)}, {"reg": "RIDGE", "eta_reg": 0.1}):
+        model, strucs = _small()
+        with pytest.raises(ValueError):
+            Trainer(_DeviceStub(model), model, strucs, device_opt=True, **kw)
+
+
+def test_device_optimizer_loop_and_restart(tmp_path):
+    """With device_opt the trainer asks for no gradient, leaves the update to the device and reads the
+    parameters back on demand; a restart written from the device state and loaded into a new trainer
+    continues the same trajectory as an uninterrupted run, and equals the host ADAM on the same
+    gradients."""
+    from ace_amd.training import Trainer
+    model, strucs = _small()
+    dev = _DeviceStub(model)
+    tr = Trainer(dev, model, strucs, gamma=1e-2, device_opt=True)
+    assert ("optimizer", "ADAM", 1e-2, 1e-16) in dev.calls
+    for _ in range(3):
+        tr.step()
+    assert [c for c in dev.calls if c[0] == "step"] == [("step", False)] * 3
+    tr.save_restart(str(tmp_path / "r.npz"))
+    for _ in range(2):
+        tr.step()
+    p_cont = tr.params.copy()
+
+    model2, strucs2 = _small()
+    dev2 = _DeviceStub(model2)
+    tr2 = Trainer(dev2, model2, strucs2, gamma=1e-2, device_opt=True)
+    tr2.load_restart(str(tmp_path / "r.npz"))
+    for _ in range(2):
+        tr2.step()
+    assert np.array_equal(tr2.params, p_cont)
+
+    model3, strucs3 = _small()
+    dev3 = _DeviceStub(model3)
+    host = Trainer(dev3, model3, strucs3, gamma=1e-2)
+    for _ in range(5):
+        host.step()
+    assert np.array_equal(host.params, p_cont)

@@ -544,7 +544,7 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
 // and row holds dE/dG (the input scaling inpw folded in).  The species' parameters are in L.nw in the
 // matrix-core layout (W_l as [k][o], row stride mlp_ld(w_{l+1}), biases at bl_off); after them: per layer
 // the activations and derivatives.  A chunk of a layer's outputs is spread over the 64 lanes, G lanes per
-// output (G a power of two); a lane's strided partial sum runs in four accumulators that are combined, and
+// output (G a power of two); a lane's strided partial sum runs in eight accumulators that are combined, and
 // then summed over the group by DPP moves, in a fixed order.  One wave needs no workgroup barrier between
 // layers (LDS accesses of a wave complete in order; wave_sync keeps the compiler from reordering them);
 // one barrier at the end publishes dE/dG to the other waves.

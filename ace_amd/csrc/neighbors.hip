@@ -239,7 +239,7 @@ __device__ inline void nb_key_decode(long long k, int& jl, int& n0, int& n1, int
 #define NB_LMAX 1024   // neighbours per atom held while sorting
 #define NB_U 4         // candidate chunks loaded together by one wave
 
-// Workgroup (4 waves) per atom; the waves take interleaved groups of NB_U x 64 candidates.
+// Workgroup (4 waves) per atom; 64-candidate chunks are dealt to the waves in turn, NB_U chunks in flight per wave.
 // MODE 0: count only (counts[i], running maximum in *maxcount).  MODE 1: sort the keys
 // (j_local << 30 | image order code: the reference order) and write them to row i of keys (cap slots); an
 // atom with more than cap neighbours raises *over and writes none.
@@ -297,13 +297,14 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
         }
         __syncthreads();
         const int total = s_cpref[64];
-        for (int tb = wv * 64 * NB_U; tb < total; tb += NB_WAVES * 64 * NB_U) {
+        // chunk q of 64 candidates goes to wave q mod 4, so a short stencil keeps every wave busy
+        for (int tb = 0; tb < total; tb += NB_WAVES * 64 * NB_U) {
             double4 pj[NB_U];
             int2 inf[NB_U];
             int sl[NB_U];
 #pragma unroll
             for (int u = 0; u < NB_U; ++u) {
-                const int x = tb + 64 * u + lane;
+                const int x = tb + 64 * (u * NB_WAVES + wv) + lane;
                 // stencil slot of candidate x: largest c with s_cpref[c] <= x
                 int c = 0;
 #pragma unroll
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
             }
 #pragma unroll
             for (int u = 0; u < NB_U; ++u) {
-                const int x = tb + 64 * u + lane;
+                const int x = tb + 64 * (u * NB_WAVES + wv) + lane;
                 bool acc = false;
                 long long key = 0;
                 if (x < total) {

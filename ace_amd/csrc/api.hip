@@ -1101,6 +1101,8 @@ int ace_set_params(ace_ctx* ctx, int type, const double* params, int n) {
 }
 
 int ace_get_params(const ace_ctx* cctx, int type, double* params, int n) {
+    // reading may refresh the host copies from the device optimiser's parameters: a cache update, the
+    // parameters themselves do not change
     ace_ctx* ctx = const_cast<ace_ctx*>(cctx);
     if (!ctx || type < 0 || type >= ctx->nt) return -1;
     if (n != ctx->sp[type].nparams()) return -1;

@@ -239,8 +239,12 @@ def main():
     dev = NNPDevice(model)
     dev.upload([s])
     dev.build_neighbors()
+    # PreBias MEAN / PreScale DEV (nnptefr.cpp:786-880) from FP64 symmetry functions, as the reference
+    # computes its input statistics in FP64
+    dev.set_precision(PREC_FP64_BASIS)
     G = dev.compute_symm()
-    configs.set_input_scaling(model, [s], G)       # PreBias MEAN / PreScale DEV from the GPU's G
+    dev.set_precision(PREC_FP32)
+    configs.set_input_scaling(model, [s], G)
     dev.sync_model_scaling()
     tune = {"fp32": dev.autotune(PREC_FP32), "fp64_basis": dev.autotune(PREC_FP64_BASIS)}
 

@@ -49,6 +49,68 @@ def taylor_augment(strucs, nadd, rdelta, rng, dist="normal"):
     return out
 
 
+PREBIAS = ("IDENTITY", "MEAN", "HAR", "RMS", "MID", "MIN")
+PRESCALE = ("IDENTITY", "DEV", "MINMAX", "MAX")
+
+
+def input_statistics(types, G, ntypes):
+    """Per species and input: count, max, min, harmonic mean, mean, sample deviation and RMS of the
+    symmetry functions over the training atoms (NNPTEFR::train, src/nnp/nnptefr.cpp:689-786; the MPI
+    reductions there are sums, minima and maxima over ranks, so one rank's full set gives the same)."""
+    types = np.asarray(types)
+    out = []
+    for I in range(ntypes):
+        g = np.asarray(G[types == I], dtype=np.float64)
+        n = len(g)
+        if n == 0:
+            out.append(None)
+            continue
+        with np.errstate(divide="ignore"):
+            har = n / np.sum(1.0 / g, axis=0)
+        avg = np.sum(g, axis=0) / n
+        dev = np.sqrt(np.sum((avg - g) * (avg - g), axis=0) / (n - 1.0)) if n > 1 else np.full(g.shape[1], np.nan)
+        rms = np.sqrt(np.sum(g * g, axis=0) / n)
+        out.append({"n": n, "max": g.max(axis=0), "min": g.min(axis=0), "har": har, "avg": avg, "dev": dev,
+                    "rms": rms})
+    return out
+
+
+def precondition(model, types, G, prebias="MEAN", prescale="DEV", inbias=0.0, inscale=1.0):
+    """The network input bias and weight of every species from the input statistics, and the output
+    bias/weight reset to (0, 1) (src/nnp/nnptefr.cpp:788-880).  G: one row per training atom (the species'
+    inputs first in each row), types: that atom's species.  prebias: IDENTITY, MEAN (-mean), HAR (-harmonic
+    mean), RMS (-rms), MID (-(max + min)/2), MIN (-min), then + inbias; prescale: IDENTITY, DEV (1/dev),
+    MINMAX (1/(max - min)), MAX (1/max), 1 where the statistic is not positive, then x inscale.  A species
+    with no training atom raises ValueError (the reference divides by its zero count)."""
+    pb, ps = prebias.upper(), prescale.upper()
+    if pb not in PREBIAS:
+        raise ValueError("Invalid input bias.")
+    if ps not in PRESCALE:
+        raise ValueError("Invalid input scaling.")
+    stats = input_statistics(types, G, model.ntypes)
+    for I in range(model.ntypes):
+        K = model.input_width(I)
+        st = stats[I]
+        if st is None:
+            raise ValueError("species %d has no atom in the training set" % I)
+        st = {k: (v[:K] if isinstance(v, np.ndarray) else v) for k, v in st.items()}
+        bias = {"IDENTITY": np.zeros(K), "MEAN": -st["avg"], "HAR": -st["har"], "RMS": -st["rms"],
+                "MID": -0.5 * (st["max"] + st["min"]), "MIN": -st["min"]}[pb]
+        if ps == "DEV":
+            w = np.where(st["dev"] == 0, 1.0, 1.0 / np.where(st["dev"] == 0, 1.0, st["dev"]))
+        elif ps == "MINMAX":
+            span = st["max"] - st["min"]
+            w = np.where(span > 0, 1.0 / np.where(span > 0, span, 1.0), 1.0)
+        elif ps == "MAX":
+            w = np.where(st["max"] > 0, 1.0 / np.where(st["max"] > 0, st["max"], 1.0), 1.0)
+        else:
+            w = np.ones(K)
+        net = model.nets[I]
+        net.inpb = bias + inbias
+        net.inpw = w * inscale
+        net.outb, net.outw = 0.0, 1.0
+
+
 class Adam:
     """ADAM::step: m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2, p -= gamma m/(1-b1^t) / (sqrt(v/(1-b2^t)) + eps)."""
 

@@ -157,3 +157,72 @@ def test_device_optimizer_loop_and_restart(tmp_path):
     for _ in range(5):
         host.step()
     assert np.array_equal(host.params, p_cont)
+
+
+def _stats_model():
+    model, [s] = configs.config1()
+    return model
+
+
+def test_input_statistics_match_library_routines():
+    """Per-species count, extremes, mean, sample deviation (N - 1), RMS and harmonic mean
+    (nnptefr.cpp:689-786) against numpy / scipy on positive random inputs of two species."""
+    from scipy import stats as sps
+    from ace_amd.training import input_statistics
+    rng = np.random.default_rng(5)
+    G = rng.uniform(0.1, 3.0, (57, 6))
+    types = rng.integers(0, 2, 57)
+    st = input_statistics(types, G, 2)
+    for I in range(2):
+        g = G[types == I]
+        assert st[I]["n"] == len(g)
+        assert np.allclose(st[I]["avg"], g.mean(axis=0), rtol=1e-14, atol=0)
+        assert np.allclose(st[I]["dev"], g.std(axis=0, ddof=1), rtol=1e-12, atol=0)
+        assert np.allclose(st[I]["rms"], np.sqrt((g ** 2).mean(axis=0)), rtol=1e-14, atol=0)
+        assert np.allclose(st[I]["har"], sps.hmean(g, axis=0), rtol=1e-13, atol=0)
+        assert np.array_equal(st[I]["max"], g.max(axis=0)) and np.array_equal(st[I]["min"], g.min(axis=0))
+
+
+def test_precondition_every_bias_and_scale():
+    """Each PreBias / PreScale of nnptefr.cpp:788-880 with inbias and inscale, the guards for zero
+    deviation, empty span and non-positive maximum, the output reset, and MEAN/DEV leaving the scaled
+    inputs with zero mean and unit sample deviation."""
+    import pytest
+    from ace_amd.training import input_statistics, precondition
+    model = _stats_model()
+    K = model.input_width(0)
+    rng = np.random.default_rng(6)
+    G = rng.uniform(-1.0, 2.0, (40, K))
+    G[:, 0] = 0.5                       # constant column (exact mean): deviation and span 0
+    G[:, 1] = -np.abs(G[:, 1]) - 0.1    # negative column: maximum < 0
+    types = np.zeros(40, dtype=int)
+    st = input_statistics(types, G, 1)[0]
+    expect_b = {"IDENTITY": np.zeros(K), "MEAN": -st["avg"], "HAR": -st["har"], "RMS": -st["rms"],
+                "MID": -0.5 * (st["max"] + st["min"]), "MIN": -st["min"]}
+    for pb, eb in expect_b.items():
+        for ps in ("IDENTITY", "DEV", "MINMAX", "MAX"):
+            model.nets[0].outb, model.nets[0].outw = 0.3, 2.0
+            precondition(model, types, G, pb, ps, inbias=0.25, inscale=1.5)
+            net = model.nets[0]
+            assert np.allclose(net.inpb, eb + 0.25, rtol=0, atol=1e-15)
+            if ps == "DEV":
+                ew = np.r_[1.0, 1.0 / st["dev"][1:]]
+            elif ps == "MINMAX":
+                ew = np.r_[1.0, 1.0 / (st["max"] - st["min"])[1:]]
+            elif ps == "MAX":
+                ew = np.r_[2.0, 1.0, 1.0 / st["max"][2:]]     # column 0: max 0.5; column 1: max < 0
+            else:
+                ew = np.ones(K)
+            assert np.allclose(net.inpw, 1.5 * ew, rtol=1e-15, atol=0), (pb, ps)
+            assert net.outb == 0.0 and net.outw == 1.0
+    precondition(model, types, G, "MEAN", "DEV")
+    x = model.nets[0].inpw * (G + model.nets[0].inpb)
+    assert np.allclose(x.mean(axis=0), 0.0, atol=1e-13)
+    assert np.allclose(x[:, 1:].std(axis=0, ddof=1), 1.0, rtol=1e-12)
+    with pytest.raises(ValueError):
+        precondition(model, types, G, "MEDIAN", "DEV")
+    with pytest.raises(ValueError):
+        precondition(model, types, G, "MEAN", "RANGE")
+    two, _ = configs.config2()
+    with pytest.raises(ValueError):
+        precondition(two, np.zeros(40, dtype=int), np.zeros((40, two.input_width(0))), "MEAN", "DEV")

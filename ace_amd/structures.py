@@ -181,6 +181,33 @@ def read_poscar(path, units="metal"):
         return read_poscar_text(f.read(), units)
 
 
+FILE_FORMATS = ("POSCAR", "XDATCAR", "VASP_XML", "LAMMPS", "XYZ", "CUBE", "QE", "CP2K")   # FILE_FORMAT::read
+
+
+def read_struc(path, fmt, units="metal"):
+    """read_struc (src/format/file_struc.cpp:13-35) by format name: POSCAR and XYZ are read here; the
+    reference's other structure readers (VASP_XML, CP2K, QE) are not restated and raise ValueError, as an
+    unknown name does."""
+    f = fmt.upper()
+    if f == "POSCAR":
+        return read_poscar(path, units)
+    if f == "XYZ":
+        return read_xyz(path, units)
+    if f in FILE_FORMATS:
+        raise ValueError("read_struc: format %s is not supported by this build" % f)
+    raise ValueError("read_struc: invalid file format")
+
+
+def write_struc(path, fmt, s: Structure, names, units="metal"):
+    """write_struc (src/format/file_struc.cpp:37-50): POSCAR or XYZ."""
+    f = fmt.upper()
+    if f == "POSCAR":
+        return write_poscar(path, s, names, units=units)
+    if f == "XYZ":
+        return write_xyz(path, s, names, units)
+    raise ValueError("write_struc: invalid file format")
+
+
 def to_model_types(s: Structure, file_names, model_names):
     """Type indices of the model's species for a structure read from a file (the NNP index of every
     atom's name, src/nnp/nnptefr.cpp:2489-2496); unknown species raise ValueError."""

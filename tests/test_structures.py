@@ -70,3 +70,20 @@ def test_model_type_mapping(golden_dir):
 def test_invalid_xyz_raises(text):
     with pytest.raises(ValueError):
         S.read_xyz_text(text)
+
+
+def test_read_write_struc_by_format_name(golden_dir, tmp_path):
+    """read_struc / write_struc (file_struc.cpp:13-50) dispatch on the format name, case-insensitively;
+    known formats without a reader here and unknown names raise."""
+    s1, n1, _ = S.read_struc(f"{golden_dir}/mixed4.xyz", "xyz")
+    s2, n2, _ = S.read_xyz(f"{golden_dir}/mixed4.xyz")
+    assert n1 == n2 and np.array_equal(s1.pos, s2.pos) and np.array_equal(s1.cell, s2.cell)
+    p, names, _ = S.read_struc(f"{golden_dir}/nacl8.poscar", "POSCAR")
+    S.write_struc(str(tmp_path / "out.poscar"), "POSCAR", p, names)
+    q, names2, _ = S.read_struc(str(tmp_path / "out.poscar"), "POSCAR")
+    assert names2 == names and np.allclose(q.pos, p.pos, atol=1e-7)
+    for fmt in ("QE", "CP2K", "VASP_XML", "PDB"):
+        with pytest.raises(ValueError):
+            S.read_struc(f"{golden_dir}/nacl8.poscar", fmt)
+    with pytest.raises(ValueError):
+        S.write_struc(str(tmp_path / "x.cube"), "CUBE", p, names)

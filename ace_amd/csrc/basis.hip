@@ -764,59 +764,87 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
         const StrucInfo& S = io.sinfo[st];
         const double pix = io.pos[3 * i], piy = io.pos[3 * i + 1], piz = io.pos[3 * i + 2];
         const bool fits = nn <= nbm;
-        int cnt[ACE_MAXT];
+        // neighbour e of the row: atom j, species Jl (-1 past the cutoff or the row) and d, |d|
+        auto decode = [&](int e, int& j, int& Jl, double& dx, double& dy, double& dz, double& r) {
+            Jl = -1;
+            j = 0;
+            dx = dy = dz = r = 0.0;
+            if (e < nn && fits) {
+                const long long key = krow[e];
+                const int jl = (int)(key >> 30), c = (int)(key & 0x3fffffff);
+                const int c0 = (c >> 20) - 512, c1 = ((c >> 10) & 1023) - 512, c2 = (c & 1023) - 512;
+                j = jl + S.atom_off;
+                dx = pix - (io.pos[3 * j] + (S.R[0] * c0 + S.R[3] * c1 + S.R[6] * c2));
+                dy = piy - (io.pos[3 * j + 1] + (S.R[1] * c0 + S.R[4] * c1 + S.R[7] * c2));
+                dz = piz - (io.pos[3 * j + 2] + (S.R[2] * c0 + S.R[5] * c1 + S.R[8] * c2));
+                r = sqrt(dx * dx + dy * dy + dz * dz);
+                if ((float)r < rcut) Jl = io.type[j];
+            }
+        };
+        int cnt[ACE_MAXT], fill[ACE_MAXT];
 #pragma unroll
         for (int J = 0; J < ACE_MAXT; ++J) cnt[J] = 0;
-        for (int pass = 0; pass < 2; ++pass) {
-            int fill[ACE_MAXT];
-            if (pass == 1) {
-                int acc = 0;
+        auto count = [&](int Jl) {
 #pragma unroll
-                for (int J = 0; J < ACE_MAXT; ++J) {
-                    fill[J] = acc;
-                    acc += (J < nt) ? cnt[J] : 0;
-                }
-                if (lane == 0) {
+            for (int J = 0; J < ACE_MAXT; ++J)
+                if (J < nt) cnt[J] += __popcll(__ballot(Jl == J));
+        };
+        // slot q of species Jl: the next free one in ballot order, i.e. the row order
+        auto place = [&](int j, int Jl, double dx, double dy, double dz, double r) {
 #pragma unroll
-                    for (int J = 0; J <= ACE_MAXT; ++J)
-                        if (J <= nt) s_cnt[J] = (J < nt) ? fill[J] : acc;
-                }
-                if (!fits) break;
-            }
-            for (int cb = 0; cb < nn; cb += 64) {
-                const int e = cb + lane;
-                int Jl = -1, j = 0;
-                double dx = 0.0, dy = 0.0, dz = 0.0, r = 0.0;
-                if (e < nn && fits) {
-                    const long long key = krow[e];
-                    const int jl = (int)(key >> 30), c = (int)(key & 0x3fffffff);
-                    const int c0 = (c >> 20) - 512, c1 = ((c >> 10) & 1023) - 512, c2 = (c & 1023) - 512;
-                    j = jl + S.atom_off;
-                    dx = pix - (io.pos[3 * j] + (S.R[0] * c0 + S.R[3] * c1 + S.R[6] * c2));
-                    dy = piy - (io.pos[3 * j + 1] + (S.R[1] * c0 + S.R[4] * c1 + S.R[7] * c2));
-                    dz = piz - (io.pos[3 * j + 2] + (S.R[2] * c0 + S.R[5] * c1 + S.R[8] * c2));
-                    r = sqrt(dx * dx + dy * dy + dz * dz);
-                    if ((float)r < rcut) Jl = io.type[j];
-                }
-#pragma unroll
-                for (int J = 0; J < ACE_MAXT; ++J) {
-                    if (J < nt) {
-                        const unsigned long long m = __ballot(Jl == J);
-                        if (pass == 0) {
-                            cnt[J] += __popcll(m);
-                        } else {
-                            if (Jl == J) {
-                                const int q = fill[J] + __popcll(m & lane_mask_lt());
-                                L.p[q] = j;
-                                L.u[4 * q] = (T)dx;
-                                L.u[4 * q + 1] = (T)dy;
-                                L.u[4 * q + 2] = (T)dz;
-                                L.u[4 * q + 3] = (T)r;
-                            }
-                            fill[J] += __popcll(m);
-                        }
+            for (int J = 0; J < ACE_MAXT; ++J) {
+                if (J < nt) {
+                    const unsigned long long m = __ballot(Jl == J);
+                    if (Jl == J) {
+                        const int q = fill[J] + __popcll(m & lane_mask_lt());
+                        L.p[q] = j;
+                        L.u[4 * q] = (T)dx;
+                        L.u[4 * q + 1] = (T)dy;
+                        L.u[4 * q + 2] = (T)dz;
+                        L.u[4 * q + 3] = (T)r;
                     }
+                    fill[J] += __popcll(m);
                 }
+            }
+        };
+        // pass 0 counts per species; the first NCH chunks of 64 neighbours stay in registers for the
+        // placing pass, so a row of up to 64 NCH neighbours is read once
+        constexpr int NCH = 2;
+        int jr[NCH], Jr[NCH];
+        double dxr[NCH], dyr[NCH], dzr[NCH], rr[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+            decode(64 * c + lane, jr[c], Jr[c], dxr[c], dyr[c], dzr[c], rr[c]);
+            count(Jr[c]);
+        }
+        for (int cb = 64 * NCH; cb < nn; cb += 64) {
+            int j, Jl;
+            double dx, dy, dz, r;
+            decode(cb + lane, j, Jl, dx, dy, dz, r);
+            count(Jl);
+        }
+        // pass 1: species offsets, then the slots
+        {
+            int acc = 0;
+#pragma unroll
+            for (int J = 0; J < ACE_MAXT; ++J) {
+                fill[J] = acc;
+                acc += (J < nt) ? cnt[J] : 0;
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int J = 0; J <= ACE_MAXT; ++J)
+                    if (J <= nt) s_cnt[J] = (J < nt) ? fill[J] : acc;
+            }
+        }
+        if (fits) {
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) place(jr[c], Jr[c], dxr[c], dyr[c], dzr[c], rr[c]);
+            for (int cb = 64 * NCH; cb < nn; cb += 64) {
+                int j, Jl;
+                double dx, dy, dz, r;
+                decode(cb + lane, j, Jl, dx, dy, dz, r);
+                place(j, Jl, dx, dy, dz, r);
             }
         }
         if (!fits && lane == 0) s_cnt[nt] = nbm + 1;

@@ -769,9 +769,8 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
             Jl = -1;
             j = 0;
             dx = dy = dz = r = 0.0;
-            // any slot of the row's capacity can be read, so the key load need not wait for the count
-            const long long key = (e < io.cap) ? krow[e] : 0ll;
             if (e < nn && fits) {
+                const long long key = krow[e];
                 const int jl = (int)(key >> 30), c = (int)(key & 0x3fffffff);
                 const int c0 = (c >> 20) - 512, c1 = ((c >> 10) & 1023) - 512, c2 = (c & 1023) - 512;
                 j = jl + S.atom_off;

@@ -115,6 +115,9 @@ struct ace_ctx {
     bool nb_async_ok = false;   // capacities known: rebuilds need no host round trip
     bool nb_same_layout = false;
     bool nb_pending = false;    // a rebuild's overflow / symmetry flags are not checked yet
+    bool nb_brute = false;      // the last build tested every atom of each structure (k_nb_brute)
+    int nb_over_idx = 5;        // d_scal index of the last build's list-overflow flag
+    int nb_parity = 0;          // which of d_scal[6], d_scal[7] the next brute-force build raises
     double nb_skin = 0;
     DBuf<int> d_nb_j, d_nb_img;
     // plain port scratch
@@ -740,7 +743,7 @@ int nb_slots(ace_ctx* ctx) { return ctx->slots_cap; }
 int nb_verify(ace_ctx* ctx) {
     if (!ctx->nb_pending) return 0;
     int over = 0;
-    HC(hipMemcpyAsync(&over, ctx->d_scal.p + 5, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipMemcpyAsync(&over, ctx->d_scal.p + ctx->nb_over_idx, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HC(hipStreamSynchronize(ctx->stream));
     ctx->nb_pending = false;
     if (over) {
@@ -1218,6 +1221,7 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
     ctx->tr.g_valid = false;
     ctx->sinfo_h.resize(ctx->S);
     int cell_off = 0;
+    bool brute = true;
     for (int s = 0; s < ctx->S; ++s) {
         StrucInfo& I = ctx->sinfo_h[s];
         memset(&I, 0, sizeof(I));
@@ -1261,6 +1265,9 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
             I.nc[d] -= 1;
         }
         for (int d = 0; d < 3; ++d) I.st[d] = std::max(1, (int)std::ceil(rsearch * I.nc[d] / hgt[d]));
+        // every structure small and with heights over twice the list radius: search without binning
+        if (ctx->natoms[s] > NB_BRUTE_MAX || !(hgt[0] > 2 * rsearch && hgt[1] > 2 * rsearch && hgt[2] > 2 * rsearch))
+            brute = false;
         const int ncell = I.nc[0] * I.nc[1] * I.nc[2];
         I.atom_off = ctx->atom_off[s];
         I.natoms = ctx->natoms[s];
@@ -1280,13 +1287,17 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
         ensure(ctx, ctx->d_cell_start, ncs) || ensure(ctx, ctx->d_cpos, N) || ensure(ctx, ctx->d_cinfo, N) ||
         ensure(ctx, ctx->d_nb_count, N + 1) || ensure(ctx, ctx->d_nb_off, N + 1))
         return -1;
-    if (timed(ctx, "nb_bin", [&] {
+    if (!brute && timed(ctx, "nb_bin", [&] {
             k_nb_bin<<<ctx->S, NB_BIN_THREADS, 0, ctx->stream>>>(ctx->d_pos.p, ctx->d_sinfo.p, ctx->d_cell_of.p,
                                                                  ctx->d_wrap_of.p, ctx->d_slot_of.p,
                                                                  ctx->d_cell_start.p, ctx->d_cpos.p, ctx->d_cinfo.p,
                                                                  ctx->d_scal.p);
         }))
         return -1;
+    ctx->nb_brute = brute;
+    // the brute-force build's overflow flag alternates between d_scal[6] and d_scal[7]: each build clears
+    // the other one for the next (the binning kernel clears all flags otherwise)
+    const int p_over = 6 + ctx->nb_parity, p_next = 7 - ctx->nb_parity;
     const double rc2 = rcl * rcl;
     // Capacity: the first build (or one after an overflow) counts first and reads the largest count back;
     // later builds reuse the key-row capacity (the LDS slots of the basis kernels, with a margin) and run
@@ -1294,11 +1305,17 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
     // call, and the basis kernels refuse it.
     const bool async = ctx->nb_async_ok && ctx->nb_same_layout;
     if (!async) {
+        if (brute) HC(hipMemsetAsync(ctx->d_scal.p, 0, 16 * sizeof(int), ctx->stream));
         if (timed(ctx, "nb_count", [&] {
-                k_nb_search<0><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
-                    N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
-                    ctx->d_cell_start.p, ctx->d_cpos.p, ctx->d_cinfo.p, ctx->d_nb_count.p, ctx->d_scal.p, nullptr, 0,
-                    nullptr);
+                if (brute)
+                    k_nb_brute<0><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
+                        N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_nb_count.p, ctx->d_scal.p,
+                        nullptr, 0, ctx->d_scal.p + p_over, ctx->d_scal.p + p_next, ctx->d_scal.p + 3);
+                else
+                    k_nb_search<0><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
+                        N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
+                        ctx->d_cell_start.p, ctx->d_cpos.p, ctx->d_cinfo.p, ctx->d_nb_count.p, ctx->d_scal.p, nullptr,
+                        0, nullptr);
             }))
             return -1;
         int mx = 0;
@@ -1309,12 +1326,20 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
     }
     if (ensure(ctx, ctx->d_nb_keys, (size_t)N * ctx->slots_cap)) return -1;
     if (timed(ctx, "nb_search", [&] {
-            k_nb_search<1><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
-                N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
-                ctx->d_cell_start.p, ctx->d_cpos.p, ctx->d_cinfo.p, ctx->d_nb_count.p, ctx->d_scal.p,
-                ctx->d_nb_keys.p, ctx->slots_cap, ctx->d_scal.p + 5);
+            if (brute)
+                k_nb_brute<1><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
+                    N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_nb_count.p, ctx->d_scal.p,
+                    ctx->d_nb_keys.p, ctx->slots_cap, ctx->d_scal.p + p_over, ctx->d_scal.p + p_next,
+                    ctx->d_scal.p + 3);
+            else
+                k_nb_search<1><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
+                    N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
+                    ctx->d_cell_start.p, ctx->d_cpos.p, ctx->d_cinfo.p, ctx->d_nb_count.p, ctx->d_scal.p,
+                    ctx->d_nb_keys.p, ctx->slots_cap, ctx->d_scal.p + 5);
         }))
         return -1;
+    ctx->nb_over_idx = brute ? p_over : 5;
+    if (brute) ctx->nb_parity ^= 1;
     ctx->nb_valid = true;
     ctx->nb_pending = true;
     ctx->csr_valid = false;
@@ -1460,7 +1485,7 @@ int ace_md_run(ace_ctx* ctx, int precision, int nsteps, double dt, int rebuild_e
     HC(hipMemsetAsync(ctx->d_md_rec.p, 0, 2 * (size_t)rec_cap * sizeof(unsigned long long), ctx->stream));
     HC(hipMemsetAsync(ctx->d_md_ctr.p, 0, 2 * sizeof(unsigned int), ctx->stream));   // step counter, flags
     auto collect_flags = [&] {
-        return timed(ctx, "md_flags", [&] { k_md_flags<<<1, 64, 0, ctx->stream>>>(ctx->d_scal.p, ctx->d_md_ctr.p + 1); });
+        return timed(ctx, "md_flags", [&] { k_md_flags<<<1, 64, 0, ctx->stream>>>(ctx->d_scal.p, ctx->nb_over_idx, ctx->d_md_ctr.p + 1); });
     };
     // A rebuild that reuses the row capacity is checked before any step uses it (one host round trip per
     // rebuild); if an atom outgrew the rows, the list is built again with a counting pass and resized rows
@@ -1470,7 +1495,7 @@ int ace_md_run(ace_ctx* ctx, int precision, int nsteps, double dt, int rebuild_e
         if ((collect && collect_flags()) || ace_build_neighbors(ctx, skin)) return -1;
         if (!ctx->nb_pending) return 0;
         int over = 0;
-        HC(hipMemcpyAsync(&over, ctx->d_scal.p + 5, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HC(hipMemcpyAsync(&over, ctx->d_scal.p + ctx->nb_over_idx, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HC(hipStreamSynchronize(ctx->stream));
         ctx->nb_pending = false;
         if (!over) return 0;

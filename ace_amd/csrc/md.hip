@@ -72,8 +72,8 @@ __global__ __launch_bounds__(MD_THREADS) void k_md_kick(int N, int S, double hdt
     }
 }
 
-// The neighbour build clears the list-overflow flag (scal[5]) and the basis kernels' refusal flag
+// The neighbour build clears the list-overflow flag (scal[over_idx]) and the basis kernels' refusal flag
 // (scal[3]) for the next list; across the rebuilds of a run they are collected here (sticky[0] != 0).
-__global__ void k_md_flags(const int* __restrict__ scal, unsigned int* __restrict__ sticky) {
-    if (threadIdx.x == 0 && (scal[3] | scal[5])) atomicOr(sticky, 1u);
+__global__ void k_md_flags(const int* __restrict__ scal, int over_idx, unsigned int* __restrict__ sticky) {
+    if (threadIdx.x == 0 && (scal[3] | scal[over_idx])) atomicOr(sticky, 1u);
 }

@@ -378,6 +378,88 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
     }
 }
 
+// Small periodic structures whose perpendicular heights all exceed twice the list radius: every atom of
+// the structure is a candidate, under the minimum image, and no binning kernel runs.  At most one image
+// of j lies within the radius (two images differ by a lattice vector longer than twice it), and each
+// component of its fractional offset from r_i - r_j is below 1/2, so rounding the fractional difference
+// finds it; the image range, the distance test and the key order are those of k_nb_search.  The build's
+// list-overflow flag is *over; the kernel clears *over_next (the flag of the next build) and the basis
+// kernels' refusal flag, which the binning kernel clears otherwise.
+#define NB_BRUTE_MAX 4096   // atoms per structure for the brute-force search
+template <int MODE>
+__global__ __launch_bounds__(NB_WAVES * 64) void k_nb_brute(
+    int natoms, double rc2, const double* __restrict__ pos, const int* __restrict__ struc_of,
+    const StrucInfo* __restrict__ sinfo, int* __restrict__ counts, int* __restrict__ maxcount,
+    long long* __restrict__ keys, int cap, int* __restrict__ over, int* __restrict__ over_next,
+    int* __restrict__ basis_err) {
+    __shared__ int s_n;
+    __shared__ long long s_key[MODE ? NB_LMAX : 1];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int i = blockIdx.x;
+    if (i == 0 && tid == 0) {
+        *over_next = 0;
+        *basis_err = 0;
+    }
+    const StrucInfo& S = sinfo[struc_of[i]];
+    const int a0 = S.atom_off, na = S.natoms;
+    const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
+    if (tid == 0) s_n = 0;
+    __syncthreads();
+    for (int cb = 0; cb < na; cb += NB_WAVES * 64) {
+        const int x = cb + tid;   // candidate j = a0 + x
+        bool acc = false;
+        long long key = 0;
+        if (x < na) {
+            const int j = a0 + x;
+            double dx, dy, dz;
+            {
+#pragma clang fp contract(off)
+                dx = pix - pos[3 * j];
+                dy = piy - pos[3 * j + 1];
+                dz = piz - pos[3 * j + 2];
+            }
+            const int n0 = (int)rint(S.Rinv[0] * dx + S.Rinv[3] * dy + S.Rinv[6] * dz);
+            const int n1 = (int)rint(S.Rinv[1] * dx + S.Rinv[4] * dy + S.Rinv[7] * dz);
+            const int n2 = (int)rint(S.Rinv[2] * dx + S.Rinv[5] * dy + S.Rinv[8] * dz);
+            if (n0 >= -S.cf[0] && n0 <= S.cf[0] && n1 >= -S.cf[1] && n1 <= S.cf[1] && n2 >= -S.cf[2] &&
+                n2 <= S.cf[2]) {
+                const double ds2 = ref_ds2(S.R, dx, dy, dz, n0, n1, n2);
+                if ((j != i || ds2 > 1e-12) && ds2 < rc2) {
+                    acc = true;
+                    key = nb_key(x, n0, n1, n2);
+                }
+            }
+        }
+        const unsigned long long m = __ballot(acc);
+        int base = 0;
+        if (lane == 0 && m) base = atomicAdd(&s_n, __popcll(m));
+        base = __shfl(base, 0, 64);
+        if (MODE == 1 && acc) {
+            const int r = base + __popcll(m & lane_mask_lt());
+            if (r < NB_LMAX) s_key[r] = key;
+        }
+    }
+    __syncthreads();
+    const int count = s_n;
+    if (tid == 0) {
+        counts[i] = count;
+        if (MODE == 0) atomicMax(maxcount, count);
+    }
+    if (MODE == 0) return;
+    if (count > cap || count > NB_LMAX) {
+        if (tid == 0) atomicMax(over, 1);
+        return;
+    }
+    // rank sort into the reference order (the keys of one atom are distinct)
+    long long* row = keys + (long long)i * cap;
+    for (int e = tid; e < count; e += NB_WAVES * 64) {
+        const long long k = s_key[e];
+        int rank = 0;
+        for (int f = 0; f < count; ++f) rank += (s_key[f] < k);
+        __builtin_nontemporal_store(k, &row[rank]);
+    }
+}
+
 // Wave per atom: CSR arrays (neighbour index and packed image, the layout of NeighborList's accessors,
 // neighbor.hpp:34-40) from the sorted key rows, for export and for the unbatched port.
 __global__ __launch_bounds__(256) void k_nb_csr(int natoms, const int* __restrict__ struc_of,

@@ -13,8 +13,10 @@
 // Pipeline (one build): k_nb_bin (workgroup per structure: cell counts, scan and a cell-ordered copy of
 // the positions, all in LDS) -> k_nb_search (workgroup per atom; candidates are contiguous runs of the
 // cell-ordered copy, loaded four chunks at a time; keys rank-sorted in LDS into the reference order and
-// written to a fixed-capacity per-atom row, which the basis kernels read directly).  CSR arrays are
-// built from the rows on demand (export, unbatched port): exclusive scan of the counts -> k_nb_csr.
+// written to a fixed-capacity per-atom row, which the basis kernels read directly).  When every structure
+// is small and its cell heights exceed twice the list radius, k_nb_brute replaces both (no cell list).
+// CSR arrays are built from the rows on demand (export, unbatched port): exclusive scan of the counts ->
+// k_nb_csr.
 #include "ace_common.hpp"
 
 // ---- binning: one workgroup per structure ---------------------------------------------------------

@@ -1,4 +1,9 @@
-"""Build recipe for libace_hip.so (gfx950) and the oracle's liboracle.so."""
+"""Build recipe for libace_hip.so (gfx950) and the oracle's liboracle.so.
+
+The library is built by ace_amd/Makefile: csrc/api.hip plus one object per basis-kernel instantiation set
+(csrc/basis_inst.hip), compiled in parallel into ace_amd/_build/.  build_library(force=True) rebuilds every
+object; force=False rebuilds what changed.
+"""
 from __future__ import annotations
 
 import os
@@ -8,28 +13,26 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
-    # no packed FP32 (v_pk_*_f32) code generation
-    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-]
 
-
-def _run(cmd):
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:
         raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
     return out
 
 
-def build_library(out=None):
-    out = out or os.path.join(HERE, "libace_hip.so")
-    src = os.path.join(HERE, "csrc", "api.hip")
-    tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-o", tmp, src])
-    os.replace(tmp, out)
-    return out
+def _jobs():
+    n = os.cpu_count() or 4
+    return str(max(1, min(n, 16)))
+
+
+def build_library(force=True):
+    cmd = ["make", "-s", "-j", _jobs(), f"HIPCC={HIPCC}"]
+    if force:
+        cmd.insert(1, "-B")
+    _run(cmd, cwd=HERE)
+    return os.path.join(HERE, "libace_hip.so")
 
 
 def build_oracle():
@@ -39,5 +42,6 @@ def build_oracle():
 
 
 if __name__ == "__main__":
-    print(build_library())
+    import sys
+    print(build_library(force="-B" in sys.argv))
     print(build_oracle())

@@ -1,6 +1,7 @@
 // api.hip -- the C-ABI (include/ace_hip.h): context, model tables, structure upload, neighbour-list
 // build, batched energy/force pass, the plain FP64 port, and per-kernel HIP-event timing.
-// One translation unit: the kernel sources are included below.
+// The kernel sources are included below; the basis kernels are instantiated in separate translation
+// units (basis_inst.hip) and reached through basis_table.hpp.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
@@ -13,6 +14,7 @@
 #include "ace_common.hpp"
 #include "neighbors.hip"
 #include "basis.hip"
+#include "basis_table.hpp"
 #include "mlp.hip"
 #include "reduce.hip"
 #include "plain.hip"
@@ -693,49 +695,9 @@ DevBasisParams basis_params(ace_ctx* ctx) {
 
 // ---- basis kernel dispatch ------------------------------------------------------------------------
 
-typedef void (*BasisFn)(int, const DevModel*, DevBasisParams, BasisIO, int, int, int);
-
-// f[0..3]: modes 0-3 with four waves per atom; f[4..7]: modes 0-3 with two
-template <class T, int NG, int NZ, int GEN>
-void pick_basis_gen(BasisFn* f) {
-    f[0] = k_basis<T, NG, NZ, 0, GEN>;
-    f[1] = k_basis<T, NG, NZ, 1, GEN>;
-    f[2] = k_basis<T, NG, NZ, 2, GEN>;
-    f[3] = k_basis<T, NG, NZ, 3, GEN>;
-    f[4] = k_basis<T, NG, NZ, 0, GEN, 2>;
-    f[5] = k_basis<T, NG, NZ, 1, GEN, 2>;
-    f[6] = k_basis<T, NG, NZ, 2, GEN, 2>;
-    f[7] = k_basis<T, NG, NZ, 3, GEN, 2>;
-}
-
-template <class T, int NG, int NZ>
-void pick_basis(BasisFn* f, bool gen) {
-    if (gen) pick_basis_gen<T, NG, NZ, 1>(f);
-    else pick_basis_gen<T, NG, NZ, 0>(f);
-}
-
-template <class T, int NG>
-void pick_basis_nz(BasisFn* f, bool gen, int nz) {
-    if (nz <= 4) pick_basis<T, NG, 4>(f, gen);
-    else pick_basis<T, NG, 7>(f, gen);
-}
-
 // the canonical angular accumulators are sized by the largest number of eta groups (1, 2 or 4) and of
-// power levels (4 or 7); the generic per-function path is compiled in only when some angular block is
-// not canonical
-void basis_fns(ace_ctx* ctx, bool f64, BasisFn* f) {
-    const bool gen = !ctx->all_canon;
-    const int nz = ctx->nzmax;
-    if (f64) {
-        if (ctx->ngk == 1) pick_basis_nz<double, 1>(f, gen, nz);
-        else if (ctx->ngk == 2) pick_basis_nz<double, 2>(f, gen, nz);
-        else pick_basis_nz<double, 4>(f, gen, nz);
-    } else {
-        if (ctx->ngk == 1) pick_basis_nz<float, 1>(f, gen, nz);
-        else if (ctx->ngk == 2) pick_basis_nz<float, 2>(f, gen, nz);
-        else pick_basis_nz<float, 4>(f, gen, nz);
-    }
-}
+// power levels; the generic per-function path is compiled in only when some angular block is not canonical
+void basis_fns(ace_ctx* ctx, bool f64, BasisFn* f) { basis_fill(f64, ctx->ngk, ctx->nzmax, !ctx->all_canon, f); }
 
 int nb_slots(ace_ctx* ctx) { return ctx->slots_cap; }
 

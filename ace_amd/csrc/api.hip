@@ -57,6 +57,7 @@ struct ace_ctx {
     double rc = 0;
     std::vector<HSpecies> sp;
     int kmax = 0, ldg = 0, maxA = 0, maxR = 0, ngmax = 1, ngk = 1, nzmax = 1;
+    int nrp = 1;                     // most radial functions of one species
     bool all_canon = true;
     DevModel hdm;
     DBuf<DevModel> d_model;
@@ -552,6 +553,8 @@ int build_tables(ace_ctx* ctx) {
     ctx->ldg = (kmax + 3) & ~3;
     ctx->maxA = maxA;
     ctx->maxR = maxR;
+    ctx->nrp = 1;
+    for (int I = 0; I < nt; ++I) ctx->nrp = std::max(ctx->nrp, ctx->sp[I].nR);
     ctx->ngmax = ngmax;
     if (upload(ctx, ctx->d_model, &dm, 1)) return -1;
     if (upload(ctx, ctx->d_r_rs, r_rs.data(), r_rs.size())) return -1;
@@ -755,10 +758,10 @@ int net_lds_floats(const ace_ctx* ctx) {
 int basis_bytes(ace_ctx* ctx, int mode, int waves = 4) {
     const int nbm = nb_slots(ctx), netf = mode == 3 ? net_lds_floats(ctx) : 0;
     if (waves == 2)
-        return ctx->basis_f64 ? basis_lds_bytes<double, 2>(nbm, ctx->ngk, ctx->kmax, mode, netf, ctx->nt)
-                              : basis_lds_bytes<float, 2>(nbm, ctx->ngk, ctx->kmax, mode, netf, ctx->nt);
-    return ctx->basis_f64 ? basis_lds_bytes<double, 4>(nbm, ctx->ngk, ctx->kmax, mode, netf, ctx->nt)
-                          : basis_lds_bytes<float, 4>(nbm, ctx->ngk, ctx->kmax, mode, netf, ctx->nt);
+        return ctx->basis_f64 ? basis_lds_bytes<double, 2>(nbm, ctx->ngk, ctx->kmax, ctx->nrp, mode, netf, ctx->nt)
+                              : basis_lds_bytes<float, 2>(nbm, ctx->ngk, ctx->kmax, ctx->nrp, mode, netf, ctx->nt);
+    return ctx->basis_f64 ? basis_lds_bytes<double, 4>(nbm, ctx->ngk, ctx->kmax, ctx->nrp, mode, netf, ctx->nt)
+                          : basis_lds_bytes<float, 4>(nbm, ctx->ngk, ctx->kmax, ctx->nrp, mode, netf, ctx->nt);
 }
 
 // The fused kernel (symmetry functions, network, adjoint of one atom per workgroup) saves two launches and
@@ -848,6 +851,7 @@ int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = 
     io.keys = ctx->d_nb_keys.p;
     io.counts = ctx->d_nb_count.p;
     io.cap = ctx->slots_cap;
+    io.nrp = ctx->nrp;
     io.row_of = ctx->d_row_of.p;
     io.ldg = ctx->ldg;
     io.G = rows;

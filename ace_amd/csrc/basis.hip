@@ -25,33 +25,46 @@
 // gradient modes, which lets more atoms share a CU where LDS limits residency.
 
 // Modes: 0 = network input rows inpw*(G + inpb); 1 = adjoint (per-pair dE/dd from dE/dG rows);
-// 2 = tangent rows v = sum_a eps_a . dG/dr_a.
-// Dynamic LDS per workgroup (nbm = neighbour slots, kp = kmax rounded up to 4), in units of T:
-//   u[4 nbm], te[4 nbm] (mode 2), E[nbm ng], Ed[nbm ng] (modes 1-3), fR[nbm], gR[nbm], ri[nbm], fr[nbm] (1, 3),
-//   row[kp], red[nred], rp[2 kp] (radial eta, rs of the species), sc[2 kp] (modes 0, 3: inpw, inpb);
-//   then p[nbm] and ci[kp] (int).  Modes 1 and 3 put g[BW 3 nbm] (double) in front; mode 3 appends the
-//   network area (netf floats: staged weights, activations, partial sums), 16-byte aligned.
-// red holds the per-wave partial sums (BW 64) and, in the adjoint, the dE/dG weights of every canonical
-// angular block in (sign, level, group) order (14 ng per TensorTri block).
+// 2 = tangent rows v = sum_a eps_a . dG/dr_a; 3 = fused (0, the network, then 1).
+// Dynamic LDS per workgroup (nbm = neighbour slots, kp = kmax rounded up to 4, nrp = the most radial
+// functions of a species):
+//   g[BW 3 nbm] (FP64, modes 1 and 3: per-wave dE/dd accumulators; LDS FP64 atomic adds run several
+//         times faster than FP32 ones on gfx950);
+//   FP64: row[kp] (G sums, then dE/dG), red[nred], rp[2 nrp] (radial eta, rs of the species),
+//         sc[2 kp] (mode 3: inpw, inpb);
+//   T: u[4 nbm], te[4 nbm] (mode 2), E[nbm ng], Ed[nbm ng] (modes 1-3), fR[nbm], gR[nbm], ri[nbm],
+//      fr[nbm] (modes 1 and 3);
+//   int: p[nbm], ci[kp];  float: rlo[nbm] (T = float);  mode 3 appends the network area (netf floats:
+//   staged weights, activations, partial sums), 16-byte aligned.
+// red holds the per-wave partial sums (BW 64, FP64) and, in the adjoint, the dE/dG weights of every
+// canonical angular block in (sign, level, group) order (14 ng FP32 values per TensorTri block).
+// Each lane sums its share of a symmetry function in T, and so does the wave reduction; the sum over the
+// waves is FP64, and so is the G row, which reaches the input scaling inpw (G + inpb) (also FP64) without the
+// FP32 rounding that the mean removal would magnify by mean / deviation.
 template <int BW>
-__host__ __device__ inline int basis_nred(int nt, int ng) { return max(BW * 64, nt * (nt + 1) / 2 * 14 * ng); }
+__host__ __device__ inline int basis_nred(int nt, int ng, int mode) {
+    const int wt = (nt * (nt + 1) / 2 * 14 * ng + 1) / 2;   // FP32 weight table, in doubles
+    return mode == 1 ? wt : (mode == 3 ? max(BW * 64, wt) : BW * 64);
+}
 
 template <class T, int BW>
-__host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int mode, int netf, int nt) {
+__host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int nrp, int mode, int netf, int nt) {
     const int kp = (kmax + 3) & ~3;
-    int n = 4 * nbm + ng * nbm + 3 * nbm + kp + basis_nred<BW>(nt, ng) + 2 * kp;
-    if (mode == 0 || mode == 3) n += 2 * kp;
+    const bool gacc = (mode == 1 || mode == 3);
+    const int gb = gacc ? 8 * BW * 3 * nbm : 0;
+    const int nd = kp + basis_nred<BW>(nt, ng, mode) + 2 * nrp + (mode == 3 ? 2 * kp : 0);
+    int n = 4 * nbm + ng * nbm + 3 * nbm;
     if (mode == 2) n += 4 * nbm;
     if (mode >= 1) n += ng * nbm;
-    if (mode == 1 || mode == 3) n += nbm;
-    const int gb = (mode == 1 || mode == 3) ? 8 * BW * 3 * nbm : 0;
-    const int base = (gb + (int)sizeof(T) * n + 4 * (nbm + kp) + 15) & ~15;
+    if (gacc) n += nbm;
+    const int nf = sizeof(T) == 4 ? nbm : 0;
+    const int base = (gb + 8 * nd + (int)sizeof(T) * n + 4 * (nbm + kp) + 4 * nf + 15) & ~15;
     return base + (mode == 3 ? 4 * netf : 0);
 }
 
 template <class T>
 struct BasisLds {
-    T* u;      // [nbm][4]
+    T* u;      // [nbm][4]: unit vector d / |d| and |d| (FP64 values rounded once)
     T* te;     // [nbm][4]
     T* E;      // [nbm][ng]: fc exp(-r^2 / eta_g^2) of each slot and eta group
     T* Ed;     // [nbm][ng]: its r derivative
@@ -59,21 +72,29 @@ struct BasisLds {
     T* gR;
     T* ri;     // 1 / r
     T* fr;     // modes 1, 3: radial dE/dr of each slot
-    double* g; // [BW][3 nbm] per-wave dE/dd accumulators (FP64: LDS FP64 atomics, exact-order-free)
-    T* row;
-    T* red;    // [BW][64] partial sums; adjoint: canonical dE/dG weights [block][14 ng]
-    T* rp;     // radial eta [nR], rs [nR] of the centre species
-    T* sc;     // network input scaling inpw [K], inpb [K] (modes 0, 3)
+    double* g; // [BW][3 nbm] per-wave dE/dd accumulators (one array per wave: a fixed order of adds)
+    double* row;
+    double* red;   // [BW][64] partial sums; adjoint: canonical dE/dG weights [block][14 ng] as FP32
+    double* rp;    // radial eta [nR], rs [nR] of the centre species
+    double* sc;    // mode 3: network input scaling inpw [K], inpb [K]
     int* p;
     int* ci;   // canonical slot of each angular function of the species
+    float* rlo;    // T = float: |d| - (float)|d|, so the radial functions see |d| to FP64 precision
     float* nw; // mode 3: the species' network parameters (matrix-core layout), then activations
 };
 
 template <class T, int BW>
-__device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, int mode, int nt) {
+__device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, int nrp, int mode, int nt) {
     BasisLds<T> L;
+    const int kp = (kmax + 3) & ~3;
+    const bool gacc = (mode == 1 || mode == 3);
     L.g = (double*)base;
-    T* c = (T*)((char*)base + ((mode == 1 || mode == 3) ? 8 * BW * 3 * nbm : 0));
+    double* d = (double*)((char*)base + (gacc ? 8 * BW * 3 * nbm : 0));
+    L.row = d; d += kp;
+    L.red = d; d += basis_nred<BW>(nt, ng, mode);
+    L.rp = d; d += 2 * nrp;
+    L.sc = d; if (mode == 3) d += 2 * kp;
+    T* c = (T*)d;
     L.u = c; c += 4 * nbm;
     L.te = c; if (mode == 2) c += 4 * nbm;
     L.E = c; c += ng * nbm;
@@ -81,17 +102,21 @@ __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, i
     L.fR = c; c += nbm;
     L.gR = c; c += nbm;
     L.ri = c; c += nbm;
-    L.fr = c; if (mode == 1 || mode == 3) c += nbm;
-    L.row = c; c += (kmax + 3) & ~3;
-    L.red = c; c += basis_nred<BW>(nt, ng);
-    const int kp = (kmax + 3) & ~3;
-    L.rp = c; c += 2 * kp;
-    L.sc = c; if (mode == 0 || mode == 3) c += 2 * kp;
+    L.fr = c; if (gacc) c += nbm;
     L.p = (int*)c;
     L.ci = L.p + nbm;
-    const long long end = ((long long)((char*)(L.ci + kp) - (char*)base) + 15) & ~15ll;
+    L.rlo = (float*)(L.ci + kp);
+    float* e = L.rlo + (sizeof(T) == 4 ? nbm : 0);
+    const long long end = ((long long)((char*)e - (char*)base) + 15) & ~15ll;
     L.nw = (float*)((char*)base + end);
     return L;
+}
+
+// |d| of slot q to FP64 precision
+template <class T>
+__device__ inline double slot_r64(const BasisLds<T>& L, int q) {
+    if constexpr (sizeof(T) == 8) return (double)L.u[4 * q + 3];
+    else return (double)L.u[4 * q + 3] + (double)L.rlo[q];
 }
 
 // (a, c) of flattened triplet t in block (J,K): rectangle a in J, c in K; triangle a<c in J.
@@ -148,14 +173,17 @@ __device__ inline T prm(const float* f, const double* d, int i) {
 // held in registers with static indices; cidx maps each function of the block to (s*7 + k)*NG + g.
 // NZ (a compile-time bound on the levels in use) keeps the loop free of branches; unused levels are
 // computed and discarded.  E / Ed are interleaved per slot: E[slot * NG + g], zero for g >= ng.
+// Each lane's sums (modes 0 and 2) and their wave reduce-scatter are T; the sum over the waves is FP64.
+// The adjoint (mode 1) adds each triplet's derivative into the wave's FP64 accumulators.
 template <class T, int NG, int NZ, int MODE, int BW>
 __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int nJ, int nK, bool same, int T3,
-                                     const int* __restrict__ cidx, const T* wtab, int nA, int goff, double* gw,
+                                     const int* __restrict__ cidx, const float* wtab, int nA, int goff, double* gw,
                                      int tid, int wv, int lane) {
     constexpr int ADJ = (MODE == 1);
     constexpr int TAN = (MODE == 2);
     constexpr int BT = BW * 64;
-    T acc[2][NZ][NG], wgt[2][NZ][NG];
+    T acc[2][NZ][NG];
+    T wgt[2][NZ][NG];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -172,7 +200,7 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
 #pragma unroll
             for (int k = 0; k < NZ; ++k)
 #pragma unroll
-                for (int g = 0; g < NG; ++g) wgt[s][k][g] = wtab[(s * 7 + k) * NG + g];
+                for (int g = 0; g < NG; ++g) wgt[s][k][g] = (T)wtab[(s * 7 + k) * NG + g];
     }
     // (row d, column x) of triplet t = d m + x, advanced by BT per iteration from one exact decode
     const int m = same ? nJ : min(nJ, nK), Mx = same ? nJ : max(nJ, nK);
@@ -293,7 +321,7 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
                 for (int g = 0; g < NG; ++g) v[(s * NZ + k) * NG + g] = acc[s][k][g];
 #pragma unroll
         for (int f = M0; f < (1 << LM); ++f) v[f] = T(0);
-        const T tot = wave_reduce_scatter<LM, T>(v, lane);
+        const double tot = (double)wave_reduce_scatter<LM, T>(v, lane);
         const int f = lane >> (6 - LM);
         if ((lane & ((1 << (6 - LM)) - 1)) == 0 && f < M0) {
             const int g = f % NG, k = (f / NG) % NZ, s = f / (NG * NZ);
@@ -301,7 +329,7 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
         }
         __syncthreads();
         for (int m = tid; m < nA; m += BT) {
-            T v = T(0);
+            double v = 0.0;
             for (int w = 0; w < BW; ++w) v += L.red[w * 64 + cidx[m]];
             L.row[goff + m] = v;
         }
@@ -336,9 +364,9 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
             const int n = lane % nrp, grp = wv * gpw + lane / nrp;
             const bool on = n < nc;
             const int rq = poff - sp.r_poff[0] + n0 + n;
-            const T eta = on ? L.rp[rq] : T(0);
-            const T rs = on ? L.rp[sp.nR + rq] : T(0);
-            const T w = (ADJ && on) ? L.row[goff + n0 + n] : T(0);
+            const double eta = on ? L.rp[rq] : 0.0;
+            const double rs = on ? L.rp[sp.nR + rq] : 0.0;
+            const T w = (ADJ && on) ? (T)L.row[goff + n0 + n] : T(0);
             T acc = T(0);
             const int rounds = (nJ + ngr - 1) / ngr;
             for (int t = 0; t < rounds; ++t) {
@@ -347,7 +375,7 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
                 if (on && q < nJ) {
                     const int slot = bJ + q;
                     T v, dv;
-                    t_radial<T>(form, eta, rs, L.u[4 * slot + 3], L.fR[slot], L.gR[slot], v, dv);
+                    t_radial_r64<T>(form, eta, rs, slot_r64(L, slot), L.fR[slot], L.gR[slot], v, dv);
                     val = ADJ ? w * dv : (TAN ? dv * L.te[4 * slot + 3] : v);
                 }
                 if (ADJ) {
@@ -360,10 +388,10 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
             }
             if (ADJ && nR > 64) __syncthreads();   // the next chunk maps slots to other lane groups
             if (!ADJ) {
-                L.red[wv * 64 + lane] = acc;   // lane n + nrp * group of each wave
+                L.red[wv * 64 + lane] = (double)acc;   // lane n + nrp * group of each wave
                 __syncthreads();
                 if (tid < nc) {
-                    T s = T(0);
+                    double s = 0.0;
                     for (int ww = 0; ww < BW; ++ww)
                         for (int g = tid; g < 64; g += nrp) s += L.red[ww * 64 + g];
                     L.row[goff + n0 + tid] = s;
@@ -376,12 +404,13 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
         // dE/dG of every canonical block scattered to its (sign, level, group) table in red: two barriers
         // for all blocks
         const int nab = nt * (nt + 1) / 2;
-        for (int q = tid; q < nab * 14 * NG; q += BT) L.red[q] = T(0);
+        float* wt = (float*)L.red;
+        for (int q = tid; q < nab * 14 * NG; q += BT) wt[q] = 0.f;
         __syncthreads();
         for (int b = 0; b < nab; ++b)
             if (!GEN || sp.a_canon[b]) {
                 const int o = sp.a_poff[b] - sp.a_poff[0];
-                for (int m = tid; m < sp.a_n[b]; m += BT) L.red[14 * NG * b + L.ci[o + m]] = L.row[sp.a_goff[b] + m];
+                for (int m = tid; m < sp.a_n[b]; m += BT) wt[14 * NG * b + L.ci[o + m]] = (float)L.row[sp.a_goff[b] + m];
             }
         __syncthreads();
     }
@@ -399,7 +428,7 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
             if (nA == 0 || T3 == 0) continue;
             if (!GEN || sp.a_canon[b]) {
                 angular_canon<T, NG, NZ, MODE, BW>(L, bJ, bK, nJ, nK, same, T3, L.ci + (sp.a_poff[b] - sp.a_poff[0]),
-                                               L.red + 14 * NG * b, nA, sp.a_goff[b], gw, tid, wv, lane);
+                                               (const float*)L.red + 14 * NG * b, nA, sp.a_goff[b], gw, tid, wv, lane);
                 continue;
             }
             const int goff = sp.a_goff[b], poff = sp.a_poff[b];
@@ -418,7 +447,7 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
                     grp[m] = on ? P.a_grp[fi] : 0;
                     sel[m] = on ? P.a_sel[fi] : 0;
                     acc[m] = T(0);
-                    w[m] = (ADJ && on) ? L.row[goff + m0 + m] : T(0);
+                    w[m] = (ADJ && on) ? (T)L.row[goff + m0 + m] : T(0);
                     zmax = max(zmax, zint[m]);
                 }
                 int nk = 1;
@@ -521,13 +550,13 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
 #pragma unroll
                     for (int m = 0; m < MAXA; ++m) {
                         if (m0 + m < nA) {
-                            const T t = t_wave_sum<T>(acc[m]);
+                            const double t = (double)t_wave_sum<T>(acc[m]);
                             if (lane == 0) L.red[wv * 64 + m] = t;
                         }
                     }
                     __syncthreads();
                     if (tid < MAXA && m0 + tid < nA) {
-                        T s = T(0);
+                        double s = 0.0;
                         for (int ww = 0; ww < BW; ++ww) s += L.red[ww * 64 + tid];
                         L.row[goff + m0 + tid] = s;
                     }
@@ -583,7 +612,7 @@ __device__ inline void block_network(const BasisLds<T>& L, const DevSpecies& sp,
         const int K = sp.K, NL = sp.L;
         const float* W = L.nw;
         float* act = L.nw + sp.wp_n;
-        for (int k = lane; k < K; k += 64) L.row[k] = L.sc[k] * (L.row[k] + L.sc[K + k]);   // inpw (G + inpb)
+        for (int k = lane; k < K; k += 64) L.row[k] = L.sc[k] * (L.row[k] + L.sc[K + k]);   // inpw (G + inpb), FP64
         wave_sync();
         // forward
         int ain = -1, aoff = 0;
@@ -638,7 +667,7 @@ __device__ inline void block_network(const BasisLds<T>& L, const DevSpecies& sp,
                 part = t_group_sum<float>(part, G);
                 if (r == 0 && on) {
                     if (l > 0) act[dm + k] = part * act[dm + wpi + k];
-                    else L.row[k] = (T)part * L.sc[k];
+                    else L.row[k] = (double)part * L.sc[k];
                 }
             }
             wave_sync();
@@ -667,6 +696,7 @@ struct BasisIO {
     const long long* keys;
     const int* counts;
     int cap;
+    int nrp;            // most radial functions of a species (LDS table size)
     const int* row_of;
     int ldg;
     float* G;
@@ -713,7 +743,7 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
     }
     const int I = io.type[i];
     const DevSpecies& gsp = M->sp[I];
-    const BasisLds<T> L = basis_lds<T, BW>(basis_smem, nbm, NG, kmax, MODE, nt);
+    const BasisLds<T> L = basis_lds<T, BW>(basis_smem, nbm, NG, kmax, io.nrp, MODE, nt);
     const int nn = io.counts[i];
     const long long* krow = io.keys + (long long)i * io.cap;
     const float rcut = gsp.rc_max;
@@ -734,19 +764,19 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
     if (wv == W_RAD) {
         const int nR = gsp.nR, r0 = gsp.r_poff[0];
         for (int q = lane; q < nR; q += 64) {
-            L.rp[q] = prm<T>(P.r_eta, P.r_eta64, r0 + q);
-            L.rp[nR + q] = prm<T>(P.r_rs, P.r_rs64, r0 + q);
+            L.rp[q] = P.r_eta64[r0 + q];
+            L.rp[nR + q] = P.r_rs64[r0 + q];
         }
         if (ADJ)
-            for (int k = lane; k < gsp.K; k += 64) L.row[k] = (T)io.dEdG[(long long)row * io.ldg + k];
+            for (int k = lane; k < gsp.K; k += 64) L.row[k] = (double)io.dEdG[(long long)row * io.ldg + k];
     }
     if (wv == W_ANG) {
         const int nA = gsp.K - gsp.nR, a0 = gsp.a_poff[0];
         for (int q = lane; q < nA; q += 64) L.ci[q] = P.a_cidx[a0 + q];
-        if (MODE == 0 || FUSED)
+        if (FUSED)
             for (int k = lane; k < gsp.K; k += 64) {
-                L.sc[k] = prm<T>(P.inpw, P.inpw64, gsp.in_off + k);
-                L.sc[gsp.K + k] = prm<T>(P.inpb, P.inpb64, gsp.in_off + k);
+                L.sc[k] = P.inpw64[gsp.in_off + k];
+                L.sc[gsp.K + k] = P.inpb64[gsp.in_off + k];
             }
     }
     if (FUSED && wv > 0) {
@@ -796,12 +826,16 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
                 if (J < nt) {
                     const unsigned long long m = __ballot(Jl == J);
                     if (Jl == J) {
+                        // unit vector, |d| and 1/|d| from the FP64 displacement, each rounded once
                         const int q = fill[J] + __popcll(m & lane_mask_lt());
+                        const double ir = 1.0 / r;
                         L.p[q] = j;
-                        L.u[4 * q] = (T)dx;
-                        L.u[4 * q + 1] = (T)dy;
-                        L.u[4 * q + 2] = (T)dz;
+                        L.u[4 * q] = (T)(dx * ir);
+                        L.u[4 * q + 1] = (T)(dy * ir);
+                        L.u[4 * q + 2] = (T)(dz * ir);
                         L.u[4 * q + 3] = (T)r;
+                        L.ri[q] = (T)ir;
+                        if (!F64) L.rlo[q] = (float)(r - (double)(float)r);
                     }
                     fill[J] += __popcll(m);
                 }
@@ -864,25 +898,23 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
 #pragma unroll
         for (int jj = 1; jj < ACE_MAXT; ++jj)
             if (jj < nt && q >= s_cnt[jj]) J = jj;
-        const T dx = L.u[4 * q], dy = L.u[4 * q + 1], dz = L.u[4 * q + 2], r = L.u[4 * q + 3];
-        const T ir = T(1) / r;
-        L.u[4 * q] = dx * ir;
-        L.u[4 * q + 1] = dy * ir;
-        L.u[4 * q + 2] = dz * ir;
-        L.ri[q] = ir;
+        // cutoffs from |d| to FP64 precision; the separable angular factors in T (their exponents are below
+        // ~1 in magnitude)
+        const double r = slot_r64(L, q);
         T fc, gc;
-        t_cutoff<T>(sp.r_cut[J], F64 ? (T)sp.r_rc64[J] : (T)sp.r_rc[J], r, fc, gc);
+        t_cutoff_r64<T>(sp.r_cut[J], sp.r_rc64[J], r, fc, gc);
         L.fR[q] = fc;
         L.gR[q] = gc;
-        t_cutoff<T>(sp.a_cut, F64 ? (T)sp.a_rc64 : (T)sp.a_rc, r, fc, gc);
+        if (sp.a_cut != sp.r_cut[J] || sp.a_rc64 != sp.r_rc64[J]) t_cutoff_r64<T>(sp.a_cut, sp.a_rc64, r, fc, gc);
+        const T fa = fc, ga = gc, rt = L.u[4 * q + 3];
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             T ev = T(0), edv = T(0);
             if (g < ng) {
                 const T ie = F64 ? (T)sp.ietar2_64[g] : (T)sp.ietar2[g];
-                const T e = t_exp<T>(-r * r * ie);
-                ev = fc * e;
-                edv = (gc - T(2) * r * ie * fc) * e;
+                const T e = t_exp<T>(-rt * rt * ie);
+                ev = fa * e;
+                edv = (ga - T(2) * rt * ie * fa) * e;
             }
             L.E[q * NG + g] = ev;
             if (MODE >= 1) L.Ed[q * NG + g] = edv;
@@ -903,11 +935,11 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
             L.te[4 * q] = ex;
             L.te[4 * q + 1] = ey;
             L.te[4 * q + 2] = ez;
-            L.te[4 * q + 3] = (dx * ex + dy * ey + dz * ez) * ir;
+            L.te[4 * q + 3] = L.u[4 * q] * ex + L.u[4 * q + 1] * ey + L.u[4 * q + 2] * ez;
         }
     }
     if (!ADJ)
-        for (int k = tid; k < sp.K; k += BT) L.row[k] = T(0);
+        for (int k = tid; k < sp.K; k += BT) L.row[k] = 0.0;
     __syncthreads();
 
     if (FUSED) {
@@ -918,10 +950,10 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
         basis_sums<T, NG, NZ, MODE, GEN, BW>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
     }
     if (MODE == 0) {
-        // network input row: the ANN's input scaling inpw*(G + inpb) (nn.cpp:1406) applied here, in T,
+        // network input row: the ANN's input scaling inpw*(G + inpb) (nn.cpp:1406) applied here, in FP64,
         // so the FP32 row keeps full relative precision after the mean is removed
         for (int k = tid; k < sp.K; k += BT) {
-            const T w = L.sc[k], b = L.sc[sp.K + k];
+            const double w = P.inpw64[sp.in_off + k], b = P.inpb64[sp.in_off + k];
             io.G[(long long)row * io.ldg + k] = (float)(w * (L.row[k] + b));
         }
     } else if (TAN) {
@@ -937,15 +969,15 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
             const double ux = (double)L.u[4 * q], uy = (double)L.u[4 * q + 1], uz = (double)L.u[4 * q + 2];
             double g3[3] = {fr * ux, fr * uy, fr * uz};
             for (int ww = 0; ww < BW; ++ww) {
-                g3[0] += L.g[ww * 3 * nbm + 3 * q];
-                g3[1] += L.g[ww * 3 * nbm + 3 * q + 1];
-                g3[2] += L.g[ww * 3 * nbm + 3 * q + 2];
+                g3[0] += (double)L.g[ww * 3 * nbm + 3 * q];
+                g3[1] += (double)L.g[ww * 3 * nbm + 3 * q + 1];
+                g3[2] += (double)L.g[ww * 3 * nbm + 3 * q + 2];
             }
             const int j = L.p[q];
             atomicAdd(&io.F_fix[3 * j], fix_of(g3[0]));
             atomicAdd(&io.F_fix[3 * j + 1], fix_of(g3[1]));
             atomicAdd(&io.F_fix[3 * j + 2], fix_of(g3[2]));
-            const double r = 1.0 / (double)L.ri[q];
+            const double r = slot_r64(L, q);
             const double d3[3] = {ux * r, uy * r, uz * r};
 #pragma unroll
             for (int a = 0; a < 3; ++a) {

@@ -40,6 +40,71 @@ __device__ inline void t_cutoff(int name, T rc, T r, T& f, T& g) {
     }
 }
 
+// 0.5 (cos(pi r / rc) + 1) and its r-derivative in FP64, for 0 <= r < rc: with t = pi r / rc - pi/2 in
+// [-pi/2, pi/2], cos(pi r / rc) = -sin t and sin(pi r / rc) = cos t, and sin t, cos t are their Taylor
+// series through t^17 and t^18 (truncation below 5e-14 on that interval) in Horner form, about twenty FP64
+// FMAs.  Not inlined: its eighteen FP64 constants would otherwise hold registers across the whole kernel.
+__device__ __attribute__((noinline)) void cos_cutoff64(double r, double rc, double& f, double& g) {
+    const double k = 3.141592653589793 / rc;
+    const double t = r * k - 1.5707963267948966;
+    const double u = t * t;
+    double p = 1.0 / 355687428096000.0;                 // 1/17!
+    p = p * u - 1.0 / 1307674368000.0;                  // 1/15!
+    p = p * u + 1.0 / 6227020800.0;                     // 1/13!
+    p = p * u - 1.0 / 39916800.0;                       // 1/11!
+    p = p * u + 1.0 / 362880.0;                         // 1/9!
+    p = p * u - 1.0 / 5040.0;                           // 1/7!
+    p = p * u + 1.0 / 120.0;                            // 1/5!
+    p = p * u - 1.0 / 6.0;                              // 1/3!
+    p = p * u + 1.0;
+    const double sin_t = t * p;
+    double q = -1.0 / 6402373705728000.0;               // 1/18!
+    q = q * u + 1.0 / 20922789888000.0;                 // 1/16!
+    q = q * u - 1.0 / 87178291200.0;                    // 1/14!
+    q = q * u + 1.0 / 479001600.0;                      // 1/12!
+    q = q * u - 1.0 / 3628800.0;                        // 1/10!
+    q = q * u + 1.0 / 40320.0;                          // 1/8!
+    q = q * u - 1.0 / 720.0;                            // 1/6!
+    q = q * u + 1.0 / 24.0;                             // 1/4!
+    q = q * u - 0.5;                                    // 1/2!
+    const double cos_t = q * u + 1.0;
+    f = 0.5 * (1.0 - sin_t);
+    g = -0.5 * k * cos_t;
+}
+
+// Cutoff value and r-derivative for |d| known to FP64 precision (r) and the cutoff radius rc, in T
+// (cutoff.cpp:62-117).  Every neighbour's cutoff multiplies all of its symmetry-function terms, so an FP32
+// cosine's absolute error (~1e-7, from rounding pi r / rc) alone puts ~1e-5 eV/A into the forces: COS runs
+// in FP64 and is rounded once.  TANH and POLY3 run in T on y = (rc - r) / rc with rc - r formed in FP64,
+// so their rounding stays relative where they vanish at rc:
+//   TANH:  tanh^3 y;                                     f' = -3 t^2 (1 - t^2) / rc
+//   POLY3: (2x - 3) x^2 + 1 = y^2 (3 - 2y), x = r / rc;  f' = -6 (1 - y) y / rc
+template <class T>
+__device__ inline void t_cutoff_r64(int name, double rc, double r, T& f, T& g) {
+    if (r >= rc) { f = T(0); g = T(0); return; }
+    const T rcT = (T)rc;
+    const T y = (T)(rc - r) / rcT;
+    switch (name) {
+        case ACE_CUT_COS: {
+            double f64, g64;
+            cos_cutoff64(r, rc, f64, g64);
+            f = (T)f64;
+            g = (T)g64;
+        } break;
+        case ACE_CUT_STEP: f = T(1); g = T(0); break;
+        case ACE_CUT_TANH: {
+            const T t = tanh(y);
+            f = t * t * t;
+            g = T(-3) * t * t * (T(1) - t * t) / rcT;
+        } break;
+        case ACE_CUT_POLY3: {
+            f = y * y * (T(3) - T(2) * y);
+            g = T(-6) * (T(1) - y) * y / rcT;
+        } break;
+        default: f = T(0); g = T(0);
+    }
+}
+
 // one radial function value and r-derivative (basis_radial.cpp:175-257)
 template <class T>
 __device__ inline void t_radial(int form, T eta, T rs, T r, T fc, T gc, T& v, T& dv) {
@@ -210,5 +275,101 @@ __device__ inline T t_wave_sum(T v) {
         return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
     } else {
         return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+    }
+}
+
+// e^y for an exponent formed in FP64: y log2(e) split into FP32 high and low parts, 2^yh (1 + yl ln 2) with
+// the hardware exp2 on yh (relative error a few FP32 ulps whatever the size of y).
+__device__ inline float exp_r64(double y) {
+    const double z = y * 1.4426950408889634;
+    const float zh = (float)z;
+    const float zl = (float)(z - (double)zh);
+    const float e = __builtin_amdgcn_exp2f(zh);
+    return fmaf(e, zl * 0.6931471805599453f, e);
+}
+
+// One radial function value and r-derivative with |d| known to FP64 precision (r) and the cutoff factors
+// in T (basis_radial.cpp:175-257).  T = float: each form's exponent is formed in FP64 from r - rs and
+// evaluated by exp_r64, the rest in FP32 (an exponent formed in FP32 carries its relative rounding times
+// its own size, up to ~25 for a steep Gaussian, into the function); TANH as e / (1 + e) with e <= 1.
+// T = double: t_radial in FP64.
+template <class T>
+__device__ inline void t_radial_r64(int form, double eta, double rs, double r, T fc, T gc, T& v, T& dv) {
+    if constexpr (sizeof(T) == 8) {
+        t_radial<double>(form, eta, rs, r, fc, gc, v, dv);
+    } else {
+        const double x = r - rs;
+        const float ef = (float)eta;
+        switch (form) {
+            case ACE_RAD_GAUSSIAN: {
+                const double a = eta * x;
+                const float e = exp_r64(-(a * a));
+                v = e * fc;
+                dv = e * (-2.f * ef * (float)a * fc + gc);
+            } break;
+            case ACE_RAD_SECH: {
+                const float e = exp_r64(-eta * x);
+                const float e2 = e * e;
+                const float den = 1.f / (1.f + e2);
+                v = 2.f * e * den * fc;
+                dv = 2.f * e * den * (-ef * (1.f - e2) * den * fc + gc);
+            } break;
+            case ACE_RAD_LOGISTIC: {
+                const float e = exp_r64(-2.0 * eta * x);
+                const float den = 1.f / (1.f + e);
+                const float t = (1.f - e) * den;
+                const float s2 = 4.f * e * den * den;
+                v = s2 * fc;
+                dv = s2 * (-2.f * ef * t * fc + gc);
+            } break;
+            case ACE_RAD_TANH: {
+                // 0.5 (tanh(-eta x) + 1) = e / (1 + e) (x >= 0) or 1 / (1 + e) (x < 0), e = exp(-2 eta |x|)
+                const float e = exp_r64(-2.0 * eta * fabs(x));
+                const float den = 1.f / (1.f + e);
+                const float h = (x >= 0.0 ? e : 1.f) * den;
+                v = h * fc;
+                dv = -2.f * ef * e * den * den * fc + h * gc;
+            } break;
+            case ACE_RAD_LOGCOSH: {
+                const float e = exp_r64(-2.0 * eta * x);
+                const float l = 0.5f * log1pf(e);
+                v = l * fc;
+                dv = -ef * e / (1.f + e) * fc + l * gc;
+            } break;
+            case ACE_RAD_LOGCOSH2: {
+                const float rad2 = 1.414213562373095f;
+                const double a = -eta * x;
+                if (a == 0.0) {
+                    v = 0.5f * fc / rad2;
+                    dv = 0.5f * (-ef * gc + fc / rad2);
+                } else {
+                    // e = exp(y), y = 2 sqrt(2) a; near y = 0, e - 1 = expm1(y) and 1 + y - e = -(expm1(y) - y)
+                    // by its series y^2 (1/2 + y/6 + ...), where the differences would cancel
+                    const double y64 = 2.0 * 1.414213562373095 * a;
+                    const float y = (float)y64, e = exp_r64(y64);
+                    const bool small = fabsf(y) < 0.5f;
+                    const float em1 = small ? expm1f(y) : e - 1.f;
+                    float q;
+                    if (small) {
+                        float p = 1.f / 3628800.f;                    // 1/10!
+                        p = p * y + 1.f / 362880.f;
+                        p = p * y + 1.f / 40320.f;
+                        p = p * y + 1.f / 5040.f;
+                        p = p * y + 1.f / 720.f;
+                        p = p * y + 1.f / 120.f;
+                        p = p * y + 1.f / 24.f;
+                        p = p * y + 1.f / 6.f;
+                        p = p * y + 0.5f;
+                        q = y * y * p;
+                    } else {
+                        q = em1 - y;
+                    }
+                    const float den = 1.f / em1, af = (float)a;
+                    v = af * e * den * fc;
+                    dv = e * (-ef * q * den * fc + af * gc) * den;
+                }
+            } break;
+            default: v = 0.f; dv = 0.f;
+        }
     }
 }

@@ -36,6 +36,7 @@ SIGNATURES = {
     "ace_symm": (C.c_int, [P, dp, C.c_int]),
     "ace_energy_forces": (C.c_int, [P, C.c_int, dp, dp, dp]),
     "ace_compute": (C.c_int, [P, C.c_int, C.c_double, dp, dp, dp]),
+    "ace_energy_forces_noalloc": (C.c_int, [P, C.c_int, P, P, P, P]),
     "ace_md_init": (C.c_int, [P, dp, dp]),
     "ace_md_run": (C.c_int, [P, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, dp]),
     "ace_md_state": (C.c_int, [P, dp, dp]),

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -136,13 +137,38 @@ struct ace_ctx {
     hipGraphExec_t md_exec[2] = {nullptr, nullptr};
     long long md_sig[2][8] = {};
     unsigned long long gen = 0;   // counts device reallocations: captured graphs hold raw pointers
+    // device of the context and its address of the no-allocation scratch array (ace_create)
+    int device = -1;
+    unsigned char* scratch = nullptr;
     // profiling
     bool prof = false;
     std::vector<ProfRec> pending;
     std::map<std::string, std::pair<double, long long>> acc;
 };
 
+// Scratch memory of ace_energy_forces_noalloc: fixed-point accumulators, per-atom energies, network rows and
+// the virial when the caller passes none.  One copy per GPU, held until the process ends.  In host code
+// `scratch` names a host-side stand-in, not device memory: its device address is looked up once per device
+// (scratch_on, from ace_create, with that device current) and only that address is used.
+#define ACE_NOALLOC_BYTES (64ull << 20)
+static __device__ __align__(256) unsigned char scratch[ACE_NOALLOC_BYTES];
+
 namespace {
+
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[64] = {};
+
+// the current device's address of `scratch` (nullptr if it cannot be looked up)
+unsigned char* scratch_on(int dev) {
+    if (dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[dev] = (unsigned char*)p;
+    }
+    return g_scratch_ptr[dev];
+}
 
 int fail(ace_ctx* c, const std::string& msg) {
     if (c) c->err = msg;
@@ -1013,6 +1039,11 @@ int ace_create(ace_ctx** out, const ace_model_desc* model) {
     }
     if (ensure(ctx, ctx->d_scal, 16)) return -5;
     if (build_tables(ctx)) return -6;
+    if (hipGetDevice(&ctx->device) != hipSuccess) {
+        ctx->err = "ace_create: no current device";
+        return -2;
+    }
+    ctx->scratch = scratch_on(ctx->device);   // nullptr: ace_energy_forces_noalloc reports it
     return 0;
 }
 
@@ -1413,6 +1444,93 @@ int ace_compute(ace_ctx* ctx, int precision, double skin, double* E_struc, doubl
     if (!ctx) return -1;
     if (ace_build_neighbors(ctx, skin)) return -1;
     return ace_energy_forces(ctx, precision, E_struc, F, virial);
+}
+
+int ace_energy_forces_noalloc(ace_ctx* ctx, int precision, const double* pos, double* E_struc, double* F,
+                              double* virial) {
+    if (!ctx) return -1;
+    if (precision != ACE_PREC_FP32 && precision != ACE_PREC_FP64_BASIS && precision != ACE_PREC_BF16L1)
+        return fail(ctx, "ace_energy_forces_noalloc: precision mode not available");
+    if (!ctx->scratch) return fail(ctx, "ace_energy_forces_noalloc: the scratch array of this device was not found");
+    int dev = -1;
+    HC(hipGetDevice(&dev));
+    if (dev != ctx->device) return fail(ctx, "ace_energy_forces_noalloc: call it with the context's device current");
+    if (!E_struc || !F) return fail(ctx, "ace_energy_forces_noalloc: E_struc and F (device pointers) are required");
+    if (ctx->S == 0) return fail(ctx, "ace_energy_forces_noalloc: no structures uploaded");
+    if (!ctx->nb_valid) return fail(ctx, "ace_energy_forces_noalloc: neighbour list not built");
+    if (ctx->params_dirty || ctx->scale_dirty)
+        return fail(ctx, "ace_energy_forces_noalloc: parameters or input scaling changed since the last "
+                         "ace_energy_forces / ace_symm call, which uploads them (this entry point does not)");
+    ctx->basis_f64 = precision == ACE_PREC_FP64_BASIS;
+    const int N = ctx->N, S = ctx->S;
+    const bool split = fused_waves(ctx, precision) == 0;
+    // scratch layout: two accumulator sets (forces 3N, energies S, virials 9S), per-atom energies, the network
+    // rows G and dE/dG of the split path, and the virial when the caller passes none
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t acc_w = 3 * (size_t)N + 10 * (size_t)S;
+    const size_t o_acc1 = al(8 * acc_w), o_e = o_acc1 + al(8 * acc_w), o_g = o_e + al(8 * (size_t)N);
+    const size_t rows = split ? al(4 * (size_t)N * ctx->ldg) : 0;
+    const size_t o_d = o_g + rows, o_w = o_d + rows;
+    const size_t need = o_w + (virial ? 0 : al(8 * 9 * (size_t)S));
+    if (need > ACE_NOALLOC_BYTES) {
+        char m[256];
+        snprintf(m, sizeof m, "ace_energy_forces_noalloc: this batch needs %zu bytes of scratch, the maximum is %llu",
+                 need, (unsigned long long)ACE_NOALLOC_BYTES);
+        return fail(ctx, m);
+    }
+    if (pos) {
+        HC(hipMemcpyAsync(ctx->d_pos.p, pos, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        ctx->tr.g_valid = false;
+    }
+    unsigned char* sp = ctx->scratch;
+    auto u64 = [&](size_t o) { return (unsigned long long*)(sp + o); };
+    // the context's buffers, pointed into scratch and the caller's outputs for this call
+    DBuf<unsigned long long> sF[2] = {ctx->d_Ffix[0], ctx->d_Ffix[1]}, sE[2] = {ctx->d_Efix[0], ctx->d_Efix[1]},
+                             sW[2] = {ctx->d_Wfix[0], ctx->d_Wfix[1]};
+    const DBuf<float> sG = ctx->d_G, sD = ctx->d_dEdG;
+    const DBuf<double> se = ctx->d_e_atom, sFo = ctx->d_F, sEo = ctx->d_E, svo = ctx->d_vir;
+    const int sfbuf = ctx->fbuf;
+    const bool sprof = ctx->prof;
+    for (int b = 0; b < 2; ++b) {
+        const size_t o = b ? o_acc1 : 0;
+        ctx->d_Ffix[b].p = u64(o);
+        ctx->d_Efix[b].p = u64(o) + 3 * (size_t)N;
+        ctx->d_Wfix[b].p = u64(o) + 3 * (size_t)N + S;
+    }
+    ctx->d_e_atom.p = (double*)(sp + o_e);
+    if (split) {
+        ctx->d_G.p = (float*)(sp + o_g);
+        ctx->d_dEdG.p = (float*)(sp + o_d);
+    }
+    ctx->d_F.p = F;
+    ctx->d_E.p = E_struc;
+    ctx->d_vir.p = virial ? virial : (double*)(sp + o_w);
+    ctx->fbuf = 0;
+    ctx->prof = false;
+    // the array keeps the last call's contents: the accumulators this call adds into start at zero, and the
+    // network rows (whose padding columns the matrix-core tiles read) too
+    auto zero = [&](size_t o, size_t words) {
+        const int grid = (int)std::min<size_t>((words + 255) / 256, 4096);
+        return timed(ctx, "zero", [&] { k_zero_u64<<<grid, 256, 0, ctx->stream>>>(words, u64(o)); });
+    };
+    int rc = zero(0, acc_w);
+    if (!rc && split) rc = zero(o_g, (o_w - o_g) / 8);
+    if (!rc) rc = run_ef(ctx, precision);
+    if (!rc) rc = run_finalize(ctx);
+    for (int b = 0; b < 2; ++b) {
+        ctx->d_Ffix[b] = sF[b];
+        ctx->d_Efix[b] = sE[b];
+        ctx->d_Wfix[b] = sW[b];
+    }
+    ctx->d_G = sG;
+    ctx->d_dEdG = sD;
+    ctx->d_e_atom = se;
+    ctx->d_F = sFo;
+    ctx->d_E = sEo;
+    ctx->d_vir = svo;
+    ctx->fbuf = sfbuf;
+    ctx->prof = sprof;
+    return rc;
 }
 
 int ace_md_init(ace_ctx* ctx, const double* mass, const double* vel) {

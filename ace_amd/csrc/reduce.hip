@@ -20,6 +20,11 @@ __global__ __launch_bounds__(256) void k_finalize(int n3, int ns, const unsigned
     if (t < 9 * ns) W[t] = (double)(long long)W_fix[t] * inv;
 }
 
+// n 64-bit words set to zero (a kernel, not hipMemsetAsync, so that callers may capture it in a HIP graph)
+__global__ __launch_bounds__(256) void k_zero_u64(size_t n, unsigned long long* __restrict__ p) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0ull;
+}
+
 // block per structure: E_s = sum e_atom (fixed-order FP64 tree); the training loss without forces
 __global__ __launch_bounds__(RED_THREADS) void k_struct_reduce(const int* __restrict__ atom_off,
                                                                 const int* __restrict__ natoms_of,

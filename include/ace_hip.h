@@ -18,6 +18,7 @@
  *   ace_energy_forces .............. NNP::compute / compute_force / compute_energy
  *                                   (src/nnp/nnp.cpp:329-338, 412-494)
  *   ace_compute .................... NNP::compute(Structure&) (src/nnp/nnp.cpp:489-494): build + energy
+ *   ace_energy_forces_noalloc ...... (no counterpart) ace_energy_forces into device memory, allocating nothing
  *   ace_train_step ................. NNPTEFR::error_cost_o1 (src/nnp/nnptefr.cpp:1292-1425)
  *   ace_train_optimizer ............ (no counterpart) the ADAM update of the training loop
  *                                    (nnptefr.cpp:1031-1066, opt::ADAM algo.cpp:280-296) on the GPU
@@ -175,6 +176,19 @@ int  ace_energy_forces(ace_ctx* ctx, int precision, double* E_struc, double* F, 
 /* NNP::compute(Structure&) (src/nnp/nnp.cpp:489-494): neighbour list with list radius rc + skin, then
  * ace_energy_forces, in one call. */
 int  ace_compute(ace_ctx* ctx, int precision, double skin, double* E_struc, double* F, double* virial);
+
+/* No counterpart in ace: for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same quantities as ace_energy_forces (same kernels, same
+ * bits) for the uploaded batch and its built neighbour list, with every pointer a DEVICE pointer: pos (N x 3,
+ * may be NULL: the uploaded positions) is copied in first, E_struc (n_struc), F (N x 3) and virial
+ * (n_struc x 9, may be NULL) are written.  The call allocates nothing and does not synchronise: its
+ * accumulators, per-atom energies and network rows live in one static 64 MiB device array per GPU (about
+ * 56 bytes per atom with the fused kernel, plus 8 * ldg bytes per atom with the split path; a larger batch
+ * is refused with a message).  Calls that overlap on one GPU share that array and race.  Parameters or
+ * input scaling changed since the last ace_energy_forces / ace_symm call are refused (those calls upload
+ * them).  An atom that outgrew the LDS slots is reported by the next synchronising call. */
+int  ace_energy_forces_noalloc(ace_ctx* ctx, int precision, const double* pos, double* E_struc, double* F,
+                               double* virial);
 
 /* Velocity-Verlet molecular dynamics of the uploaded batch, metal units (A, ps, amu, eV).  No counterpart in
  * ace's NNP API: ace has no NNP molecular dynamics; the step follows its Verlet template

@@ -170,6 +170,26 @@ unsigned char* scratch_on(int dev) {
     return g_scratch_ptr[dev];
 }
 
+// inverse of a 3x3 matrix, both column-major (Rinv[r + 3 c] = (R^-1)(r, c)); R must be non-singular
+void mat3_inverse(const double* R, double* Rinv) {
+    const double m[3][3] = {{R[0], R[3], R[6]}, {R[1], R[4], R[7]}, {R[2], R[5], R[8]}};
+    const double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                       m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    const double id = 1.0 / det;
+    double inv[3][3];
+    inv[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) * id;
+    inv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
+    inv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
+    inv[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * id;
+    inv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id;
+    inv[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id;
+    inv[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * id;
+    inv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * id;
+    inv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) Rinv[r + 3 * c] = inv[r][c];
+}
+
 int fail(ace_ctx* c, const std::string& msg) {
     if (c) c->err = msg;
     return -1;
@@ -1230,26 +1250,14 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
         for (int q = 0; q < 9; ++q) fro += R[q] * R[q];
         if (!(std::fabs(det) > 1e-9) || !(std::sqrt(fro) > 1e-12))
             return fail(ctx, "ace_build_neighbors: singular cell (non-periodic structures are not supported)");
-        // inverse, column-major: Rinv(r,c)
-        const double id = 1.0 / det;
-        double inv[3][3];
-        const double m[3][3] = {{R[0], R[3], R[6]}, {R[1], R[4], R[7]}, {R[2], R[5], R[8]}};
-        inv[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) * id;
-        inv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
-        inv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
-        inv[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) * id;
-        inv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id;
-        inv[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id;
-        inv[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * id;
-        inv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * id;
-        inv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
-        for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 3; ++c) I.Rinv[r + 3 * c] = inv[r][c];
+        mat3_inverse(R, I.Rinv);
         double hgt[3];
         for (int d = 0; d < 3; ++d) {
             const double coln = std::sqrt(R[3 * d] * R[3 * d] + R[3 * d + 1] * R[3 * d + 1] + R[3 * d + 2] * R[3 * d + 2]);
             I.cf[d] = (int)std::ceil(3.0 * rcl / coln);                       // neighbor.cpp:49-53
-            const double rown = std::sqrt(inv[d][0] * inv[d][0] + inv[d][1] * inv[d][1] + inv[d][2] * inv[d][2]);
+            // row d of R^-1 (column-major storage: (d, c) at d + 3 c)
+            const double* Ri = I.Rinv;
+            const double rown = std::sqrt(Ri[d] * Ri[d] + Ri[d + 3] * Ri[d + 3] + Ri[d + 6] * Ri[d + 6]);
             hgt[d] = 1.0 / rown;                                              // perpendicular height
             I.nc[d] = std::max(1, std::min((int)std::floor(hgt[d] / rsearch), 1024));
             if (I.cf[d] > 511) return fail(ctx, "ace_build_neighbors: image range too large for this cell");
@@ -1576,7 +1584,13 @@ int ace_md_run(ace_ctx* ctx, int precision, int nsteps, double dt, int rebuild_e
     // (the graphs are then captured again).  The flags left by the previous list are collected first, except
     // before the run's first build (they belong to an earlier call).
     auto rebuild = [&](bool collect) -> int {
-        if ((collect && collect_flags()) || ace_build_neighbors(ctx, skin)) return -1;
+        if (collect && collect_flags()) return -1;
+        if (timed(ctx, "md_wrap", [&] {
+                k_md_wrap<<<cdiv(ctx->N, MD_THREADS), MD_THREADS, 0, ctx->stream>>>(ctx->N, ctx->d_struc_of.p,
+                                                                                  ctx->d_cell.p, ctx->d_pos.p);
+            }))
+            return -1;
+        if (ace_build_neighbors(ctx, skin)) return -1;
         if (!ctx->nb_pending) return 0;
         int over = 0;
         HC(hipMemcpyAsync(&over, ctx->d_scal.p + ctx->nb_over_idx, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -1630,6 +1644,20 @@ int ace_md_state(ace_ctx* ctx, double* pos, double* vel) {
     if (pos) HC(hipMemcpyAsync(pos, ctx->d_pos.p, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (vel) HC(hipMemcpyAsync(vel, ctx->d_vel.p, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HC(hipStreamSynchronize(ctx->stream));
+    if (pos) {
+        // positions in their cells, as the reference's Verlet keeps them (Cell::returnToCell after each drift);
+        // on the device they are returned to the cells at each list rebuild
+        for (int i = 0; i < ctx->N; ++i) {
+            const double* R = &ctx->cell[9 * (size_t)ctx->struc_of_h[i]];
+            double Ri[9];
+            mat3_inverse(R, Ri);
+            double* r = pos + 3 * (size_t)i;
+            double f[3];
+            for (int a = 0; a < 3; ++a) f[a] = Ri[a] * r[0] + Ri[3 + a] * r[1] + Ri[6 + a] * r[2];
+            for (int a = 0; a < 3; ++a) f[a] -= std::floor(f[a]);
+            for (int a = 0; a < 3; ++a) r[a] = R[a] * f[0] + R[3 + a] * f[1] + R[6 + a] * f[2];
+        }
+    }
     return 0;
 }
 

@@ -72,6 +72,32 @@ __global__ __launch_bounds__(MD_THREADS) void k_md_kick(int N, int S, double hdt
     }
 }
 
+// Returns every atom to its cell, r -> R (f - floor(f)) with f = R^-1 r (Cell::returnToCell, cell.cpp:73-83,
+// which the reference's Verlet applies after every drift, integrator.cpp:553).  Run just before each
+// neighbour-list rebuild: between rebuilds the list's image vectors refer to the positions as they were
+// at the build, and the build's image range (ceil(3 rc / |a_d|), neighbor.cpp:49-53) must cover them.
+__global__ __launch_bounds__(MD_THREADS) void k_md_wrap(int N, const int* __restrict__ struc_of,
+                                                        const double* __restrict__ cell, double* __restrict__ pos) {
+    const int i = blockIdx.x * MD_THREADS + threadIdx.x;
+    if (i >= N) return;
+    const double* R = cell + 9 * struc_of[i];   // column-major: R[3 c + r] = component r of lattice vector c
+    // R^-1 by cofactors
+    const double c00 = R[4] * R[8] - R[7] * R[5], c01 = R[7] * R[2] - R[1] * R[8], c02 = R[1] * R[5] - R[4] * R[2];
+    const double c10 = R[6] * R[5] - R[3] * R[8], c11 = R[0] * R[8] - R[6] * R[2], c12 = R[3] * R[2] - R[0] * R[5];
+    const double c20 = R[3] * R[7] - R[6] * R[4], c21 = R[6] * R[1] - R[0] * R[7], c22 = R[0] * R[4] - R[3] * R[1];
+    const double idet = 1.0 / (R[0] * c00 + R[3] * c01 + R[6] * c02);
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    double f0 = (c00 * x + c10 * y + c20 * z) * idet;
+    double f1 = (c01 * x + c11 * y + c21 * z) * idet;
+    double f2 = (c02 * x + c12 * y + c22 * z) * idet;
+    f0 -= floor(f0);
+    f1 -= floor(f1);
+    f2 -= floor(f2);
+    pos[3 * i] = R[0] * f0 + R[3] * f1 + R[6] * f2;
+    pos[3 * i + 1] = R[1] * f0 + R[4] * f1 + R[7] * f2;
+    pos[3 * i + 2] = R[2] * f0 + R[5] * f1 + R[8] * f2;
+}
+
 // The neighbour build clears the list-overflow flag (scal[over_idx]) and the basis kernels' refusal flag
 // (scal[3]) for the next list; across the rebuilds of a run they are collected here (sticky[0] != 0).
 __global__ void k_md_flags(const int* __restrict__ scal, int over_idx, unsigned int* __restrict__ sticky) {

@@ -356,52 +356,79 @@ def test_bf16_first_layer():
     check_bf16(m2, strucs2, "bf16-c2")
 
 
-def test_c5_full_size():
-    """The 256-structure, 568k-atom mixed-precision inference batch: FP32 and BF16L1 energies and forces
-    of the two smallest structures vs the oracle, and those two structures uploaded alone give the same
-    bits as inside the batch on the same path (every per-atom result is independent of the batch; sums
-    are exact).  The path is the one the size rules take for the full batch: split, two waves per atom."""
+def test_c5_full_size(golden_dir):
+    """The 256-structure, 568k-atom mixed-precision inference batch: FP32 and BF16L1 energies and forces of
+    the smallest, the median and the largest (8,126 atoms) structure vs the FP64 oracle's values in
+    tests/golden/c5_ref.npz (written by tests/golden/make_c5_ref.py, with the input scaling of that file),
+    and the two smallest structures uploaded alone give the same bits as inside the batch on the same path
+    (every per-atom result is independent of the batch; sums are exact).  The path is the one the size rules
+    take for the full batch: split, two waves per atom."""
+    g = np.load(os.path.join(golden_dir, "c5_ref.npz"))
     model, strucs = configs.config5()
+    o = 0
+    for I in range(model.ntypes):
+        K = model.input_width(I)
+        model.nets[I].inpw, model.nets[I].inpb = g["inpw"][o:o + K], g["inpb"][o:o + K]
+        o += K
     N = sum(s.natoms for s in strucs)
     path = "split_2w"
     dev = NNPDevice(model)
     dev.upload(strucs)
     dev.build_neighbors()
-    dev.set_precision(PREC_FP64_BASIS)
-    G = dev.compute_symm()
-    configs.set_input_scaling(model, strucs, G)     # PreBias MEAN / PreScale DEV over the whole batch
-    del G
-    dev.sync_model_scaling()
-    dev.set_precision(PREC_FP32)
     for p in (PREC_FP32, PREC_BF16L1):
         dev.set_path(path, p)
     res = {p: dev.compute(p) for p in (PREC_FP32, PREC_BF16L1)}
     dev.close()
     off = np.concatenate([[0], np.cumsum([s.natoms for s in strucs])])
+    for q, k in enumerate(g["idx"]):
+        n = strucs[k].natoms
+        Er, Fr = float(g["E"][q]), g[f"F_{q}"]
+        for p, te, tf in ((PREC_FP32, TOL[PREC_FP32]["E"], TOL[PREC_FP32]["F"]),
+                          (PREC_BF16L1, BF16_TOL["E"], BF16_TOL["F"])):
+            E, F, _ = res[p]
+            de = abs(E[k] - Er) / n
+            df = np.max(np.abs(F[off[k]:off[k] + n] - Fr))
+            log(f"c5 ({N} atoms) struct {k} n={n} prec {p}: dE/atom {de:.1e} dF {df:.1e}")
+            assert de < te and df < tf, (k, p)
     small = [int(k) for k in np.argsort([s.natoms for s in strucs])[:2]]
-    orc = O.Oracle(model)
     alone = NNPDevice(model)
     alone.upload([strucs[k] for k in small])
     alone.build_neighbors()
     for p in (PREC_FP32, PREC_BF16L1):
         alone.set_path(path, p)
-    Ea = {p: alone.compute(p) for p in (PREC_FP32, PREC_BF16L1)}
-    alone.close()
-    a0 = 0
-    for q, k in enumerate(small):
-        s = strucs[k]
-        n = s.natoms
-        Er, Fr, _ = orc.compute(s)
-        for p, te, tf in ((PREC_FP32, TOL[PREC_FP32]["E"], TOL[PREC_FP32]["F"]),
-                          (PREC_BF16L1, BF16_TOL["E"], BF16_TOL["F"])):
-            E, F, W = res[p]
-            de = abs(E[k] - Er) / n
-            df = np.max(np.abs(F[off[k]:off[k] + n] - Fr))
-            log(f"c5 ({N} atoms) struct {k} n={n} prec {p}: dE/atom {de:.1e} dF {df:.1e}")
-            assert de < te and df < tf, (k, p)
-            e1, f1, w1 = Ea[p]
+        e1, f1, w1 = alone.compute(p)
+        E, F, W = res[p]
+        a0 = 0
+        for q, k in enumerate(small):
+            n = strucs[k].natoms
             assert e1[q] == E[k] and np.array_equal(f1[a0:a0 + n], F[off[k]:off[k] + n]) and np.array_equal(w1[q], W[k])
-        a0 += n
+            a0 += n
+    alone.close()
+
+
+def test_c4_fp32_forces_vs_plain_fp64():
+    """The 108,000-atom two-species fcc cell of the MD workload (list radius 6 A, output layers scaled to an
+    rms force of 0.1 eV/A as for the MD runs, configs.calibrate_forces): FP32 energies and forces against
+    the unbatched FP64 port (which matches the oracle to 1e-9 on the smaller cells above) -- the oracle
+    itself would need ~10^11 distance tests for this cell's reference neighbour list."""
+    model, s, _ = configs.config4()
+    dev = NNPDevice(model)
+    dev.upload([s])
+    dev.build_neighbors()
+    dev.set_precision(PREC_FP64_BASIS)
+    configs.set_input_scaling(model, [s], dev.compute_symm())
+    dev.sync_model_scaling()
+    _, F, _ = dev.compute(PREC_FP64_BASIS)
+    configs.calibrate_forces(model, F, rms=0.1)
+    for I in range(model.ntypes):
+        dev.set_params(I, model.nets[I].params)
+    E32, F32, _ = dev.compute(PREC_FP32)
+    Ep, Fp = dev.plain_compute()
+    dev.close()
+    de = abs(E32[0] - Ep[0]) / s.natoms
+    df = np.abs(F32 - Fp).max()
+    log(f"c4 ({s.natoms} atoms) fp32 vs plain fp64: dE/atom {de:.1e} dF {df:.1e} (|F| rms {np.sqrt(np.mean(Fp ** 2)):.2f})")
+    assert de < TOL[PREC_FP32]["E"] and df < TOL[PREC_FP32]["F"]
 
 
 def test_potential_file_on_gpu(tmp_path):

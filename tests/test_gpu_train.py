@@ -1,6 +1,7 @@
 """Batched training step (energy + force-matching loss, d(F)/dW) vs the FP64 oracle and vs FP64 central
 finite differences of the oracle's loss on 50 random weights.  Tolerance: loss relative 1e-5, gradient
-1e-3 of the gradient's largest entry (FP32 basis, network and adjoints; FP64 reductions)."""
+1e-5 of the gradient's largest entry (measured ~1e-7; FP32 symmetry-function terms, networks and adjoints
+with FP64 geometry, sums and reductions)."""
 import numpy as np
 import pytest
 
@@ -42,7 +43,8 @@ def oracle_loss(model, strucs, G, ld):
     return tot, grad
 
 
-@pytest.mark.parametrize("loss,norm,wf,prec", [(0, 0, 0.0, 0), (0, 1, 0.1, 0), (2, 2, 0.1, 0), (0, 1, 0.1, 2)])
+@pytest.mark.parametrize("loss,norm,wf,prec", [(0, 0, 0.0, 0), (0, 1, 0.1, 0), (2, 2, 0.1, 0), (0, 1, 0.1, 2),
+                                               (1, 3, 0.1, 0), (3, 4, 0.1, 0), (3, 3, 0.0, 2)])
 def test_train_step_matches_oracle(loss, norm, wf, prec):
     model, strucs, G = batch()
     nat = sum(s.natoms for s in strucs)
@@ -58,7 +60,7 @@ def test_train_step_matches_oracle(loss, norm, wf, prec):
     lf = abs(out[2] - ref[1]) / max(abs(ref[1]), 1e-12) if wf else 0.0
     ge = np.max(np.abs(g - gref)) / np.max(np.abs(gref))
     log(f"train loss={loss} norm={norm} wf={wf} prec={prec}: loss_E rel {le:.1e} loss_F rel {lf:.1e} grad {ge:.1e}")
-    assert le < 1e-5 and lf < 1e-5 and ge < 1e-3
+    assert le < 1e-5 and lf < 1e-5 and ge < 1e-5
     # a second step with unchanged parameters reproduces the first (cached G, deterministic reductions)
     out2, g2 = dev.train_step(loss, norm, 0.5, 1.0 / len(strucs), wf / (3 * nat))
     assert np.array_equal(out, out2) and np.array_equal(g, g2)
@@ -170,3 +172,33 @@ def test_device_adam_matches_host_adam(tmp_path):
         out = tr.step()
         assert np.array_equal(out, runs[1][k][0]) and np.array_equal(tr.params, runs[1][k][1])
     dev.close()
+
+
+@pytest.mark.parametrize("force_only", [False, True])
+def test_train_step_full_config3_batch(golden_dir, force_only):
+    """One ace_train_step on the whole 64-structure, 20,245-atom force-matching batch (structures of 32 to
+    1013 atoms, small cells with several images, the cell search and the multi-round adjoint and tangent
+    kernels) against the oracle's loss and gradient in tests/golden/c3_train.npz (written by
+    tests/golden/make_c3_train.py: the oracle needs minutes for this batch).  The force-only loss checks the
+    d(F)/dW part, which the energy term would otherwise dominate."""
+    import os
+    g = np.load(os.path.join(golden_dir, "c3_train.npz"))
+    model, _, strucs = configs.config3()
+    o = 0
+    for I in range(model.ntypes):
+        K = model.input_width(I)
+        model.nets[I].inpw, model.nets[I].inpb = g["inpw"][o:o + K], g["inpb"][o:o + K]
+        o += K
+    we = 0.0 if force_only else float(g["w_energy"])
+    ref, gref = (g["loss_force"], g["grad_force"]) if force_only else (g["loss"], g["grad"])
+    dev = NNPDevice(model)
+    dev.upload(strucs)
+    dev.build_neighbors()
+    dev.set_targets(g["E_ref"], g["F_ref"].astype(np.float64))
+    out, grad = dev.train_step(0, 0, 1.0, we, float(g["w_force"]))
+    dev.close()
+    le = abs(out[1] - ref[0]) / max(abs(ref[0]), 1e-300) if not force_only else abs(out[1])
+    lf = abs(out[2] - ref[1]) / abs(ref[1])
+    ge = np.max(np.abs(grad - gref)) / np.max(np.abs(gref))
+    log(f"c3 full batch force_only={force_only}: loss_E rel {le:.1e} loss_F rel {lf:.1e} grad rel {ge:.1e}")
+    assert le < 1e-5 and lf < 1e-5 and ge < 1e-5

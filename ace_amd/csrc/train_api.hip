@@ -302,6 +302,12 @@ int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* 
         return fail(ctx, "ace_train_step: delta must be positive");
     ctx->basis_f64 = ctx->precision == ACE_PREC_FP64_BASIS;
     if (require_nb(ctx)) return -1;
+    // a pending asynchronous rebuild is verified before any training kernel or optimiser update runs; one
+    // that outgrew its capacity is rebuilt with exact sizes (as ace_energy_forces does)
+    if (ctx->nb_pending && nb_verify(ctx)) {
+        if (ctx->nb_async_ok) return -1;
+        if (ace_build_neighbors(ctx, ctx->nb_skin) || require_nb(ctx)) return -1;
+    }
     if (train_tiles(ctx)) return -1;
     const int N = ctx->N, S = ctx->S;
     if (ensure(ctx, T.d_gfac, S) || ensure(ctx, T.d_eps, 3 * (size_t)N) || ensure(ctx, T.d_out, 3 * (size_t)S))

@@ -126,6 +126,24 @@ def test_small_and_triclinic_cells():
     check_against_oracle(m, strucs, "small")
 
 
+def test_large_triclinic_cells_binning_free_search():
+    """A batch of skewed cells whose heights all exceed twice the list radius, with atoms one or more cells
+    outside their cell: the build tests every atom of each structure under the minimum image (no binning)
+    and must still give the reference's pairs, images and order bit for bit."""
+    rng = np.random.default_rng(31)
+    m = configs.c3_model(rng)
+    strucs = []
+    for L_, skew, n in [(14.0, 0.25, 90), (15.5, 0.4, 120), (13.2, 0.1, 70)]:
+        cell = np.array([[L_, skew * L_, 0.3 * skew * L_], [0, L_ * 1.05, 0.45 * skew * L_], [0, 0, L_ * 0.97]])
+        heights = 1.0 / np.linalg.norm(np.linalg.inv(cell), axis=1)
+        assert heights.min() > 2 * 6.0
+        f = rng.uniform(0, 1, (n, 3))
+        f[:6] += rng.integers(-2, 3, (6, 3))          # atoms one or two cells outside
+        strucs.append(structure(cell, (cell @ f.T).T, rng.integers(0, 3, n)))
+    scaled_model(m, strucs)
+    check_against_oracle(m, strucs, "triclinic")
+
+
 def test_isolated_and_single_atom():
     rng = np.random.default_rng(6)
     m = configs.c3_model(rng)

@@ -1,10 +1,14 @@
-"""Benchmark: batched E+F throughput (atoms/s) of the Behler-Parrinello hot path on one MI355X.
+"""Benchmark of the Behler-Parrinello energy/force/training hot path on one MI355X.  Prints one JSON line.
 
-Workload (one step = NNP::compute of the whole batch: neighbour-list build, symmetry functions,
-networks, adjoint forces, FP64 energy/virial reductions): 2 elements (50/50), 1000 atoms on a
-jittered (0.15 A) bcc lattice at 0.085 atoms/A^3, periodic cell, cutoff 6.0 A, 16 G2 per neighbour
-species + 16 G4 per species pair = 80 inputs, 80-32-32-1 tanh networks, FP32 arithmetic with FP64
-reductions.  Inputs are resident in HBM before the timed region.  Prints one JSON line.
+Headline workload (`value`): the mixed-precision batched inference batch -- 256 structures of 256-8192
+atoms (log-uniform; 568,089 atoms), 4 elements, rc 6.5 A, 32 G2 + 96 G4 per element network, 128-128-64-1
+tanh, FP32 arithmetic with FP64 geometry, sums and reductions.  One step = NNP::compute of the whole batch:
+neighbour-list build, symmetry functions, networks, adjoint forces, per-structure FP64 energies and virials
+(ace_compute), inputs resident in HBM, results left in HBM.  The same batch with the first network layer
+on BF16 matrix cores is a sub-line.  Secondary lines: the 1000-atom bcc structure (FP32 and FP64 symmetry
+functions), the 64-structure force-matching training step, the 108,000-atom MD run, the plain unbatched
+FP64 GPU port, and the reference algorithm on the host's cores (the FP64 oracle, structures block-sharded
+over worker processes as thread::Dist does over MPI ranks).
 """
 from __future__ import annotations
 
@@ -21,32 +25,37 @@ sys.path.insert(0, ROOT)
 
 METRIC = "atoms/s E+F eval and train structs/s: batched adjoint vs plain FP64 GPU"
 PEAK_FP32_TFLOPS = 157.3
-PEAK_HBM_GBS = 8000.0
+CPU_WORKERS_MAX = 16        # host cores of one GPU's share on the GPU boxes
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def work_counts(model, strucs, off, j, typ):
-    """Pairs P, triplets T, radial evaluations E_R and angular evaluations E_A of one E+F pass,
-    counted from the exported neighbour list (skin 0: every listed pair is inside rc)."""
+# ---- algorithmic work -------------------------------------------------------------------------------
+
+def work_counts(model, strucs, off, j):
+    """Pairs P, triplets T, radial evaluations E_R and angular evaluations E_A of one E+F pass, from the
+    exported neighbour list (skin 0: every listed pair is inside rc); vectorised over atoms."""
     from ace_amd.model import tri_index
     nt = model.ntypes
-    base = np.repeat(np.concatenate([[0], np.cumsum([s.natoms for s in strucs])[:-1]]),
-                     [s.natoms for s in strucs])
-    P = T = ER = EA = 0
-    for i in range(len(typ)):
-        I = typ[i]
-        n = int(off[i + 1] - off[i])
-        cnt = np.bincount(typ[base[i] + j[off[i]:off[i + 1]]], minlength=nt)
-        P += n
-        T += n * (n - 1) // 2
-        ER += sum(int(cnt[J]) * len(model.radial[I][J].rs) for J in range(nt))
-        for J in range(nt):
-            for K in range(J, nt):
-                tr = cnt[J] * (cnt[J] - 1) // 2 if J == K else cnt[J] * cnt[K]
-                EA += int(tr) * len(model.angular[I][tri_index(J, K)].eta)
+    nat = np.array([s.natoms for s in strucs])
+    typ = np.concatenate([s.type for s in strucs]).astype(np.int64)
+    base = np.repeat(np.concatenate([[0], np.cumsum(nat)[:-1]]), nat)
+    n_i = np.diff(off)
+    atom_of_pair = np.repeat(np.arange(len(typ)), n_i)
+    spec = typ[base[atom_of_pair] + j.astype(np.int64)]
+    cnt = np.bincount(atom_of_pair * nt + spec, minlength=len(typ) * nt).reshape(len(typ), nt).astype(np.int64)
+    P = int(n_i.sum())
+    T = int((n_i * (n_i - 1) // 2).sum())
+    nR = np.array([[len(model.radial[I][J].rs) for J in range(nt)] for I in range(nt)])
+    ER = int((cnt * nR[typ]).sum())
+    EA = 0
+    for J in range(nt):
+        for K in range(J, nt):
+            pairs = cnt[:, J] * (cnt[:, J] - 1) // 2 if J == K else cnt[:, J] * cnt[:, K]
+            nA = np.array([len(model.angular[I][tri_index(J, K)].eta) for I in range(nt)])
+            EA += int((pairs * nA[typ]).sum())
     return P, T, ER, EA
 
 
@@ -54,12 +63,140 @@ def mlp_flops(model, typ):
     f = 0
     for I in range(model.ntypes):
         w = model.nets[I].widths
-        f += 4 * np.count_nonzero(typ == I) * sum(w[l] * w[l + 1] for l in range(len(w) - 1))
+        f += 4 * int(np.count_nonzero(typ == I)) * sum(w[l] * w[l + 1] for l in range(len(w) - 1))
     return f
 
 
-def cpu_baseline(model, s, samples):
-    """The FP64 oracle (reference loop structure, brute-force neighbour list) on one host thread."""
+def kernel_flops(model, strucs, dev):
+    """SURVEY's FLOP model per kernel of one E+F pass over the uploaded batch (adds, muls, compares)."""
+    off, j, _ = dev.neighbors()
+    P, T, ER, EA = work_counts(model, strucs, off, j)
+    typ = np.concatenate([s.type for s in strucs])
+    fl = {"basis_symm": 10 * P + 6 * ER + 15 * T + 7 * EA, "basis_adjoint": 22 * P + 6 * ER + 45 * T + 7 * EA,
+          "mlp_energy_grad": mlp_flops(model, typ)}
+    fl["basis_fused"] = fl["basis_symm"] + fl["basis_adjoint"] + fl["mlp_energy_grad"]
+    fl["mlp_energy_grad_bf16"] = fl["mlp_energy_grad"]
+    return fl, {"pairs": P, "triplets": T, "radial_evals": ER, "angular_evals": EA}
+
+
+# ---- timing helpers ------------------------------------------------------------------------------------
+
+def timed_steps(dev, prec, steps):
+    """K steps of ace_compute (neighbour build + energies/forces/virials), one synchronisation before the
+    first and after each step: (ms per step over the K steps, per-step milliseconds)."""
+    dev.sync()
+    per = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        dev.compute_step(prec)
+        dev.sync()
+        per.append((time.perf_counter() - a) * 1e3)
+    total = (time.perf_counter() - t0) * 1e3
+    return total / steps, per
+
+
+KERNELS = ("nb_bin", "nb_search", "basis_fused", "basis_symm", "mlp_energy_grad", "mlp_energy_grad_bf16",
+           "basis_adjoint", "finalize")
+
+
+def kernel_times(dev, prec, steps):
+    """Average HIP-event duration (ms) of each kernel over `steps` more steps, on the library's stream."""
+    sfx = "_f64" if prec == 2 else ""
+    dev.profile(True)
+    for _ in range(steps):
+        dev.compute_step(prec)
+    dev.sync()
+    out = {}
+    for k in KERNELS:
+        name = k + sfx if k.startswith("basis") else k
+        t, n = dev.profile_read(name)
+        if n:
+            out[k] = t / n
+    dev.profile(False)
+    return out
+
+
+def roofline(flops, kern, workload):
+    dom = max((k for k in ("basis_fused", "basis_symm", "basis_adjoint") if k in kern), key=lambda k: kern[k])
+    achieved = flops[dom] / (kern[dom] * 1e-3) / 1e12
+    traffic = None     # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary, when present
+    tf = os.path.join(ROOT, "profiles", f"{dom}_{workload}_traffic.txt")
+    if os.path.exists(tf):
+        for line in open(tf):
+            if line.startswith("bytes_per_launch:"):
+                traffic = int(float(line.split(":")[1]))
+    return {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "flops_per_launch": int(flops[dom]),
+            "avg_launch_ms": kern[dom],
+            "note": "FP32/FP64 vector-ALU kernel (no matrix cores) priced against MI355X's FP32 peak; the build "
+                    "disables packed FP32, so half that peak is this kernel's vector-ALU ceiling"}
+
+
+# ---- the reference algorithm on the host's cores (cpu_baseline leg: the FP64 oracle) --------------------
+
+def _oracle_worker(args):
+    """One worker = one MPI rank of the reference: its contiguous block of the sample's structures
+    (thread::Dist::init, dist.cpp:29-41), each through NeighborList::build, compute_symm and compute_force;
+    returns (structure index, atoms, E, F) per structure and the worker's busy time."""
+    cfg, block = args
+    from oracle import oracle as O
+    model, strucs = cfg_load(cfg)
+    orc = O.Oracle(model)
+    done = []
+    t0 = time.perf_counter()
+    for k in block:
+        E, F, _ = orc.compute(strucs[k])
+        done.append((k, strucs[k].natoms, E, F))
+    return done, time.perf_counter() - t0
+
+
+def cfg_load(cfg):
+    from ace_amd import configs
+    name, scaling = cfg
+    model, strucs = configs.config5() if name == "c5" else configs.config2()
+    o = 0
+    for I in range(model.ntypes):
+        K = model.input_width(I)
+        model.nets[I].inpw, model.nets[I].inpb = scaling[0][o:o + K], scaling[1][o:o + K]
+        o += K
+    return model, strucs
+
+
+def dist_blocks(items, workers):
+    """thread::Dist::init (dist.cpp:29-41): contiguous blocks, the first len % workers one longer."""
+    q, r = divmod(len(items), workers)
+    out, s = [], 0
+    for w in range(workers):
+        c = q + (1 if w < r else 0)
+        out.append(list(items[s:s + c]))
+        s += c
+    return out
+
+
+def cpu_baseline_sharded(cfg, sample, note):
+    """The FP64 oracle on the host's cores (capped at one GPU's share), the sample's structures
+    block-sharded over worker processes; atoms/s = atoms of the sample / the longest worker's busy time."""
+    from multiprocessing import get_context
+    ncpu = os.cpu_count() or 1
+    workers = max(1, min(ncpu, CPU_WORKERS_MAX, len(sample)))
+    blocks = dist_blocks(sample, workers)
+    t0 = time.perf_counter()
+    with get_context("spawn").Pool(workers) as pool:
+        res = pool.map(_oracle_worker, [(cfg, b) for b in blocks])
+    wall = time.perf_counter() - t0
+    done = [d for r in res for d in r[0]]
+    busy = max(r[1] for r in res)
+    atoms = sum(d[1] for d in done)
+    return {"value": atoms / busy, "unit": "atoms/s", "cores": workers, "kind": "port", "nproc": ncpu,
+            "structures": len(done), "atoms": int(atoms), "busy_s": busy, "wall_s": wall,
+            "sample": f"the FP64 oracle (reference loops: brute-force NeighborList::build, compute_symm, "
+                      f"compute_force) on {note}, block-sharded over {workers} worker processes "
+                      f"(thread::Dist::init); atoms / the longest worker's busy time"}, {d[0]: (d[2], d[3]) for d in done}
+
+
+def cpu_baseline_one(model, s, samples):
+    """The FP64 oracle on one host thread (the reference has no intra-structure parallelism)."""
     from oracle import oracle as O
     orc = O.Oracle(model)
     t0 = time.perf_counter()
@@ -70,7 +207,134 @@ def cpu_baseline(model, s, samples):
     return s.natoms / dt, out
 
 
-def train_bench(steps):
+# ---- workloads --------------------------------------------------------------------------------------------
+
+def c5_headline(args):
+    from ace_amd import PREC_BF16L1, PREC_FP32, PREC_FP64_BASIS, NNPDevice, configs
+    model, strucs = configs.config5()
+    N = sum(x.natoms for x in strucs)
+    log(f"c5: {len(strucs)} structures, {N} atoms")
+    dev = NNPDevice(model)
+    dev.upload(strucs)
+    dev.build_neighbors()
+    # PreBias MEAN / PreScale DEV (nnptefr.cpp:786-880) over the batch, from FP64 symmetry functions
+    dev.set_precision(PREC_FP64_BASIS)
+    configs.set_input_scaling(model, strucs, dev.compute_symm())
+    dev.sync_model_scaling()
+    dev.set_precision(PREC_FP32)
+    tune = {"fp32": dev.autotune(PREC_FP32), "bf16_layer1": dev.autotune(PREC_BF16L1)}
+    for _ in range(args.warmup):
+        dev.compute_step(PREC_FP32)
+    ms, per = timed_steps(dev, PREC_FP32, args.steps)
+    kern = kernel_times(dev, PREC_FP32, max(3, args.steps // 2))
+    for _ in range(2):
+        dev.compute_step(PREC_BF16L1)
+    ms_bf, per_bf = timed_steps(dev, PREC_BF16L1, args.steps)
+    kern_bf = kernel_times(dev, PREC_BF16L1, max(3, args.steps // 2))
+    flops, counts = kernel_flops(model, strucs, dev)
+    rl = roofline(flops, kern, "c5")
+    E32, F32, _ = dev.compute(PREC_FP32)
+    Ebf, Fbf, _ = dev.compute(PREC_BF16L1)
+    dev.close()
+    scaling = (np.concatenate([model.nets[I].inpw for I in range(model.ntypes)]),
+               np.concatenate([model.nets[I].inpb for I in range(model.ntypes)]))
+    mlp_ms = kern.get("mlp_energy_grad", 0.0)
+    out = {
+        "value": N / (ms * 1e-3), "ms_per_step": ms, "median_ms_per_step": float(np.median(per)),
+        "min_ms_per_step": float(np.min(per)), "max_ms_per_step": float(np.max(per)),
+        "config": {"workload": "256 structures, N log-uniform [256,8192], 4 elements 25/25/25/25, rc 6.5 A, 32 G2 + "
+                               "96 G4 per element, 128-128-64-1 tanh; step = neighbour build + E + F + virial",
+                   "global_batch": len(strucs), "atoms": int(N), **{k: int(v) for k, v in counts.items()}},
+        "roofline": rl,
+        "kernels_avg_ms": {k: round(v, 4) for k, v in kern.items()},
+        "network_mfma": {"kernel": "mlp_energy_grad", "flops_per_launch": int(flops["mlp_energy_grad"]),
+                         "avg_launch_ms": mlp_ms,
+                         "achieved_tflops": flops["mlp_energy_grad"] / (mlp_ms * 1e-3) / 1e12 if mlp_ms else None,
+                         "peak_tflops": PEAK_FP32_TFLOPS},
+        "autotune": tune,
+        "bf16_layer1": {"value": N / (ms_bf * 1e-3), "ms_per_step": ms_bf, "median_ms_per_step": float(np.median(per_bf)),
+                        "kernels_avg_ms": {k: round(v, 4) for k, v in kern_bf.items()}},
+    }
+    return out, (model, strucs, scaling, E32, F32, Ebf, Fbf)
+
+
+def c2_line(args):
+    from ace_amd import PREC_FP32, PREC_FP64_BASIS, NNPDevice, configs
+    model, [s] = configs.config2()
+    dev = NNPDevice(model)
+    dev.upload([s])
+    dev.build_neighbors()
+    dev.set_precision(PREC_FP64_BASIS)
+    configs.set_input_scaling(model, [s], dev.compute_symm())
+    dev.sync_model_scaling()
+    dev.set_precision(PREC_FP32)
+    tune = {"fp32": dev.autotune(PREC_FP32), "fp64_basis": dev.autotune(PREC_FP64_BASIS)}
+    for prec in (PREC_FP32, PREC_FP64_BASIS):
+        for _ in range(20):
+            dev.compute_step(prec)
+    dev.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.c2_steps):
+        dev.compute_step(PREC_FP32)
+    dev.sync()
+    ms = (time.perf_counter() - t0) / args.c2_steps * 1e3
+    t0 = time.perf_counter()
+    for _ in range(args.c2_steps):
+        dev.compute_step(PREC_FP64_BASIS)
+    dev.sync()
+    ms64 = (time.perf_counter() - t0) / args.c2_steps * 1e3
+    kern = kernel_times(dev, PREC_FP32, args.c2_steps)
+    kern64 = kernel_times(dev, PREC_FP64_BASIS, args.c2_steps)
+    flops, counts = kernel_flops(model, [s], dev)
+    E, F, _ = dev.compute(PREC_FP32)
+    E64, F64, _ = dev.compute(PREC_FP64_BASIS)
+    # the plain unbatched FP64 GPU port on the same structure, same step shape
+    for _ in range(3):
+        dev.build_neighbors()
+        dev.plain_compute()
+    npl = 5
+    t0 = time.perf_counter()
+    for _ in range(npl):
+        dev.build_neighbors()
+        dev.plain_compute()
+    plain_ms = (time.perf_counter() - t0) / npl * 1e3
+    dev.close()
+    scaling = (np.concatenate([model.nets[I].inpw for I in range(model.ntypes)]),
+               np.concatenate([model.nets[I].inpb for I in range(model.ntypes)]))
+    out = {"value": s.natoms / (ms * 1e-3), "unit": "atoms/s", "ms_per_step": ms,
+           "config": {"workload": "bcc 1000 atoms, 2 elements 50/50, rc 6.0 A, 80 inputs (16 G2 + 16 G4 per "
+                                  "species pair), 80-32-32-1 tanh; step = neighbour build + E + F + virial",
+                      "atoms": int(s.natoms), **{k: int(v) for k, v in counts.items()}},
+           "roofline": roofline(flops, kern, "c2"),
+           "kernels_avg_ms": {k: round(v, 5) for k, v in kern.items()}, "autotune": tune,
+           "fp64_basis": {"value": s.natoms / (ms64 * 1e-3), "ms_per_step": ms64,
+                          "kernels_avg_ms": {k: round(v, 5) for k, v in kern64.items()}},
+           "plain_fp64_port": {"value": s.natoms / (plain_ms * 1e-3), "ms_per_step": plain_ms}}
+    return out, (model, s, scaling, E, F, E64, F64)
+
+
+def plain_port_c5_sample(strucs_model, stride=16):
+    """The plain unbatched FP64 GPU port (one structure per launch, one thread per atom) on every `stride`-th
+    structure of the 568k-atom batch (a bounded sample: the port needs minutes for the whole batch)."""
+    from ace_amd import NNPDevice
+    model, strucs = strucs_model
+    sub = strucs[::stride]
+    n = sum(x.natoms for x in sub)
+    dev = NNPDevice(model)
+    dev.upload(sub)
+    dev.build_neighbors()
+    dev.plain_compute()
+    dev.sync()
+    t0 = time.perf_counter()
+    dev.build_neighbors()
+    dev.plain_compute()
+    ms = (time.perf_counter() - t0) * 1e3
+    dev.close()
+    return {"value": n / (ms * 1e-3), "unit": "atoms/s", "ms": ms, "structures": len(sub), "atoms": int(n),
+            "sample": f"every {stride}th structure of the batch ({len(sub)} structures, {n} atoms), one pass"}
+
+
+def train_line(steps):
     """Force-matching training throughput (structures/s): 64 structures of 32-1024 atoms (log-uniform),
     3 elements, random packing at 0.08 atoms/A^3, 24 G2 + 32 G4, 56-64-64-1; targets from a seeded teacher
     network of the same architecture (evaluated on the GPU in FP64-basis mode).  One step = loss and
@@ -83,8 +347,7 @@ def train_bench(steps):
     tdev.upload(strucs)
     tdev.build_neighbors()
     tdev.set_precision(PREC_FP64_BASIS)
-    G = tdev.compute_symm()
-    configs.set_input_scaling(teacher, strucs, G)
+    configs.set_input_scaling(teacher, strucs, tdev.compute_symm())
     tdev.sync_model_scaling()
     E, F, _ = tdev.compute(PREC_FP64_BASIS)
     off = np.concatenate([[0], np.cumsum([s.natoms for s in strucs])])
@@ -96,32 +359,44 @@ def train_bench(steps):
     dev = NNPDevice(model)
     dev.upload(strucs)
     dev.build_neighbors()
+    flops, counts = kernel_flops(model, strucs, dev)
+    typ = np.concatenate([s.type for s in strucs])
+    # SURVEY's training-step model: twice the basis forward + adjoint (the tangent recomputes dG/dr) plus
+    # forward, tangent-forward, weight-gradient and backward network products
+    flop_train = 2 * (flops["basis_symm"] + flops["basis_adjoint"]) + 4 * mlp_flops(model, typ)
     tr = Trainer(dev, model, strucs, gamma=1e-3, w_force=0.1, graph=True, device_opt=True)
     for _ in range(3):
         tr.step()
     dev.sync()
+    per = []
     t0 = time.perf_counter()
     first = last = None
     for k in range(steps):
+        a = time.perf_counter()
         out = tr.step()
+        per.append((time.perf_counter() - a) * 1e3)
         first = out if k == 0 else first
         last = out
     dev.sync()
     dt = (time.perf_counter() - t0) / steps
     dev.close()
+    achieved = flop_train / dt / 1e12
     return {"metric": "train structs/s", "value": len(strucs) / dt, "unit": "structures/s",
-            "ms_per_step": dt * 1e3, "structures": len(strucs), "atoms": int(off[-1]), "steps": steps, "graph": True,
-            "optimizer": "ADAM on the GPU",
+            "ms_per_step": dt * 1e3, "median_ms_per_step": float(np.median(per)), "structures": len(strucs),
+            "atoms": int(off[-1]), "steps": steps, "graph": True, "optimizer": "ADAM on the GPU",
             "loss_first": float(first[0]), "loss_last": float(last[0]), "dtype": "fp32",
+            "roofline": {"bound": "valu", "kernel": "whole training step", "achieved": achieved,
+                         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                         "flops_per_step": int(flop_train)},
             "workload": "64 structures, N log-uniform [32,1024], 3 elements, 56-64-64-1, energy + 0.1 force MSE"}
 
 
-def md_bench(steps):
+def md_line(steps):
     """Molecular dynamics throughput: 108,000-atom fcc cell (30^3 conventional cells, 0.085 atoms/A^3), 2
     elements 70/30 with masses 27/16 amu, Maxwell-Boltzmann velocities at 300 K, list radius 6 + 1 A rebuilt
     every 20 steps, 80-32-32-1 networks (output layers scaled to an rms force component of 0.1 eV/A),
     velocity Verlet with dt = 1 fs, steps between rebuilds replayed from HIP graphs.  The FP32 run's energy
-    drift is reported next to the same trajectory length with FP64 symmetry functions."""
+    drift is reported next to the same trajectory with FP64 symmetry functions."""
     from ace_amd import PREC_FP32, PREC_FP64_BASIS, NNPDevice, configs
     model, s, masses = configs.config4()
     v0 = configs.maxwell_boltzmann(masses[s.type], 300.0, np.random.default_rng(0xACE4))
@@ -162,43 +437,13 @@ def md_bench(steps):
             "workload": "fcc 108,000 atoms, 2 elements 70/30, 300 K, velocity Verlet 1 fs, list 7 A rebuilt every 20 steps"}
 
 
-def c5_bench(steps):
-    """Mixed-precision batched inference: 256 structures of 256-8192 atoms (log-uniform, 568,089 atoms), 4
-    elements, rc 6.5 A, 32 G2 + 96 G4, 128-128-64-1; FP32, and the first layer on BF16 matrix cores.
-    Per-structure errors against the FP64 oracle for the two smallest structures."""
-    from ace_amd import PREC_BF16L1, PREC_FP32, PREC_FP64_BASIS, NNPDevice, configs
-    from oracle import oracle as O
-    model, strucs = configs.config5()
-    N = sum(x.natoms for x in strucs)
-    dev = NNPDevice(model)
-    dev.upload(strucs)
-    dev.build_neighbors()
-    dev.set_precision(PREC_FP64_BASIS)
-    configs.set_input_scaling(model, strucs, dev.compute_symm())
-    dev.sync_model_scaling()
-    dev.set_precision(PREC_FP32)
-    res = {"metric": "atoms/s E+F", "unit": "atoms/s", "structures": len(strucs), "atoms": int(N), "steps": steps,
-           "workload": "256 structures, N log-uniform [256,8192], 4 elements, rc 6.5, 128-128-64-1"}
-    off = np.concatenate([[0], np.cumsum([x.natoms for x in strucs])])
-    small = [int(k) for k in np.argsort([x.natoms for x in strucs])[:2]]
-    orc = O.Oracle(model)
-    ref = {k: orc.compute(strucs[k]) for k in small}
-    for prec, name in ((PREC_FP32, "fp32"), (PREC_BF16L1, "bf16_layer1")):
-        dev.compute_step(prec)
-        dev.sync()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            dev.compute_step(prec)
-        dev.sync()
-        ms = (time.perf_counter() - t0) / steps * 1e3
-        E, F, _ = dev.compute(prec)
-        err = [{"structure": k, "atoms": strucs[k].natoms,
-                "energy_per_atom": abs(E[k] - ref[k][0]) / strucs[k].natoms,
-                "force_max_abs": float(np.max(np.abs(F[off[k]:off[k + 1]] - ref[k][1])))} for k in small]
-        res[name] = {"value": N / (ms * 1e-3), "ms_per_step": ms, "error_vs_fp64_oracle": err}
-    dev.close()
-    res["value"] = res["fp32"]["value"]
-    return res
+def errors(ref, E, F, off, strucs):
+    out = []
+    for k, (Er, Fr) in sorted(ref.items()):
+        n = strucs[k].natoms
+        out.append({"structure": int(k), "atoms": int(n), "energy_per_atom": float(abs(E[k] - Er) / n),
+                    "force_max_abs": float(np.max(np.abs(F[off[k]:off[k] + n] - Fr)))})
+    return out
 
 
 def copy_calibration(mib):
@@ -214,13 +459,16 @@ def copy_calibration(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-samples", type=int, default=12)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--steps", type=int, default=10, help="timed steps of the headline batch")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--c2-steps", type=int, default=200)
     ap.add_argument("--train-steps", type=int, default=20)
     ap.add_argument("--md-steps", type=int, default=1000)
-    ap.add_argument("--c5-steps", type=int, default=5)
+    ap.add_argument("--cpu-stride", type=int, default=2,
+                    help="the CPU baseline's sample: every n-th structure of at most 2048 atoms")
+    ap.add_argument("--cpu-samples", type=int, default=5, help="one-thread evaluations of the 1000-atom structure")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-headline", action="store_true", help="the headline batch only (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 elementwise copies of MIB MiB (reads MIB, writes MIB): the reference point "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -233,132 +481,49 @@ def main():
               "batched inference, spatial bricks with a ghost layer for MD) is not built", file=sys.stderr)
         sys.exit(2)
 
-    from ace_amd import PREC_FP32, PREC_FP64_BASIS, NNPDevice, configs
-
-    model, [s] = configs.config2()
-    dev = NNPDevice(model)
-    dev.upload([s])
-    dev.build_neighbors()
-    # PreBias MEAN / PreScale DEV (nnptefr.cpp:786-880) from FP64 symmetry functions, as the reference
-    # computes its input statistics in FP64
-    dev.set_precision(PREC_FP64_BASIS)
-    G = dev.compute_symm()
-    dev.set_precision(PREC_FP32)
-    configs.set_input_scaling(model, [s], G)
-    dev.sync_model_scaling()
-    tune = {"fp32": dev.autotune(PREC_FP32), "fp64_basis": dev.autotune(PREC_FP64_BASIS)}
-
-    def run(prec, steps, events):
-        """K steps of NNP::compute (neighbour build + energy/forces, one ace_compute call each); returns ms per
-        step and per-kernel event times."""
-        dev.profile(events)
-        dev.sync()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            dev.compute_step(prec)
-        dev.sync()
-        t1 = time.perf_counter()
-        kern = {}
-        if events:
-            sfx = "_f64" if prec == PREC_FP64_BASIS else ""
-            for k in ("nb_bin", "nb_search", "basis_fused" + sfx, "basis_symm" + sfx, "mlp_energy_grad",
-                      "basis_adjoint" + sfx):
-                v = dev.profile_read(k)
-                if v[1]:
-                    kern[k.replace(sfx, "") if sfx else k] = v
-            dev.profile(False)
-        return (t1 - t0) / steps * 1e3, kern
-
-    for prec in (PREC_FP32, PREC_FP64_BASIS):
-        run(prec, args.warmup, False)
-    # timed region: K steps, no per-kernel events
-    ms, _ = run(PREC_FP32, args.steps, False)
-    value = s.natoms / (ms * 1e-3)
-    # the same K steps again with HIP events around every kernel (per-kernel durations)
-    ms_ev, kern = run(PREC_FP32, args.steps, True)
-    ms64, _ = run(PREC_FP64_BASIS, args.steps, False)
-    _, kern64 = run(PREC_FP64_BASIS, args.steps, True)
-
-    # algorithmic work of the dominant kernel
-    off, j, img = dev.neighbors()
-    P, T, ER, EA = work_counts(model, [s], off, j, s.type)
-    flops = {
-        "basis_symm": 10 * P + 6 * ER + 15 * T + 7 * EA,
-        "basis_adjoint": 22 * P + 6 * ER + 45 * T + 7 * EA,
-        "mlp_energy_grad": mlp_flops(model, s.type),
-    }
-    # the fused kernel does all three: symmetry functions, networks and the adjoint
-    flops["basis_fused"] = flops["basis_symm"] + flops["basis_adjoint"] + flops["mlp_energy_grad"]
-    avg = {k: v[0] / v[1] for k, v in kern.items()}
-    dom = max((k for k in ("basis_fused", "basis_symm", "basis_adjoint") if k in avg), key=lambda k: avg[k])
-    achieved = flops[dom] / (avg[dom] * 1e-3) / 1e12
-    traffic = None   # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary, when present
-    tf = os.path.join(ROOT, "profiles", f"{dom}_c2_traffic.txt")
-    if os.path.exists(tf):
-        for line in open(tf):
-            if line.startswith("bytes_per_launch:"):
-                traffic = int(float(line.split(":")[1]))
-
-    # plain unbatched FP64 GPU port (the 5x denominator), same step shape
-    for _ in range(3):
-        dev.build_neighbors()
-        dev.plain_compute()
-    nplain = max(3, args.steps // 20)
-    t0 = time.perf_counter()
-    for _ in range(nplain):
-        dev.build_neighbors()
-        dev.plain_compute()
-    plain_ms = (time.perf_counter() - t0) / nplain * 1e3
-    plain_value = s.natoms / (plain_ms * 1e-3)
-
-    E, F, _ = dev.compute(PREC_FP32)
-    E64, F64, _ = dev.compute(PREC_FP64_BASIS)
-    train = train_bench(args.train_steps) if args.train_steps > 0 else None
-    md = md_bench(args.md_steps) if args.md_steps > 0 else None
-    c5 = c5_bench(args.c5_steps) if args.c5_steps > 0 else None
-    avg64 = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kern64.items()}
+    head, (m5, s5, sc5, E32, F32, Ebf, Fbf) = c5_headline(args)
+    log(f"c5: {head['value'] / 1e6:.2f} M atoms/s ({head['ms_per_step']:.2f} ms per step)")
     res = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "atoms/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic (seeded jittered bcc, random-init networks)",
-        "config": {"workload": "bcc 1000 atoms, 2 elements 50/50, rc 6.0 A, 80 inputs (16 G2 + 16 G4 per "
-                               "species pair), 80-32-32-1 tanh; step = neighbour build + E + F + virial",
-                   "global_batch": 1, "atoms": int(s.natoms), "pairs": int(P), "triplets": int(T)},
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "flops_per_launch": int(flops[dom]), "avg_launch_ms": avg[dom],
-                     "note": "FP32 vector-ALU kernel priced against the FP32 dense peak"},
-        "kernels_avg_us": {k: round(v * 1e3, 2) for k, v in avg.items()},
-        "autotune": tune,
-        "ms_per_step_with_kernel_events": ms_ev,
-        "fp64_basis": {"value": s.natoms / (ms64 * 1e-3), "unit": "atoms/s", "ms_per_step": ms64,
-                       "kernels_avg_us": {k: round(v * 1e3, 2) for k, v in avg64.items()}},
-        "plain_fp64_port": {"value": plain_value, "unit": "atoms/s", "ms_per_step": plain_ms,
-                            "speedup_batched_vs_plain": value / plain_value},
-        "train": train,
-        "md": md,
-        "mixed_precision_inference": c5,
+        "metric": METRIC, "value": head["value"], "unit": "atoms/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded jittered fcc blocks, random-init networks with batch input statistics)",
+        "config": head["config"], "roofline": head["roofline"],
+        "median_ms_per_step": head["median_ms_per_step"], "min_ms_per_step": head["min_ms_per_step"],
+        "max_ms_per_step": head["max_ms_per_step"], "kernels_avg_ms": head["kernels_avg_ms"],
+        "network_mfma": head["network_mfma"], "autotune": head["autotune"], "bf16_layer1": head["bf16_layer1"],
     }
+    if args.only_headline:
+        print(json.dumps(res))
+        return
+    res["plain_fp64_port"] = plain_port_c5_sample((m5, s5))
+    res["plain_fp64_port"]["speedup_batched_vs_plain"] = head["value"] / res["plain_fp64_port"]["value"]
+    c2, (m2, s2, sc2, E2, F2, E264, F264) = c2_line(args)
+    res["c2"] = c2
+    res["train"] = train_line(args.train_steps) if args.train_steps > 0 else None
+    res["md"] = md_line(args.md_steps) if args.md_steps > 0 else None
     if not args.no_cpu:
-        cpu_value, (E_ref, F_ref, _) = cpu_baseline(model, s, args.cpu_samples)
-        res["cpu_baseline"] = {"value": cpu_value, "unit": "atoms/s", "cores": 1, "kind": "port",
-                               "sample": f"{args.cpu_samples} full E+F evaluations of the same 1000-atom structure "
-                                         "(brute-force neighbour list, compute_symm, compute_force), FP64, 1 thread"}
-        res["speedup_vs_cpu"] = value / cpu_value
-        res["error_vs_fp64_oracle"] = {"energy_per_atom": abs(E[0] - E_ref) / s.natoms,
-                                       "force_max_abs": float(np.max(np.abs(F - F_ref)))}
-        res["fp64_basis"]["error_vs_fp64_oracle"] = {"energy_per_atom": abs(E64[0] - E_ref) / s.natoms,
-                                                     "force_max_abs": float(np.max(np.abs(F64 - F_ref)))}
-    dev.close()
+        # cpu_baseline leg: the reference algorithm (FP64 oracle) on the host, timed on the headline batch;
+        # its results double as the FP64 reference for the error report
+        # sample: every 2nd of the structures with at most 2048 atoms (the oracle's time per atom grows with N
+        # through its O(N^2) brute-force list, so this sample favours the CPU); ~10 s on 16 workers
+        small = [k for k in range(len(s5)) if s5[k].natoms <= 2048]
+        sample = small[::args.cpu_stride]
+        cpu, ref = cpu_baseline_sharded(("c5", sc5), sample, f"every {args.cpu_stride}th of the batch's structures of at "
+                                        f"most 2048 atoms ({len(sample)} structures, "
+                                        f"{sum(s5[k].natoms for k in sample)} atoms)")
+        res["cpu_baseline"] = cpu
+        res["speedup_vs_cpu"] = head["value"] / cpu["value"]
+        off = np.concatenate([[0], np.cumsum([x.natoms for x in s5])])
+        res["error_vs_fp64_oracle"] = {"fp32": errors(ref, E32, F32, off, s5), "bf16_layer1": errors(ref, Ebf, Fbf, off, s5)}
+        v1, (Er, Fr, _) = cpu_baseline_one(m2, s2, args.cpu_samples)
+        res["c2"]["cpu_one_thread"] = {"value": v1, "unit": "atoms/s", "cores": 1, "kind": "port",
+                                       "sample": f"{args.cpu_samples} evaluations of the 1000-atom structure, one thread "
+                                                 "(the reference has no intra-structure parallelism)"}
+        res["c2"]["error_vs_fp64_oracle"] = {
+            "fp32": {"energy_per_atom": float(abs(E2[0] - Er) / s2.natoms), "force_max_abs": float(np.max(np.abs(F2 - Fr)))},
+            "fp64_basis": {"energy_per_atom": float(abs(E264[0] - Er) / s2.natoms),
+                           "force_max_abs": float(np.max(np.abs(F264 - Fr)))}}
     print(json.dumps(res))
 
 

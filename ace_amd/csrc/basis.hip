@@ -32,10 +32,12 @@
 //         times faster than FP32 ones on gfx950);
 //   FP64: row[kp] (G sums, then dE/dG), red[nred], rp[2 nrp] (radial eta, rs of the species),
 //         sc[2 kp] (mode 3: inpw, inpb);
-//   T: u[4 nbm], te[4 nbm] (mode 2), E[nbm ng], Ed[nbm ng] (modes 1-3), fR[nbm], gR[nbm], ri[nbm],
+//   T (16-byte aligned): u[4 nbm] (unit vector, 1/|d|), E[nbm][2 NG] (the angular factors E and their
+//      r-derivatives Ed of a slot side by side), te[4 nbm] (mode 2), fR[nbm], gR[nbm], r[nbm],
 //      fr[nbm] (modes 1 and 3);
 //   int: p[nbm], ci[kp];  float: rlo[nbm] (T = float);  mode 3 appends the network area (netf floats:
 //   staged weights, activations, partial sums), 16-byte aligned.
+// The triplet loops read a slot's geometry and its angular factors as whole 16-byte records.
 // red holds the per-wave partial sums (BW 64, FP64) and, in the adjoint, the dE/dG weights of every
 // canonical angular block in (sign, level, group) order (14 ng FP32 values per TensorTri block).
 // Each lane sums its share of a symmetry function in T, and so does the wave reduction; the sum over the
@@ -47,30 +49,37 @@ __host__ __device__ inline int basis_nred(int nt, int ng, int mode) {
     return mode == 1 ? wt : (mode == 3 ? max(BW * 64, wt) : BW * 64);
 }
 
-template <class T, int BW>
-__host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int nrp, int mode, int netf, int nt) {
+// byte offset of the T area: the FP64 arrays, rounded up to 16
+template <int BW>
+__host__ __device__ inline int basis_t_off(int nbm, int ng, int kmax, int nrp, int mode, int nt) {
     const int kp = (kmax + 3) & ~3;
     const bool gacc = (mode == 1 || mode == 3);
     const int gb = gacc ? 8 * BW * 3 * nbm : 0;
     const int nd = kp + basis_nred<BW>(nt, ng, mode) + 2 * nrp + (mode == 3 ? 2 * kp : 0);
-    int n = 4 * nbm + ng * nbm + 3 * nbm;
+    return (gb + 8 * nd + 15) & ~15;
+}
+
+template <class T, int BW>
+__host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int nrp, int mode, int netf, int nt) {
+    const int kp = (kmax + 3) & ~3;
+    const bool gacc = (mode == 1 || mode == 3);
+    int n = 4 * nbm + 2 * ng * nbm + 3 * nbm;
     if (mode == 2) n += 4 * nbm;
-    if (mode >= 1) n += ng * nbm;
     if (gacc) n += nbm;
     const int nf = sizeof(T) == 4 ? nbm : 0;
-    const int base = (gb + 8 * nd + (int)sizeof(T) * n + 4 * (nbm + kp) + 4 * nf + 15) & ~15;
+    const int base = (basis_t_off<BW>(nbm, ng, kmax, nrp, mode, nt) + (int)sizeof(T) * n + 4 * (nbm + kp) + 4 * nf +
+                      15) & ~15;
     return base + (mode == 3 ? 4 * netf : 0);
 }
 
 template <class T>
 struct BasisLds {
-    T* u;      // [nbm][4]: unit vector d / |d| and |d| (FP64 values rounded once)
+    T* u;      // [nbm][4]: unit vector d / |d| and 1 / |d| (FP64 values rounded once)
+    T* E;      // [nbm][2 NG]: fc exp(-r^2 / eta_g^2) of each slot and eta group, then its r derivative
     T* te;     // [nbm][4]
-    T* E;      // [nbm][ng]: fc exp(-r^2 / eta_g^2) of each slot and eta group
-    T* Ed;     // [nbm][ng]: its r derivative
     T* fR;
     T* gR;
-    T* ri;     // 1 / r
+    T* r;      // |d|
     T* fr;     // modes 1, 3: radial dE/dr of each slot
     double* g; // [BW][3 nbm] per-wave dE/dd accumulators (one array per wave: a fixed order of adds)
     double* row;
@@ -93,15 +102,14 @@ __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, i
     L.row = d; d += kp;
     L.red = d; d += basis_nred<BW>(nt, ng, mode);
     L.rp = d; d += 2 * nrp;
-    L.sc = d; if (mode == 3) d += 2 * kp;
-    T* c = (T*)d;
+    L.sc = d;
+    T* c = (T*)((char*)base + basis_t_off<BW>(nbm, ng, kmax, nrp, mode, nt));
     L.u = c; c += 4 * nbm;
+    L.E = c; c += 2 * ng * nbm;
     L.te = c; if (mode == 2) c += 4 * nbm;
-    L.E = c; c += ng * nbm;
-    L.Ed = c; if (mode >= 1) c += ng * nbm;
     L.fR = c; c += nbm;
     L.gR = c; c += nbm;
-    L.ri = c; c += nbm;
+    L.r = c; c += nbm;
     L.fr = c; if (gacc) c += nbm;
     L.p = (int*)c;
     L.ci = L.p + nbm;
@@ -115,8 +123,8 @@ __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, i
 // |d| of slot q to FP64 precision
 template <class T>
 __device__ inline double slot_r64(const BasisLds<T>& L, int q) {
-    if constexpr (sizeof(T) == 8) return (double)L.u[4 * q + 3];
-    else return (double)L.u[4 * q + 3] + (double)L.rlo[q];
+    if constexpr (sizeof(T) == 8) return (double)L.r[q];
+    else return (double)L.r[q] + (double)L.rlo[q];
 }
 
 // (a, c) of flattened triplet t in block (J,K): rectangle a in J, c in K; triangle a<c in J.
@@ -166,24 +174,89 @@ __device__ inline T prm(const float* f, const double* d, int i) {
     else return f[i];
 }
 
+// floor(a / b) for 0 <= a < 2^20 and 1 <= b < 2^20, from an approximate reciprocal rb of b: the estimate
+// is within one of the quotient, and one integer step corrects it
+__device__ inline int div_small(int a, int b, float rb) {
+    int q = (int)(((float)a + 0.5f) * rb);
+    const int r = a - q * b;
+    q += (r >= b) - (r < 0);
+    return q;
+}
+
+// a slot's 4 T values (unit vector and 1/|d|, or a tangent record) in one or two 16-byte LDS reads
+template <class T>
+__device__ inline void ld4(const T* p, T& a, T& b, T& c, T& d) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 v = *(const float4*)p;
+        a = v.x; b = v.y; c = v.z; d = v.w;
+    } else {
+        const double2 v0 = *(const double2*)p, v1 = *(const double2*)(p + 2);
+        a = v0.x; b = v0.y; c = v1.x; d = v1.y;
+    }
+}
+
+// Chunk length C (1 .. 64) of an angular block with m rows of at most Dmax pairs: the one that minimises
+// rounds (C + 1/8), rounds = ceil(m ceil(Dmax / C) / BT) passes of the BT threads over the units and 1/8 the
+// cost of a chunk's set-up and flush relative to one pair (ties: the longer chunk).  Lane l tries C = l + 1;
+// every wave finds the same minimum, so C is uniform over the workgroup.
+template <int BT>
+__device__ inline int pick_chunk(int m, int Dmax, int lane) {
+    const int C = min(lane + 1, max(Dmax, 1));
+    const int nq = div_small(Dmax + C - 1, C, __builtin_amdgcn_rcpf((float)C));
+    const int rounds = (m * nq + BT - 1) / BT;
+    const int key = wave_min_i(rounds * (8 * C + 1) * 64 + (64 - C));
+    return 64 - (key & 63);
+}
+
+// the 2 NG angular factors of a slot (E then Ed; NG = 1, 2 or 4) in 8- or 16-byte LDS reads
+template <class T, int NG>
+__device__ inline void ld_e(const T* p, T* e, T* ed) {
+    if constexpr (sizeof(T) == 4 && NG == 1) {
+        const float2 v = *(const float2*)p;
+        e[0] = v.x; ed[0] = v.y;
+    } else if constexpr (sizeof(T) == 4 && NG == 2) {
+        const float4 v = *(const float4*)p;
+        e[0] = v.x; e[1] = v.y; ed[0] = v.z; ed[1] = v.w;
+    } else if constexpr (sizeof(T) == 4) {
+        const float4 a = *(const float4*)p, b = *(const float4*)(p + 4);
+        e[0] = a.x; e[1] = a.y; e[2] = a.z; e[3] = a.w;
+        ed[0] = b.x; ed[1] = b.y; ed[2] = b.z; ed[3] = b.w;
+    } else {
+#pragma unroll
+        for (int h = 0; h < NG; ++h) { e[h] = p[h]; ed[h] = p[NG + h]; }
+    }
+}
+
 // One angular block whose functions are all canonical: lambda = +-1 and zeta = 2^k, k < NZ <= 7, one
 // function per (sign s, level k, eta group g).  Per triplet the windows b_s = |1/2 + s cos/2| are raised
 // to 2^k (cp) and 2^k - 1 (cq) by squaring, and the block's sums are the outer products
 //   G[s][k][g] += b_s^(2^k) E_j[g] E_k[g]
 // held in registers with static indices; cidx maps each function of the block to (s*7 + k)*NG + g.
 // NZ (a compile-time bound on the levels in use) keeps the loop free of branches; unused levels are
-// computed and discarded.  E / Ed are interleaved per slot: E[slot * NG + g], zero for g >= ng.
+// computed and discarded.  E / Ed of slot q are E[q][g] and E[q][NG + g], zero for g >= ng.
+//
+// Triplets: every summand is symmetric in its two neighbours, so a block is a set of slot pairs.  A pair is
+// (f, v) with the fixed slot f = x of the smaller species side (x < m) and v = (x + d) mod M on the other
+// (M its size): a rectangle J x K takes every d < M; the triangle j < k of J x J takes d = 1 .. (n - 1) / 2
+// for every x and, for even n, d = n / 2 for x < n / 2 (each unordered pair once).  A row (x, all its d) is
+// cut into chunks of C consecutive d, and the workgroup's threads take the units (x, chunk) in turn: f's
+// record stays in registers over a chunk, and in the adjoint its dE/dd builds up there and reaches the
+// LDS accumulator once per chunk; pick_chunk sets C.
 // Each lane's sums (modes 0 and 2) and their wave reduce-scatter are T; the sum over the waves is FP64.
-// The adjoint (mode 1) adds each triplet's derivative into the wave's FP64 accumulators.
-template <class T, int NG, int NZ, int MODE, int BW>
-__device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int nJ, int nK, bool same, int T3,
+// The adjoint (mode 1) adds each pair's derivative for v into the wave's FP64 accumulators.
+template <class T, int NG, int NZ, int MODE>
+__device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int nJ, int nK, bool same,
                                      const int* __restrict__ cidx, const float* wtab, int nA, int goff, double* gw,
-                                     int tid, int wv, int lane) {
+                                     int wv, int lane) {
     constexpr int ADJ = (MODE == 1);
     constexpr int TAN = (MODE == 2);
-    constexpr int BT = BW * 64;
+    constexpr int BT = 64;   // one wave per block
+    constexpr int SE = 2 * NG;
+    const int tid = lane;
     T acc[2][NZ][NG];
-    T wgt[2][NZ][NG];
+    // adjoint: dE/dG weights w and the weights of d b^(2^k)/d cos = s 2^(k-1) b^(2^k - 1) (levels k >= 1; the
+    // level-0 terms are the constant wd0)
+    T wgt[2][NZ][NG], wgd[2][NZ][NG], wd0[NG];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -192,7 +265,10 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
             for (int g = 0; g < NG; ++g) {
                 acc[s][k][g] = T(0);
                 wgt[s][k][g] = T(0);
+                wgd[s][k][g] = T(0);
             }
+#pragma unroll
+    for (int g = 0; g < NG; ++g) wd0[g] = T(0);
     if (ADJ) {
         // the block's dE/dG weights in (s, k, g) order, from the table built before the angular loop
 #pragma unroll
@@ -200,111 +276,135 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
 #pragma unroll
             for (int k = 0; k < NZ; ++k)
 #pragma unroll
-                for (int g = 0; g < NG; ++g) wgt[s][k][g] = (T)wtab[(s * 7 + k) * NG + g];
+                for (int g = 0; g < NG; ++g) {
+                    wgt[s][k][g] = (T)wtab[(s * 7 + k) * NG + g];
+                    wgd[s][k][g] = wgt[s][k][g] * (T(1 << k) * (s ? T(-0.5) : T(0.5)));
+                }
+#pragma unroll
+        for (int g = 0; g < NG; ++g) wd0[g] = wgd[0][0][g] + wgd[1][0][g];
     }
-    // (row d, column x) of triplet t = d m + x, advanced by BT per iteration from one exact decode
+    // the block's pairs: fixed slots bF + x (x < m), partners bV + y
     const int m = same ? nJ : min(nJ, nK), Mx = same ? nJ : max(nJ, nK);
     const bool sw = !same && nJ > nK;
-    const int qB = BT / m, rB = BT - qB * m;
-    int x, d;
-    {
-        int a0, c0;
-        tri_decode(tid, same, nJ, nK, 1.f / (float)m, a0, c0);
-        x = sw ? c0 : a0;
-        d = (tid - x) / m;
-    }
-    for (int t = tid; t < T3; t += BT) {
-        int y = x + d + (same ? 1 : 0);
+    const int bF = sw ? bK : bJ, bV = sw ? bJ : bK;
+    const int d0 = same ? 1 : 0;
+    const int Dmax = same ? nJ / 2 : Mx;                   // longest row
+    const int xs = (same && !(nJ & 1)) ? nJ / 2 : m;       // rows x >= xs are one shorter
+    const float rm = __builtin_amdgcn_rcpf((float)m);
+    const int qB = div_small(BT, m, rm), rB = BT - qB * m; // unit u -> u + BT: chunk + qB, row + rB
+    const int C = __builtin_amdgcn_readfirstlane(pick_chunk<BT>(m, Dmax, lane));
+    const int nq = div_small(Dmax + C - 1, C, __builtin_amdgcn_rcpf((float)C));
+    const int U = m * nq;
+    int q = div_small(tid, m, rm), x = tid - q * m;       // unit u = q m + x
+    for (int u = tid; u < U; u += BT) {
+        const int j0 = q * C;
+        const int cnt = Dmax - (x >= xs ? 1 : 0) - j0;    // pairs of the row from this chunk on
+        const int sf = bF + x;
+        T fx, fy, fz, fi, Ef[NG], Edf[NG];
+        ld4<T>(L.u + 4 * sf, fx, fy, fz, fi);
+        ld_e<T, NG>(L.E + SE * sf, Ef, Edf);
+        T tfx = T(0), tfy = T(0), tfz = T(0), tfp = T(0);
+        if (TAN) ld4<T>(L.te + 4 * sf, tfx, tfy, tfz, tfp);
+        T gfx = T(0), gfy = T(0), gfz = T(0);
+        int y = x + d0 + j0;
         y -= (y >= Mx) ? Mx : 0;
-        const int a = sw ? y : x, c = sw ? x : y;
-        x += rB;
-        d += qB;
-        if (x >= m) {
-            x -= m;
-            ++d;
-        }
-        const int sa = bJ + a, sc = bK + c;
-        const T uax = L.u[4 * sa], uay = L.u[4 * sa + 1], uaz = L.u[4 * sa + 2];
-        const T ucx = L.u[4 * sc], ucy = L.u[4 * sc + 1], ucz = L.u[4 * sc + 2];
-        const T cs = uax * ucx + uay * ucy + uaz * ucz;
-        T Da[NG], Dc[NG], Dda[NG], Ddc[NG], D[NG];
+        for (int j = 0; j < C; ++j) {
+            if (j < cnt) {
+                const int sv = bV + y;
+                T vx, vy, vz, vi, Ev[NG], Edv[NG];
+                ld4<T>(L.u + 4 * sv, vx, vy, vz, vi);
+                ld_e<T, NG>(L.E + SE * sv, Ev, Edv);
+                const T cs = fx * vx + fy * vy + fz * vz;
+                T D[NG];
 #pragma unroll
-        for (int g = 0; g < NG; ++g) {
-            Da[g] = L.E[sa * NG + g];
-            Dc[g] = L.E[sc * NG + g];
-            Dda[g] = (ADJ || TAN) ? L.Ed[sa * NG + g] : T(0);
-            Ddc[g] = (ADJ || TAN) ? L.Ed[sc * NG + g] : T(0);
-            D[g] = Da[g] * Dc[g];
-        }
-        T cp[2][NZ], cq[2][NZ];
-        cp[0][0] = fabs(T(0.5) + T(0.5) * cs);
-        cp[1][0] = fabs(T(0.5) - T(0.5) * cs);
-        cq[0][0] = T(1);
-        cq[1][0] = T(1);
+                for (int g = 0; g < NG; ++g) D[g] = Ef[g] * Ev[g];
+                T cp[2][NZ], cq[2][NZ];
+                cp[0][0] = fabs(T(0.5) + T(0.5) * cs);
+                cp[1][0] = fabs(T(0.5) - T(0.5) * cs);
+                cq[0][0] = T(1);
+                cq[1][0] = T(1);
 #pragma unroll
-        for (int k = 1; k < NZ; ++k)
+                for (int k = 1; k < NZ; ++k)
 #pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                cq[s][k] = cq[s][k - 1] * cp[s][k - 1];
-                cp[s][k] = cp[s][k - 1] * cp[s][k - 1];
-            }
-        if (MODE == 0) {
+                    for (int s = 0; s < 2; ++s) {
+                        cq[s][k] = (k == 1) ? cp[s][0] : cq[s][k - 1] * cp[s][k - 1];
+                        cp[s][k] = cp[s][k - 1] * cp[s][k - 1];
+                    }
+                if (MODE == 0) {
 #pragma unroll
-            for (int k = 0; k < NZ; ++k)
+                    for (int k = 0; k < NZ; ++k)
 #pragma unroll
-                for (int s = 0; s < 2; ++s)
+                        for (int s = 0; s < 2; ++s)
 #pragma unroll
-                    for (int g = 0; g < NG; ++g) acc[s][k][g] += cp[s][k] * D[g];
-        } else if (TAN) {
-            const T tax = L.te[4 * sa], tay = L.te[4 * sa + 1], taz = L.te[4 * sa + 2], tap = L.te[4 * sa + 3];
-            const T tcx = L.te[4 * sc], tcy = L.te[4 * sc + 1], tcz = L.te[4 * sc + 2], tcp = L.te[4 * sc + 3];
-            const T dcs = (ucx * tax + ucy * tay + ucz * taz - cs * tap) * L.ri[sa] +
-                          (uax * tcx + uay * tcy + uaz * tcz - cs * tcp) * L.ri[sc];
-            T X[NG], Y[NG];
-#pragma unroll
-            for (int g = 0; g < NG; ++g) {
-                X[g] = D[g] * dcs;
-                Y[g] = Dda[g] * Dc[g] * tap + Da[g] * Ddc[g] * tcp;
-            }
-#pragma unroll
-            for (int k = 0; k < NZ; ++k)
-#pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    const T dp = T(1 << k) * (s ? T(-0.5) : T(0.5)) * cq[s][k];   // d b^(2^k) / d cos
-#pragma unroll
-                    for (int g = 0; g < NG; ++g) acc[s][k][g] += dp * X[g] + cp[s][k] * Y[g];
-                }
-        } else {
-            T S[NG], Sp[NG];
-#pragma unroll
-            for (int g = 0; g < NG; ++g) { S[g] = T(0); Sp[g] = T(0); }
-#pragma unroll
-            for (int k = 0; k < NZ; ++k)
-#pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    const T dp = T(1 << k) * (s ? T(-0.5) : T(0.5)) * cq[s][k];
+                            for (int g = 0; g < NG; ++g) acc[s][k][g] += cp[s][k] * D[g];
+                } else if (TAN) {
+                    T tvx, tvy, tvz, tvp;
+                    ld4<T>(L.te + 4 * sv, tvx, tvy, tvz, tvp);
+                    const T dcs = (vx * tfx + vy * tfy + vz * tfz - cs * tfp) * fi +
+                                  (fx * tvx + fy * tvy + fz * tvz - cs * tvp) * vi;
+                    T X[NG], Y[NG];
 #pragma unroll
                     for (int g = 0; g < NG; ++g) {
-                        S[g] += wgt[s][k][g] * cp[s][k];
-                        Sp[g] += wgt[s][k][g] * dp;
+                        X[g] = D[g] * dcs;
+                        Y[g] = Edf[g] * Ev[g] * tfp + Ef[g] * Edv[g] * tvp;
                     }
-                }
-            T dEdc = T(0), dEda = T(0), dEdcc = T(0);
 #pragma unroll
-            for (int g = 0; g < NG; ++g) {
-                dEdc += Sp[g] * D[g];
-                dEda += S[g] * Dda[g] * Dc[g];
-                dEdcc += S[g] * Da[g] * Ddc[g];
+                    for (int k = 0; k < NZ; ++k)
+#pragma unroll
+                        for (int s = 0; s < 2; ++s) {
+                            const T dp = T(1 << k) * (s ? T(-0.5) : T(0.5)) * cq[s][k];   // d b^(2^k) / d cos
+#pragma unroll
+                            for (int g = 0; g < NG; ++g) acc[s][k][g] += dp * X[g] + cp[s][k] * Y[g];
+                        }
+                } else {
+                    T S[NG], Sp[NG];
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) {
+                        S[g] = T(0);
+                        Sp[g] = wd0[g];
+                    }
+#pragma unroll
+                    for (int k = 0; k < NZ; ++k)
+#pragma unroll
+                        for (int s = 0; s < 2; ++s)
+#pragma unroll
+                            for (int g = 0; g < NG; ++g) {
+                                S[g] += wgt[s][k][g] * cp[s][k];
+                                if (k > 0) Sp[g] += wgd[s][k][g] * cq[s][k];
+                            }
+                    T dEdc = T(0), dEdf = T(0), dEdv = T(0);
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) {
+                        dEdc += Sp[g] * D[g];
+                        dEdf += S[g] * Edf[g] * Ev[g];
+                        dEdv += S[g] * Ef[g] * Edv[g];
+                    }
+                    const T cc = dEdc * cs;
+                    const T kf = dEdf - cc * fi, kv = dEdv - cc * vi;
+                    const T xf = dEdc * fi, xv = dEdc * vi;
+                    gfx += kf * fx + xf * vx;
+                    gfy += kf * fy + xf * vy;
+                    gfz += kf * fz + xf * vz;
+                    double* gv = gw + 3 * sv;
+                    atomicAdd(gv, (double)(kv * vx + xv * fx));
+                    atomicAdd(gv + 1, (double)(kv * vy + xv * fy));
+                    atomicAdd(gv + 2, (double)(kv * vz + xv * fz));
+                }
             }
-            const T ia = L.ri[sa], ic = L.ri[sc];
-            const T ka = dEda - dEdc * cs * ia, kc = dEdcc - dEdc * cs * ic;
-            const T xa = dEdc * ia, xc = dEdc * ic;
-            atomicAdd(&gw[3 * sa], (double)(ka * uax + xa * ucx));
-            atomicAdd(&gw[3 * sa + 1], (double)(ka * uay + xa * ucy));
-            atomicAdd(&gw[3 * sa + 2], (double)(ka * uaz + xa * ucz));
-            atomicAdd(&gw[3 * sc], (double)(kc * ucx + xc * uax));
-            atomicAdd(&gw[3 * sc + 1], (double)(kc * ucy + xc * uay));
-            atomicAdd(&gw[3 * sc + 2], (double)(kc * ucz + xc * uaz));
+            ++y;
+            y -= (y >= Mx) ? Mx : 0;
+        }
+        if (ADJ && cnt > 0) {
+            double* gf = gw + 3 * sf;
+            atomicAdd(gf, (double)gfx);
+            atomicAdd(gf + 1, (double)gfy);
+            atomicAdd(gf + 2, (double)gfz);
+        }
+        x += rB;
+        q += qB;
+        if (x >= m) {
+            x -= m;
+            ++q;
         }
     }
     if (!ADJ) {
@@ -327,14 +427,22 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
             const int g = f % NG, k = (f / NG) % NZ, s = f / (NG * NZ);
             L.red[wv * 64 + (s * 7 + k) * NG + g] = tot;
         }
-        __syncthreads();
-        for (int m = tid; m < nA; m += BT) {
-            double v = 0.0;
-            for (int w = 0; w < BW; ++w) v += L.red[w * 64 + cidx[m]];
-            L.row[goff + m] = v;
-        }
-        __syncthreads();
+        wave_sync();
+        for (int m = lane; m < nA; m += 64) L.row[goff + m] = L.red[wv * 64 + cidx[m]];
+        wave_sync();   // the wave's next block reuses its red slots
     }
+}
+
+// The wave that takes the next block of a phase: the least loaded one (the first on ties), charged with the
+// block's cost.  Every wave runs the same deal over the same blocks, so the owners agree without a barrier.
+template <int BW>
+__device__ inline int deal(int* load, int cost) {
+    int w = 0;
+#pragma unroll
+    for (int k = 1; k < BW; ++k) w = (load[k] < load[w]) ? k : w;
+#pragma unroll
+    for (int k = 0; k < BW; ++k) load[k] += (k == w) ? cost : 0;
+    return w;
 }
 
 // Radial and angular sums of one atom (slots, tables and row already in LDS).  PM 0: G into row;
@@ -349,54 +457,54 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
     constexpr int TAN = (PM == 2);
     constexpr int MAXA = 16;   // functions per register chunk of the generic path
     const int ng = sp.ng;
-    // --- radial blocks: lane = function, lane group = neighbour subset ---------------------------
-    for (int J = 0; J < nt; ++J) {
-        const int bJ = s_cnt[J], nJ = s_cnt[J + 1] - s_cnt[J];
-        const int nR = sp.r_n[J];
-        if (nJ == 0 || nR == 0) continue;
-        const int form = sp.r_form[J];
-        const int goff = sp.r_goff[J], poff = sp.r_poff[J];
-        for (int n0 = 0; n0 < nR; n0 += 64) {
-            const int nc = min(64, nR - n0);
-            int nrp = 1;
-            while (nrp < nc) nrp <<= 1;
-            const int gpw = 64 / nrp, ngr = BW * gpw;
-            const int n = lane % nrp, grp = wv * gpw + lane / nrp;
-            const bool on = n < nc;
-            const int rq = poff - sp.r_poff[0] + n0 + n;
-            const double eta = on ? L.rp[rq] : 0.0;
-            const double rs = on ? L.rp[sp.nR + rq] : 0.0;
-            const T w = (ADJ && on) ? (T)L.row[goff + n0 + n] : T(0);
-            T acc = T(0);
-            const int rounds = (nJ + ngr - 1) / ngr;
-            for (int t = 0; t < rounds; ++t) {
-                const int q = grp + t * ngr;
-                T val = T(0);
-                if (on && q < nJ) {
-                    const int slot = bJ + q;
-                    T v, dv;
-                    t_radial_r64<T>(form, eta, rs, slot_r64(L, slot), L.fR[slot], L.gR[slot], v, dv);
-                    val = ADJ ? w * dv : (TAN ? dv * L.te[4 * slot + 3] : v);
+    // --- radial blocks: one wave per neighbour species J; lane = function, lane group = neighbour subset;
+    // the lanes of a function sum their groups in FP64 ----------------------------------------------
+    {
+        int load[BW];
+#pragma unroll
+        for (int w = 0; w < BW; ++w) load[w] = 0;
+        for (int J = 0; J < nt; ++J) {
+            const int bJ = s_cnt[J], nJ = s_cnt[J + 1] - s_cnt[J];
+            const int nR = sp.r_n[J];
+            if (nJ == 0 || nR == 0) continue;
+            if (deal<BW>(load, nJ * nR) != wv) continue;
+            const int form = sp.r_form[J];
+            const int goff = sp.r_goff[J], poff = sp.r_poff[J];
+            for (int n0 = 0; n0 < nR; n0 += 64) {
+                const int nc = min(64, nR - n0);
+                int nrp = 1;
+                while (nrp < nc) nrp <<= 1;
+                const int gpw = 64 / nrp;
+                const int n = lane % nrp, grp = lane / nrp;
+                const bool on = n < nc;
+                const int rq = poff - sp.r_poff[0] + n0 + n;
+                const double eta = on ? L.rp[rq] : 0.0;
+                const double rs = on ? L.rp[sp.nR + rq] : 0.0;
+                const T w = (ADJ && on) ? (T)L.row[goff + n0 + n] : T(0);
+                T acc = T(0);
+                const int rounds = (nJ + gpw - 1) / gpw;
+                for (int t = 0; t < rounds; ++t) {
+                    const int q = grp + t * gpw;
+                    T val = T(0);
+                    if (on && q < nJ) {
+                        const int slot = bJ + q;
+                        T v, dv;
+                        t_radial_r64<T>(form, eta, rs, slot_r64(L, slot), L.fR[slot], L.gR[slot], v, dv);
+                        val = ADJ ? w * dv : (TAN ? dv * L.te[4 * slot + 3] : v);
+                    }
+                    if (ADJ) {
+                        // dE/dr of the slot: sum over the nrp function lanes of its group; one lane group per slot
+                        val = t_group_sum<T>(val, nrp);
+                        if (n == 0 && q < nJ) L.fr[bJ + q] += val;
+                    } else {
+                        acc += val;
+                    }
                 }
-                if (ADJ) {
-                    // dE/dr of the slot: sum over the nrp function lanes of its group; one lane group per slot
-                    val = t_group_sum<T>(val, nrp);
-                    if (n == 0 && q < nJ) L.fr[bJ + q] += val;
-                } else {
-                    acc += val;
+                if (ADJ) wave_sync();   // the next chunk maps slots to other lane groups
+                if (!ADJ) {
+                    const double tot = t_cross_sum<double>((double)acc, nrp);
+                    if (lane < nc) L.row[goff + n0 + lane] = tot;
                 }
-            }
-            if (ADJ && nR > 64) __syncthreads();   // the next chunk maps slots to other lane groups
-            if (!ADJ) {
-                L.red[wv * 64 + lane] = (double)acc;   // lane n + nrp * group of each wave
-                __syncthreads();
-                if (tid < nc) {
-                    double s = 0.0;
-                    for (int ww = 0; ww < BW; ++ww)
-                        for (int g = tid; g < 64; g += nrp) s += L.red[ww * 64 + g];
-                    L.row[goff + n0 + tid] = s;
-                }
-                __syncthreads();
             }
         }
     }
@@ -415,9 +523,28 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
         __syncthreads();
     }
 
-    // --- angular blocks (J <= K) -------------------------------------------------------------------
+    // --- angular blocks (J <= K): canonical ones one wave each, then the others with all waves ----------
     double* gw = L.g + wv * 3 * nbm;
-    for (int K = 0; K < nt; ++K) {
+    {
+        int load[BW];
+#pragma unroll
+        for (int w = 0; w < BW; ++w) load[w] = 0;
+        for (int K = 0; K < nt; ++K) {
+            for (int J = 0; J <= K; ++J) {
+                const int b = K * (K + 1) / 2 + J;
+                const int nA = sp.a_n[b];
+                const bool same = (J == K);
+                const int bJ = s_cnt[J], bK = s_cnt[K];
+                const int nJ = s_cnt[J + 1] - bJ, nK = s_cnt[K + 1] - bK;
+                const int T3 = same ? nJ * (nJ - 1) / 2 : nJ * nK;
+                if (nA == 0 || T3 == 0 || (GEN && !sp.a_canon[b])) continue;
+                if (deal<BW>(load, T3) != wv) continue;
+                angular_canon<T, NG, NZ, MODE>(L, bJ, bK, nJ, nK, same, L.ci + (sp.a_poff[b] - sp.a_poff[0]),
+                                               (const float*)L.red + 14 * NG * b, nA, sp.a_goff[b], gw, wv, lane);
+            }
+        }
+    }
+    for (int K = 0; K < (GEN ? nt : 0); ++K) {
         for (int J = 0; J <= K; ++J) {
             const int b = K * (K + 1) / 2 + J;
             const int nA = sp.a_n[b];
@@ -425,12 +552,7 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
             const int bJ = s_cnt[J], bK = s_cnt[K];
             const int nJ = s_cnt[J + 1] - bJ, nK = s_cnt[K + 1] - bK;
             const int T3 = same ? nJ * (nJ - 1) / 2 : nJ * nK;
-            if (nA == 0 || T3 == 0) continue;
-            if (!GEN || sp.a_canon[b]) {
-                angular_canon<T, NG, NZ, MODE, BW>(L, bJ, bK, nJ, nK, same, T3, L.ci + (sp.a_poff[b] - sp.a_poff[0]),
-                                               (const float*)L.red + 14 * NG * b, nA, sp.a_goff[b], gw, tid, wv, lane);
-                continue;
-            }
+            if (nA == 0 || T3 == 0 || sp.a_canon[b]) continue;
             const int goff = sp.a_goff[b], poff = sp.a_poff[b];
             const float rnK = 1.f / (float)(same ? nJ : min(nJ, nK));
             for (int m0 = 0; m0 < nA; m0 += MAXA) {
@@ -463,10 +585,10 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
 #pragma unroll
                     for (int g = 0; g < ACE_MAXG; ++g) {
                         const bool on = g < ng;
-                        Da[g] = on ? L.E[sa * NG + g] : T(0);
-                        Dc[g] = on ? L.E[sc * NG + g] : T(0);
-                        Dda[g] = (on && (ADJ || TAN)) ? L.Ed[sa * NG + g] : T(0);
-                        Ddc[g] = (on && (ADJ || TAN)) ? L.Ed[sc * NG + g] : T(0);
+                        Da[g] = on ? L.E[sa * 2 * NG + g] : T(0);
+                        Dc[g] = on ? L.E[sc * 2 * NG + g] : T(0);
+                        Dda[g] = (on && (ADJ || TAN)) ? L.E[sa * 2 * NG + NG + g] : T(0);
+                        Ddc[g] = (on && (ADJ || TAN)) ? L.E[sc * 2 * NG + NG + g] : T(0);
                     }
                     // cp[k + 7s] = b_s^(2^k), cq[k + 7s] = b_s^(2^k - 1), b_0 = (1+cos)/2, b_1 = (1-cos)/2
                     T cp[14], cq[14];
@@ -493,8 +615,8 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
                     if (TAN) {
                         const T tax = L.te[4 * sa], tay = L.te[4 * sa + 1], taz = L.te[4 * sa + 2], tap = L.te[4 * sa + 3];
                         const T tcx = L.te[4 * sc], tcy = L.te[4 * sc + 1], tcz = L.te[4 * sc + 2], tcp = L.te[4 * sc + 3];
-                        const T dcs = (ucx * tax + ucy * tay + ucz * taz - cs * tap) * L.ri[sa] +
-                                      (uax * tcx + uay * tcy + uaz * tcz - cs * tcp) * L.ri[sc];
+                        const T dcs = (ucx * tax + ucy * tay + ucz * taz - cs * tap) * L.u[4 * sa + 3] +
+                                      (uax * tcx + uay * tcy + uaz * tcz - cs * tcp) * L.u[4 * sc + 3];
 #pragma unroll
                         for (int g = 0; g < ACE_MAXG; ++g) {
                             Y[g] = Dda[g] * Dc[g] * tap + Da[g] * Ddc[g] * tcp;
@@ -535,7 +657,7 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
                             dEda += S[g] * Dda[g] * Dc[g];
                             dEdcc += S[g] * Da[g] * Ddc[g];
                         }
-                        const T ia = L.ri[sa], ic = L.ri[sc];
+                        const T ia = L.u[4 * sa + 3], ic = L.u[4 * sc + 3];
                         const T ka = dEda - dEdc * cs * ia, kc = dEdcc - dEdc * cs * ic;
                         const T xa = dEdc * ia, xc = dEdc * ic;
                         atomicAdd(&gw[3 * sa], (double)(ka * uax + xa * ucx));
@@ -833,8 +955,8 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
                         L.u[4 * q] = (T)(dx * ir);
                         L.u[4 * q + 1] = (T)(dy * ir);
                         L.u[4 * q + 2] = (T)(dz * ir);
-                        L.u[4 * q + 3] = (T)r;
-                        L.ri[q] = (T)ir;
+                        L.u[4 * q + 3] = (T)ir;
+                        L.r[q] = (T)r;
                         if (!F64) L.rlo[q] = (float)(r - (double)(float)r);
                     }
                     fill[J] += __popcll(m);
@@ -906,7 +1028,7 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
         L.fR[q] = fc;
         L.gR[q] = gc;
         if (sp.a_cut != sp.r_cut[J] || sp.a_rc64 != sp.r_rc64[J]) t_cutoff_r64<T>(sp.a_cut, sp.a_rc64, r, fc, gc);
-        const T fa = fc, ga = gc, rt = L.u[4 * q + 3];
+        const T fa = fc, ga = gc, rt = L.r[q];
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             T ev = T(0), edv = T(0);
@@ -916,8 +1038,8 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
                 ev = fa * e;
                 edv = (ga - T(2) * rt * ie * fa) * e;
             }
-            L.E[q * NG + g] = ev;
-            if (MODE >= 1) L.Ed[q * NG + g] = edv;
+            L.E[q * 2 * NG + g] = ev;
+            L.E[q * 2 * NG + NG + g] = edv;
         }
         if (GACC) {
             L.fr[q] = T(0);

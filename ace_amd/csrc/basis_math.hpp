@@ -176,7 +176,7 @@ __device__ inline T t_pow_window(T x, T z, int zint) {
 
 // Cross-lane moves through DPP (data-parallel primitives: a VALU operand modifier, no LDS round trip).
 // CTRL: 0xB1 quad_perm(1,0,3,2), 0x4E quad_perm(2,3,0,1), 0x141 row_half_mirror, 0x140 row_mirror,
-// 0x142 row_bcast:15, 0x143 row_bcast:31.  Rows outside ROWMASK receive 0.
+// 0x142 row_bcast:15, 0x143 row_bcast:31, 0x120 + r row_ror:r (rotation by r within the 16-lane row).  Rows outside ROWMASK receive 0.
 template <int CTRL, int ROWMASK>
 __device__ inline float dpp_mov(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWMASK, 0xF, false));
@@ -209,6 +209,51 @@ __device__ inline T t_wave_sum63(T v) {
     return v;
 }
 
+// gfx950 lane swaps between two registers: B = 5 exchanges the upper 32 lanes of a with the lower 32 lanes
+// of b (v_permlane32_swap), B = 4 the odd 16-lane rows of a with the even rows of b (v_permlane16_swap).
+// Afterwards a + b holds a(l) + a(l ^ 2^B) in the lanes l whose bit B is clear and b(l) + b(l ^ 2^B) in the
+// others.
+template <int B>
+__device__ inline void swap_rows(float& a, float& b) {
+    const unsigned x = __builtin_bit_cast(unsigned, a), y = __builtin_bit_cast(unsigned, b);
+    unsigned p, q;
+    if constexpr (B == 5) {
+        const auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+        p = r[0]; q = r[1];
+    } else {
+        static_assert(B == 4, "lane swaps exchange 32- or 16-lane halves");
+        const auto r = __builtin_amdgcn_permlane16_swap(x, y, false, false);
+        p = r[0]; q = r[1];
+    }
+    a = __builtin_bit_cast(float, p);
+    b = __builtin_bit_cast(float, q);
+}
+template <int B>
+__device__ inline void swap_rows(double& a, double& b) {
+    const long long x = __builtin_bit_cast(long long, a), y = __builtin_bit_cast(long long, b);
+    float xl = __builtin_bit_cast(float, (unsigned)(x & 0xffffffffll)), xh = __builtin_bit_cast(float, (unsigned)(x >> 32));
+    float yl = __builtin_bit_cast(float, (unsigned)(y & 0xffffffffll)), yh = __builtin_bit_cast(float, (unsigned)(y >> 32));
+    swap_rows<B>(xl, yl);
+    swap_rows<B>(xh, yh);
+    a = __builtin_bit_cast(double, ((long long)__builtin_bit_cast(unsigned, xh) << 32) | (long long)__builtin_bit_cast(unsigned, xl));
+    b = __builtin_bit_cast(double, ((long long)__builtin_bit_cast(unsigned, yh) << 32) | (long long)__builtin_bit_cast(unsigned, yl));
+}
+
+// minimum over the wave, in every lane: row rotations, then lane swaps across rows
+__device__ inline int wave_min_i(int v) {
+    v = min(v, __builtin_amdgcn_update_dpp(0, v, 0x121, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(0, v, 0x122, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(0, v, 0x124, 0xF, 0xF, false));
+    v = min(v, __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false));
+    float a = __builtin_bit_cast(float, v), b = a;
+    swap_rows<4>(a, b);
+    v = min(__builtin_bit_cast(int, a), __builtin_bit_cast(int, b));
+    a = __builtin_bit_cast(float, v);
+    b = a;
+    swap_rows<5>(a, b);
+    return min(__builtin_bit_cast(int, a), __builtin_bit_cast(int, b));
+}
+
 // sum over aligned groups of n lanes (n a power of two, wave-uniform), returned in every lane of the group
 template <class T>
 __device__ inline T t_group_sum(T v, int n) {
@@ -216,8 +261,16 @@ __device__ inline T t_group_sum(T v, int n) {
     if (n >= 4) v += dpp_mov<0x4E, 0xF>(v);
     if (n >= 8) v += dpp_mov<0x141, 0xF>(v);
     if (n >= 16) v += dpp_mov<0x140, 0xF>(v);
-    if (n >= 32) v += __shfl_xor(v, 16, 64);
-    if (n >= 64) v += __shfl_xor(v, 32, 64);
+    if (n >= 32) {
+        T a = v, b = v;
+        swap_rows<4>(a, b);
+        v = a + b;
+    }
+    if (n >= 64) {
+        T a = v, b = v;
+        swap_rows<5>(a, b);
+        v = a + b;
+    }
     return v;
 }
 
@@ -235,16 +288,25 @@ __device__ inline T lane_swap(T v) {
 }
 
 // halving steps from bit B down: lanes with bit B set keep the upper half of v, the others the lower half,
-// and add the partner's copy of the half they keep
+// and add the partner's copy of the half they keep (bits 5 and 4: one lane swap of the pair of halves)
 template <int B, int H, class T>
 __device__ inline void rs_steps(T* v, int lane) {
     if constexpr (H >= 1) {
-        const bool hi = (lane >> B) & 1;
+        if constexpr (B >= 4) {
 #pragma unroll
-        for (int j = 0; j < H; ++j) {
-            const T send = hi ? v[j] : v[j + H];
-            const T keep = hi ? v[j + H] : v[j];
-            v[j] = keep + lane_swap<B>(send);
+            for (int j = 0; j < H; ++j) {
+                T a = v[j], b = v[j + H];
+                swap_rows<B>(a, b);
+                v[j] = a + b;
+            }
+        } else {
+            const bool hi = (lane >> B) & 1;
+#pragma unroll
+            for (int j = 0; j < H; ++j) {
+                const T send = hi ? v[j] : v[j + H];
+                const T keep = hi ? v[j + H] : v[j];
+                v[j] = keep + lane_swap<B>(send);
+            }
         }
         rs_steps<B - 1, H / 2>(v, lane);
     }
@@ -252,8 +314,36 @@ __device__ inline void rs_steps(T* v, int lane) {
 
 template <int B, class T>
 __device__ inline T sum_steps(T v) {
-    if constexpr (B >= 0) return sum_steps<B - 1>(v + lane_swap<B>(v));
-    else return v;
+    if constexpr (B >= 4) {
+        T a = v, b = v;
+        swap_rows<B>(a, b);
+        return sum_steps<B - 1>(a + b);
+    } else if constexpr (B >= 0) {
+        return sum_steps<B - 1>(v + lane_swap<B>(v));
+    } else {
+        return v;
+    }
+}
+
+// sum over the lanes l' with l' = l mod n (n a power of two up to 32, wave-uniform), in every lane:
+// row rotations by n, 2n, .. 8 (DPP row_ror) sum within the 16-lane row, lane swaps across rows
+template <class T>
+__device__ inline T t_cross_sum(T v, int n) {
+    if (n <= 1) v += dpp_mov<0x121, 0xF>(v);
+    if (n <= 2) v += dpp_mov<0x122, 0xF>(v);
+    if (n <= 4) v += dpp_mov<0x124, 0xF>(v);
+    if (n <= 8) v += dpp_mov<0x128, 0xF>(v);
+    if (n <= 16) {
+        T a = v, b = v;
+        swap_rows<4>(a, b);
+        v = a + b;
+    }
+    if (n <= 32) {
+        T a = v, b = v;
+        swap_rows<5>(a, b);
+        v = a + b;
+    }
+    return v;
 }
 
 // Reduce-scatter of M = 2^LM values per lane over the wave: returns the wave total of value

@@ -196,7 +196,7 @@ __device__ inline void ld4(const T* p, T& a, T& b, T& c, T& d) {
 }
 
 // Chunk length C (1 .. 64) of an angular block with m rows of at most Dmax pairs: the one that minimises
-// rounds (C + 1/8), rounds = ceil(m ceil(Dmax / C) / BT) passes of the BT threads over the units and 1/8 the
+// rounds (C + 3/8), rounds = ceil(m ceil(Dmax / C) / BT) passes of the BT threads over the units and 3/8 the
 // cost of a chunk's set-up and flush relative to one pair (ties: the longer chunk).  Lane l tries C = l + 1;
 // every wave finds the same minimum, so C is uniform over the workgroup.
 template <int BT>
@@ -204,7 +204,7 @@ __device__ inline int pick_chunk(int m, int Dmax, int lane) {
     const int C = min(lane + 1, max(Dmax, 1));
     const int nq = div_small(Dmax + C - 1, C, __builtin_amdgcn_rcpf((float)C));
     const int rounds = (m * nq + BT - 1) / BT;
-    const int key = wave_min_i(rounds * (8 * C + 1) * 64 + (64 - C));
+    const int key = wave_min_i(rounds * (8 * C + 3) * 64 + (64 - C));
     return 64 - (key & 63);
 }
 
@@ -254,9 +254,8 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
     constexpr int SE = 2 * NG;
     const int tid = lane;
     T acc[2][NZ][NG];
-    // adjoint: dE/dG weights w and the weights of d b^(2^k)/d cos = s 2^(k-1) b^(2^k - 1) (levels k >= 1; the
-    // level-0 terms are the constant wd0)
-    T wgt[2][NZ][NG], wgd[2][NZ][NG], wd0[NG];
+    // adjoint: dE/dG weights w and the weights u = w s 2^(k-1) of d b^(2^k)/d cos = s 2^(k-1) b^(2^k - 1)
+    T wgt[2][NZ][NG], wgd[2][NZ][NG];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -267,8 +266,6 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
                 wgt[s][k][g] = T(0);
                 wgd[s][k][g] = T(0);
             }
-#pragma unroll
-    for (int g = 0; g < NG; ++g) wd0[g] = T(0);
     if (ADJ) {
         // the block's dE/dG weights in (s, k, g) order, from the table built before the angular loop
 #pragma unroll
@@ -280,8 +277,6 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
                     wgt[s][k][g] = (T)wtab[(s * 7 + k) * NG + g];
                     wgd[s][k][g] = wgt[s][k][g] * (T(1 << k) * (s ? T(-0.5) : T(0.5)));
                 }
-#pragma unroll
-        for (int g = 0; g < NG; ++g) wd0[g] = wgd[0][0][g] + wgd[1][0][g];
     }
     // the block's pairs: fixed slots bF + x (x < m), partners bV + y
     const int m = same ? nJ : min(nJ, nK), Mx = same ? nJ : max(nJ, nK);
@@ -318,6 +313,7 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
                 T D[NG];
 #pragma unroll
                 for (int g = 0; g < NG; ++g) D[g] = Ef[g] * Ev[g];
+                // cp[s][k] = b_s^(2^k) (the adjoint needs k < NZ - 1 only), cq[s][k] = b_s^(2^k - 1) (tangent)
                 T cp[2][NZ], cq[2][NZ];
                 cp[0][0] = fabs(T(0.5) + T(0.5) * cs);
                 cp[1][0] = fabs(T(0.5) - T(0.5) * cs);
@@ -327,8 +323,8 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
                 for (int k = 1; k < NZ; ++k)
 #pragma unroll
                     for (int s = 0; s < 2; ++s) {
-                        cq[s][k] = (k == 1) ? cp[s][0] : cq[s][k - 1] * cp[s][k - 1];
-                        cp[s][k] = cp[s][k - 1] * cp[s][k - 1];
+                        cq[s][k] = TAN ? ((k == 1) ? cp[s][0] : cq[s][k - 1] * cp[s][k - 1]) : T(0);
+                        cp[s][k] = (ADJ && k == NZ - 1) ? T(0) : cp[s][k - 1] * cp[s][k - 1];
                     }
                 if (MODE == 0) {
 #pragma unroll
@@ -357,34 +353,36 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
                             for (int g = 0; g < NG; ++g) acc[s][k][g] += dp * X[g] + cp[s][k] * Y[g];
                         }
                 } else {
+                    // S_g = sum_s b (w_0 + b (w_1 + b^2 (w_2 + b^4 (w_3 + ...)))) = sum_s,k w_k b^(2^k) and
+                    // Sp_g = sum_s (u_0 + b (u_1 + b^2 (u_2 + ...))) = sum_s,k u_k b^(2^k - 1), u_k = w_k d b^(2^k)/d cos
+                    // / b^(2^k - 1): Horner over the squares b^(2^k), k < NZ - 1
                     T S[NG], Sp[NG];
 #pragma unroll
-                    for (int g = 0; g < NG; ++g) {
-                        S[g] = T(0);
-                        Sp[g] = wd0[g];
-                    }
+                    for (int s = 0; s < 2; ++s)
 #pragma unroll
-                    for (int k = 0; k < NZ; ++k)
+                        for (int g = 0; g < NG; ++g) {
+                            T h = wgt[s][NZ - 1][g], hd = wgd[s][NZ - 1][g];
 #pragma unroll
-                        for (int s = 0; s < 2; ++s)
-#pragma unroll
-                            for (int g = 0; g < NG; ++g) {
-                                S[g] += wgt[s][k][g] * cp[s][k];
-                                if (k > 0) Sp[g] += wgd[s][k][g] * cq[s][k];
+                            for (int k = NZ - 2; k >= 0; --k) {
+                                h = h * cp[s][k] + wgt[s][k][g];
+                                hd = hd * cp[s][k] + wgd[s][k][g];
                             }
-                    T dEdc = T(0), dEdf = T(0), dEdv = T(0);
+                            S[g] = s ? S[g] + h * cp[s][0] : h * cp[s][0];
+                            Sp[g] = s ? Sp[g] + hd : hd;
+                        }
+                    T dEdc = Sp[0] * D[0], dEdf = S[0] * (Edf[0] * Ev[0]), dEdv = S[0] * (Ef[0] * Edv[0]);
 #pragma unroll
-                    for (int g = 0; g < NG; ++g) {
+                    for (int g = 1; g < NG; ++g) {
                         dEdc += Sp[g] * D[g];
-                        dEdf += S[g] * Edf[g] * Ev[g];
-                        dEdv += S[g] * Ef[g] * Edv[g];
+                        dEdf += S[g] * (Edf[g] * Ev[g]);
+                        dEdv += S[g] * (Ef[g] * Edv[g]);
                     }
                     const T cc = dEdc * cs;
                     const T kf = dEdf - cc * fi, kv = dEdv - cc * vi;
                     const T xf = dEdc * fi, xv = dEdc * vi;
-                    gfx += kf * fx + xf * vx;
-                    gfy += kf * fy + xf * vy;
-                    gfz += kf * fz + xf * vz;
+                    gfx = fma(xf, vx, fma(kf, fx, gfx));
+                    gfy = fma(xf, vy, fma(kf, fy, gfy));
+                    gfz = fma(xf, vz, fma(kf, fz, gfz));
                     double* gv = gw + 3 * sv;
                     atomicAdd(gv, (double)(kv * vx + xv * fx));
                     atomicAdd(gv + 1, (double)(kv * vy + xv * fy));

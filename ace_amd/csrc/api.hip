@@ -319,8 +319,8 @@ int parse_model(ace_ctx* ctx, const ace_model_desc* m) {
             s.a_zeta.emplace_back(m->a_zeta + o, m->a_zeta + o + n);
             s.a_lambda.emplace_back(m->a_lambda + o, m->a_lambda + o + n);
             if (n < 0) return fail(ctx, "ace_create: negative angular block size");
-            if (n > 0 && m->a_form[q] != ACE_ANG_GAUSS)
-                return fail(ctx, "ace_create: angular form not supported by the GPU path (GAUSS only)");
+            if (n > 0 && (m->a_form[q] < ACE_ANG_GAUSS || m->a_form[q] > ACE_ANG_STUDENT5))
+                return fail(ctx, "ace_create: invalid angular function name");
             if (n > 0 && (m->a_cut[q] < ACE_CUT_STEP || m->a_cut[q] > ACE_CUT_POLY3))
                 return fail(ctx, "ace_create: invalid cutoff name");
             if (n > 0 && !(m->a_rc[q] > 0)) return fail(ctx, "ace_create: invalid basis cutoff");
@@ -572,7 +572,7 @@ int build_tables(ace_ctx* ctx) {
         DevSpecies& d = dm.sp[I];
         int nz = 1;
         for (int b = 0; b < nab; ++b) {
-            bool canon = d.a_n[b] > 0;
+            bool canon = d.a_n[b] > 0 && d.a_form[b] == ACE_ANG_GAUSS;   // the others are not separable
             std::vector<char> used(14 * ngk, 0);
             for (int q = 0; q < d.a_n[b] && canon; ++q) {
                 const int fi = d.a_poff[b] + q, sel = a_sel[fi];
@@ -643,6 +643,7 @@ int build_tables(ace_ctx* ctx) {
             p.a_n[b] = s.a_n[b]; p.a_goff[b] = goff; p.a_poff[b] = (int)aeta.size(); p.a_rc[b] = s.a_rc[b];
             goff += s.a_n[b];
             if (s.a_n[b] && s.a_cut[b] != ACE_CUT_COS) { ctx->plain_ok = false; ctx->plain_why = "angular cutoff"; }
+            if (s.a_n[b] && s.a_form[b] != ACE_ANG_GAUSS) { ctx->plain_ok = false; ctx->plain_why = "angular form"; }
             for (int q = 0; q < s.a_n[b]; ++q) {
                 aeta.push_back(s.a_eta[b][q]); azeta.push_back(s.a_zeta[b][q]); alamf.push_back(0.5 * s.a_lambda[b][q]);
             }
@@ -804,10 +805,10 @@ int net_lds_floats(const ace_ctx* ctx) {
 int basis_bytes(ace_ctx* ctx, int mode, int waves = 4) {
     const int nbm = nb_slots(ctx), netf = mode == 3 ? net_lds_floats(ctx) : 0;
     if (waves == 2)
-        return ctx->basis_f64 ? basis_lds_bytes<double, 2>(nbm, ctx->ngk, ctx->kmax, ctx->nrp, mode, netf, ctx->nt)
-                              : basis_lds_bytes<float, 2>(nbm, ctx->ngk, ctx->kmax, ctx->nrp, mode, netf, ctx->nt);
-    return ctx->basis_f64 ? basis_lds_bytes<double, 4>(nbm, ctx->ngk, ctx->kmax, ctx->nrp, mode, netf, ctx->nt)
-                          : basis_lds_bytes<float, 4>(nbm, ctx->ngk, ctx->kmax, ctx->nrp, mode, netf, ctx->nt);
+        return ctx->basis_f64 ? basis_lds_bytes<double, 2>(nbm, ctx->ngk, ctx->kmax, ctx->nrp, mode, netf, ctx->nt, !ctx->all_canon)
+                              : basis_lds_bytes<float, 2>(nbm, ctx->ngk, ctx->kmax, ctx->nrp, mode, netf, ctx->nt, !ctx->all_canon);
+    return ctx->basis_f64 ? basis_lds_bytes<double, 4>(nbm, ctx->ngk, ctx->kmax, ctx->nrp, mode, netf, ctx->nt, !ctx->all_canon)
+                          : basis_lds_bytes<float, 4>(nbm, ctx->ngk, ctx->kmax, ctx->nrp, mode, netf, ctx->nt, !ctx->all_canon);
 }
 
 // The fused kernel (symmetry functions, network, adjoint of one atom per workgroup) saves two launches and

@@ -34,7 +34,7 @@
 //         sc[2 kp] (mode 3: inpw, inpb);
 //   T (16-byte aligned): u[4 nbm] (unit vector, 1/|d|), E[nbm][2 NG] (the angular factors E and their
 //      r-derivatives Ed of a slot side by side), te[4 nbm] (mode 2), fR[nbm], gR[nbm], r[nbm],
-//      fr[nbm] (modes 1 and 3);
+//      fr[nbm] (modes 1 and 3), fA[nbm], gA[nbm] (generic angular path: angular cutoff and derivative);
 //   int: p[nbm], ci[kp];  float: rlo[nbm] (T = float);  mode 3 appends the network area (netf floats:
 //   staged weights, activations, partial sums), 16-byte aligned.
 // The triplet loops read a slot's geometry and its angular factors as whole 16-byte records.
@@ -60,12 +60,13 @@ __host__ __device__ inline int basis_t_off(int nbm, int ng, int kmax, int nrp, i
 }
 
 template <class T, int BW>
-__host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int nrp, int mode, int netf, int nt) {
+__host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int nrp, int mode, int netf, int nt, bool gen) {
     const int kp = (kmax + 3) & ~3;
     const bool gacc = (mode == 1 || mode == 3);
     int n = 4 * nbm + 2 * ng * nbm + 3 * nbm;
     if (mode == 2) n += 4 * nbm;
     if (gacc) n += nbm;
+    if (gen) n += 2 * nbm;
     const int nf = sizeof(T) == 4 ? nbm : 0;
     const int base = (basis_t_off<BW>(nbm, ng, kmax, nrp, mode, nt) + (int)sizeof(T) * n + 4 * (nbm + kp) + 4 * nf +
                       15) & ~15;
@@ -81,6 +82,8 @@ struct BasisLds {
     T* gR;
     T* r;      // |d|
     T* fr;     // modes 1, 3: radial dE/dr of each slot
+    T* fA;     // kernels with the generic angular path: angular cutoff of each slot and its r derivative
+    T* gA;
     double* g; // [BW][3 nbm] per-wave dE/dd accumulators (one array per wave: a fixed order of adds)
     double* row;
     double* red;   // [BW][64] partial sums; adjoint: canonical dE/dG weights [block][14 ng] as FP32
@@ -93,7 +96,7 @@ struct BasisLds {
 };
 
 template <class T, int BW>
-__device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, int nrp, int mode, int nt) {
+__device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, int nrp, int mode, int nt, bool gen) {
     BasisLds<T> L;
     const int kp = (kmax + 3) & ~3;
     const bool gacc = (mode == 1 || mode == 3);
@@ -111,6 +114,8 @@ __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, i
     L.gR = c; c += nbm;
     L.r = c; c += nbm;
     L.fr = c; if (gacc) c += nbm;
+    L.fA = c; if (gen) c += nbm;
+    L.gA = c; if (gen) c += nbm;
     L.p = (int*)c;
     L.ci = L.p + nbm;
     L.rlo = (float*)(L.ci + kp);
@@ -551,7 +556,7 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
             const int nJ = s_cnt[J + 1] - bJ, nK = s_cnt[K + 1] - bK;
             const int T3 = same ? nJ * (nJ - 1) / 2 : nJ * nK;
             if (nA == 0 || T3 == 0 || sp.a_canon[b]) continue;
-            const int goff = sp.a_goff[b], poff = sp.a_poff[b];
+            const int goff = sp.a_goff[b], poff = sp.a_poff[b], aform = sp.a_form[b];
             const float rnK = 1.f / (float)(same ? nJ : min(nJ, nK));
             for (int m0 = 0; m0 < nA; m0 += MAXA) {
                 T lamf[MAXA], zeta[MAXA], acc[MAXA], w[MAXA];
@@ -579,14 +584,30 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
                     const T uax = L.u[4 * sa], uay = L.u[4 * sa + 1], uaz = L.u[4 * sa + 2];
                     const T ucx = L.u[4 * sc], ucy = L.u[4 * sc + 1], ucz = L.u[4 * sc + 2];
                     const T cs = uax * ucx + uay * ucy + uaz * ucz;
+                    // distance factors: X = Da Dc, dX/d|d_a| = Dda Dc, dX/d|d_c| = Da Ddc
                     T Da[ACE_MAXG], Dc[ACE_MAXG], Dda[ACE_MAXG], Ddc[ACE_MAXG];
+                    if (aform == ACE_ANG_GAUSS) {
 #pragma unroll
-                    for (int g = 0; g < ACE_MAXG; ++g) {
-                        const bool on = g < ng;
-                        Da[g] = on ? L.E[sa * 2 * NG + g] : T(0);
-                        Dc[g] = on ? L.E[sc * 2 * NG + g] : T(0);
-                        Dda[g] = (on && (ADJ || TAN)) ? L.E[sa * 2 * NG + NG + g] : T(0);
-                        Ddc[g] = (on && (ADJ || TAN)) ? L.E[sc * 2 * NG + NG + g] : T(0);
+                        for (int g = 0; g < ACE_MAXG; ++g) {
+                            const bool on = g < ng;
+                            Da[g] = on ? L.E[sa * 2 * NG + g] : T(0);
+                            Dc[g] = on ? L.E[sc * 2 * NG + g] : T(0);
+                            Dda[g] = (on && (ADJ || TAN)) ? L.E[sa * 2 * NG + NG + g] : T(0);
+                            Ddc[g] = (on && (ADJ || TAN)) ? L.E[sc * 2 * NG + NG + g] : T(0);
+                        }
+                    } else {
+                        // not separable in the two distances: F(r_a^2 + r_c^2) per triplet, in FP64
+                        const double ra = slot_r64(L, sa), rc = slot_r64(L, sc), r2s = ra * ra + rc * rc;
+                        const T fa = L.fA[sa], fc = L.fA[sc], ga = L.gA[sa], gc = L.gA[sc];
+#pragma unroll
+                        for (int g = 0; g < ACE_MAXG; ++g) {
+                            double F = 0.0, q = 0.0;
+                            if (g < ng) angular_factor(aform, sp.ietar2_64[g], r2s, F, q);
+                            Da[g] = (g < ng) ? fa : T(0);
+                            Dc[g] = fc * (T)F;
+                            Dda[g] = ga + (T)(2.0 * ra * q) * fa;
+                            Ddc[g] = (T)F * (gc + (T)(2.0 * rc * q) * fc);
+                        }
                     }
                     // cp[k + 7s] = b_s^(2^k), cq[k + 7s] = b_s^(2^k - 1), b_0 = (1+cos)/2, b_1 = (1-cos)/2
                     T cp[14], cq[14];
@@ -863,7 +884,7 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
     }
     const int I = io.type[i];
     const DevSpecies& gsp = M->sp[I];
-    const BasisLds<T> L = basis_lds<T, BW>(basis_smem, nbm, NG, kmax, io.nrp, MODE, nt);
+    const BasisLds<T> L = basis_lds<T, BW>(basis_smem, nbm, NG, kmax, io.nrp, MODE, nt, GEN != 0);
     const int nn = io.counts[i];
     const long long* krow = io.keys + (long long)i * io.cap;
     const float rcut = gsp.rc_max;
@@ -1027,6 +1048,10 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
         L.gR[q] = gc;
         if (sp.a_cut != sp.r_cut[J] || sp.a_rc64 != sp.r_rc64[J]) t_cutoff_r64<T>(sp.a_cut, sp.a_rc64, r, fc, gc);
         const T fa = fc, ga = gc, rt = L.r[q];
+        if (GEN) {
+            L.fA[q] = fa;
+            L.gA[q] = ga;
+        }
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             T ev = T(0), edv = T(0);

@@ -378,6 +378,35 @@ __device__ inline float exp_r64(double y) {
     return fmaf(e, zl * 0.6931471805599453f, e);
 }
 
+// Distance factor F(s) of an angular form, s = r_ij^2 + r_ik^2, and its logarithmic derivative q = d ln F / ds
+// (basis_angular.cpp:211-249 values, 261-343 derivatives), in FP64: GAUSS exp(-s/eta^2); GAUSS2 2e/(1+e) and
+// SECH 2e/(1+e^2), e = exp(-pi/2 s/eta^2); STUDENT3/4/5 (1 + s/eta^2)^(-p/2), p = 4, 5, 6.  ie = 1/eta^2.
+__device__ inline void angular_factor(int form, double ie, double s, double& F, double& q) {
+    const double hp = 1.5707963267948966;   // pi / 2
+    switch (form) {
+        case ACE_ANG_GAUSS2: {
+            const double e = exp(-hp * ie * s), den = 1.0 / (1.0 + e);
+            F = 2.0 * e * den;
+            q = -hp * ie * den;
+        } break;
+        case ACE_ANG_SECH: {
+            const double e = exp(-hp * ie * s), e2 = e * e, den = 1.0 / (1.0 + e2);
+            F = 2.0 * e * den;
+            q = -hp * ie * (1.0 - e2) * den;
+        } break;
+        case ACE_ANG_STUDENT3: case ACE_ANG_STUDENT4: case ACE_ANG_STUDENT5: {
+            const int p = form == ACE_ANG_STUDENT3 ? 4 : (form == ACE_ANG_STUDENT4 ? 5 : 6);
+            const double d2 = 1.0 / (1.0 + ie * s);
+            F = p == 4 ? d2 * d2 : (p == 5 ? d2 * d2 * sqrt(d2) : d2 * d2 * d2);
+            q = -0.5 * p * ie * d2;
+        } break;
+        default: {
+            F = exp(-ie * s);
+            q = -ie;
+        }
+    }
+}
+
 // One radial function value and r-derivative with |d| known to FP64 precision (r) and the cutoff factors
 // in T (basis_radial.cpp:175-257).  T = float: each form's exponent is formed in FP64 from r - rs and
 // evaluated by exp_r64, the rest in FP32 (an exponent formed in FP32 carries its relative rounding times

@@ -195,6 +195,33 @@ def test_every_radial_and_cutoff_form_and_generic_angular():
     check_against_oracle(m, strucs, "forms", check_plain=False, paths=NNPDevice.PATHS)
 
 
+def test_every_angular_form():
+    """The five angular forms whose distance factor couples the two distances (GAUSS2, SECH, STUDENT3/4/5,
+    basis_angular.cpp:211-249, 455-560): one form per block beside GAUSS blocks, through the generic angular
+    path with its per-triplet FP64 factor, on every energy/force path and both precisions."""
+    from ace_amd.model import Angular, CUT_COS
+    rng = np.random.default_rng(29)
+    m = configs.c3_model(rng)
+    funcs = [(0.005, 1, 1), (0.005, 2, -1), (0.005, 4, 1), (0.02, 1, -1), (0.02, 3, 1), (0.02, 8, -1)]
+    eta = np.array([1.0 / np.sqrt(f[0]) for f in funcs])
+    zeta = np.array([float(f[1]) for f in funcs])
+    lam = np.array([int(f[2]) for f in funcs], dtype=np.int32)
+    for I in range(3):
+        for b in range(6):
+            form = 2 + (I + b) % 5 if b != 5 else 1
+            m.angular[I][b] = Angular(form, CUT_COS, 6.0, eta, zeta, lam)
+    for I in range(3):
+        K = m.input_width(I)
+        w = [K] + list(m.nets[I].widths[1:])
+        m.nets[I] = configs.Network(w, m.nets[I].act, 1.0, configs.xavier_params(w, rng), np.ones(K), np.zeros(K))
+    strucs = []
+    for n in (33, 52):
+        cell, pos = configs.random_packed(n, 0.08, 1.6, rng)
+        strucs.append(structure(cell, pos, rng.integers(0, 3, n)))
+    scaled_model(m, strucs)
+    check_against_oracle(m, strucs, "angular forms", check_plain=False, paths=NNPDevice.PATHS)
+
+
 @pytest.mark.parametrize("hidden,act", [((128, 128), 3), ((40, 24, 17), 2), ((8,), 10), ((33, 31), 15)])
 def test_network_shapes_and_activations(hidden, act):
     """Widths that are not multiples of the 16x16 matrix tiles, 1 to 3 hidden layers, each activation,

@@ -14,10 +14,21 @@ from conftest import log
 pytestmark = pytest.mark.gpu
 
 
-def batch(seed=21, sizes=(24, 40, 57, 33)):
+def non_gaussian_angular(model):
+    """Every angular block of the model on one of GAUSS2, SECH, STUDENT3/4/5 (basis_angular.cpp:211-249)."""
+    from ace_amd.model import Angular
+    for I in range(model.ntypes):
+        for b, a in enumerate(model.angular[I]):
+            model.angular[I][b] = Angular(2 + (I + b) % 5, a.cut, a.rc, a.eta, a.zeta, a.lam)
+
+
+def batch(seed=21, sizes=(24, 40, 57, 33), angular_forms=False):
     rng = np.random.default_rng(seed)
     model = configs.c3_model(rng)
     teacher = configs.c3_model(np.random.default_rng(seed + 1))
+    if angular_forms:
+        non_gaussian_angular(model)
+        non_gaussian_angular(teacher)
     strucs = []
     for n in sizes:
         cell, pos = configs.random_packed(n, 0.08, 1.6, rng)
@@ -64,6 +75,28 @@ def test_train_step_matches_oracle(loss, norm, wf, prec):
     # a second step with unchanged parameters reproduces the first (cached G, deterministic reductions)
     out2, g2 = dev.train_step(loss, norm, 0.5, 1.0 / len(strucs), wf / (3 * nat))
     assert np.array_equal(out, out2) and np.array_equal(g, g2)
+    dev.close()
+
+
+@pytest.mark.parametrize("prec", [0, 2])
+def test_train_step_every_angular_form(prec):
+    """The force-matching gradient through the generic angular path's per-triplet distance factors (tangent and
+    adjoint modes) for GAUSS2, SECH and STUDENT3/4/5 blocks."""
+    model, strucs, G = batch(seed=33, angular_forms=True)
+    nat = sum(s.natoms for s in strucs)
+    ld = LossDesc(0, 1, 0.5, 1.0 / len(strucs), 0.1 / (3 * nat))
+    ref, gref = oracle_loss(model, strucs, G, ld)
+    dev = NNPDevice(model)
+    dev.upload(strucs)
+    dev.build_neighbors()
+    dev.set_precision(prec)
+    dev.set_targets([s.energy for s in strucs], np.concatenate([s.forces for s in strucs]))
+    out, g = dev.train_step(0, 1, 0.5, 1.0 / len(strucs), 0.1 / (3 * nat))
+    le = abs(out[1] - ref[0]) / max(abs(ref[0]), 1e-12)
+    lf = abs(out[2] - ref[1]) / max(abs(ref[1]), 1e-12)
+    ge = np.max(np.abs(g - gref)) / np.max(np.abs(gref))
+    log(f"train angular forms prec={prec}: loss_E rel {le:.1e} loss_F rel {lf:.1e} grad {ge:.1e}")
+    assert le < 1e-5 and lf < 1e-5 and ge < 1e-5
     dev.close()
 
 

@@ -48,6 +48,8 @@ SIGNATURES = {
     "ace_plain_energy_forces": (C.c_int, [P, dp, dp]),
     "ace_set_targets": (C.c_int, [P, dp, dp]),
     "ace_train_step": (C.c_int, [P, P, dp, dp]),
+    "ace_forward": (C.c_int, [P, dp, dp, dp]),
+    "ace_backward": (C.c_int, [P, dp, dp, dp, dp]),
     "ace_profile_enable": (C.c_int, [P, C.c_int]),
     "ace_profile_read": (C.c_int, [P, C.c_char_p, dp, lp]),
     "ace_sync": (C.c_int, [P]),

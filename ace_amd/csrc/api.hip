@@ -89,6 +89,8 @@ struct ace_ctx {
     bool nn64_dirty = true;          // FP64 parameter copy of the plain port is stale
     int precision = ACE_PREC_FP32;   // default of ace_symm / ace_train_step
     bool basis_f64 = false;          // precision of the basis kernels of the current call
+    unsigned long long seq = 0;      // advances with every device pass and every change of positions,
+                                     // parameters, input scaling or precision (ace_backward's validity check)
     // structures
     int S = 0, N = 0;
     std::vector<int> natoms, atom_off, type_h, row_of_h, struc_of_h;
@@ -874,6 +876,7 @@ int split_waves(ace_ctx* ctx, int mode, int precision) {
 // split_waves of the precision (modes 0-2; -1: the context's precision) or four (mode 3)
 int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = nullptr, int waves = -1,
               int precision = -1) {
+    ++ctx->seq;
     BasisFn fns[8];
     const bool f64 = ctx->basis_f64;
     basis_fns(ctx, f64, fns);
@@ -920,6 +923,7 @@ int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = 
 
 // the accumulators of the last gradient launch as FP64 forces, energies and virials (device)
 int run_finalize(ace_ctx* ctx) {
+    ++ctx->seq;
     const int b = 1 - ctx->fbuf;
     const int n = std::max(3 * ctx->N, 9 * ctx->S);
     return timed(ctx, "finalize", [&] {
@@ -929,6 +933,7 @@ int run_finalize(ace_ctx* ctx) {
 }
 
 int run_mlp(ace_ctx* ctx, bool bf16, int precision = ACE_PREC_FP32) {
+    ++ctx->seq;
     int lds = 0, lds_w = 0;
     for (int I = 0; I < ctx->nt; ++I) {
         lds = std::max(lds, mlp_lds_floats(ctx->hdm.sp[I]) * 4);
@@ -1116,6 +1121,7 @@ int ace_set_params(ace_ctx* ctx, int type, const double* params, int n) {
     if (sync_host_params(ctx)) return -1;
     ctx->sp[type].params.assign(params, params + n);
     ctx->params_dirty = true;
+    ++ctx->seq;
     ctx->nn64_dirty = true;
     ctx->tr.opt_dev_stale = ctx->tr.opt_on;
     return 0;
@@ -1140,6 +1146,7 @@ int ace_set_input_scaling(ace_ctx* ctx, int type, const double* inpw, const doub
     ctx->sp[type].inpb.assign(inpb, inpb + n);
     ctx->scale_dirty = true;
     ctx->tr.g_valid = false;     // cached network-input rows include the scaling
+    ++ctx->seq;
     return 0;
 }
 
@@ -1223,6 +1230,7 @@ int ace_update_positions(ace_ctx* ctx, const double* pos) {
     HC(hipMemcpyAsync(ctx->d_pos.p, pos, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HC(hipStreamSynchronize(ctx->stream));
     ctx->tr.g_valid = false;
+    ++ctx->seq;
     return 0;
 }
 
@@ -1237,6 +1245,7 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
     ctx->nb_skin = skin;
     ctx->nb_valid = false;
     ctx->tr.g_valid = false;
+    ++ctx->seq;
     ctx->sinfo_h.resize(ctx->S);
     int cell_off = 0;
     bool brute = true;
@@ -1394,6 +1403,7 @@ int ace_set_precision(ace_ctx* ctx, int precision) {
     if (precision != ACE_PREC_FP32 && precision != ACE_PREC_FP64_BASIS)
         return fail(ctx, "ace_set_precision: precision mode not available");
     if (precision != ctx->precision) ctx->tr.g_valid = false;
+    ++ctx->seq;
     ctx->precision = precision;
     return 0;
 }
@@ -1490,6 +1500,7 @@ int ace_energy_forces_noalloc(ace_ctx* ctx, int precision, const double* pos, do
     if (pos) {
         HC(hipMemcpyAsync(ctx->d_pos.p, pos, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         ctx->tr.g_valid = false;
+    ++ctx->seq;
     }
     unsigned char* sp = ctx->scratch;
     auto u64 = [&](size_t o) { return (unsigned long long*)(sp + o); };
@@ -1606,6 +1617,7 @@ int ace_md_run(ace_ctx* ctx, int precision, int nsteps, double dt, int rebuild_e
     if (require_nb(ctx)) return -1;
     if (run_ef(ctx, precision)) return -1;
     ctx->tr.g_valid = false;
+    ++ctx->seq;
     for (int k = 0; k < nsteps;) {
         if (k > 0 && k % rebuild_every == 0 && rebuild(true)) return -1;
         const int c = std::min(nsteps - k, rebuild_every - k % rebuild_every);

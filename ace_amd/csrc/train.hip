@@ -15,6 +15,8 @@ struct TrainState {
     bool have_targets = false;
     bool have_forces = false;
     bool g_valid = false;            // G rows of the current positions are in HBM
+    bool fwd_ok = false;             // ace_forward ran; valid while ctx->seq == fwd_seq
+    unsigned long long fwd_seq = 0;
     int tm = 16;                     // rows per training tile
     std::vector<int4> tiles;
     std::vector<int> tile_first;     // first tile of each species (ntypes + 1)

@@ -59,6 +59,7 @@ void train_on_upload(ace_ctx* ctx) {
     ctx->tr.have_targets = false;
     ctx->tr.have_forces = false;
     ctx->tr.g_valid = false;
+    ++ctx->seq;
     ctx->tr.tiles.clear();
 }
 
@@ -86,6 +87,8 @@ int train_tiles(ace_ctx* ctx) {
     return 0;
 }
 
+int train_grad(ace_ctx* ctx, bool wf);
+
 // The device part of a training step (network, forces, loss, tangent, gradient), after the G rows and the
 // buffers are in place: a fixed sequence of launches, captured into a HIP graph when the caller asks.
 int train_launch(ace_ctx* ctx, const ace_loss_desc& L, bool wf, bool want_grad) {
@@ -110,7 +113,15 @@ int train_launch(ace_ctx* ctx, const ace_loss_desc& L, bool wf, bool want_grad) 
     if (wf) {
         if (run_basis(ctx, 2, T.d_V.p, T.d_eps.p)) return -1;
     }
-    if (want_grad) {
+    if (want_grad && train_grad(ctx, wf)) return -1;
+    return 0;
+}
+
+// parameter gradient from the seeds in gfac (dL/dE per structure) and, when wf, the tangent rows V of the
+// force seeds eps (dL/dF): matrix-core pass per tile, then the two-stage reduction into d_grad
+int train_grad(ace_ctx* ctx, bool wf) {
+    TrainState& T = ctx->tr;
+    {
         int lds = 0;
         for (int I = 0; I < ctx->nt; ++I) lds = std::max(lds, train_mc_lds_floats(ctx->hdm.sp[I]) * 4);
         if (lds > 160 * 1024) return fail(ctx, "ace_train_step: network too wide for the training kernel's LDS");
@@ -269,6 +280,7 @@ int ace_train_opt_state(ace_ctx* ctx, int set, double* p, double* m, double* v, 
         for (int I = 0; I < ctx->nt; ++I)
             for (double& x : ctx->sp[I].params) x = p[q++];
         ctx->params_dirty = true;
+        ++ctx->seq;
         ctx->nn64_dirty = true;
         T.opt_host_stale = false;
         T.opt_dev_stale = false;
@@ -369,6 +381,84 @@ int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* 
         loss[2] = lf;
         loss[3] = rm;
     }
+    return flush_prof(ctx);
+}
+
+// Autograd-style pair.  The forward is the split energy/force pass (network-input rows G kept in HBM, network,
+// adjoint, fixed-point sums); the backward seeds the training kernels with the caller's dL/dE and dL/dF
+// instead of a loss's.
+int ace_forward(ace_ctx* ctx, double* E_struc, double* F, double* virial) {
+    if (!ctx) return -1;
+    TrainState& T = ctx->tr;
+    T.fwd_ok = false;
+    ctx->basis_f64 = ctx->precision == ACE_PREC_FP64_BASIS;
+    if (require_nb(ctx)) return -1;
+    if (ctx->nb_pending && nb_verify(ctx)) {
+        if (ctx->nb_async_ok) return -1;
+        if (ace_build_neighbors(ctx, ctx->nb_skin) || require_nb(ctx)) return -1;
+    }
+    if (ctx->S == 0) return fail(ctx, "ace_forward: no structures uploaded");
+    HC(hipMemsetAsync(ctx->d_scal.p + 3, 0, sizeof(int), ctx->stream));
+    if (!T.g_valid) {
+        if (run_basis(ctx, 0)) return -1;
+        T.g_valid = true;
+    }
+    if (run_mlp(ctx, false) || run_basis(ctx, 1) || run_finalize(ctx)) return -1;
+    if (E_struc) HC(hipMemcpyAsync(E_struc, ctx->d_E.p, ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (F) HC(hipMemcpyAsync(F, ctx->d_F.p, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (virial) HC(hipMemcpyAsync(virial, ctx->d_vir.p, 9 * ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC(hipStreamSynchronize(ctx->stream));
+    if (check_err_flag(ctx, "ace_forward")) return -1;
+    T.fwd_ok = true;
+    T.fwd_seq = ctx->seq;
+    return flush_prof(ctx);
+}
+
+int ace_backward(ace_ctx* ctx, const double* dL_dE, const double* dL_dF, double* dL_dparams, double* dL_dpos) {
+    if (!ctx) return -1;
+    TrainState& T = ctx->tr;
+    if (!T.fwd_ok || T.fwd_seq != ctx->seq)
+        return fail(ctx, "ace_backward: no ace_forward at the current positions, parameters, scaling and precision "
+                         "(or another device pass ran since)");
+    if (dL_dpos && dL_dF)
+        return fail(ctx, "ace_backward: dL/dpos through the forces (a Hessian-vector product) is not computed; "
+                         "pass dL_dF = NULL for dL/dpos of the energy term");
+    const int N = ctx->N, S = ctx->S;
+    if (dL_dparams) {
+        if (train_tiles(ctx)) return -1;
+        size_t ntot = 0;
+        long long gofs = 0;
+        train_sizes(ctx, ntot, gofs);
+        const bool wf = dL_dF != nullptr;
+        if (ensure(ctx, T.d_gfac, S) || ensure(ctx, T.d_eps, 3 * (size_t)N) || ensure(ctx, T.d_grad, ntot) ||
+            ensure(ctx, T.d_gsum, (size_t)std::max(gofs, 1LL)) || (wf && ensure(ctx, T.d_V, (size_t)N * ctx->ldg)))
+            return -1;
+        if (dL_dE) {
+            HC(hipMemcpyAsync(T.d_gfac.p, dL_dE, S * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        } else {
+            HC(hipMemsetAsync(T.d_gfac.p, 0, S * sizeof(double), ctx->stream));
+        }
+        if (wf) {
+            HC(hipMemcpyAsync(T.d_eps.p, dL_dF, 3 * (size_t)N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            if (run_basis(ctx, 2, T.d_V.p, T.d_eps.p)) return -1;
+        }
+        if (train_grad(ctx, wf)) return -1;
+        HC(hipMemcpyAsync(dL_dparams, T.d_grad.p, ntot * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    std::vector<double> F;
+    if (dL_dpos) {
+        F.resize(3 * (size_t)N);
+        HC(hipMemcpyAsync(F.data(), ctx->d_F.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HC(hipStreamSynchronize(ctx->stream));
+    if (check_err_flag(ctx, "ace_backward")) return -1;
+    if (dL_dpos) {
+        // d/dpos sum_s dL/dE_s E_s = -dL/dE_s F_a for the atoms a of structure s
+        for (int st = 0; st < S; ++st)
+            for (int a = ctx->atom_off[st]; a < ctx->atom_off[st] + ctx->natoms[st]; ++a)
+                for (int d = 0; d < 3; ++d) dL_dpos[3 * a + d] = dL_dE ? -dL_dE[st] * F[3 * a + d] : 0.0;
+    }
+    T.fwd_seq = ctx->seq;   // the forward stays valid for further backward calls
     return flush_prof(ctx);
 }
 

@@ -241,6 +241,26 @@ class NNPDevice:
                     "ace_train_step")
         return out, g
 
+    def forward(self):
+        """Autograd-style forward at the current parameters and positions (precision of set_precision):
+        per-structure energies (with E0), forces, virials; keeps the network inputs for backward()."""
+        E = np.zeros(len(self.natoms))
+        F = np.zeros((self.N, 3))
+        W = np.zeros((len(self.natoms), 9))
+        self._check(self.lib.ace_forward(self.ctx, _dp(E), _dp(F), _dp(W)), "ace_forward")
+        return E, F, W.reshape(-1, 3, 3)
+
+    def backward(self, dL_dE=None, dL_dF=None, want_params=True, want_pos=False):
+        """Backward of forward(): from upstream dL/dE (per structure) and dL/dF (per atom, 3) of a loss L(E, F),
+        dL/dparams (species concatenated, get_all_params order) and dL/dpos (energy term only; needs
+        dL_dF None)."""
+        e = None if dL_dE is None else np.ascontiguousarray(dL_dE, dtype=np.float64)
+        f = None if dL_dF is None else np.ascontiguousarray(dL_dF, dtype=np.float64).reshape(-1)
+        gp = np.zeros(sum(self.model.param_counts())) if want_params else None
+        gx = np.zeros((self.N, 3)) if want_pos else None
+        self._check(self.lib.ace_backward(self.ctx, _dp(e), _dp(f), _dp(gp), _dp(gx)), "ace_backward")
+        return gp, gx
+
     def train_optimizer(self, algo=None, gamma=1e-3, eps=1e-16):
         """algo "ADAM": every later train_step also applies one ADAM step on the GPU (FP64 master
         parameters, device layouts refreshed in place); None: the caller updates the parameters."""

@@ -23,6 +23,10 @@
  *   ace_train_optimizer ............ (no counterpart) the ADAM update of the training loop
  *                                    (nnptefr.cpp:1031-1066, opt::ADAM algo.cpp:280-296) on the GPU
  *   ace_train_opt_state ............ (no counterpart) its state, for restart files
+ *   ace_forward / ace_backward ..... autograd-style pair: the energy/force pass, then d(loss)/d(params) from
+ *                                    the caller's dL/dE and dL/dF (the network factor NN::D2ODZDI::grad,
+ *                                    src/ml/nn.cpp:2047-2140, seeded as NNPTEFR::error_cost_o1 seeds it,
+ *                                    nnptefr.cpp:1353-1378, with the upstream vectors in place of the loss's)
  *                                   extended with the force-matching term d(F)/dW
  *                                   (network factor NN::D2ODZDI, src/ml/nn.cpp:2047-2140)
  *   ace_md_init / ace_md_run / ace_md_state  velocity-Verlet steps (src/sim/integrator.cpp:548-571), HIP graphs
@@ -236,6 +240,22 @@ int  ace_set_targets(ace_ctx* ctx, const double* E_ref, const double* F_ref);
  * captured HIP graph (no counterpart in ace; same results as the plain launches). */
 int  ace_train_graph(ace_ctx* ctx, int on);
 int  ace_train_step(ace_ctx* ctx, const ace_loss_desc* loss_desc, double* loss, double* grad);
+
+/*
+ * Autograd-style boundary over the same kernels as ace_train_step.  ace_forward: energies E_struc[S] (with
+ * E0), forces F[3N] and virials[9S] at the current parameters and positions, in the precision of
+ * ace_set_precision (any output may be NULL); it keeps the network-input rows for ace_backward.
+ * ace_backward: with upstream gradients dL/dE_struc[S] and dL/dF[3N] (either may be NULL, meaning zero) of a
+ * caller's loss L(E, F), writes dL/dparams (the parameter count of every species, concatenated in
+ * ace_get_params order) and dL/dpos[3N] = -dL/dE_s F_a (the energy term; with dL_dF given, dL/dpos would need
+ * the Hessian-vector product, which is not computed, and the call fails).  Either output may be NULL.  Valid
+ * after ace_forward until positions, parameters, input scaling or precision change or another device pass
+ * (ace_energy_forces, ace_train_step, ...) runs; it may be called several times per forward.  For the seeds of
+ * ace_train_step's loss (dL/dE_s = w_energy nf(N_s) dU_s, dL/dF = w_force (F - F_ref)) dL/dparams equals
+ * ace_train_step's gradient.
+ */
+int  ace_forward(ace_ctx* ctx, double* E_struc, double* F, double* virial);
+int  ace_backward(ace_ctx* ctx, const double* dL_dE, const double* dL_dF, double* dL_dparams, double* dL_dpos);
 
 /* Device optimiser (no counterpart in ace, whose training loop updates the parameters on the host,
  * nnptefr.cpp:1031-1066).  algo ACE_OPT_ADAM: every later ace_train_step also applies one ADAM step

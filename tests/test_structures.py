@@ -82,8 +82,68 @@ def test_read_write_struc_by_format_name(golden_dir, tmp_path):
     S.write_struc(str(tmp_path / "out.poscar"), "POSCAR", p, names)
     q, names2, _ = S.read_struc(str(tmp_path / "out.poscar"), "POSCAR")
     assert names2 == names and np.allclose(q.pos, p.pos, atol=1e-7)
-    for fmt in ("QE", "CP2K", "VASP_XML", "PDB"):
+    for fmt in ("XDATCAR", "CUBE", "PDB"):
         with pytest.raises(ValueError):
+            S.read_struc(f"{golden_dir}/nacl8.poscar", fmt)
+    for fmt in ("QE", "CP2K", "VASP_XML"):       # readers present, but the file is not in their format
+        with pytest.raises((ValueError, IndexError)):
             S.read_struc(f"{golden_dir}/nacl8.poscar", fmt)
     with pytest.raises(ValueError):
         S.write_struc(str(tmp_path / "x.cube"), "CUBE", p, names)
+
+
+# ---- VASP_XML, CP2K and QE outputs: hand-written excerpts in each program's layout (tests/golden/formats) ----
+EH2EV, BOHR2ANG = 27.211386245988, 0.529177210903     # src/chem/units.hpp:24,30
+
+
+def test_vasp_xml(golden_dir):
+    """VASP::XML::read (vasp_struc.cpp:303-470): types from <atominfo>, the first basis and positions, the last
+    e_fr_energy, the first forces block."""
+    s, names, per = S.read_struc(f"{golden_dir}/formats/vasprun_si2.xml", "VASP_XML")
+    cell = np.array([[5.1, 0.2, 0.0], [0.0, 5.3, 0.1], [0.0, 0.0, 5.5]])
+    frac = np.array([[0, 0, 0], [0.25, 0.5, 0.1], [0.5, 0.25, 0.75]])
+    assert names == ["Si", "O"] and per == ["Si", "O", "Si"] and list(s.type) == [0, 1, 0]
+    assert np.array_equal(s.cell, cell)
+    assert np.allclose(s.pos, frac @ cell.T, atol=1e-14)
+    assert s.energy == -12.3456789
+    assert np.array_equal(s.forces, np.array([[0.1, -0.2, 0.3], [-0.4, 0.5, -0.6], [0.3, -0.3, 0.3]]))
+
+
+def test_cp2k(golden_dir):
+    """CP2K::read (cp2k_struc.cpp:15-168): names and positions (A) from ATOMIC COORDINATES, forces in
+    Hartree/Bohr -> eV/A, the last ENERGY| line in Hartree -> eV, the cell from the three lines after the first
+    CELL| line (the line after the CELL| block is skipped, as the reference's scan does)."""
+    s, names, per = S.read_struc(f"{golden_dir}/formats/cp2k_h2o.out", "CP2K")
+    assert names == ["O", "H"] and per == ["O", "H", "H"] and list(s.type) == [0, 1, 1]
+    assert np.array_equal(s.cell, np.array([[10.0, 0.0, 0.0], [0.0, 10.0, 0.5], [0.0, 0.0, 10.0]]))
+    assert np.array_equal(s.pos, np.array([[5.0, 5.0, 5.12], [5.76, 5.0, 4.52], [4.24, 5.0, 4.52]]))
+    assert s.energy == -17.15 * EH2EV
+    f = np.array([[0.01, -0.002, 0.003], [-0.005, 0.001, -0.0015], [-0.005, 0.001, -0.0015]])
+    assert np.allclose(s.forces, f * (EH2EV / BOHR2ANG), rtol=1e-15, atol=0)
+    au, _, _ = S.read_cp2k(f"{golden_dir}/formats/cp2k_h2o.out", units="au")
+    assert au.energy == -17.15 and np.array_equal(au.forces, f) and np.array_equal(au.pos, s.pos)
+
+
+def test_qe(golden_dir):
+    """QE::OUT::read (qe_struc.cpp:40-306): alat (Bohr) x crystal axes, tau (alat units) returned to the cell,
+    the total energy in Ry and the forces in Ry/Bohr converted to eV and eV/A, types in species order."""
+    s, names, per = S.read_struc(f"{golden_dir}/formats/qe_si2.out", "QE")
+    alat = 10.2 * BOHR2ANG
+    lv = np.array([[1.0, 0.05, 0.0], [0.0, 0.95, 0.1], [0.0, 0.0, 1.05]])
+    cell = alat * lv
+    assert names == ["Si", "O"] and per == ["Si", "O", "Si"] and list(s.type) == [0, 1, 0]
+    assert np.allclose(s.cell, cell, rtol=1e-15, atol=0)
+    tau = np.array([[0, 0, 0], [0.25, 0.25, 0.25], [1.1, 0.5, 0.5]]) * alat
+    f = np.linalg.solve(cell, tau.T).T
+    assert np.allclose(s.pos, (cell @ (f - np.floor(f)).T).T, atol=1e-12)
+    assert s.pos[2, 0] < tau[2, 0]                       # atom 3 came back into the cell
+    assert s.energy == 0.5 * EH2EV * -93.45678
+    fr = np.array([[0.001, 0.002, -0.003], [-0.0004, 0.0005, 0.0006], [-0.0006, -0.0025, 0.0024]])
+    assert np.allclose(s.forces, fr * (0.5 * EH2EV / BOHR2ANG), rtol=1e-15, atol=0)
+
+
+def test_dft_readers_feed_the_model_type_mapping(golden_dir):
+    """Species names of any reader map onto a potential's species (nnptefr.cpp:2489-2496)."""
+    s, names, per = S.read_struc(f"{golden_dir}/formats/qe_si2.out", "QE")
+    t = S.to_model_types(s, names, ["O", "Si"])
+    assert list(t.type) == [1, 0, 1]

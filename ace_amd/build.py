@@ -41,7 +41,18 @@ def build_oracle():
     return os.path.join(ROOT, "oracle", "liboracle.so")
 
 
+def build_adapter_test(force=True):
+    """The integration guide's C++ adapters compiled into a test program (tests/adapter), linked with the
+    library; the GPU tests run it."""
+    cmd = ["make", "-s", "-C", os.path.join(ROOT, "tests", "adapter")]
+    if force:
+        cmd.insert(1, "-B")
+    _run(cmd)
+    return os.path.join(ROOT, "tests", "adapter", "adapter_test")
+
+
 if __name__ == "__main__":
     import sys
     print(build_library(force="-B" in sys.argv))
     print(build_oracle())
+    print(build_adapter_test())

@@ -19,6 +19,8 @@
 #include "mlp.hip"
 #include "reduce.hip"
 #include "plain.hip"
+
+#define NB_CELL_DIV 1   // cells per list radius along each perpendicular height (binned search)
 #include "train.hip"
 #include "md.hip"
 
@@ -1269,7 +1271,9 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
             const double* Ri = I.Rinv;
             const double rown = std::sqrt(Ri[d] * Ri[d] + Ri[d + 3] * Ri[d + 3] + Ri[d + 6] * Ri[d + 6]);
             hgt[d] = 1.0 / rown;                                              // perpendicular height
-            I.nc[d] = std::max(1, std::min((int)std::floor(hgt[d] / rsearch), 1024));
+            // cells of about half the list radius: a 5-cell stencil per dimension (st = 2) scans (5/2)^3 rc^3 of
+            // candidates instead of 27 rc^3 or more with cells of one radius
+            I.nc[d] = std::max(1, std::min((int)std::floor(NB_CELL_DIV * hgt[d] / rsearch), 1024));
             if (I.cf[d] > 511) return fail(ctx, "ace_build_neighbors: image range too large for this cell");
         }
         // the binning workgroup holds one counter per cell: coarsen the grid to NB_MAXC cells
@@ -1327,7 +1331,7 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
                         N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_nb_count.p, ctx->d_scal.p,
                         nullptr, 0, ctx->d_scal.p + p_over, ctx->d_scal.p + p_next, ctx->d_scal.p + 3);
                 else
-                    k_nb_search<0><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
+                    k_nb_search<0><<<N, NB_SEARCH_WAVES * 64, 0, ctx->stream>>>(
                         N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
                         ctx->d_cell_start.p, ctx->d_cpos.p, ctx->d_cinfo.p, ctx->d_nb_count.p, ctx->d_scal.p, nullptr,
                         0, nullptr);
@@ -1347,7 +1351,7 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
                     ctx->d_nb_keys.p, ctx->slots_cap, ctx->d_scal.p + p_over, ctx->d_scal.p + p_next,
                     ctx->d_scal.p + 3);
             else
-                k_nb_search<1><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
+                k_nb_search<1><<<N, NB_SEARCH_WAVES * 64, ctx->slots_cap * sizeof(long long), ctx->stream>>>(
                     N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_cell_of.p, ctx->d_wrap_of.p,
                     ctx->d_cell_start.p, ctx->d_cpos.p, ctx->d_cinfo.p, ctx->d_nb_count.p, ctx->d_scal.p,
                     ctx->d_nb_keys.p, ctx->slots_cap, ctx->d_scal.p + 5);

@@ -237,16 +237,19 @@ __device__ inline void nb_key_decode(long long k, int& jl, int& n0, int& n1, int
     n2 = (c & 1023) - 512;
 }
 
-#define NB_WAVES 4
+#define NB_WAVES 4          // waves per atom of the binning-free search
+#define NB_SEARCH_WAVES 1   // waves per atom of the binned search
 #define NB_LMAX 1024   // neighbours per atom held while sorting
 #define NB_U 4         // candidate chunks loaded together by one wave
 
-// Workgroup (4 waves) per atom; 64-candidate chunks are dealt to the waves in turn, NB_U chunks in flight per wave.
+// Workgroup of NB_SEARCH_WAVES waves per atom (one: the most atoms in flight per CU, which hides the dependent
+// loads of the stencil and the candidates best); 64-candidate chunks are dealt to the waves in turn, NB_U chunks
+// in flight per wave.
 // MODE 0: count only (counts[i], running maximum in *maxcount).  MODE 1: sort the keys
 // (j_local << 30 | image order code: the reference order) and write them to row i of keys (cap slots); an
 // atom with more than cap neighbours raises *over and writes none.
 template <int MODE>
-__global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
+__global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
     int natoms, double rc2, const double* __restrict__ pos, const int* __restrict__ struc_of,
     const StrucInfo* __restrict__ sinfo, const int* __restrict__ cell_of, const int* __restrict__ wrap_of,
     const int* __restrict__ cell_start, const double4* __restrict__ cpos, const int2* __restrict__ cinfo,
@@ -256,7 +259,7 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
     __shared__ int s_cpref[65];
     __shared__ int s_cimg[64];
     __shared__ int s_n;
-    __shared__ long long s_key[MODE ? NB_LMAX : 1];
+    extern __shared__ long long s_key[];   // MODE 1: cap keys (dynamic LDS)
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int i = blockIdx.x;
     const int s = struc_of[i];
@@ -299,14 +302,14 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
         }
         __syncthreads();
         const int total = s_cpref[64];
-        // chunk q of 64 candidates goes to wave q mod 4, so a short stencil keeps every wave busy
-        for (int tb = 0; tb < total; tb += NB_WAVES * 64 * NB_U) {
+        // chunk q of 64 candidates goes to wave q mod NB_SEARCH_WAVES, so a short stencil keeps every wave busy
+        for (int tb = 0; tb < total; tb += NB_SEARCH_WAVES * 64 * NB_U) {
             double4 pj[NB_U];
             int2 inf[NB_U];
             int sl[NB_U];
 #pragma unroll
             for (int u = 0; u < NB_U; ++u) {
-                const int x = tb + 64 * (u * NB_WAVES + wv) + lane;
+                const int x = tb + 64 * (u * NB_SEARCH_WAVES + wv) + lane;
                 // stencil slot of candidate x: largest c with s_cpref[c] <= x
                 int c = 0;
 #pragma unroll
@@ -321,7 +324,7 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
             }
 #pragma unroll
             for (int u = 0; u < NB_U; ++u) {
-                const int x = tb + 64 * (u * NB_WAVES + wv) + lane;
+                const int x = tb + 64 * (u * NB_SEARCH_WAVES + wv) + lane;
                 bool acc = false;
                 long long key = 0;
                 if (x < total) {
@@ -352,7 +355,7 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
                 base = __shfl(base, 0, 64);
                 if (MODE == 1 && acc) {
                     const int r = base + __popcll(m & lane_mask_lt());
-                    if (r < NB_LMAX) s_key[r] = key;
+                    if (r < cap) s_key[r] = key;
                 }
             }
         }
@@ -372,7 +375,7 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_search(
     }
     // rank sort into the reference order (the keys of one atom are distinct)
     long long* row = keys + (long long)i * cap;
-    for (int e = tid; e < count; e += NB_WAVES * 64) {
+    for (int e = tid; e < count; e += NB_SEARCH_WAVES * 64) {
         const long long k = s_key[e];
         int rank = 0;
         for (int f = 0; f < count; ++f) rank += (s_key[f] < k);
@@ -395,7 +398,7 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_brute(
     long long* __restrict__ keys, int cap, int* __restrict__ over, int* __restrict__ over_next,
     int* __restrict__ basis_err) {
     __shared__ int s_n;
-    __shared__ long long s_key[MODE ? NB_LMAX : 1];
+    extern __shared__ long long s_key[];   // MODE 1: cap keys (dynamic LDS)
     const int tid = threadIdx.x, lane = tid & 63;
     const int i = blockIdx.x;
     if (i == 0 && tid == 0) {

@@ -252,7 +252,7 @@ __device__ inline void ld_e(const T* p, T* e, T* ed) {
 template <class T, int NG, int NZ, int MODE>
 __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int nJ, int nK, bool same,
                                      const int* __restrict__ cidx, const float* wtab, int nA, int goff, double* gw,
-                                     int wv, int lane) {
+                                     int wv, int lane, int rs_f, bool rs_writer) {
     constexpr int ADJ = (MODE == 1);
     constexpr int TAN = (MODE == 2);
     constexpr int BT = 64;   // one wave per block
@@ -411,22 +411,21 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
         }
     }
     if (!ADJ) {
-        // wave reduce-scatter of the 2 NZ NG sums (padded to a power of two); the lanes holding sum f write
-        // it to its (sign, level, group) slot
+        // wave reduce-scatter of the 2 NZ NG sums; the lane that writes sum f puts it in its (sign, level, group)
+        // slot
         constexpr int M0 = 2 * NZ * NG;
-        constexpr int LM = M0 <= 2 ? 1 : M0 <= 4 ? 2 : M0 <= 8 ? 3 : M0 <= 16 ? 4 : M0 <= 32 ? 5 : 6;
-        T v[1 << LM];
+        static_assert(M0 <= 64, "one sum per lane at most");
+        T v[M0];
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll
             for (int k = 0; k < NZ; ++k)
 #pragma unroll
                 for (int g = 0; g < NG; ++g) v[(s * NZ + k) * NG + g] = acc[s][k][g];
-#pragma unroll
-        for (int f = M0; f < (1 << LM); ++f) v[f] = T(0);
-        const double tot = (double)wave_reduce_scatter<LM, T>(v, lane);
-        const int f = lane >> (6 - LM);
-        if ((lane & ((1 << (6 - LM)) - 1)) == 0 && f < M0) {
+        rs_any<5, M0, T>(v, lane);
+        const double tot = (double)v[0];
+        if (rs_writer) {
+            const int f = rs_f;
             const int g = f % NG, k = (f / NG) % NZ, s = f / (NG * NZ);
             L.red[wv * 64 + (s * 7 + k) * NG + g] = tot;
         }
@@ -528,6 +527,9 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
 
     // --- angular blocks (J <= K): canonical ones one wave each, then the others with all waves ----------
     double* gw = L.g + wv * 3 * nbm;
+    int rs_f = 0;
+    bool rs_writer = false;
+    if (!ADJ) rs_any_owner<2 * NZ * NG>(lane, rs_f, rs_writer);
     {
         int load[BW];
 #pragma unroll
@@ -543,7 +545,8 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
                 if (nA == 0 || T3 == 0 || (GEN && !sp.a_canon[b])) continue;
                 if (deal<BW>(load, T3) != wv) continue;
                 angular_canon<T, NG, NZ, MODE>(L, bJ, bK, nJ, nK, same, L.ci + (sp.a_poff[b] - sp.a_poff[0]),
-                                               (const float*)L.red + 14 * NG * b, nA, sp.a_goff[b], gw, wv, lane);
+                                               (const float*)L.red + 14 * NG * b, nA, sp.a_goff[b], gw, wv, lane,
+                                               rs_f, rs_writer);
             }
         }
     }

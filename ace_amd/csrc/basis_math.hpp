@@ -355,6 +355,63 @@ __device__ inline T wave_reduce_scatter(T* v, int lane) {
     return sum_steps<5 - LM>(v[0]);
 }
 
+// Reduce-scatter of any M <= 64 values per lane (no padding to a power of two): halving steps from lane bit B
+// down.  A step splits the list of m values into P = floor(m / 2) pairs (j, j + m - P) -- lanes with the bit
+// set keep the second of each pair, the others the first, and add their partner's copy -- and, for odd m, one
+// single value (index P) that both halves keep, summed with the partner's copy.  The list shrinks to m - P.
+template <int B, int M, class T>
+__device__ inline void rs_any(T* v, int lane) {
+    if constexpr (B >= 0 && M >= 1) {
+        constexpr int P = M / 2, H = M - P;
+        if constexpr (B >= 4) {
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                T a = v[j], b = v[j + H];
+                swap_rows<B>(a, b);
+                v[j] = a + b;
+            }
+            if constexpr (M & 1) {
+                T a = v[P], b = v[P];
+                swap_rows<B>(a, b);
+                v[P] = a + b;
+            }
+        } else {
+            const bool hi = (lane >> B) & 1;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const T send = hi ? v[j] : v[j + H];
+                const T keep = hi ? v[j + H] : v[j];
+                v[j] = keep + lane_swap<B>(send);
+            }
+            if constexpr (M & 1) v[P] = v[P] + lane_swap<B>(v[P]);
+        }
+        rs_any<B - 1, H>(v, lane);
+    }
+}
+
+// The value index whose wave total rs_any<5, M> leaves in this lane, and whether the lane is the one of its
+// group that writes it (lanes that differ only in the bits at which that value was a single hold copies).
+template <int M>
+__device__ inline void rs_any_owner(int lane, int& f, bool& writer) {
+    int sz[6];
+    int m = M;
+#pragma unroll
+    for (int B = 5; B >= 0; --B) {
+        sz[B] = m;
+        m -= m / 2;
+    }
+    int pos = 0;
+    writer = true;
+#pragma unroll
+    for (int B = 0; B <= 5; ++B) {   // back from the last step
+        const int P = sz[B] / 2, H = sz[B] - P;
+        const bool hi = (lane >> B) & 1;
+        if (pos < P) pos = hi ? pos + H : pos;
+        else writer = writer && !hi;
+    }
+    f = pos;
+}
+
 template <class T>
 __device__ inline T t_wave_sum(T v) {
     v = t_wave_sum63<T>(v);

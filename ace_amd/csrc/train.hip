@@ -35,6 +35,7 @@ struct TrainState {
     int opt_n = 0;
     DBuf<double> d_p, d_m, d_v, d_bt;
     DBuf<int> d_pmap;
+    DBuf<int> d_adone;               // completion counter of the ADAM launch (left at zero)
 };
 
 __device__ inline void act_fd2(int act, float c, float z, float& a, float& d, float& d2) {
@@ -78,60 +79,99 @@ __device__ inline void act_fd2(int act, float c, float z, float& a, float& d, fl
     }
 }
 
-// block per structure: energy loss terms, gfac; force residual eps and force loss (FP64)
-#define LOSS_THREADS 256
-__global__ __launch_bounds__(LOSS_THREADS) void k_loss(
-    const int* __restrict__ atom_off, const int* __restrict__ natoms_of, const double* __restrict__ E,
-    const double* __restrict__ E_ref, const double* __restrict__ F, const double* __restrict__ F_ref,
-    ace_loss_desc Ld, double* __restrict__ gfac, double* __restrict__ eps, double* __restrict__ out /*S x 3*/) {
-    __shared__ double s[LOSS_THREADS];
-    const int st = blockIdx.x, t = threadIdx.x;
-    const int a0 = atom_off[st], n = natoms_of[st];
-    double lf = 0;
-    if (F_ref) {
-        for (int q = t; q < 3 * n; q += LOSS_THREADS) {
-            const double r = F[3 * a0 + q] - F_ref[3 * a0 + q];
-            lf += 0.5 * r * r;
-            eps[3 * a0 + q] = Ld.w_force * r;
-        }
+// the per-structure loss terms from the energy error and the summed force term: gfac = dL/dE, out = (energy
+// loss, force loss, (dE / N)^2)
+__device__ inline void loss_terms(int st, int n, double dE, double lf, const ace_loss_desc& Ld, double* gfac,
+                                  double* out) {
+    const double nA = n;
+    double nf = 1.0;
+    switch (Ld.norm) {                       // nnptefr.cpp:609-632
+        case ACE_NORM_LINEAR: nf = 1.0 / nA; break;
+        case ACE_NORM_SQRT: nf = 1.0 / sqrt(nA); break;
+        case ACE_NORM_CBRT: nf = 1.0 / cbrt(nA); break;
+        case ACE_NORM_LOG: nf = 1.0 / (log(nA) + 1.0); break;
+        default: nf = 1.0;
     }
-    s[t] = lf;
+    const double dU = dE * nf;
+    double g = nf, le = 0;
+    switch (Ld.loss) {                       // nnptefr.cpp:1353-1378
+        case ACE_LOSS_MAE: le = fabs(dU); g *= (dU > 0) - (dU < 0); break;
+        case ACE_LOSS_HUBER: {
+            const double arg = dU / Ld.delta, sq = sqrt(1.0 + arg * arg);
+            le = Ld.delta * Ld.delta * (sq - 1.0);
+            g *= dU / sq;
+        } break;
+        case ACE_LOSS_ASINH: {
+            const double arg = dU / Ld.delta, sq = sqrt(1.0 + arg * arg), lg = log(arg + sq);
+            le = Ld.delta * Ld.delta * (1.0 - sq + arg * lg);
+            g *= lg * Ld.delta;
+        } break;
+        default: le = 0.5 * dU * dU; g *= dU;
+    }
+    gfac[st] = Ld.w_energy * g;
+    out[3 * st] = Ld.w_energy * le;
+    out[3 * st + 1] = Ld.w_force * lf;
+    out[3 * st + 2] = dE * dE / (nA * nA);
+}
+
+// fixed-order FP64 tree over the block
+#define LOSS_THREADS 256
+__device__ inline double loss_block_sum(double* s, double v) {
+    const int t = threadIdx.x;
+    s[t] = v;
     __syncthreads();
     for (int h = LOSS_THREADS / 2; h > 0; h >>= 1) {
         if (t < h) s[t] += s[t + h];
         __syncthreads();
     }
+    return s[0];
+}
+
+// Block per structure, for the step with forces: the structure's fixed-point force, energy and virial sums
+// converted to FP64 (as k_finalize), then the loss on them -- force residual eps = w_force (F - F_ref) and the
+// force loss in the same pass over the forces, the energy terms and gfac -- in one launch.
+__global__ __launch_bounds__(LOSS_THREADS) void k_finalize_loss(
+    const int* __restrict__ atom_off, const int* __restrict__ natoms_of, const unsigned long long* __restrict__ F_fix,
+    const unsigned long long* __restrict__ E_fix, const unsigned long long* __restrict__ W_fix, double* __restrict__ F,
+    double* __restrict__ E, double* __restrict__ W, const double* __restrict__ E_ref, const double* __restrict__ F_ref,
+    ace_loss_desc Ld, double* __restrict__ gfac, double* __restrict__ eps, double* __restrict__ out /*S x 3*/) {
+    __shared__ double s[LOSS_THREADS];
+    constexpr double inv = 1.0 / 17179869184.0;
+    const int st = blockIdx.x, t = threadIdx.x;
+    const int a0 = atom_off[st], n = natoms_of[st];
+    double lf = 0;
+    for (int q = t; q < 3 * n; q += LOSS_THREADS) {
+        const double f = (double)(long long)F_fix[3 * a0 + q] * inv;
+        F[3 * a0 + q] = f;
+        const double r = f - F_ref[3 * a0 + q];
+        lf += 0.5 * r * r;
+        eps[3 * a0 + q] = Ld.w_force * r;
+    }
+    if (t < 9) W[9 * st + t] = (double)(long long)W_fix[9 * st + t] * inv;
+    const double tot = loss_block_sum(s, lf);
     if (t == 0) {
-        const double nA = n;
-        double nf = 1.0;
-        switch (Ld.norm) {                       // nnptefr.cpp:609-632
-            case ACE_NORM_LINEAR: nf = 1.0 / nA; break;
-            case ACE_NORM_SQRT: nf = 1.0 / sqrt(nA); break;
-            case ACE_NORM_CBRT: nf = 1.0 / cbrt(nA); break;
-            case ACE_NORM_LOG: nf = 1.0 / (log(nA) + 1.0); break;
-            default: nf = 1.0;
-        }
-        const double dE = E[st] - E_ref[st];
-        const double dU = dE * nf;
-        double g = nf, le = 0;
-        switch (Ld.loss) {                       // nnptefr.cpp:1353-1378
-            case ACE_LOSS_MAE: le = fabs(dU); g *= (dU > 0) - (dU < 0); break;
-            case ACE_LOSS_HUBER: {
-                const double arg = dU / Ld.delta, sq = sqrt(1.0 + arg * arg);
-                le = Ld.delta * Ld.delta * (sq - 1.0);
-                g *= dU / sq;
-            } break;
-            case ACE_LOSS_ASINH: {
-                const double arg = dU / Ld.delta, sq = sqrt(1.0 + arg * arg), lg = log(arg + sq);
-                le = Ld.delta * Ld.delta * (1.0 - sq + arg * lg);
-                g *= lg * Ld.delta;
-            } break;
-            default: le = 0.5 * dU * dU; g *= dU;
-        }
-        gfac[st] = Ld.w_energy * g;
-        out[3 * st] = Ld.w_energy * le;
-        out[3 * st + 1] = Ld.w_force * s[0];
-        out[3 * st + 2] = dE * dE / (nA * nA);
+        const double e = (double)(long long)E_fix[st] * inv;
+        E[st] = e;
+        loss_terms(st, n, e - E_ref[st], tot, Ld, gfac, out);
+    }
+}
+
+// Block per structure, for the step without forces: E_s = sum of the atomic energies (the fixed-order tree of
+// k_struct_reduce), then the energy loss terms and gfac.
+__global__ __launch_bounds__(LOSS_THREADS) void k_reduce_loss(const int* __restrict__ atom_off,
+                                                              const int* __restrict__ natoms_of,
+                                                              const double* __restrict__ e_atom, double* __restrict__ E,
+                                                              const double* __restrict__ E_ref, ace_loss_desc Ld,
+                                                              double* __restrict__ gfac, double* __restrict__ out) {
+    __shared__ double s[LOSS_THREADS];
+    const int st = blockIdx.x, t = threadIdx.x;
+    const int a0 = atom_off[st], n = natoms_of[st];
+    double acc = 0.0;
+    for (int k = t; k < n; k += LOSS_THREADS) acc += e_atom[a0 + k];
+    const double e = loss_block_sum(s, acc);
+    if (t == 0) {
+        E[st] = e;
+        loss_terms(st, n, e - E_ref[st], 0.0, Ld, gfac, out);
     }
 }
 

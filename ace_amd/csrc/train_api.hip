@@ -4,37 +4,43 @@
 // FP64 master parameters, in the operation order of the host restatement and without contraction, so
 // device and host trajectories agree bit for bit; each thread also writes its parameter into the FP32
 // matrix-core layout and, for first-layer weights, the BF16 high/low copies (the indices of param_map).
-__global__ void k_adam(int n, double* __restrict__ p, double* __restrict__ m, double* __restrict__ v,
-                       const double* __restrict__ g, const double* __restrict__ bt, double gamma, double b1, double b2,
-                       double eps, const int* __restrict__ map, float* __restrict__ wpad, uint16_t* __restrict__ w0bf) {
+__global__ __launch_bounds__(256) void k_adam(int n, double* __restrict__ p, double* __restrict__ m,
+                                              double* __restrict__ v, const double* __restrict__ g,
+                                              double* __restrict__ bt, double gamma, double b1, double b2, double eps,
+                                              const int* __restrict__ map, float* __restrict__ wpad,
+                                              uint16_t* __restrict__ w0bf, int* __restrict__ done) {
 #pragma clang fp contract(off)
+    __shared__ int s_last;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const double c1 = 1.0 / (1.0 - bt[0]), c2 = 1.0 / (1.0 - bt[1]);
-    const double gq = g[q];
-    const double mq = m[q] * b1 + (1.0 - b1) * gq;
-    const double vq = v[q] * b2 + (1.0 - b2) * gq * gq;
-    const double pq = p[q] - gamma * c1 * mq / (sqrt(c2 * vq) + eps);
-    m[q] = mq;
-    v[q] = vq;
-    p[q] = pq;
-    const int* e = map + 5 * (size_t)q;
-    const float x = (float)pq;
-    wpad[e[0]] = x;
-    if (e[1] >= 0) {
-        const uint16_t hi = bf16_rne(x), lo = bf16_rne(x - bf16_value(hi));
-        w0bf[e[1]] = hi;
-        w0bf[e[2]] = lo;
-        w0bf[e[3]] = hi;
-        w0bf[e[4]] = lo;
+    if (q < n) {
+        const double c1 = 1.0 / (1.0 - bt[0]), c2 = 1.0 / (1.0 - bt[1]);
+        const double gq = g[q];
+        const double mq = m[q] * b1 + (1.0 - b1) * gq;
+        const double vq = v[q] * b2 + (1.0 - b2) * gq * gq;
+        const double pq = p[q] - gamma * c1 * mq / (sqrt(c2 * vq) + eps);
+        m[q] = mq;
+        v[q] = vq;
+        p[q] = pq;
+        const int* e = map + 5 * (size_t)q;
+        const float x = (float)pq;
+        wpad[e[0]] = x;
+        if (e[1] >= 0) {
+            const uint16_t hi = bf16_rne(x), lo = bf16_rne(x - bf16_value(hi));
+            w0bf[e[1]] = hi;
+            w0bf[e[2]] = lo;
+            w0bf[e[3]] = hi;
+            w0bf[e[4]] = lo;
+        }
     }
-}
-
-// (beta1^t, beta2^t) -> (beta1^(t+1), beta2^(t+1)) after the step that read them
-__global__ void k_adam_advance(double* bt, double b1, double b2) {
-    if (threadIdx.x == 0) {
+    // (beta1^t, beta2^t) -> (beta1^(t+1), beta2^(t+1)) by the last workgroup, after every workgroup read them
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = (atomicAdd(done, 1) == (int)gridDim.x - 1);
+    __syncthreads();
+    if (s_last && threadIdx.x == 0) {
+        __threadfence();
         bt[0] *= b1;
         bt[1] *= b2;
+        atomicExch(done, 0);
     }
 }
 
@@ -48,6 +54,7 @@ void train_release(ace_ctx* ctx) {
     release(T.d_tiles); release(T.d_Eref); release(T.d_Fref); release(T.d_gfac); release(T.d_eps);
     release(T.d_out); release(T.d_grad); release(T.d_gsum); release(T.d_V); release(T.d_part);
     release(T.d_p); release(T.d_m); release(T.d_v); release(T.d_bt); release(T.d_pmap);
+    release(T.d_adone);
     for (int q = 0; q < 2; ++q)
         if (ctx->tr.graph[q]) {
             (void)hipGraphExecDestroy(ctx->tr.graph[q]);
@@ -89,6 +96,7 @@ int train_tiles(ace_ctx* ctx) {
 
 int train_grad(ace_ctx* ctx, bool wf);
 
+
 // The device part of a training step (network, forces, loss, tangent, gradient), after the G rows and the
 // buffers are in place: a fixed sequence of launches, captured into a HIP graph when the caller asks.
 int train_launch(ace_ctx* ctx, const ace_loss_desc& L, bool wf, bool want_grad) {
@@ -97,19 +105,21 @@ int train_launch(ace_ctx* ctx, const ace_loss_desc& L, bool wf, bool want_grad) 
     if (run_mlp(ctx, false)) return -1;
     if (wf) {
         if (run_basis(ctx, 1)) return -1;
-        if (run_finalize(ctx)) return -1;
-    } else if (timed(ctx, "struct_reduce", [&] {
-                   k_struct_reduce<<<S, RED_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p,
-                                                                       ctx->d_e_atom.p, ctx->d_E.p);
+        ++ctx->seq;
+        const int b = 1 - ctx->fbuf;   // the accumulators the adjoint added into
+        if (timed(ctx, "loss", [&] {
+                k_finalize_loss<<<S, LOSS_THREADS, 0, ctx->stream>>>(
+                    ctx->d_atom_off.p, ctx->d_natoms.p, ctx->d_Ffix[b].p, ctx->d_Efix[b].p, ctx->d_Wfix[b].p, ctx->d_F.p,
+                    ctx->d_E.p, ctx->d_vir.p, T.d_Eref.p, T.d_Fref.p, L, T.d_gfac.p, T.d_eps.p, T.d_out.p);
+            }))
+            return -1;
+    } else if (timed(ctx, "loss", [&] {
+                   k_reduce_loss<<<S, LOSS_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p,
+                                                                      ctx->d_e_atom.p, ctx->d_E.p, T.d_Eref.p, L,
+                                                                      T.d_gfac.p, T.d_out.p);
                })) {
         return -1;
     }
-    if (timed(ctx, "loss", [&] {
-            k_loss<<<S, LOSS_THREADS, 0, ctx->stream>>>(ctx->d_atom_off.p, ctx->d_natoms.p, ctx->d_E.p, T.d_Eref.p,
-                                                        ctx->d_F.p, wf ? T.d_Fref.p : nullptr, L, T.d_gfac.p,
-                                                        T.d_eps.p, T.d_out.p);
-        }))
-        return -1;
     if (wf) {
         if (run_basis(ctx, 2, T.d_V.p, T.d_eps.p)) return -1;
     }
@@ -150,6 +160,8 @@ int train_grad(ace_ctx* ctx, bool wf) {
             npmax = std::max(npmax, R.np[I]);
             ngmax = std::max(ngmax, ng);
         }
+        // two launches: the kernel boundary orders the group sums before the per-parameter sum at no cost, where
+        // one launch would need a device-scope fence per workgroup (an L2 write-back)
         const dim3 g1(cdiv(npmax, 256), ngmax, ctx->nt), g2(cdiv(npmax, 256), 1, ctx->nt);
         if (timed(ctx, "grad_reduce", [&] {
                 k_grad_reduce1<<<g1, 256, 0, ctx->stream>>>(R, T.ldpart, T.d_part.p, T.d_gsum.p);
@@ -251,10 +263,11 @@ int ace_train_optimizer(ace_ctx* ctx, int algo, double gamma, double eps) {
     const int n = (int)p.size();
     const double bt[2] = {ADAM_B1, ADAM_B2};
     if (upload(ctx, T.d_p, p.data(), n) || upload(ctx, T.d_pmap, map.data(), map.size()) ||
-        upload(ctx, T.d_bt, bt, 2) || ensure(ctx, T.d_m, n) || ensure(ctx, T.d_v, n))
+        upload(ctx, T.d_bt, bt, 2) || ensure(ctx, T.d_m, n) || ensure(ctx, T.d_v, n) || ensure(ctx, T.d_adone, 1))
         return -1;
     HC(hipMemsetAsync(T.d_m.p, 0, n * sizeof(double), ctx->stream));
     HC(hipMemsetAsync(T.d_v.p, 0, n * sizeof(double), ctx->stream));
+    HC(hipMemsetAsync(T.d_adone.p, 0, sizeof(int), ctx->stream));   // the step's completion counter
     HC(hipStreamSynchronize(ctx->stream));
     T.opt_n = n;
     T.opt_gamma = gamma;
@@ -356,8 +369,7 @@ int ace_train_step(ace_ctx* ctx, const ace_loss_desc* Ld, double* loss, double* 
         if (timed(ctx, "adam", [&] {
                 k_adam<<<cdiv(n, 256), 256, 0, ctx->stream>>>(n, T.d_p.p, T.d_m.p, T.d_v.p, T.d_grad.p, T.d_bt.p,
                                                               T.opt_gamma, ADAM_B1, ADAM_B2, T.opt_eps, T.d_pmap.p,
-                                                              ctx->d_wpad.p, ctx->d_w0bf.p);
-                k_adam_advance<<<1, 64, 0, ctx->stream>>>(T.d_bt.p, ADAM_B1, ADAM_B2);
+                                                              ctx->d_wpad.p, ctx->d_w0bf.p, T.d_adone.p);
             }))
             return -1;
         T.opt_host_stale = true;

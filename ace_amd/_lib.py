@@ -49,6 +49,7 @@ SIGNATURES = {
     "ace_set_targets": (C.c_int, [P, dp, dp]),
     "ace_train_step": (C.c_int, [P, P, dp, dp]),
     "ace_forward": (C.c_int, [P, dp, dp, dp]),
+    "ace_tile_shape": (C.c_int, [P, C.c_int, dp]),
     "ace_backward": (C.c_int, [P, dp, dp, dp, dp]),
     "ace_profile_enable": (C.c_int, [P, C.c_int]),
     "ace_profile_read": (C.c_int, [P, C.c_char_p, dp, lp]),

@@ -75,6 +75,7 @@ struct DevBasisParams {
     const float* inpb;
     const double* inpw64;
     const double* inpb64;
+    int chunk_w;           // angular triplet loops: a chunk's set-up and flush cost in eighths of one pair
 };
 
 // Per-structure geometry for the neighbour build.

@@ -83,6 +83,8 @@ struct ace_ctx {
                                         // symm -> network -> adjoint (0)
     int tune_stage[3] = {-1, -1, -1};   // per precision: network weights staged in LDS (1) or read from memory
     int tune_waves[3] = {-1, -1, -1};   // per precision: waves per atom of the split path's kernels (4 or 2)
+    int tune_chunk[3] = {-1, -1, -1};   // per precision: the angular chunk weight w (1, 3 or 8; -1: default 3)
+    double tune_chunk_ms[3][3] = {{-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}};   // its timings, w = 1, 3, 8
     long long tune_N = -1;
     int tune_slots = -1;
     bool pinned[3] = {false, false, false};   // ace_set_path: the choice holds for every batch
@@ -744,6 +746,7 @@ DevBasisParams basis_params(ace_ctx* ctx) {
     P.inpb = ctx->d_inpb.p;
     P.inpw64 = ctx->d_inpw64.p;
     P.inpb64 = ctx->d_inpb64.p;
+    P.chunk_w = 3;
     return P;
 }
 
@@ -890,6 +893,10 @@ int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = 
     BasisFn f = fns[mode + (waves == 2 ? 4 : 0)];
     if (bytes > 64 * 1024) HC(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
     DevBasisParams P = basis_params(ctx);
+    {
+        const int pr = precision < 0 ? ctx->precision : precision;
+        if (tuned(ctx, pr) && ctx->tune_chunk[pr] > 0) P.chunk_w = ctx->tune_chunk[pr];
+    }
     int* err = ctx->d_scal.p + 3;
     float* rows = out ? out : ctx->d_G.p;
     static const char* names[2][4] = {{"basis_symm", "basis_adjoint", "basis_tangent", "basis_fused"},
@@ -1702,10 +1709,8 @@ int ace_autotune(ace_ctx* ctx, int precision, double* ms) {
     hipEvent_t e0, e1;
     HC(hipEventCreate(&e0));
     HC(hipEventCreate(&e1));
-    int best = -1;
-    for (int c = 0; c < ACE_NPATHS; ++c) {
-        if (!ok[c]) continue;
-        set_path(ctx, precision, c);
+    // milliseconds of one energy/force pass in the current setting (a warm-up, then 3 or 20 timed passes)
+    auto time_ef = [&](double& ms_out) -> int {
         if (run_ef(ctx, precision)) return -1;   // warm-up (kernel loads, LDS attributes)
         float el = 0;
         int reps = 3;
@@ -1719,19 +1724,63 @@ int ace_autotune(ace_ctx* ctx, int precision, double* ms) {
             if (el / reps > 0.5f) break;
             reps = 20;                               // short launches: time more of them
         }
-        t[c] = (double)el / reps;
+        ms_out = (double)el / reps;
+        return 0;
+    };
+    ctx->tune_chunk[precision] = -1;
+    for (int k = 0; k < 3; ++k) ctx->tune_chunk_ms[precision][k] = -1;
+    int best = -1;
+    for (int c = 0; c < ACE_NPATHS; ++c) {
+        if (!ok[c]) continue;
+        set_path(ctx, precision, c);
+        if (time_ef(t[c])) return -1;
         if (best < 0 || t[c] < t[best]) best = c;
     }
+    // the angular tile shape on the chosen path: the chunk weight w of the triplet loops
+    set_path(ctx, precision, best);
+    ctx->pinned[precision] = false;
+    static const int wts[3] = {1, 3, 8};
+    int wbest = -1;
+    for (int k = 0; k < 3; ++k) {
+        ctx->tune_chunk[precision] = wts[k];
+        if (time_ef(ctx->tune_chunk_ms[precision][k])) return -1;
+        if (wbest < 0 || ctx->tune_chunk_ms[precision][k] < ctx->tune_chunk_ms[precision][wbest]) wbest = k;
+    }
+    ctx->tune_chunk[precision] = wts[wbest];
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     ctx->prof = prof;
-    set_path(ctx, precision, best);
-    ctx->pinned[precision] = false;
     if (check_err_flag(ctx, "ace_autotune")) return -1;
     if (ms) {
         for (int c = 0; c < ACE_NPATHS; ++c) ms[c] = t[c];
         ms[ACE_NPATHS] = best;
     }
+    return 0;
+}
+
+int ace_tile_shape(const ace_ctx* cctx, int precision, double* shape) {
+    if (!cctx) return -1;
+    ace_ctx* ctx = const_cast<ace_ctx*>(cctx);
+    if (precision != ACE_PREC_FP32 && precision != ACE_PREC_FP64_BASIS && precision != ACE_PREC_BF16L1)
+        return fail(ctx, "ace_tile_shape: precision mode not available");
+    if (!shape) return fail(ctx, "ace_tile_shape: null array");
+    const bool tu = tuned(ctx, precision);
+    int path = -1;
+    if (tu && ctx->tune_fused[precision] >= 0) {
+        const int f = ctx->tune_fused[precision], w = ctx->tune_waves[precision], st = ctx->tune_stage[precision];
+        path = f == 1 ? ACE_PATH_FUSED : f == 2 ? ACE_PATH_FUSED_2W
+             : (w == 2 ? (st == 1 ? ACE_PATH_SPLIT_STAGED_2W : ACE_PATH_SPLIT_2W)
+                       : (st == 1 ? ACE_PATH_SPLIT_STAGED : ACE_PATH_SPLIT));
+    }
+    const int fw = fused_waves(ctx, precision);
+    shape[0] = path;
+    shape[1] = fw ? fw : split_waves(ctx, 0, precision);
+    shape[2] = (tu && ctx->tune_stage[precision] >= 0) ? ctx->tune_stage[precision] : -1;
+    const int w = (tu && ctx->tune_chunk[precision] > 0) ? ctx->tune_chunk[precision] : 3;
+    shape[3] = w;
+    const int kw = w == 1 ? 0 : (w == 3 ? 1 : 2);
+    shape[4] = tu ? ctx->tune_chunk_ms[precision][kw] : -1;
+    for (int k = 0; k < 3; ++k) shape[5 + k] = tu ? ctx->tune_chunk_ms[precision][k] : -1;
     return 0;
 }
 

@@ -201,15 +201,15 @@ __device__ inline void ld4(const T* p, T& a, T& b, T& c, T& d) {
 }
 
 // Chunk length C (1 .. 64) of an angular block with m rows of at most Dmax pairs: the one that minimises
-// rounds (C + 3/8), rounds = ceil(m ceil(Dmax / C) / BT) passes of the BT threads over the units and 3/8 the
-// cost of a chunk's set-up and flush relative to one pair (ties: the longer chunk).  Lane l tries C = l + 1;
-// every wave finds the same minimum, so C is uniform over the workgroup.
+// rounds (C + w/8), rounds = ceil(m ceil(Dmax / C) / BT) passes of the BT threads over the units and w/8 the
+// cost of a chunk's set-up and flush relative to one pair (w from the autotuner, 3 by default; ties: the
+// longer chunk).  Lane l tries C = l + 1; every wave finds the same minimum, so C is uniform.
 template <int BT>
-__device__ inline int pick_chunk(int m, int Dmax, int lane) {
+__device__ inline int pick_chunk(int m, int Dmax, int lane, int w) {
     const int C = min(lane + 1, max(Dmax, 1));
     const int nq = div_small(Dmax + C - 1, C, __builtin_amdgcn_rcpf((float)C));
     const int rounds = (m * nq + BT - 1) / BT;
-    const int key = wave_min_i(rounds * (8 * C + 3) * 64 + (64 - C));
+    const int key = wave_min_i(rounds * (8 * C + w) * 64 + (64 - C));
     return 64 - (key & 63);
 }
 
@@ -252,7 +252,7 @@ __device__ inline void ld_e(const T* p, T* e, T* ed) {
 template <class T, int NG, int NZ, int MODE>
 __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int nJ, int nK, bool same,
                                      const int* __restrict__ cidx, const float* wtab, int nA, int goff, double* gw,
-                                     int wv, int lane, int rs_f, bool rs_writer) {
+                                     int wv, int lane, int rs_f, bool rs_writer, int chunk_w) {
     constexpr int ADJ = (MODE == 1);
     constexpr int TAN = (MODE == 2);
     constexpr int BT = 64;   // one wave per block
@@ -292,7 +292,7 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
     const int xs = (same && !(nJ & 1)) ? nJ / 2 : m;       // rows x >= xs are one shorter
     const float rm = __builtin_amdgcn_rcpf((float)m);
     const int qB = div_small(BT, m, rm), rB = BT - qB * m; // unit u -> u + BT: chunk + qB, row + rB
-    const int C = __builtin_amdgcn_readfirstlane(pick_chunk<BT>(m, Dmax, lane));
+    const int C = __builtin_amdgcn_readfirstlane(pick_chunk<BT>(m, Dmax, lane, chunk_w));
     const int nq = div_small(Dmax + C - 1, C, __builtin_amdgcn_rcpf((float)C));
     const int U = m * nq;
     int q = div_small(tid, m, rm), x = tid - q * m;       // unit u = q m + x
@@ -546,7 +546,7 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
                 if (deal<BW>(load, T3) != wv) continue;
                 angular_canon<T, NG, NZ, MODE>(L, bJ, bK, nJ, nK, same, L.ci + (sp.a_poff[b] - sp.a_poff[0]),
                                                (const float*)L.red + 14 * NG * b, nA, sp.a_goff[b], gw, wv, lane,
-                                               rs_f, rs_writer);
+                                               rs_f, rs_writer, P.chunk_w);
             }
         }
     }

@@ -186,9 +186,19 @@ class NNPDevice:
         ms = np.zeros(n + 1)
         self._check(self.lib.ace_autotune(self.ctx, precision, _dp(ms)), "ace_autotune")
         names = self.PATHS
-        return {"ms": {names[i]: float(ms[i]) for i in range(n) if ms[i] >= 0}, "choice": names[int(ms[n])]}
+        return {"ms": {names[i]: float(ms[i]) for i in range(n) if ms[i] >= 0}, "choice": names[int(ms[n])],
+                "tile": self.tile_shape(precision)}
 
     PATHS = ("fused", "split_staged", "split", "fused_2w", "split_staged_2w", "split_2w")
+
+    def tile_shape(self, precision=PREC_FP32):
+        """The shapes the energy/force pass uses: path (None: size rule), waves per atom, network weights staged
+        (None: not tuned), the angular chunk weight w and the milliseconds measured for w = 1, 3, 8."""
+        v = np.zeros(8)
+        self._check(self.lib.ace_tile_shape(self.ctx, precision, _dp(v)), "ace_tile_shape")
+        return {"path": None if v[0] < 0 else self.PATHS[int(v[0])], "waves": int(v[1]),
+                "staged": None if v[2] < 0 else bool(v[2]), "chunk_w": int(v[3]),
+                "chunk_ms": {w: float(v[5 + k]) for k, w in enumerate((1, 3, 8)) if v[5 + k] >= 0}}
 
     def set_path(self, path, precision=PREC_FP32):
         """Fixes the energy/force path (a name of PATHS, or None for the size rule) for every batch."""

@@ -20,6 +20,8 @@
  *   ace_compute .................... NNP::compute(Structure&) (src/nnp/nnp.cpp:489-494): build + energy
  *   ace_energy_forces_noalloc ...... (no counterpart) ace_energy_forces into device memory, allocating nothing
  *   ace_train_step ................. NNPTEFR::error_cost_o1 (src/nnp/nnptefr.cpp:1292-1425)
+ *                                    extended with the force-matching term d(F)/dW
+ *                                    (network factor NN::D2ODZDI, src/ml/nn.cpp:2047-2140)
  *   ace_train_optimizer ............ (no counterpart) the ADAM update of the training loop
  *                                    (nnptefr.cpp:1031-1066, opt::ADAM algo.cpp:280-296) on the GPU
  *   ace_train_opt_state ............ (no counterpart) its state, for restart files
@@ -27,10 +29,10 @@
  *                                    the caller's dL/dE and dL/dF (the network factor NN::D2ODZDI::grad,
  *                                    src/ml/nn.cpp:2047-2140, seeded as NNPTEFR::error_cost_o1 seeds it,
  *                                    nnptefr.cpp:1353-1378, with the upstream vectors in place of the loss's)
- *                                   extended with the force-matching term d(F)/dW
- *                                   (network factor NN::D2ODZDI, src/ml/nn.cpp:2047-2140)
  *   ace_md_init / ace_md_run / ace_md_state  velocity-Verlet steps (src/sim/integrator.cpp:548-571), HIP graphs
- *   ace_autotune ................... (no counterpart) measured choice of the energy/force path
+ *   ace_autotune ................... (no counterpart) measured choice of the energy/force path and of
+ *                                    its tile shapes
+ *   ace_tile_shape ................. (no counterpart) the shapes in use
  *   ace_set_path ................... (no counterpart) fixed choice of the energy/force path
  *   ace_plain_energy_forces ........ same as ace_energy_forces, unbatched FP64 one-thread-per-atom
  *                                   GPU port (the baseline the batched path is measured against)
@@ -215,10 +217,18 @@ int  ace_md_state(ace_ctx* ctx, double* pos, double* vel);
 enum { ACE_PATH_FUSED = 0, ACE_PATH_SPLIT_STAGED = 1, ACE_PATH_SPLIT = 2, ACE_PATH_FUSED_2W = 3,
        ACE_PATH_SPLIT_STAGED_2W = 4, ACE_PATH_SPLIT_2W = 5, ACE_NPATHS = 6 };
 /* Autotuner (no counterpart in ace): times the paths on the uploaded batch (neighbour list built) and
- * keeps the fastest for this precision.  ms: NULL or ACE_NPATHS + 1 entries, the milliseconds of each
- * path (-1 where not feasible) and the index chosen.  The choice holds while the batch size and
- * neighbour-row capacity stay as measured; otherwise size rules decide. */
+ * keeps the fastest for this precision, then, on that path, the tile shape of the angular triplet loops:
+ * the chunk of partner slots a lane takes per fixed slot, chosen per block to minimise
+ * rounds x (chunk + w/8) with w in {1, 3, 8} (w: the cost of a chunk's set-up and flush in eighths of one
+ * pair; a larger w favours long chunks, a smaller one fuller rounds).  ms: NULL or ACE_NPATHS + 1 entries,
+ * the milliseconds of each path (-1 where not feasible) and the index chosen.  The choice holds while the
+ * batch size and neighbour-row capacity stay as measured; otherwise size rules decide. */
 int  ace_autotune(ace_ctx* ctx, int precision, double* ms);
+/* The shapes the energy/force pass of this precision uses now: shape[0] path (ACE_PATH_*, -1: size rule),
+ * shape[1] waves per atom of its basis kernels, shape[2] network weights staged in LDS (1) or not (0),
+ * shape[3] the angular chunk weight w, shape[4] the milliseconds of the chosen chunk weight's timing (or -1),
+ * shape[5..7] the milliseconds measured for w = 1, 3, 8 (-1 where not measured). */
+int  ace_tile_shape(const ace_ctx* ctx, int precision, double* shape);
 /* Fixes the energy/force path of a precision (ACE_PATH_*) for every batch, the wave count of the
  * symmetry-function, adjoint and tangent kernels of training included (a fused path whose workgroup does
  * not fit in LDS runs split); -1 returns to the size rules.  The BF16 first-layer mode has the split

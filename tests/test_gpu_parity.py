@@ -509,6 +509,10 @@ def test_autotune_keeps_results_within_tolerance():
     r = dev.autotune(PREC_FP32)
     log(f"autotune c2: {r}")
     assert r["choice"] in r["ms"] and all(v > 0 for v in r["ms"].values()) and len(r["ms"]) >= 2
+    # the tile shape of the angular triplet loops was measured on the chosen path and the fastest kept
+    t = r["tile"]
+    assert t["path"] == r["choice"] and sorted(t["chunk_ms"]) == [1, 3, 8]
+    assert t["chunk_w"] == min(t["chunk_ms"], key=t["chunk_ms"].get)
     E, F, _ = dev.compute(PREC_FP32)
     Er, Fr, _ = O.Oracle(m).compute(strucs[0])
     assert abs(E[0] - Er) / strucs[0].natoms < TOL[PREC_FP32]["E"] and np.abs(F - Fr).max() < TOL[PREC_FP32]["F"]

@@ -183,7 +183,7 @@ __device__ inline T prm(const float* f, const double* d, int i) {
 // is within one of the quotient, and one integer step corrects it
 __device__ inline int div_small(int a, int b, float rb) {
     int q = (int)(((float)a + 0.5f) * rb);
-    const int r = a - q * b;
+    const int r = a - (int)__umul24(q, b);   // q, b < 2^24: the full-rate 24-bit product
     q += (r >= b) - (r < 0);
     return q;
 }
@@ -208,8 +208,8 @@ template <int BT>
 __device__ inline int pick_chunk(int m, int Dmax, int lane, int w) {
     const int C = min(lane + 1, max(Dmax, 1));
     const int nq = div_small(Dmax + C - 1, C, __builtin_amdgcn_rcpf((float)C));
-    const int rounds = (m * nq + BT - 1) / BT;
-    const int key = wave_min_i(rounds * (8 * C + w) * 64 + (64 - C));
+    const int rounds = ((int)__umul24(m, nq) + BT - 1) / BT;   // m nq < 2^24
+    const int key = wave_min_i((int)__umul24(rounds, 8 * C + w) * 64 + (64 - C));
     return 64 - (key & 63);
 }
 
@@ -295,7 +295,12 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
     const int C = __builtin_amdgcn_readfirstlane(pick_chunk<BT>(m, Dmax, lane, chunk_w));
     const int nq = div_small(Dmax + C - 1, C, __builtin_amdgcn_rcpf((float)C));
     const int U = m * nq;
-    int q = div_small(tid, m, rm), x = tid - q * m;       // unit u = q m + x
+    int q = div_small(tid, m, rm), x = tid - (int)__umul24(q, m);   // unit u = q m + x
+    // partner records and accumulators from the side's first slot, indexed by y < Mx
+    const T* uV = L.u + 4 * bV;
+    const T* eV = L.E + SE * bV;
+    const T* tV = L.te + 4 * bV;
+    double* gV = gw + 3 * bV;
     for (int u = tid; u < U; u += BT) {
         const int j0 = q * C;
         const int cnt = Dmax - (x >= xs ? 1 : 0) - j0;    // pairs of the row from this chunk on
@@ -306,14 +311,13 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
         T tfx = T(0), tfy = T(0), tfz = T(0), tfp = T(0);
         if (TAN) ld4<T>(L.te + 4 * sf, tfx, tfy, tfz, tfp);
         T gfx = T(0), gfy = T(0), gfz = T(0);
-        int y = x + d0 + j0;
-        y -= (y >= Mx) ? Mx : 0;
+        unsigned y = x + d0 + j0;
+        y = min(y, y - (unsigned)Mx);   // y mod Mx for y < 2 Mx: the difference wraps above y when y < Mx
         for (int j = 0; j < C; ++j) {
             if (j < cnt) {
-                const int sv = bV + y;
                 T vx, vy, vz, vi, Ev[NG], Edv[NG];
-                ld4<T>(L.u + 4 * sv, vx, vy, vz, vi);
-                ld_e<T, NG>(L.E + SE * sv, Ev, Edv);
+                ld4<T>(uV + 4 * y, vx, vy, vz, vi);
+                ld_e<T, NG>(eV + SE * y, Ev, Edv);
                 const T cs = fx * vx + fy * vy + fz * vz;
                 T D[NG];
 #pragma unroll
@@ -340,7 +344,7 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
                             for (int g = 0; g < NG; ++g) acc[s][k][g] += cp[s][k] * D[g];
                 } else if (TAN) {
                     T tvx, tvy, tvz, tvp;
-                    ld4<T>(L.te + 4 * sv, tvx, tvy, tvz, tvp);
+                    ld4<T>(tV + 4 * y, tvx, tvy, tvz, tvp);
                     const T dcs = (vx * tfx + vy * tfy + vz * tfz - cs * tfp) * fi +
                                   (fx * tvx + fy * tvy + fz * tvz - cs * tvp) * vi;
                     T X[NG], Y[NG];
@@ -388,14 +392,14 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
                     gfx = fma(xf, vx, fma(kf, fx, gfx));
                     gfy = fma(xf, vy, fma(kf, fy, gfy));
                     gfz = fma(xf, vz, fma(kf, fz, gfz));
-                    double* gv = gw + 3 * sv;
+                    double* gv = (double*)((char*)gV + __umul24(y, 24u));   // 24-bit product: one full-rate op
                     atomicAdd(gv, (double)(kv * vx + xv * fx));
                     atomicAdd(gv + 1, (double)(kv * vy + xv * fy));
                     atomicAdd(gv + 2, (double)(kv * vz + xv * fz));
                 }
             }
             ++y;
-            y -= (y >= Mx) ? Mx : 0;
+            y = min(y, y - (unsigned)Mx);
         }
         if (ADJ && cnt > 0) {
             double* gf = gw + 3 * sf;
@@ -459,8 +463,10 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
     constexpr int TAN = (PM == 2);
     constexpr int MAXA = 16;   // functions per register chunk of the generic path
     const int ng = sp.ng;
-    // --- radial blocks: one wave per neighbour species J; lane = function, lane group = neighbour subset;
-    // the lanes of a function sum their groups in FP64 ----------------------------------------------
+    // --- radial blocks: one wave per neighbour species J.  Sums over neighbours (G, tangent rows): lane =
+    // function, lane group = neighbour subset, the lanes of a function sum their groups in FP64.  Sums over
+    // functions (the adjoint's dE/dr per neighbour): lane = (neighbour, function group), the neighbour's
+    // |d| and cutoff values in registers, the groups summed across lanes once per neighbour ------------
     {
         int load[BW];
 #pragma unroll
@@ -472,6 +478,36 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
             if (deal<BW>(load, nJ * nR) != wv) continue;
             const int form = sp.r_form[J];
             const int goff = sp.r_goff[J], poff = sp.r_poff[J];
+            if (ADJ) {
+                int ns = 1;   // lanes per function group: neighbours side by side, at most 64
+                while (ns < nJ && ns < 64) ns <<= 1;
+                const int fg = 64 / ns, h = lane / ns;
+                const int rq0 = poff - sp.r_poff[0];
+                for (int q0 = 0; q0 < nJ; q0 += ns) {
+                    const int q = q0 + lane % ns;
+                    const bool on = q < nJ;
+                    const int slot = bJ + (on ? q : 0);
+                    const double r = slot_r64(L, slot);
+                    const T fc = L.fR[slot], gc = L.gR[slot];
+                    T acc = T(0);
+                    // the form is uniform: the Gaussian (every benchmark model) without the dispatch per function
+                    if (on && form == ACE_RAD_GAUSSIAN)
+                        for (int n = h; n < nR; n += fg) {
+                            T v, dv;
+                            t_radial_r64<T>(ACE_RAD_GAUSSIAN, L.rp[rq0 + n], L.rp[sp.nR + rq0 + n], r, fc, gc, v, dv);
+                            acc += (T)L.row[goff + n] * dv;
+                        }
+                    else if (on)
+                        for (int n = h; n < nR; n += fg) {
+                            T v, dv;
+                            t_radial_r64<T>(form, L.rp[rq0 + n], L.rp[sp.nR + rq0 + n], r, fc, gc, v, dv);
+                            acc += (T)L.row[goff + n] * dv;
+                        }
+                    acc = t_cross_sum<T>(acc, ns);
+                    if (h == 0 && on) L.fr[slot] += acc;
+                }
+                continue;
+            }
             for (int n0 = 0; n0 < nR; n0 += 64) {
                 const int nc = min(64, nR - n0);
                 int nrp = 1;
@@ -482,31 +518,30 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
                 const int rq = poff - sp.r_poff[0] + n0 + n;
                 const double eta = on ? L.rp[rq] : 0.0;
                 const double rs = on ? L.rp[sp.nR + rq] : 0.0;
-                const T w = (ADJ && on) ? (T)L.row[goff + n0 + n] : T(0);
                 T acc = T(0);
                 const int rounds = (nJ + gpw - 1) / gpw;
-                for (int t = 0; t < rounds; ++t) {
-                    const int q = grp + t * gpw;
-                    T val = T(0);
-                    if (on && q < nJ) {
-                        const int slot = bJ + q;
-                        T v, dv;
-                        t_radial_r64<T>(form, eta, rs, slot_r64(L, slot), L.fR[slot], L.gR[slot], v, dv);
-                        val = ADJ ? w * dv : (TAN ? dv * L.te[4 * slot + 3] : v);
+                if (form == ACE_RAD_GAUSSIAN)
+                    for (int t = 0; t < rounds; ++t) {
+                        const int q = grp + t * gpw;
+                        if (on && q < nJ) {
+                            const int slot = bJ + q;
+                            T v, dv;
+                            t_radial_r64<T>(ACE_RAD_GAUSSIAN, eta, rs, slot_r64(L, slot), L.fR[slot], L.gR[slot], v, dv);
+                            acc += TAN ? dv * L.te[4 * slot + 3] : v;
+                        }
                     }
-                    if (ADJ) {
-                        // dE/dr of the slot: sum over the nrp function lanes of its group; one lane group per slot
-                        val = t_group_sum<T>(val, nrp);
-                        if (n == 0 && q < nJ) L.fr[bJ + q] += val;
-                    } else {
-                        acc += val;
+                else
+                    for (int t = 0; t < rounds; ++t) {
+                        const int q = grp + t * gpw;
+                        if (on && q < nJ) {
+                            const int slot = bJ + q;
+                            T v, dv;
+                            t_radial_r64<T>(form, eta, rs, slot_r64(L, slot), L.fR[slot], L.gR[slot], v, dv);
+                            acc += TAN ? dv * L.te[4 * slot + 3] : v;
+                        }
                     }
-                }
-                if (ADJ) wave_sync();   // the next chunk maps slots to other lane groups
-                if (!ADJ) {
-                    const double tot = t_cross_sum<double>((double)acc, nrp);
-                    if (lane < nc) L.row[goff + n0 + lane] = tot;
-                }
+                const double tot = t_cross_sum<double>((double)acc, nrp);
+                if (lane < nc) L.row[goff + n0 + lane] = tot;
             }
         }
     }
@@ -873,7 +908,7 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
     __shared__ int s_cnt[ACE_MAXT + 1];
     __shared__ DevSpecies s_sp;
     __shared__ double s_red[BW][16];
-    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x, wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int i = blockIdx.x;
     if (i >= natoms) return;
     const int st = io.struc_of[i];

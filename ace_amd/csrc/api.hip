@@ -62,6 +62,8 @@ struct ace_ctx {
     int kmax = 0, ldg = 0, maxA = 0, maxR = 0, ngmax = 1, ngk = 1, nzmax = 1;
     int nrp = 1;                     // most radial functions of one species
     bool all_canon = true;
+    int lsplit = 0;                  // level split of the two eta groups' canonical levels (0: none)
+    bool lsplit_on = true;           // ace_set_level_split
     DevModel hdm;
     DBuf<DevModel> d_model;
     DBuf<float> d_r_rs, d_r_eta, d_a_lamf, d_a_zeta, d_inpw, d_inpb, d_wpad;
@@ -599,6 +601,24 @@ int build_tables(ace_ctx* ctx) {
     ctx->nzmax = nzmax_all;
     ctx->ngk = ngk;
     ctx->all_canon = all_canon;
+    // two eta groups whose canonical levels do not overlap -- group 0 below level s, group 1 from s on --
+    // take the kernels that form only those (level, group) products
+    ctx->lsplit = 0;
+    if (ngk == 2) {
+        int top0 = -1, low1 = 99;
+        for (int I = 0; I < nt; ++I) {
+            const DevSpecies& d = dm.sp[I];
+            for (int b = 0; b < nab; ++b) {
+                if (!d.a_canon[b]) continue;
+                for (int q = 0; q < d.a_n[b]; ++q) {
+                    const int fi = d.a_poff[b] + q, k = a_sel[fi] % 7;
+                    if (a_grp[fi] == 0) top0 = std::max(top0, k);
+                    else low1 = std::min(low1, k);
+                }
+            }
+        }
+        if (top0 >= 0 && low1 < 99 && top0 < low1) ctx->lsplit = top0 + 1;
+    }
     dm.kmax = kmax;
     dm.ng_max = ngmax;
     ctx->kmax = kmax;
@@ -753,8 +773,13 @@ DevBasisParams basis_params(ace_ctx* ctx) {
 // ---- basis kernel dispatch ------------------------------------------------------------------------
 
 // the canonical angular accumulators are sized by the largest number of eta groups (1, 2 or 4) and of
-// power levels; the generic per-function path is compiled in only when some angular block is not canonical
-void basis_fns(ace_ctx* ctx, bool f64, BasisFn* f) { basis_fill(f64, ctx->ngk, ctx->nzmax, !ctx->all_canon, f); }
+// power levels; the generic per-function path is compiled in only when some angular block is not canonical;
+// FP32 kernels of two eta groups take the model's level split
+void basis_fns(ace_ctx* ctx, bool f64, BasisFn* f) {
+    basis_fill(f64, ctx->ngk, ctx->nzmax, !ctx->all_canon, ctx->lsplit_on ? ctx->lsplit : 0, f);
+}
+
+int level_split(const ace_ctx* ctx, bool f64) { return (!f64 && ctx->ngk == 2 && ctx->lsplit_on) ? ctx->lsplit : 0; }
 
 int nb_slots(ace_ctx* ctx) { return ctx->slots_cap; }
 
@@ -1781,6 +1806,13 @@ int ace_tile_shape(const ace_ctx* cctx, int precision, double* shape) {
     const int kw = w == 1 ? 0 : (w == 3 ? 1 : 2);
     shape[4] = tu ? ctx->tune_chunk_ms[precision][kw] : -1;
     for (int k = 0; k < 3; ++k) shape[5 + k] = tu ? ctx->tune_chunk_ms[precision][k] : -1;
+    shape[8] = level_split(ctx, precision == ACE_PREC_FP64_BASIS);
+    return 0;
+}
+
+int ace_set_level_split(ace_ctx* ctx, int enable) {
+    if (!ctx) return -1;
+    ctx->lsplit_on = enable != 0;
     return 0;
 }
 

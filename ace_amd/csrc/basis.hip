@@ -232,10 +232,25 @@ __device__ inline void ld_e(const T* p, T* e, T* ed) {
     }
 }
 
+// Level bands of the canonical blocks.  SPL = 0: every eta group g uses every level k < NZ.  SPL > 0 (two
+// groups): group 0 uses the levels k < SPL and group 1 the levels SPL <= k < NZ -- a model whose small
+// zetas come with one eta and large zetas with the other -- so only NZ of the 2 NZ (level, group) products
+// are formed per sign.  NA sums per sign; sum c of a sign is (level lev(c), group grp(c)).
+template <int NG, int NZ, int SPL>
+struct Bands {
+    static_assert(SPL == 0 || (NG == 2 && SPL > 0 && SPL < NZ), "a level split takes two eta groups");
+    static constexpr int NA = SPL > 0 ? NZ : NZ * NG;
+    static constexpr int lo(int g) { return (SPL > 0 && g == 1) ? SPL : 0; }
+    static constexpr int hi(int g) { return (SPL > 0 && g == 0) ? SPL : NZ; }   // one past the last level
+    static constexpr bool on(int k, int g) { return k >= lo(g) && k < hi(g); }
+    static constexpr int lev(int c) { return SPL > 0 ? c : c / NG; }
+    static constexpr int grp(int c) { return SPL > 0 ? (c >= SPL ? 1 : 0) : c % NG; }
+};
+
 // One angular block whose functions are all canonical: lambda = +-1 and zeta = 2^k, k < NZ <= 7, one
 // function per (sign s, level k, eta group g).  Per triplet the windows b_s = |1/2 + s cos/2| are raised
 // to 2^k (cp) and 2^k - 1 (cq) by squaring, and the block's sums are the outer products
-//   G[s][k][g] += b_s^(2^k) E_j[g] E_k[g]
+//   G[s][k][g] += b_s^(2^k) E_j[g] E_k[g]      ((k, g) in the level bands of SPL)
 // held in registers with static indices; cidx maps each function of the block to (s*7 + k)*NG + g.
 // NZ (a compile-time bound on the levels in use) keeps the loop free of branches; unused levels are
 // computed and discarded.  E / Ed of slot q are E[q][g] and E[q][NG + g], zero for g >= ng.
@@ -249,10 +264,11 @@ __device__ inline void ld_e(const T* p, T* e, T* ed) {
 // LDS accumulator once per chunk; pick_chunk sets C.
 // Each lane's sums (modes 0 and 2) and their wave reduce-scatter are T; the sum over the waves is FP64.
 // The adjoint (mode 1) adds each pair's derivative for v into the wave's FP64 accumulators.
-template <class T, int NG, int NZ, int MODE>
+template <class T, int NG, int NZ, int MODE, int SPL>
 __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int nJ, int nK, bool same,
                                      const int* __restrict__ cidx, const float* wtab, int nA, int goff, double* gw,
                                      int wv, int lane, int rs_f, bool rs_writer, int chunk_w) {
+    using B = Bands<NG, NZ, SPL>;
     constexpr int ADJ = (MODE == 1);
     constexpr int TAN = (MODE == 2);
     constexpr int BT = 64;   // one wave per block
@@ -322,7 +338,8 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
                 T D[NG];
 #pragma unroll
                 for (int g = 0; g < NG; ++g) D[g] = Ef[g] * Ev[g];
-                // cp[s][k] = b_s^(2^k) (the adjoint needs k < NZ - 1 only), cq[s][k] = b_s^(2^k - 1) (tangent)
+                // cp[s][k] = b_s^(2^k), cq[s][k] = b_s^(2^k - 1) (tangent); the levels no band reads are
+                // dropped by the compiler
                 T cp[2][NZ], cq[2][NZ];
                 cp[0][0] = fabs(T(0.5) + T(0.5) * cs);
                 cp[1][0] = fabs(T(0.5) - T(0.5) * cs);
@@ -333,7 +350,7 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
 #pragma unroll
                     for (int s = 0; s < 2; ++s) {
                         cq[s][k] = TAN ? ((k == 1) ? cp[s][0] : cq[s][k - 1] * cp[s][k - 1]) : T(0);
-                        cp[s][k] = (ADJ && k == NZ - 1) ? T(0) : cp[s][k - 1] * cp[s][k - 1];
+                        cp[s][k] = cp[s][k - 1] * cp[s][k - 1];
                     }
                 if (MODE == 0) {
 #pragma unroll
@@ -341,7 +358,8 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
 #pragma unroll
                         for (int s = 0; s < 2; ++s)
 #pragma unroll
-                            for (int g = 0; g < NG; ++g) acc[s][k][g] += cp[s][k] * D[g];
+                            for (int g = 0; g < NG; ++g)
+                                if (B::on(k, g)) acc[s][k][g] += cp[s][k] * D[g];
                 } else if (TAN) {
                     T tvx, tvy, tvz, tvp;
                     ld4<T>(tV + 4 * y, tvx, tvy, tvz, tvp);
@@ -359,25 +377,34 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
                         for (int s = 0; s < 2; ++s) {
                             const T dp = T(1 << k) * (s ? T(-0.5) : T(0.5)) * cq[s][k];   // d b^(2^k) / d cos
 #pragma unroll
-                            for (int g = 0; g < NG; ++g) acc[s][k][g] += dp * X[g] + cp[s][k] * Y[g];
+                            for (int g = 0; g < NG; ++g)
+                                if (B::on(k, g)) acc[s][k][g] += dp * X[g] + cp[s][k] * Y[g];
                         }
                 } else {
-                    // S_g = sum_s b (w_0 + b (w_1 + b^2 (w_2 + b^4 (w_3 + ...)))) = sum_s,k w_k b^(2^k) and
-                    // Sp_g = sum_s (u_0 + b (u_1 + b^2 (u_2 + ...))) = sum_s,k u_k b^(2^k - 1), u_k = w_k d b^(2^k)/d cos
-                    // / b^(2^k - 1): Horner over the squares b^(2^k), k < NZ - 1
+                    // Over the levels lo <= k < hi of group g's band:
+                    // S_g = sum_s b^(2^lo) (w_lo + b^(2^lo) (w_lo+1 + b^(2^(lo+1)) (...))) = sum_s,k w_k b^(2^k) and
+                    // Sp_g = sum_s b^(2^lo - 1) (u_lo + b^(2^lo) (u_lo+1 + ...)) = sum_s,k u_k b^(2^k - 1),
+                    // u_k = w_k d b^(2^k)/d cos / b^(2^k - 1): Horner over the squares b^(2^k)
                     T S[NG], Sp[NG];
 #pragma unroll
                     for (int s = 0; s < 2; ++s)
 #pragma unroll
                         for (int g = 0; g < NG; ++g) {
-                            T h = wgt[s][NZ - 1][g], hd = wgd[s][NZ - 1][g];
+                            T h = T(0), hd = T(0), q = T(1);
 #pragma unroll
-                            for (int k = NZ - 2; k >= 0; --k) {
-                                h = h * cp[s][k] + wgt[s][k][g];
-                                hd = hd * cp[s][k] + wgd[s][k][g];
+                            for (int k = NZ - 1; k >= 0; --k) {
+                                if (k == B::hi(g) - 1) {
+                                    h = wgt[s][k][g];
+                                    hd = wgd[s][k][g];
+                                } else if (B::on(k, g)) {
+                                    h = h * cp[s][k] + wgt[s][k][g];
+                                    hd = hd * cp[s][k] + wgd[s][k][g];
+                                }
+                                if (k < B::lo(g)) q = (k == B::lo(g) - 1) ? cp[s][k] : q * cp[s][k];   // b^(2^lo - 1)
                             }
-                            S[g] = s ? S[g] + h * cp[s][0] : h * cp[s][0];
-                            Sp[g] = s ? Sp[g] + hd : hd;
+                            const T hs = h * cp[s][B::lo(g)], hds = (B::lo(g) > 0) ? hd * q : hd;
+                            S[g] = s ? S[g] + hs : hs;
+                            Sp[g] = s ? Sp[g] + hds : hds;
                         }
                     T dEdc = Sp[0] * D[0], dEdf = S[0] * (Edf[0] * Ev[0]), dEdv = S[0] * (Ef[0] * Edv[0]);
 #pragma unroll
@@ -415,23 +442,21 @@ __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int n
         }
     }
     if (!ADJ) {
-        // wave reduce-scatter of the 2 NZ NG sums; the lane that writes sum f puts it in its (sign, level, group)
+        // wave reduce-scatter of the 2 NA sums; the lane that writes sum f puts it in its (sign, level, group)
         // slot
-        constexpr int M0 = 2 * NZ * NG;
+        constexpr int M0 = 2 * B::NA;
         static_assert(M0 <= 64, "one sum per lane at most");
         T v[M0];
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll
-            for (int k = 0; k < NZ; ++k)
-#pragma unroll
-                for (int g = 0; g < NG; ++g) v[(s * NZ + k) * NG + g] = acc[s][k][g];
+            for (int c = 0; c < B::NA; ++c) v[s * B::NA + c] = acc[s][B::lev(c)][B::grp(c)];
         rs_any<5, M0, T>(v, lane);
         const double tot = (double)v[0];
         if (rs_writer) {
             const int f = rs_f;
-            const int g = f % NG, k = (f / NG) % NZ, s = f / (NG * NZ);
-            L.red[wv * 64 + (s * 7 + k) * NG + g] = tot;
+            const int s = f / B::NA, c = f - s * B::NA;
+            L.red[wv * 64 + (s * 7 + B::lev(c)) * NG + B::grp(c)] = tot;
         }
         wave_sync();
         for (int m = lane; m < nA; m += 64) L.row[goff + m] = L.red[wv * 64 + cidx[m]];
@@ -454,7 +479,7 @@ __device__ inline int deal(int* load, int cost) {
 // Radial and angular sums of one atom (slots, tables and row already in LDS).  PM 0: G into row;
 // PM 1: per-slot dE/dd into the per-wave accumulators g from the dE/dG weights in row; PM 2: tangent
 // rows into row.  Ends with a workgroup barrier.
-template <class T, int NG, int NZ, int PM, int GEN, int BW>
+template <class T, int NG, int NZ, int PM, int GEN, int BW, int SPL>
 __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, const DevBasisParams& P,
                                   const int* s_cnt, int nt, int nbm, int tid, int wv, int lane) {
     constexpr int MODE = PM;
@@ -564,7 +589,7 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
     double* gw = L.g + wv * 3 * nbm;
     int rs_f = 0;
     bool rs_writer = false;
-    if (!ADJ) rs_any_owner<2 * NZ * NG>(lane, rs_f, rs_writer);
+    if (!ADJ) rs_any_owner<2 * Bands<NG, NZ, SPL>::NA>(lane, rs_f, rs_writer);
     {
         int load[BW];
 #pragma unroll
@@ -579,7 +604,7 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
                 const int T3 = same ? nJ * (nJ - 1) / 2 : nJ * nK;
                 if (nA == 0 || T3 == 0 || (GEN && !sp.a_canon[b])) continue;
                 if (deal<BW>(load, T3) != wv) continue;
-                angular_canon<T, NG, NZ, MODE>(L, bJ, bK, nJ, nK, same, L.ci + (sp.a_poff[b] - sp.a_poff[0]),
+                angular_canon<T, NG, NZ, MODE, SPL>(L, bJ, bK, nJ, nK, same, L.ci + (sp.a_poff[b] - sp.a_poff[0]),
                                                (const float*)L.red + 14 * NG * b, nA, sp.a_goff[b], gw, wv, lane,
                                                rs_f, rs_writer, P.chunk_w);
             }
@@ -890,9 +915,10 @@ struct BasisIO {
 
 // GEN = 0 compiles the kernel for models whose angular blocks are all canonical (no generic path).
 // NG: eta groups per species (1, 2 or 4, at least the model's largest count); NZ: canonical power levels
-// (4 or 7, at least the model's largest log2(zeta) + 1).
+// (3, 4, 5 or 7, at least the model's largest log2(zeta) + 1); SPL: the level split of two eta groups
+// (0: none; Bands).
 // MODE 0: network input rows; 1: gradients from dE/dG rows; 2: tangent rows; 3: fused energy and forces.
-template <class T, int NG, int NZ, int MODE, int GEN, int BW = 4>
+template <class T, int NG, int NZ, int MODE, int GEN, int BW = 4, int SPL = 0>
 __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* __restrict__ M, DevBasisParams P, BasisIO io,
                                               int nbm, int nt, int kmax) {
     constexpr int ADJ = (MODE == 1);
@@ -1126,11 +1152,11 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
     __syncthreads();
 
     if (FUSED) {
-        basis_sums<T, NG, NZ, 0, GEN, BW>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
+        basis_sums<T, NG, NZ, 0, GEN, BW, SPL>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
         block_network<T>(L, sp, tid, &io.e_atom[i]);
-        basis_sums<T, NG, NZ, 1, GEN, BW>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
+        basis_sums<T, NG, NZ, 1, GEN, BW, SPL>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
     } else {
-        basis_sums<T, NG, NZ, MODE, GEN, BW>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
+        basis_sums<T, NG, NZ, MODE, GEN, BW, SPL>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
     }
     if (MODE == 0) {
         // network input row: the ANN's input scaling inpw*(G + inpb) (nn.cpp:1406) applied here, in FP64,

@@ -194,11 +194,16 @@ class NNPDevice:
     def tile_shape(self, precision=PREC_FP32):
         """The shapes the energy/force pass uses: path (None: size rule), waves per atom, network weights staged
         (None: not tuned), the angular chunk weight w and the milliseconds measured for w = 1, 3, 8."""
-        v = np.zeros(8)
+        v = np.zeros(9)
         self._check(self.lib.ace_tile_shape(self.ctx, precision, _dp(v)), "ace_tile_shape")
         return {"path": None if v[0] < 0 else self.PATHS[int(v[0])], "waves": int(v[1]),
                 "staged": None if v[2] < 0 else bool(v[2]), "chunk_w": int(v[3]),
-                "chunk_ms": {w: float(v[5 + k]) for k, w in enumerate((1, 3, 8)) if v[5 + k] >= 0}}
+                "chunk_ms": {w: float(v[5 + k]) for k, w in enumerate((1, 3, 8)) if v[5 + k] >= 0},
+                "level_split": int(v[8])}
+
+    def set_level_split(self, enable):
+        """Level-split angular kernels (two eta groups over disjoint zeta levels) on or off."""
+        self._check(self.lib.ace_set_level_split(self.ctx, 1 if enable else 0), "ace_set_level_split")
 
     def set_path(self, path, precision=PREC_FP32):
         """Fixes the energy/force path (a name of PATHS, or None for the size rule) for every batch."""

@@ -33,6 +33,7 @@
  *   ace_autotune ................... (no counterpart) measured choice of the energy/force path and of
  *                                    its tile shapes
  *   ace_tile_shape ................. (no counterpart) the shapes in use
+ *   ace_set_level_split ............ (no counterpart) level-split angular kernels on or off
  *   ace_set_path ................... (no counterpart) fixed choice of the energy/force path
  *   ace_plain_energy_forces ........ same as ace_energy_forces, unbatched FP64 one-thread-per-atom
  *                                   GPU port (the baseline the batched path is measured against)
@@ -227,8 +228,13 @@ int  ace_autotune(ace_ctx* ctx, int precision, double* ms);
 /* The shapes the energy/force pass of this precision uses now: shape[0] path (ACE_PATH_*, -1: size rule),
  * shape[1] waves per atom of its basis kernels, shape[2] network weights staged in LDS (1) or not (0),
  * shape[3] the angular chunk weight w, shape[4] the milliseconds of the chosen chunk weight's timing (or -1),
- * shape[5..7] the milliseconds measured for w = 1, 3, 8 (-1 where not measured). */
+ * shape[5..7] the milliseconds measured for w = 1, 3, 8 (-1 where not measured), shape[8] the level split s
+ * of the canonical angular sums (0: none; with two eta groups whose power levels do not overlap, group 0
+ * below level s and group 1 from s on, the FP32 kernels form only those (level, group) products). */
 int  ace_tile_shape(const ace_ctx* ctx, int precision, double* shape);
+/* (no counterpart) Enables (default) or disables the level-split kernels; both give the oracle's results
+ * within the FP32 bounds, in a different rounding order. */
+int  ace_set_level_split(ace_ctx* ctx, int enable);
 /* Fixes the energy/force path of a precision (ACE_PATH_*) for every batch, the wave count of the
  * symmetry-function, adjoint and tangent kernels of training included (a fused path whose workgroup does
  * not fit in LDS runs split); -1 returns to the size rules.  The BF16 first-layer mode has the split

@@ -34,11 +34,13 @@ def structure(cell, pos, typ):
     return configs.Structure(np.asarray(cell, float), np.asarray(pos, float), np.asarray(typ, dtype=np.int32))
 
 
-def check_against_oracle(model, strucs, name, check_plain=True, paths=(None,), tol=None):
+def check_against_oracle(model, strucs, name, check_plain=True, paths=(None,), tol=None, setup=None):
     """Neighbour lists bit-exact, G, energies, forces and virials within TOL (or tol[(path, prec)]), for
-    each energy/force path in `paths` (None: the size rule)."""
+    each energy/force path in `paths` (None: the size rule); setup(dev) runs before the upload."""
     orc = O.Oracle(model)
     dev = NNPDevice(model)
+    if setup is not None:
+        setup(dev)
     dev.upload(strucs)
     dev.build_neighbors()
     off, j, img = dev.neighbors()
@@ -220,6 +222,35 @@ def test_every_angular_form():
         strucs.append(structure(cell, pos, rng.integers(0, 3, n)))
     scaled_model(m, strucs)
     check_against_oracle(m, strucs, "angular forms", check_plain=False, paths=NNPDevice.PATHS)
+
+
+@pytest.mark.parametrize("split", [True, False])
+def test_level_split_angular_kernels(split):
+    """Two eta groups over disjoint zeta levels (the headline basis: zeta 1-4 with one eta, 8-16 with the other,
+    blocks of 10 and 9 functions; the training basis: zeta 1-2 and 4) take the level-split kernels, which
+    form only those (level, group) products; with the split switched off the full kernels run.  Every
+    path, both precisions, against the oracle; the FP64-basis mode never splits."""
+    rng = np.random.default_rng(41)
+    models = [("c5 basis", configs.c5_model(rng), 4, 3), ("c3 basis", configs.c3_model(rng), 3, 2)]
+    for name, m, nt, spl in models:
+        for I in range(nt):
+            K = m.input_width(I)
+            w = [K, 24, 16, 1]
+            m.nets[I] = configs.Network(w, m.nets[I].act, 1.0, configs.xavier_params(w, rng), np.ones(K), np.zeros(K))
+        strucs = []
+        for n in (45, 70):
+            cell, pos = configs.random_packed(n, 0.085, 1.6, rng)
+            strucs.append(structure(cell, pos, rng.integers(0, nt, n)))
+        scaled_model(m, strucs)
+        seen = {}
+
+        def setup(dev):
+            dev.set_level_split(split)
+            seen["fp32"] = dev.tile_shape(PREC_FP32)["level_split"]
+            seen["fp64b"] = dev.tile_shape(PREC_FP64_BASIS)["level_split"]
+
+        check_against_oracle(m, strucs, f"{name} split={split}", check_plain=False, paths=NNPDevice.PATHS, setup=setup)
+        assert seen == {"fp32": spl if split else 0, "fp64b": 0}, (name, seen)
 
 
 @pytest.mark.parametrize("hidden,act", [((128, 128), 3), ((40, 24, 17), 2), ((8,), 10), ((33, 31), 15)])

@@ -78,6 +78,30 @@ def test_train_step_matches_oracle(loss, norm, wf, prec):
     dev.close()
 
 
+@pytest.mark.parametrize("split", [True, False])
+def test_train_step_level_split(split):
+    """The training basis (zeta 1-2 with one eta, 4 with the other) runs its symmetry-function, adjoint and
+    tangent kernels level-split by default; with the split off the full kernels run.  Both against the
+    oracle's loss and gradient."""
+    model, strucs, G = batch(seed=45)
+    nat = sum(s.natoms for s in strucs)
+    ld = LossDesc(0, 1, 0.5, 1.0 / len(strucs), 0.1 / (3 * nat))
+    ref, gref = oracle_loss(model, strucs, G, ld)
+    dev = NNPDevice(model)
+    dev.set_level_split(split)
+    assert dev.tile_shape(0)["level_split"] == (2 if split else 0)
+    dev.upload(strucs)
+    dev.build_neighbors()
+    dev.set_targets([s.energy for s in strucs], np.concatenate([s.forces for s in strucs]))
+    out, g = dev.train_step(0, 1, 0.5, 1.0 / len(strucs), 0.1 / (3 * nat))
+    le = abs(out[1] - ref[0]) / max(abs(ref[0]), 1e-12)
+    lf = abs(out[2] - ref[1]) / max(abs(ref[1]), 1e-12)
+    ge = np.max(np.abs(g - gref)) / np.max(np.abs(gref))
+    log(f"train level split={split}: loss_E rel {le:.1e} loss_F rel {lf:.1e} grad {ge:.1e}")
+    assert le < 1e-5 and lf < 1e-5 and ge < 1e-5
+    dev.close()
+
+
 @pytest.mark.parametrize("prec", [0, 2])
 def test_train_step_every_angular_form(prec):
     """The force-matching gradient through the generic angular path's per-triplet distance factors (tangent and

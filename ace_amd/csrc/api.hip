@@ -84,6 +84,10 @@ struct ace_ctx {
     int tune_fused[3] = {-1, -1, -1};   // per precision: fused kernel with 4 (1) or 2 (2) waves per atom, or
                                         // symm -> network -> adjoint (0)
     int tune_stage[3] = {-1, -1, -1};   // per precision: network weights staged in LDS (1) or read from memory
+    int tune_mlpw[3] = {0, 0, 0};       // per precision: waves per network tile (0: the default, 4)
+    int pin_chunk[3] = {0, 0, 0};       // ace_set_tile_shape: fixed chunk weight w / waves per network tile
+    int pin_mlpw[3] = {0, 0, 0};        // (0: tuned or default)
+    double tune_mlpw_ms[3][3] = {{-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}};   // timings for 1, 2, 4 waves
     int tune_waves[3] = {-1, -1, -1};   // per precision: waves per atom of the split path's kernels (4 or 2)
     int tune_chunk[3] = {-1, -1, -1};   // per precision: the angular chunk weight w (1, 3 or 8; -1: default 3)
     double tune_chunk_ms[3][3] = {{-1, -1, -1}, {-1, -1, -1}, {-1, -1, -1}};   // its timings, w = 1, 3, 8
@@ -921,6 +925,7 @@ int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = 
     {
         const int pr = precision < 0 ? ctx->precision : precision;
         if (tuned(ctx, pr) && ctx->tune_chunk[pr] > 0) P.chunk_w = ctx->tune_chunk[pr];
+        if (ctx->pin_chunk[pr] > 0) P.chunk_w = ctx->pin_chunk[pr];
     }
     int* err = ctx->d_scal.p + 3;
     float* rows = out ? out : ctx->d_G.p;
@@ -966,6 +971,13 @@ int run_finalize(ace_ctx* ctx) {
     });
 }
 
+// waves per 16-atom network tile: the autotuner's choice, else four
+int mlp_waves(ace_ctx* ctx, int precision) {
+    if (ctx->pin_mlpw[precision] > 0) return ctx->pin_mlpw[precision];
+    const int w = tuned(ctx, precision) ? ctx->tune_mlpw[precision] : 0;
+    return (w == 1 || w == 2 || w == 4) ? w : 4;
+}
+
 int run_mlp(ace_ctx* ctx, bool bf16, int precision = ACE_PREC_FP32) {
     ++ctx->seq;
     int lds = 0, lds_w = 0;
@@ -977,11 +989,14 @@ int run_mlp(ace_ctx* ctx, bool bf16, int precision = ACE_PREC_FP32) {
     if (tuned(ctx, precision) && ctx->tune_stage[precision] >= 0 && lds_w <= 160 * 1024) stage = ctx->tune_stage[precision] == 1;
     if (stage) lds = lds_w;
     if (lds > 160 * 1024) return fail(ctx, "mlp: network too wide for LDS");
-    auto f = stage ? (bf16 ? k_mlp_energy_grad<true, true> : k_mlp_energy_grad<true, false>)
-                   : (bf16 ? k_mlp_energy_grad<false, true> : k_mlp_energy_grad<false, false>);
+    const int W = mlp_waves(ctx, precision);
+#define ACE_MLP_F(W_) (stage ? (bf16 ? k_mlp_energy_grad<true, true, W_> : k_mlp_energy_grad<true, false, W_>) \
+                             : (bf16 ? k_mlp_energy_grad<false, true, W_> : k_mlp_energy_grad<false, false, W_>))
+    auto f = W == 1 ? ACE_MLP_F(1) : (W == 2 ? ACE_MLP_F(2) : ACE_MLP_F(4));
+#undef ACE_MLP_F
     if (lds > 64 * 1024) HC(hipFuncSetAttribute((const void*)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
     return timed(ctx, bf16 ? "mlp_energy_grad_bf16" : "mlp_energy_grad", [&] {
-        f<<<(int)ctx->tiles_h.size(), 64, lds, ctx->stream>>>(ctx->d_model.p, ctx->d_wpad.p,
+        f<<<(int)ctx->tiles_h.size(), 64 * W, lds, ctx->stream>>>(ctx->d_model.p, ctx->d_wpad.p,
                                                               (const __bf16*)ctx->d_w0bf.p, ctx->d_inpw.p,
                                                               ctx->d_tiles.p, ctx->ldg, ctx->d_G.p, ctx->d_dEdG.p,
                                                               ctx->d_atom_of_row.p, ctx->d_e_atom.p);
@@ -1754,6 +1769,7 @@ int ace_autotune(ace_ctx* ctx, int precision, double* ms) {
     };
     ctx->tune_chunk[precision] = -1;
     for (int k = 0; k < 3; ++k) ctx->tune_chunk_ms[precision][k] = -1;
+    ctx->tune_mlpw[precision] = 0;
     int best = -1;
     for (int c = 0; c < ACE_NPATHS; ++c) {
         if (!ok[c]) continue;
@@ -1772,6 +1788,19 @@ int ace_autotune(ace_ctx* ctx, int precision, double* ms) {
         if (wbest < 0 || ctx->tune_chunk_ms[precision][k] < ctx->tune_chunk_ms[precision][wbest]) wbest = k;
     }
     ctx->tune_chunk[precision] = wts[wbest];
+    // the network tile shape on a split path: waves per 16-atom tile
+    ctx->tune_mlpw[precision] = 0;
+    for (int k = 0; k < 3; ++k) ctx->tune_mlpw_ms[precision][k] = -1;
+    if (!fused_waves(ctx, precision)) {
+        static const int nws[3] = {1, 2, 4};
+        int nbest = -1;
+        for (int k = 0; k < 3; ++k) {
+            ctx->tune_mlpw[precision] = nws[k];
+            if (time_ef(ctx->tune_mlpw_ms[precision][k])) return -1;
+            if (nbest < 0 || ctx->tune_mlpw_ms[precision][k] < ctx->tune_mlpw_ms[precision][nbest]) nbest = k;
+        }
+        ctx->tune_mlpw[precision] = nws[nbest];
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     ctx->prof = prof;
@@ -1801,12 +1830,28 @@ int ace_tile_shape(const ace_ctx* cctx, int precision, double* shape) {
     shape[0] = path;
     shape[1] = fw ? fw : split_waves(ctx, 0, precision);
     shape[2] = (tu && ctx->tune_stage[precision] >= 0) ? ctx->tune_stage[precision] : -1;
-    const int w = (tu && ctx->tune_chunk[precision] > 0) ? ctx->tune_chunk[precision] : 3;
+    const int w = ctx->pin_chunk[precision] > 0 ? ctx->pin_chunk[precision]
+                : ((tu && ctx->tune_chunk[precision] > 0) ? ctx->tune_chunk[precision] : 3);
     shape[3] = w;
     const int kw = w == 1 ? 0 : (w == 3 ? 1 : 2);
     shape[4] = tu ? ctx->tune_chunk_ms[precision][kw] : -1;
     for (int k = 0; k < 3; ++k) shape[5 + k] = tu ? ctx->tune_chunk_ms[precision][k] : -1;
     shape[8] = level_split(ctx, precision == ACE_PREC_FP64_BASIS);
+    shape[9] = fw ? 0 : mlp_waves(ctx, precision);
+    for (int k = 0; k < 3; ++k) shape[10 + k] = tu ? ctx->tune_mlpw_ms[precision][k] : -1;
+    return 0;
+}
+
+int ace_set_tile_shape(ace_ctx* ctx, int precision, int chunk_w, int net_waves) {
+    if (!ctx) return -1;
+    if (precision != ACE_PREC_FP32 && precision != ACE_PREC_FP64_BASIS && precision != ACE_PREC_BF16L1)
+        return fail(ctx, "ace_set_tile_shape: precision mode not available");
+    if (chunk_w != 0 && chunk_w != 1 && chunk_w != 3 && chunk_w != 8)
+        return fail(ctx, "ace_set_tile_shape: chunk weight must be 1, 3, 8 or 0 (tuned or default)");
+    if (net_waves != 0 && net_waves != 1 && net_waves != 2 && net_waves != 4)
+        return fail(ctx, "ace_set_tile_shape: waves per network tile must be 1, 2, 4 or 0 (tuned or default)");
+    ctx->pin_chunk[precision] = chunk_w;
+    ctx->pin_mlpw[precision] = net_waves;
     return 0;
 }
 

@@ -33,13 +33,13 @@ __host__ __device__ inline int mlp_lds_floats(const DevSpecies& sp) {
     return n;
 }
 
-// C[16 x n] = A[16 x k] * B for NT 16-wide output tiles starting at o0, o0 + 16, ...; A at LDS offset a
-// (row stride lda), B[kk][o] at LDS offset b (BLDS) or global Bg, element o * so + kk * sk.  Two
-// accumulators per tile (even and odd k-steps) give 2 NT independent matrix-core chains; the A operand
-// of a k-step is shared by the NT tiles.  acc[t] receives the lane's 4 values of tile t.
+// C[16 x n] = A[16 x k] * B for NT 16-wide output tiles starting at o0, o0 + ts, o0 + 2 ts, ... (ts = 16:
+// adjacent tiles); A at LDS offset a (row stride lda), B[kk][o] at LDS offset b (BLDS) or global Bg, element
+// o * so + kk * sk.  Two accumulators per tile (even and odd k-steps) give 2 NT independent matrix-core
+// chains; the A operand of a k-step is shared by the NT tiles.  acc[t] receives the lane's 4 values of tile t.
 template <bool BLDS, int NT>
 __device__ inline void mfma_group(int a, int lda, int k, int b, const float* __restrict__ Bg, int so, int sk, int n,
-                                  int o0, int lane, mfma_f4* acc) {
+                                  int o0, int lane, mfma_f4* acc, int ts = 16) {
     mfma_f4 c0[NT], c1[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -57,7 +57,7 @@ __device__ inline void mfma_group(int a, int lda, int k, int b, const float* __r
             av[u] = mlp_smem[a + r * lda + kk];
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-                const int o = o0 + 16 * t + r;
+                const int o = o0 + ts * t + r;
                 if (BLDS) bv[u][t] = mlp_smem[b + o * so + kk * sk];
                 else bv[u][t] = Bg[o * so + kk * sk];
             }
@@ -98,7 +98,7 @@ __device__ inline void bf_split8(const float* v, bf16x8& hi, bf16x8& lo) {
 // so each lane's 8 values are one 16-byte load.
 template <int NT>
 __device__ inline void bf3_group(int a, int lda, int k, const __bf16* __restrict__ Bh, int lo, int sb, int o0,
-                                 int lane, mfma_f4* acc) {
+                                 int lane, mfma_f4* acc, int ts) {
     mfma_f4 c0[NT], c1[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -118,7 +118,7 @@ __device__ inline void bf3_group(int a, int lda, int k, const __bf16* __restrict
         bf_split8(av, ah, al);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            const int e = (o0 + 16 * t + r) * sb + kb + 8 * q;
+            const int e = (o0 + ts * t + r) * sb + kb + 8 * q;
             const bf16x8 bh = *(const bf16x8*)(Bh + e);
             const bf16x8 bl = *(const bf16x8*)(Bh + lo + e);
             c0[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c0[t], 0, 0, 0);
@@ -130,51 +130,55 @@ __device__ inline void bf3_group(int a, int lda, int k, const __bf16* __restrict
     for (int t = 0; t < NT; ++t) acc[t] = c0[t] + c1[t];
 }
 
+// The 16-wide output tiles of a layer dealt to the W waves of a tile's workgroup: wave wv takes tiles wv,
+// wv + W, wv + 2W, ... in groups of 4, 2 or 1; epi(acc, tile count, first column, column stride).
 template <class EPI>
-__device__ inline void bf3_layer(int a, int lda, int k, const __bf16* __restrict__ Bh, int lo, int n, int lane, EPI epi) {
+__device__ inline void bf3_layer(int a, int lda, int k, const __bf16* __restrict__ Bh, int lo, int n, int lane, EPI epi,
+                                 int wv = 0, int W = 1) {
     const int sb = (k + 31) & ~31;
-    for (int o0 = 0; o0 < n;) {
-        const int rem = (n - o0 + 15) >> 4;
+    const int ts = 16 * W;
+    for (int o0 = 16 * wv; o0 < n;) {
+        const int rem = (n - o0 + ts - 1) / ts;   // this wave's tiles from o0 on
         if (rem >= 4) {
             mfma_f4 acc[4];
-            bf3_group<4>(a, lda, k, Bh, lo, sb, o0, lane, acc);
-            epi(acc, 4, o0);
-            o0 += 64;
+            bf3_group<4>(a, lda, k, Bh, lo, sb, o0, lane, acc, ts);
+            epi(acc, 4, o0, ts);
+            o0 += 4 * ts;
         } else if (rem >= 2) {
             mfma_f4 acc[2];
-            bf3_group<2>(a, lda, k, Bh, lo, sb, o0, lane, acc);
-            epi(acc, 2, o0);
-            o0 += 32;
+            bf3_group<2>(a, lda, k, Bh, lo, sb, o0, lane, acc, ts);
+            epi(acc, 2, o0, ts);
+            o0 += 2 * ts;
         } else {
             mfma_f4 acc[1];
-            bf3_group<1>(a, lda, k, Bh, lo, sb, o0, lane, acc);
-            epi(acc, 1, o0);
-            o0 += 16;
+            bf3_group<1>(a, lda, k, Bh, lo, sb, o0, lane, acc, ts);
+            epi(acc, 1, o0, ts);
+            o0 += ts;
         }
     }
 }
 
-// every output tile of a layer, in groups of 4, 2 or 1 tiles; epi(acc, tile count, first column)
 template <bool BLDS, class EPI>
 __device__ inline void mfma_layer(int a, int lda, int k, int b, const float* __restrict__ Bg, int so, int sk, int n,
-                                  int lane, EPI epi) {
-    for (int o0 = 0; o0 < n;) {
-        const int rem = (n - o0 + 15) >> 4;
+                                  int lane, EPI epi, int wv = 0, int W = 1) {
+    const int ts = 16 * W;
+    for (int o0 = 16 * wv; o0 < n;) {
+        const int rem = (n - o0 + ts - 1) / ts;
         if (rem >= 4) {
             mfma_f4 acc[4];
-            mfma_group<BLDS, 4>(a, lda, k, b, Bg, so, sk, n, o0, lane, acc);
-            epi(acc, 4, o0);
-            o0 += 64;
+            mfma_group<BLDS, 4>(a, lda, k, b, Bg, so, sk, n, o0, lane, acc, ts);
+            epi(acc, 4, o0, ts);
+            o0 += 4 * ts;
         } else if (rem >= 2) {
             mfma_f4 acc[2];
-            mfma_group<BLDS, 2>(a, lda, k, b, Bg, so, sk, n, o0, lane, acc);
-            epi(acc, 2, o0);
-            o0 += 32;
+            mfma_group<BLDS, 2>(a, lda, k, b, Bg, so, sk, n, o0, lane, acc, ts);
+            epi(acc, 2, o0, ts);
+            o0 += 2 * ts;
         } else {
             mfma_f4 acc[1];
-            mfma_group<BLDS, 1>(a, lda, k, b, Bg, so, sk, n, o0, lane, acc);
-            epi(acc, 1, o0);
-            o0 += 16;
+            mfma_group<BLDS, 1>(a, lda, k, b, Bg, so, sk, n, o0, lane, acc, ts);
+            epi(acc, 1, o0, ts);
+            o0 += ts;
         }
     }
 }
@@ -184,17 +188,27 @@ __device__ inline void mfma_layer(int a, int lda, int k, int b, const float* __r
 // loads (one memory round trip instead of one per k-step); otherwise it is read from global memory.
 // BF: the first layer's forward product and the dE/dG product run on BF16 matrix cores (three-product
 // split above) with the weights from w0bf (DevSpecies::bf_off); the other layers stay FP32.
-template <bool STAGE, bool BF>
-__global__ __launch_bounds__(64) void k_mlp_energy_grad(
+// W (1, 2 or 4) waves share a tile: they stage its rows together and split every layer's 16-wide output
+// tiles (wave w takes tiles w, w + W, ...), with a workgroup barrier between layers; more waves per tile
+// keep more matrix-core chains and memory loads in flight per SIMD for the same LDS.
+template <bool STAGE, bool BF, int W>
+__global__ __launch_bounds__(64 * W) void k_mlp_energy_grad(
     const DevModel* __restrict__ M, const float* __restrict__ wpad, const __bf16* __restrict__ w0bf,
     const float* __restrict__ inpw, const int4* __restrict__ tiles, int ldg, const float* __restrict__ G,
     float* __restrict__ dEdG, const int* __restrict__ atom_of_row, double* __restrict__ e_atom) {
+    static_assert(W == 1 || W == 2 || W == 4, "one, two or four waves per tile");
+    constexpr int BT = 64 * W;
     __shared__ int s_off[2 * ACE_MAXL + 2];   // A_l, D_l offsets; weights offset
     __shared__ DevSpecies s_sp;                 // the species' tables, read from LDS in the layer loops
     const int4 tile = tiles[blockIdx.x];
     const int I = tile.x, r0 = tile.y, nr = tile.z;
-    const int lane = threadIdx.x;
-    {
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = W > 1 ? __builtin_amdgcn_readfirstlane(tid >> 6) : 0;
+    auto sync = [] {
+        if constexpr (W > 1) __syncthreads();
+        else wave_sync();
+    };
+    if (wv == 0) {
         constexpr int n16 = (int)(sizeof(DevSpecies) / 16);
         static_assert(n16 >= 64 && n16 <= 128, "DevSpecies copy covers 64 to 128 words");
         const int4* src = (const int4*)&M->sp[I];
@@ -204,11 +218,11 @@ __global__ __launch_bounds__(64) void k_mlp_energy_grad(
         dst[lane] = v0;
         if (64 + lane < n16) dst[64 + lane] = v1;
     }
-    wave_sync();
+    sync();
     const DevSpecies& sp = s_sp;
     const int K = sp.K, L = sp.L;
     const int ldx = mlp_ld(K);
-    if (lane == 0) {
+    if (tid == 0) {
         int q = MLP_ROWS * ldx;
         for (int l = 0; l < L; ++l) {
             const int w = MLP_ROWS * mlp_ld(sp.width[l + 1]);
@@ -218,23 +232,23 @@ __global__ __launch_bounds__(64) void k_mlp_energy_grad(
         }
         s_off[2 * ACE_MAXL] = q;   // 16-byte aligned: every ld is odd and the row count 16
     }
-    wave_sync();
+    sync();
     const int wofs = s_off[2 * ACE_MAXL];
     const float* Wg = wpad + sp.wp_off;
     if (STAGE) {
         const float4* src = (const float4*)Wg;
         float4* dst = (float4*)(mlp_smem + wofs);
         const int n4 = sp.wp_n >> 2;
-        for (int b0 = 0; b0 < n4; b0 += 64 * 8) {
+        for (int b0 = 0; b0 < n4; b0 += BT * 8) {
             float4 v[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int e = b0 + 64 * u + lane;
+                const int e = b0 + BT * u + tid;
                 v[u] = (e < n4) ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int e = b0 + 64 * u + lane;
+                const int e = b0 + BT * u + tid;
                 if (e < n4) dst[e] = v[u];
             }
         }
@@ -244,18 +258,18 @@ __global__ __launch_bounds__(64) void k_mlp_energy_grad(
     {
         const int k4 = (K + 3) >> 2;
         const int n = MLP_ROWS * k4;
-        for (int b0 = 0; b0 < n; b0 += 64 * 8) {
+        for (int b0 = 0; b0 < n; b0 += BT * 8) {
             float4 v[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int e = b0 + 64 * u + lane;
+                const int e = b0 + BT * u + tid;
                 const int r = e / k4, c = e - r * k4;
                 v[u] = (e < n && r < nr) ? *(const float4*)(G + (long long)(r0 + r) * ldg + 4 * c)
                                          : make_float4(0.f, 0.f, 0.f, 0.f);
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                const int e = b0 + 64 * u + lane;
+                const int e = b0 + BT * u + tid;
                 if (e < n) {
                     const int r = e / k4, c = e - r * k4;
                     const int x = r * ldx + 4 * c;
@@ -264,10 +278,10 @@ __global__ __launch_bounds__(64) void k_mlp_energy_grad(
             }
         }
         const int pad = ldx - 4 * k4;   // columns [4 k4, ldx) of every row
-        for (int e = lane; e < MLP_ROWS * pad; e += 64) mlp_smem[(e / pad) * ldx + 4 * k4 + e % pad] = 0.f;
+        for (int e = tid; e < MLP_ROWS * pad; e += BT) mlp_smem[(e / pad) * ldx + 4 * k4 + e % pad] = 0.f;
     }
-    for (int idx = MLP_ROWS * ldx + lane; idx < wofs; idx += 64) mlp_smem[idx] = 0.f;
-    wave_sync();
+    for (int idx = MLP_ROWS * ldx + tid; idx < wofs; idx += BT) mlp_smem[idx] = 0.f;
+    sync();
     const int cr = lane & 15, rq = 4 * (lane >> 4);
     // forward: z_l = b_l + W_l a_{l-1}; W_l[o][k] at wl_off + k mlp_ld(no) + o
     int ain = 0, lda = ldx;
@@ -278,9 +292,9 @@ __global__ __launch_bounds__(64) void k_mlp_energy_grad(
         const float* bias = STAGE ? nullptr : Wg + sp.bl_off[l];
         const int boff = wofs + sp.bl_off[l];
         const float cc = sp.c;
-        auto epi = [&](const mfma_f4* acc, int nt, int o0) {
+        auto epi = [&](const mfma_f4* acc, int nt, int o0, int ts) {
             for (int t = 0; t < nt; ++t) {
-                const int o = o0 + 16 * t + cr;
+                const int o = o0 + ts * t + cr;
                 if (o < no) {
                     const float bo = STAGE ? mlp_smem[boff + o] : bias[o];
 #pragma unroll
@@ -294,29 +308,29 @@ __global__ __launch_bounds__(64) void k_mlp_energy_grad(
             }
         };
         if (BF && l == 0)   // forward block: B[k][o] = W_0[o][k] at [o][k], round16(w_1) x round32(K) per part
-            bf3_layer(ain, lda, ni, w0bf + sp.bf_off, ((no + 15) & ~15) * ((ni + 31) & ~31), no, lane, epi);
+            bf3_layer(ain, lda, ni, w0bf + sp.bf_off, ((no + 15) & ~15) * ((ni + 31) & ~31), no, lane, epi, wv, W);
         else
-            mfma_layer<STAGE>(ain, lda, ni, wofs + sp.wl_off[l], Wg + sp.wl_off[l], 1, mlp_ld(no), no, lane, epi);
+            mfma_layer<STAGE>(ain, lda, ni, wofs + sp.wl_off[l], Wg + sp.wl_off[l], 1, mlp_ld(no), no, lane, epi, wv, W);
         ain = ao;
         lda = ldo;
-        wave_sync();
+        sync();
     }
     // energies: out = outb + outw a_last (the last layer has width 1)
     const int aL = s_off[2 * (L - 1)], dL = s_off[2 * (L - 1) + 1], ldL = mlp_ld(1);
-    if (lane < nr) e_atom[atom_of_row[r0 + lane]] = (double)(sp.outb + sp.outw * mlp_smem[aL + lane * ldL]) + sp.E0;
-    wave_sync();
+    if (wv == 0 && lane < nr) e_atom[atom_of_row[r0 + lane]] = (double)(sp.outb + sp.outw * mlp_smem[aL + lane * ldL]) + sp.E0;
+    sync();
     // backward: delta_l overwrites A_l; delta_{l-1} = (delta_l W_l) * f'(z_{l-1})
-    if (lane < MLP_ROWS) mlp_smem[aL + lane * ldL] = sp.outw * mlp_smem[dL + lane * ldL];
-    wave_sync();
+    if (wv == 0 && lane < MLP_ROWS) mlp_smem[aL + lane * ldL] = sp.outw * mlp_smem[dL + lane * ldL];
+    sync();
     for (int l = L - 1; l > 0; --l) {
         const int no = sp.width[l + 1], ni = sp.width[l];
         const int al = s_off[2 * l], am = s_off[2 * (l - 1)], dm = s_off[2 * (l - 1) + 1];
         const int ldl = mlp_ld(no), ldm = mlp_ld(ni);
         // B[o][k] = W_l[o][k]: output column k, reduction index o
         mfma_layer<STAGE>(al, ldl, no, wofs + sp.wl_off[l], Wg + sp.wl_off[l], mlp_ld(no), 1, ni, lane,
-                          [&](const mfma_f4* acc, int nt, int k0) {
+                          [&](const mfma_f4* acc, int nt, int k0, int ts) {
                               for (int t = 0; t < nt; ++t) {
-                                  const int k = k0 + 16 * t + cr;
+                                  const int k = k0 + ts * t + cr;
                                   if (k < ni) {
 #pragma unroll
                                       for (int r = 0; r < 4; ++r) {
@@ -325,17 +339,17 @@ __global__ __launch_bounds__(64) void k_mlp_energy_grad(
                                       }
                                   }
                               }
-                          });
-        wave_sync();
+                          }, wv, W);
+        sync();
     }
     // dE/dG = (delta_0 W_0) diag(inpw)
     {
         const int no = sp.width[1];
         const float* iw = inpw + sp.in_off;
         const int a0 = s_off[0], ld0 = mlp_ld(no);
-        auto epi = [&](const mfma_f4* acc, int nt, int k0) {
+        auto epi = [&](const mfma_f4* acc, int nt, int k0, int ts) {
             for (int t = 0; t < nt; ++t) {
-                const int k = k0 + 16 * t + cr;
+                const int k = k0 + ts * t + cr;
                 if (k < K) {
                     const float w = iw[k];
 #pragma unroll
@@ -346,9 +360,9 @@ __global__ __launch_bounds__(64) void k_mlp_energy_grad(
         };
         if (BF) {   // backward block after the forward one: B[o][k] = W_0[o][k] at [k][o], round16(K) x round32(w_1)
             const int fwd = ((no + 15) & ~15) * ((K + 31) & ~31);
-            bf3_layer(a0, ld0, no, w0bf + sp.bf_off + 2 * fwd, ((K + 15) & ~15) * ((no + 31) & ~31), K, lane, epi);
+            bf3_layer(a0, ld0, no, w0bf + sp.bf_off + 2 * fwd, ((K + 15) & ~15) * ((no + 31) & ~31), K, lane, epi, wv, W);
         } else {
-            mfma_layer<STAGE>(a0, ld0, no, wofs + sp.wl_off[0], Wg + sp.wl_off[0], mlp_ld(no), 1, K, lane, epi);
+            mfma_layer<STAGE>(a0, ld0, no, wofs + sp.wl_off[0], Wg + sp.wl_off[0], mlp_ld(no), 1, K, lane, epi, wv, W);
         }
     }
 }

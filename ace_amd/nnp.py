@@ -193,13 +193,19 @@ class NNPDevice:
 
     def tile_shape(self, precision=PREC_FP32):
         """The shapes the energy/force pass uses: path (None: size rule), waves per atom, network weights staged
-        (None: not tuned), the angular chunk weight w and the milliseconds measured for w = 1, 3, 8."""
-        v = np.zeros(9)
+        (None: not tuned), the angular chunk weight w and the milliseconds measured for w = 1, 3, 8, the level
+        split, and the waves per network tile with the milliseconds measured for 1, 2, 4."""
+        v = np.zeros(13)
         self._check(self.lib.ace_tile_shape(self.ctx, precision, _dp(v)), "ace_tile_shape")
         return {"path": None if v[0] < 0 else self.PATHS[int(v[0])], "waves": int(v[1]),
                 "staged": None if v[2] < 0 else bool(v[2]), "chunk_w": int(v[3]),
                 "chunk_ms": {w: float(v[5 + k]) for k, w in enumerate((1, 3, 8)) if v[5 + k] >= 0},
-                "level_split": int(v[8])}
+                "level_split": int(v[8]), "net_waves": int(v[9]),
+                "net_waves_ms": {w: float(v[10 + k]) for k, w in enumerate((1, 2, 4)) if v[10 + k] >= 0}}
+
+    def set_tile_shape(self, chunk_w=0, net_waves=0, precision=PREC_FP32):
+        """Fixes the angular chunk weight (1, 3, 8) and the waves per network tile (1, 2, 4); 0: tuned/default."""
+        self._check(self.lib.ace_set_tile_shape(self.ctx, precision, chunk_w, net_waves), "ace_set_tile_shape")
 
     def set_level_split(self, enable):
         """Level-split angular kernels (two eta groups over disjoint zeta levels) on or off."""

@@ -33,6 +33,7 @@
  *   ace_autotune ................... (no counterpart) measured choice of the energy/force path and of
  *                                    its tile shapes
  *   ace_tile_shape ................. (no counterpart) the shapes in use
+ *   ace_set_tile_shape ............. (no counterpart) fixed tile shapes (chunk weight, network waves)
  *   ace_set_level_split ............ (no counterpart) level-split angular kernels on or off
  *   ace_set_path ................... (no counterpart) fixed choice of the energy/force path
  *   ace_plain_energy_forces ........ same as ace_energy_forces, unbatched FP64 one-thread-per-atom
@@ -221,7 +222,8 @@ enum { ACE_PATH_FUSED = 0, ACE_PATH_SPLIT_STAGED = 1, ACE_PATH_SPLIT = 2, ACE_PA
  * keeps the fastest for this precision, then, on that path, the tile shape of the angular triplet loops:
  * the chunk of partner slots a lane takes per fixed slot, chosen per block to minimise
  * rounds x (chunk + w/8) with w in {1, 3, 8} (w: the cost of a chunk's set-up and flush in eighths of one
- * pair; a larger w favours long chunks, a smaller one fuller rounds).  ms: NULL or ACE_NPATHS + 1 entries,
+ * pair; a larger w favours long chunks, a smaller one fuller rounds), and on a split path the waves that
+ * share a 16-atom network tile (1, 2 or 4).  ms: NULL or ACE_NPATHS + 1 entries,
  * the milliseconds of each path (-1 where not feasible) and the index chosen.  The choice holds while the
  * batch size and neighbour-row capacity stay as measured; otherwise size rules decide. */
 int  ace_autotune(ace_ctx* ctx, int precision, double* ms);
@@ -230,8 +232,15 @@ int  ace_autotune(ace_ctx* ctx, int precision, double* ms);
  * shape[3] the angular chunk weight w, shape[4] the milliseconds of the chosen chunk weight's timing (or -1),
  * shape[5..7] the milliseconds measured for w = 1, 3, 8 (-1 where not measured), shape[8] the level split s
  * of the canonical angular sums (0: none; with two eta groups whose power levels do not overlap, group 0
- * below level s and group 1 from s on, the FP32 kernels form only those (level, group) products). */
+ * below level s and group 1 from s on, the FP32 kernels form only those (level, group) products),
+ * shape[9] the waves per 16-atom network tile of the split path (1, 2 or 4; 0 on a fused path),
+ * shape[10..12] the milliseconds measured with 1, 2, 4 waves per tile (-1 where not measured).  shape holds
+ * 13 entries. */
 int  ace_tile_shape(const ace_ctx* ctx, int precision, double* shape);
+/* (no counterpart) Fixes tile shapes of a precision over the autotuner's choice and the defaults: the
+ * chunk weight w of the angular triplet loops (1, 3 or 8) and the waves per 16-atom network tile (1, 2 or
+ * 4); 0 returns either to the tuned or default value. */
+int  ace_set_tile_shape(ace_ctx* ctx, int precision, int chunk_w, int net_waves);
 /* (no counterpart) Enables (default) or disables the level-split kernels; both give the oracle's results
  * within the FP32 bounds, in a different rounding order. */
 int  ace_set_level_split(ace_ctx* ctx, int enable);

@@ -395,10 +395,14 @@ def test_compute_entry_matches_build_then_energy():
 BF16_TOL = dict(E=1e-4, F=1e-3)
 
 
-def check_bf16(model, strucs, name):
+def check_bf16(model, strucs, name, net_waves=0):
     """BF16L1 energies and forces vs the FP64 oracle, next to the FP32 mode on the same batch."""
     orc = O.Oracle(model)
     dev = NNPDevice(model)
+    if net_waves:
+        for prec in (PREC_BF16L1, PREC_FP32):
+            dev.set_path("split", prec)
+            dev.set_tile_shape(0, net_waves, prec)
     dev.upload(strucs)
     dev.build_neighbors()
     Eb, Fb, Wb = dev.compute(PREC_BF16L1)
@@ -430,6 +434,40 @@ def test_bf16_first_layer():
     m2, strucs2 = configs.config2()
     scaled_model(m2, strucs2)
     check_bf16(m2, strucs2, "bf16-c2")
+
+
+@pytest.mark.parametrize("net_waves,chunk_w", [(1, 1), (2, 8), (4, 3)])
+def test_network_and_chunk_tile_shapes(net_waves, chunk_w):
+    """Each count of waves sharing a 16-atom network tile (1, 2, 4) and each angular chunk weight (1, 3, 8),
+    pinned, on the split paths (weights staged in LDS or read from memory, two or four waves per atom), with
+    layer widths that are not multiples of 16 next to a 128-wide layer; then the BF16 first layer."""
+    rng = np.random.default_rng(53)
+    m = configs.c3_model(rng)
+    for I in range(3):
+        K = m.input_width(I)
+        w = [K, 128, 40, 17, 1]
+        m.nets[I] = configs.Network(w, 3, 1.0, configs.xavier_params(w, rng), np.ones(K), np.zeros(K))
+    strucs = []
+    for n in (21, 52, 37):
+        cell, pos = configs.random_packed(n, 0.08, 1.6, rng)
+        strucs.append(structure(cell, pos, rng.integers(0, 3, n)))
+    scaled_model(m, strucs)
+
+    def setup(dev):
+        for prec in PRECS:
+            dev.set_tile_shape(chunk_w, net_waves, prec)
+            t = dev.tile_shape(prec)
+            assert t["chunk_w"] == chunk_w, t
+
+    check_against_oracle(m, strucs, f"tile w={chunk_w} net_waves={net_waves}", check_plain=False,
+                         paths=("split", "split_staged", "split_2w", "split_staged_2w"), setup=setup)
+    m5 = configs.c5_model(rng)
+    strucs5 = []
+    for n in (40, 95):
+        cell, pos = configs.random_packed(n, 0.085, 1.5, rng)
+        strucs5.append(structure(cell, pos, rng.integers(0, 4, n)))
+    scaled_model(m5, strucs5)
+    check_bf16(m5, strucs5, f"bf16 net_waves={net_waves}", net_waves=net_waves)
 
 
 def test_c5_full_size(golden_dir):

@@ -59,15 +59,21 @@ __host__ __device__ inline int basis_t_off(int nbm, int ng, int kmax, int nrp, i
     return (gb + 8 * nd + 15) & ~15;
 }
 
+// The split adjoint of an all-canonical model (mode 1 without the generic path) keeps no per-slot radial
+// arrays: its per-slot pass adds each neighbour's radial dE/dd to wave 0's accumulators directly, and |d| in
+// FP64 passes from the bucketing to that pass through wave 1's accumulators, which are cleared afterwards.
+__host__ __device__ inline bool basis_lean(int mode, bool gen) { return mode == 1 && !gen; }
+
 template <class T, int BW>
 __host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int nrp, int mode, int netf, int nt, bool gen) {
     const int kp = (kmax + 3) & ~3;
     const bool gacc = (mode == 1 || mode == 3);
-    int n = 4 * nbm + 2 * ng * nbm + 3 * nbm;
+    const bool lean = basis_lean(mode, gen);
+    int n = 4 * nbm + 2 * ng * nbm + (lean ? 0 : 3 * nbm);
     if (mode == 2) n += 4 * nbm;
-    if (gacc) n += nbm;
+    if (gacc && !lean) n += nbm;
     if (gen) n += 2 * nbm;
-    const int nf = sizeof(T) == 4 ? nbm : 0;
+    const int nf = (sizeof(T) == 4 && !lean) ? nbm : 0;
     const int base = (basis_t_off<BW>(nbm, ng, kmax, nrp, mode, nt) + (int)sizeof(T) * n + 4 * (nbm + kp) + 4 * nf +
                       15) & ~15;
     return base + (mode == 3 ? 4 * netf : 0);
@@ -81,7 +87,7 @@ struct BasisLds {
     T* fR;
     T* gR;
     T* r;      // |d|
-    T* fr;     // modes 1, 3: radial dE/dr of each slot
+    T* fr;     // modes 1 (not lean), 3: radial dE/dr of each slot
     T* fA;     // kernels with the generic angular path: angular cutoff of each slot and its r derivative
     T* gA;
     double* g; // [BW][3 nbm] per-wave dE/dd accumulators (one array per wave: a fixed order of adds)
@@ -110,16 +116,17 @@ __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, i
     L.u = c; c += 4 * nbm;
     L.E = c; c += 2 * ng * nbm;
     L.te = c; if (mode == 2) c += 4 * nbm;
-    L.fR = c; c += nbm;
-    L.gR = c; c += nbm;
-    L.r = c; c += nbm;
-    L.fr = c; if (gacc) c += nbm;
+    const bool lean = basis_lean(mode, gen);
+    L.fR = c; if (!lean) c += nbm;
+    L.gR = c; if (!lean) c += nbm;
+    L.r = c; if (!lean) c += nbm;
+    L.fr = c; if (gacc && !lean) c += nbm;
     L.fA = c; if (gen) c += nbm;
     L.gA = c; if (gen) c += nbm;
     L.p = (int*)c;
     L.ci = L.p + nbm;
     L.rlo = (float*)(L.ci + kp);
-    float* e = L.rlo + (sizeof(T) == 4 ? nbm : 0);
+    float* e = L.rlo + ((sizeof(T) == 4 && !lean) ? nbm : 0);
     const long long end = ((long long)((char*)e - (char*)base) + 15) & ~15ll;
     L.nw = (float*)((char*)base + end);
     return L;
@@ -479,7 +486,7 @@ __device__ inline int deal(int* load, int cost) {
 // Radial and angular sums of one atom (slots, tables and row already in LDS).  PM 0: G into row;
 // PM 1: per-slot dE/dd into the per-wave accumulators g from the dE/dG weights in row; PM 2: tangent
 // rows into row.  Ends with a workgroup barrier.
-template <class T, int NG, int NZ, int PM, int GEN, int BW, int SPL>
+template <class T, int NG, int NZ, int PM, int GEN, int BW, int SPL, bool LEAN = false>
 __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, const DevBasisParams& P,
                                   const int* s_cnt, int nt, int nbm, int tid, int wv, int lane) {
     constexpr int MODE = PM;
@@ -487,12 +494,19 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
     constexpr int ADJ = (PM == 1);
     constexpr int TAN = (PM == 2);
     constexpr int MAXA = 16;   // functions per register chunk of the generic path
+    static_assert(!LEAN || (ADJ && !GEN), "the lean layout is the split adjoint's without the generic path");
     const int ng = sp.ng;
-    // --- radial blocks: one wave per neighbour species J.  Sums over neighbours (G, tangent rows): lane =
+    if (LEAN && wv > 0) {
+        // waves 1.. clear their own accumulators (wave 1's held |d| until the per-slot pass); the barriers
+        // below order this before any wave adds into them
+        double* gw0 = L.g + wv * 3 * nbm;
+        for (int k = lane; k < 3 * s_cnt[nt]; k += 64) gw0[k] = 0.0;
+    }
+    // --- radial blocks (the lean adjoint added them in the per-slot pass): one wave per neighbour species J.  Sums over neighbours (G, tangent rows): lane =
     // function, lane group = neighbour subset, the lanes of a function sum their groups in FP64.  Sums over
     // functions (the adjoint's dE/dr per neighbour): lane = (neighbour, function group), the neighbour's
     // |d| and cutoff values in registers, the groups summed across lanes once per neighbour ------------
-    {
+    if (!LEAN) {
         int load[BW];
 #pragma unroll
         for (int w = 0; w < BW; ++w) load[w] = 0;
@@ -925,6 +939,7 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
     constexpr int TAN = (MODE == 2);
     constexpr int FUSED = (MODE == 3);        // symmetry functions, network and adjoint in one pass
     constexpr int GACC = (ADJ || FUSED);      // per-slot gradient accumulators
+    constexpr bool LEAN = (MODE == 1 && !GEN);   // basis_lean: no per-slot radial arrays
     constexpr bool F64 = sizeof(T) == 8;
     constexpr int BT = BW * 64;
     // the waves that stage the radial and the angular tables (wave 1 copies the species record)
@@ -932,8 +947,8 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
     static_assert(BW >= 2 && BW <= 4, "two to four waves per atom");
     extern __shared__ double4 basis_smem[];
     __shared__ int s_cnt[ACE_MAXT + 1];
-    __shared__ DevSpecies s_sp;
     __shared__ double s_red[BW][16];
+    __shared__ DevSpecies s_sp;   // the fused kernel's copy of the species record (LDS does not limit it)
     const int tid = threadIdx.x, wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int i = blockIdx.x;
     if (i >= natoms) return;
@@ -954,17 +969,19 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
     const float rcut = gsp.rc_max;
     const int row = io.row_of[i];
 
-    // --- waves 1.. stage the species tables in LDS while wave 0 buckets the neighbours --------------
-    if (wv == 1) {
+    // --- waves 1.. stage the species tables in LDS while wave 0 buckets the neighbours; the split kernels
+    // read the species record from memory with uniform (scalar) loads, which saves its LDS for more atoms
+    // in flight, the fused one (a round of workgroups or a few, latency-bound) from an LDS copy ----------
+    if (FUSED && wv == 1) {
         constexpr int n16 = (int)(sizeof(DevSpecies) / 16);
         static_assert(sizeof(DevSpecies) % 16 == 0, "DevSpecies is copied in 16-byte words");
+        static_assert(n16 >= 64 && n16 <= 128, "DevSpecies copy covers 64 to 128 words");
         const int4* src = (const int4*)&gsp;
         int4* dst = (int4*)&s_sp;
         const int4 v0 = src[lane];
         const int4 v1 = (64 + lane < n16) ? src[64 + lane] : v0;
         dst[lane] = v0;
         if (64 + lane < n16) dst[64 + lane] = v1;
-        static_assert(n16 >= 64 && n16 <= 128, "DevSpecies copy covers 64 to 128 words");
     }
     if (wv == W_RAD) {
         const int nR = gsp.nR, r0 = gsp.r_poff[0];
@@ -1039,8 +1056,12 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
                         L.u[4 * q + 1] = (T)(dy * ir);
                         L.u[4 * q + 2] = (T)(dz * ir);
                         L.u[4 * q + 3] = (T)ir;
-                        L.r[q] = (T)r;
-                        if (!F64) L.rlo[q] = (float)(r - (double)(float)r);
+                        if (LEAN) {
+                            L.g[3 * nbm + q] = r;   // |d| in FP64 for the per-slot pass (wave 1's accumulators)
+                        } else {
+                            L.r[q] = (T)r;
+                            if (!F64) L.rlo[q] = (float)(r - (double)(float)r);
+                        }
                     }
                     fill[J] += __popcll(m);
                 }
@@ -1089,7 +1110,7 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
         if (!fits && lane == 0) s_cnt[nt] = nbm + 1;
     }
     __syncthreads();
-    const DevSpecies& sp = s_sp;
+    const DevSpecies& sp = *(FUSED ? &s_sp : &gsp);
     const int ng = sp.ng;
     const int ntot = s_cnt[nt];
     if (ntot > nbm) {
@@ -1098,6 +1119,51 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
     }
 
     // --- per-slot quantities ---------------------------------------------------------------------
+    if (LEAN) {
+        // cutoffs and angular factors, and the slot's radial dE/d|d| over the functions of its species block
+        // (weights dE/dG in row, basis_radial.cpp:316-376) as wave 0's first gradient term
+        for (int q = tid; q < ntot; q += BT) {
+            int J = 0;
+#pragma unroll
+            for (int jj = 1; jj < ACE_MAXT; ++jj)
+                if (jj < nt && q >= s_cnt[jj]) J = jj;
+            const double r = L.g[3 * nbm + q];
+            T fc, gc;
+            t_cutoff_r64<T>(sp.r_cut[J], sp.r_rc64[J], r, fc, gc);
+            T acc = T(0);
+            const int nR = sp.r_n[J], goff = sp.r_goff[J], rq0 = sp.r_poff[J] - sp.r_poff[0], form = sp.r_form[J];
+            if (form == ACE_RAD_GAUSSIAN)
+                for (int n = 0; n < nR; ++n) {
+                    T v, dv;
+                    t_radial_r64<T>(ACE_RAD_GAUSSIAN, L.rp[rq0 + n], L.rp[sp.nR + rq0 + n], r, fc, gc, v, dv);
+                    acc += (T)L.row[goff + n] * dv;
+                }
+            else
+                for (int n = 0; n < nR; ++n) {
+                    T v, dv;
+                    t_radial_r64<T>(form, L.rp[rq0 + n], L.rp[sp.nR + rq0 + n], r, fc, gc, v, dv);
+                    acc += (T)L.row[goff + n] * dv;
+                }
+            if (sp.a_cut != sp.r_cut[J] || sp.a_rc64 != sp.r_rc64[J]) t_cutoff_r64<T>(sp.a_cut, sp.a_rc64, r, fc, gc);
+            const T fa = fc, ga = gc, rt = (T)r;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                T ev = T(0), edv = T(0);
+                if (g < ng) {
+                    const T ie = F64 ? (T)sp.ietar2_64[g] : (T)sp.ietar2[g];
+                    const T e = t_exp<T>(-rt * rt * ie);
+                    ev = fa * e;
+                    edv = (ga - T(2) * rt * ie * fa) * e;
+                }
+                L.E[q * 2 * NG + g] = ev;
+                L.E[q * 2 * NG + NG + g] = edv;
+            }
+            const double a = (double)acc;
+            L.g[3 * q] = a * (double)L.u[4 * q];
+            L.g[3 * q + 1] = a * (double)L.u[4 * q + 1];
+            L.g[3 * q + 2] = a * (double)L.u[4 * q + 2];
+        }
+    } else
     for (int q = tid; q < ntot; q += BT) {
         int J = 0;
 #pragma unroll
@@ -1156,7 +1222,7 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
         block_network<T>(L, sp, tid, &io.e_atom[i]);
         basis_sums<T, NG, NZ, 1, GEN, BW, SPL>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
     } else {
-        basis_sums<T, NG, NZ, MODE, GEN, BW, SPL>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
+        basis_sums<T, NG, NZ, MODE, GEN, BW, SPL, LEAN>(L, sp, P, s_cnt, nt, nbm, tid, wv, lane);
     }
     if (MODE == 0) {
         // network input row: the ANN's input scaling inpw*(G + inpb) (nn.cpp:1406) applied here, in FP64,
@@ -1174,7 +1240,7 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
 #pragma unroll
         for (int q9 = 0; q9 < 9; ++q9) w[q9] = 0.0;
         for (int q = tid; q < ntot; q += BT) {
-            const double fr = (double)L.fr[q];
+            const double fr = LEAN ? 0.0 : (double)L.fr[q];   // the lean adjoint's radial part is in g
             const double ux = (double)L.u[4 * q], uy = (double)L.u[4 * q + 1], uz = (double)L.u[4 * q + 2];
             double g3[3] = {fr * ux, fr * uy, fr * uz};
             for (int ww = 0; ww < BW; ++ww) {
@@ -1186,7 +1252,8 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
             atomicAdd(&io.F_fix[3 * j], fix_of(g3[0]));
             atomicAdd(&io.F_fix[3 * j + 1], fix_of(g3[1]));
             atomicAdd(&io.F_fix[3 * j + 2], fix_of(g3[2]));
-            const double r = slot_r64(L, q);
+            // |d| for the virial: the lean layout keeps 1/|d| only (its rounding is the unit vector's)
+            const double r = LEAN ? 1.0 / (double)L.u[4 * q + 3] : slot_r64(L, q);
             const double d3[3] = {ux * r, uy * r, uz * r};
 #pragma unroll
             for (int a = 0; a < 3; ++a) {

@@ -1388,7 +1388,9 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
         HC(hipMemcpyAsync(&mx, ctx->d_scal.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HC(hipStreamSynchronize(ctx->stream));
         if (mx > NB_LMAX) return fail(ctx, "ace_build_neighbors: an atom has more than 1024 neighbours");
-        ctx->slots_cap = std::min(NB_LMAX, ((mx + 16 + 31) & ~31));
+        // the largest count plus a margin of 16 for later asynchronous rebuilds, rounded up to 8: the basis
+        // kernels' LDS is sized by it, and every slot less keeps more atoms in flight per CU
+        ctx->slots_cap = std::min(NB_LMAX, ((mx + 16 + 7) & ~7));
     }
     if (ensure(ctx, ctx->d_nb_keys, (size_t)N * ctx->slots_cap)) return -1;
     if (timed(ctx, "nb_search", [&] {

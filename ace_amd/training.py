@@ -191,6 +191,7 @@ class Trainer:
         o.gamma = self.decay.step(o.gamma, self.it)
         self.algo.step(o)
         weight_decay(o, self.mask, self.eta_reg, self.reg)
+        self.dv, self.dp = abs(o.val - o.val_old), float(np.linalg.norm(o.p - o.p_old))   # nnptefr.cpp:1060-1061
         o.val_old, o.p_old, o.g_old = o.val, o.p.copy(), o.g.copy()
         self.it += 1
         return o.p
@@ -207,6 +208,68 @@ class Trainer:
         for k, v in vars(self.algo).items():
             state["algo_" + k] = np.asarray(v)
         np.savez(path, **state)
+
+    def write_restart(self, path, files=("", "", "", ""), n_print=1, n_write=1, max_iter=0):
+        """NNPTEFR::write_restart (src/nnp/nnptefr.cpp:509-537) in the reference's binary layout (restart.py):
+        the potential, the full training batch in order, the objective (gamma, last loss, parameters, last
+        gradient), the iteration count, the optimiser's state, decay, loss, norm and weight decay."""
+        from . import restart as R
+        if self.device_opt:
+            self._pull_device_state()
+        o = self.obj
+        self.model.set_all_params(o.p)
+        name = type(self.algo).__name__
+        scal, vecs = R.ALGO_FIELDS[name]
+        dim = len(o.p)
+        state = {ref: float(getattr(self.algo, attr)) for ref, attr in scal}
+        for ref, attr in vecs:
+            v = getattr(self.algo, attr, None)
+            state[ref] = np.zeros((dim, dim) if ref in ("B", "BOld") else dim) if v is None else np.asarray(v)
+        order = np.arange(self.nbatch, dtype=np.int32)
+        st = R.RestartState(
+            model=self.model, files=tuple(files), batch_train=R.Batch(0, self.nbatch, self.nbatch, order, order),
+            gamma=o.gamma, val=o.val_old, val_old=o.val_old, dv=getattr(self, "dv", 0.0), dp=getattr(self, "dp", 0.0),
+            p=o.p, p_old=o.p_old, g=o.g_old, g_old=o.g_old, n_print=n_print, n_write=n_write, step=self.it,
+            count=self.it, max_iter=max_iter, loss=R.LOSS_NAMES[self.loss + 1], algo=name, algo_state=state,
+            decay=self.decay.name, decay_period=self.decay.period, decay_alpha=self.decay.alpha,
+            norm=R.NORM_NAMES[self.norm], reg=self.reg.upper(), delta=self.delta, eta=self.eta_reg)
+        R.write_restart(path, st)
+
+    def read_restart(self, path):
+        """NNPTEFR::read_restart (nnptefr.cpp:539-574) of a file in the reference's layout: parameters, the
+        objective, the optimiser (as the reference unpacks it: a fresh optimiser holding the file's state,
+        so ADAM-type bias-correction powers restart), decay, loss, norm, weight decay and the iteration
+        count.  The file's potential must have this trainer's species, bases and network shapes."""
+        from . import restart as R
+        from .optimizers import Decay, make
+        st = R.read_restart(path)
+        m = self.model
+        if st.model.ntypes != m.ntypes or any(a.widths != b.widths for a, b in zip(st.model.nets, m.nets)) or \
+                any(st.model.input_width(I) != m.input_width(I) for I in range(m.ntypes)):
+            raise ValueError("read_restart: the restart's potential does not match the trainer's model")
+        if st.algo is None or st.p is None:
+            raise ValueError("read_restart: the restart holds no optimiser state")
+        o = self.obj
+        o.p, o.p_old, o.g_old = st.p.copy(), st.p_old.copy(), st.g_old.copy()
+        o.val_old, o.gamma, self.it = st.val_old, st.gamma, st.count
+        self.dv, self.dp = st.dv, st.dp
+        self.algo = make(st.algo, len(o.p))
+        scal, vecs = R.ALGO_FIELDS[st.algo]
+        for ref, attr in scal:
+            setattr(self.algo, attr, st.algo_state[ref])
+        for ref, attr in vecs:
+            if hasattr(self.algo, attr) and ref in st.algo_state:
+                setattr(self.algo, attr, np.array(st.algo_state[ref]))
+        self.decay = Decay(st.decay, st.decay_alpha, st.decay_period)
+        self.loss = R.LOSS_NAMES.index(st.loss) - 1
+        self.norm = R.NORM_NAMES.index(st.norm)
+        self.delta, self.reg, self.eta_reg = st.delta, st.reg, st.eta
+        self.model.set_all_params(o.p)
+        if self.device_opt:
+            self.dev.train_opt_state({"p": o.p, "m": self.algo.m, "v": self.algo.v,
+                                      "bt": np.array([self.algo.b1t, self.algo.b2t])})
+        elif hasattr(self.dev, "set_all_params"):
+            self.dev.set_all_params(o.p)
 
     def load_restart(self, path):
         """Continues from a state written by save_restart (NNPTEFR::read_restart, nnptefr.cpp:539-574)."""

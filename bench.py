@@ -469,6 +469,8 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=5, help="one-thread evaluations of the 1000-atom structure")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-headline", action="store_true", help="the headline batch only (profiling runs)")
+    ap.add_argument("--only", choices=("headline", "c2", "train", "md"), default=None,
+                    help="one leg only, its JSON object printed (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 elementwise copies of MIB MiB (reads MIB, writes MIB): the reference point "
                          "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -481,6 +483,13 @@ def main():
               "batched inference, spatial bricks with a ghost layer for MD) is not built", file=sys.stderr)
         sys.exit(2)
 
+    if args.only_headline:
+        args.only = "headline"
+    if args.only in ("c2", "train", "md"):
+        leg = {"c2": lambda: c2_line(args)[0], "train": lambda: train_line(args.train_steps),
+               "md": lambda: md_line(args.md_steps)}[args.only]()
+        print(json.dumps(leg))
+        return
     head, (m5, s5, sc5, E32, F32, Ebf, Fbf) = c5_headline(args)
     log(f"c5: {head['value'] / 1e6:.2f} M atoms/s ({head['ms_per_step']:.2f} ms per step)")
     res = {
@@ -493,7 +502,7 @@ def main():
         "max_ms_per_step": head["max_ms_per_step"], "kernels_avg_ms": head["kernels_avg_ms"],
         "network_mfma": head["network_mfma"], "autotune": head["autotune"], "bf16_layer1": head["bf16_layer1"],
     }
-    if args.only_headline:
+    if args.only == "headline":
         print(json.dumps(res))
         return
     res["plain_fp64_port"] = plain_port_c5_sample((m5, s5))

@@ -20,7 +20,7 @@
 #include "reduce.hip"
 #include "plain.hip"
 
-#define NB_CELL_DIV 1   // cells per list radius along each perpendicular height (binned search)
+#define NB_CELL_DIV 2   // cells per list radius along each perpendicular height (binned search)
 #include "train.hip"
 #include "md.hip"
 

@@ -237,13 +237,61 @@ __device__ inline void nb_key_decode(long long k, int& jl, int& n0, int& n1, int
     n2 = (c & 1023) - 512;
 }
 
+// Rank sort of one atom's keys with order-preserving 32-bit keys: (j_local, n0 + m, n1 + m, n2 + m) packed
+// in fields of b bits for the offsets (2 m < 2^b, m the largest image range of the structure) and the rest
+// for j_local.  Thread t holds keys t + k BT (k < NK): it reads them, the workgroup overwrites the front of
+// the key buffer with the 32-bit keys (padded to a multiple of four with 0xffffffff, larger than any key:
+// a key of all ones needs n + m = 2^b - 1, odd, above 2 m), and every thread counts the smaller ones for
+// its NK keys in one sweep.  count <= NK BT, and count rounded up to four stays within the buffer (cap is
+// a multiple of eight).
+template <int NK, int BT>
+__device__ inline void nb_rank_sort32(long long* s_key, int count, int tid, int m, int b, long long* row) {
+    long long a[NK];
+    unsigned c[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        const int e = tid + k * BT;
+        a[k] = 0;
+        c[k] = 0xffffffffu;
+        if (e < count) {
+            a[k] = s_key[e];
+            int jl, n0, n1, n2;
+            nb_key_decode(a[k], jl, n0, n1, n2);
+            c[k] = ((unsigned)jl << (3 * b)) | ((unsigned)(n0 + m) << (2 * b)) | ((unsigned)(n1 + m) << b) |
+                   (unsigned)(n2 + m);
+        }
+    }
+    __syncthreads();
+    unsigned* s_c = reinterpret_cast<unsigned*>(s_key);
+    const int count4 = (count + 3) & ~3;
+#pragma unroll
+    for (int k = 0; k < NK; ++k)
+        if (tid + k * BT < count4) s_c[tid + k * BT] = c[k];
+    __syncthreads();
+    int r[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) r[k] = 0;
+    const uint2* s_c2 = reinterpret_cast<const uint2*>(s_key);
+    for (int f = 0; f < count4; f += 4) {
+        const uint2 p = s_c2[f >> 1], q = s_c2[(f >> 1) + 1];
+#pragma unroll
+        for (int k = 0; k < NK; ++k)
+            r[k] += (int)(p.x < c[k]) + (int)(p.y < c[k]) + (int)(q.x < c[k]) + (int)(q.y < c[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < NK; ++k)
+        if (tid + k * BT < count) __builtin_nontemporal_store(a[k], &row[r[k]]);   // read once, by the next kernel
+}
+
 #define NB_WAVES 4          // waves per atom of the binning-free search
 #define NB_SEARCH_WAVES 1   // waves per atom of the binned search
 #define NB_LMAX 1024   // neighbours per atom held while sorting
-#define NB_U 4         // candidate chunks loaded together by one wave
+#define NB_U 2         // candidate chunks loaded together by one wave
+#define NB_SC 128      // stencil cells per pass (two per lane): the 5 x 5 x 5 stencil in one pass
 
 // Workgroup of NB_SEARCH_WAVES waves per atom (one: the most atoms in flight per CU, which hides the dependent
-// loads of the stencil and the candidates best); 64-candidate chunks are dealt to the waves in turn, NB_U chunks
+// loads of the stencil and the candidates best); the stencil is taken NB_SC cells at a time (each pass is two
+// dependent loads: cell table, then candidates), 64-candidate chunks are dealt to the waves in turn, NB_U chunks
 // in flight per wave.
 // MODE 0: count only (counts[i], running maximum in *maxcount).  MODE 1: sort the keys
 // (j_local << 30 | image order code: the reference order) and write them to row i of keys (cap slots); an
@@ -255,9 +303,9 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
     const int* __restrict__ cell_start, const double4* __restrict__ cpos, const int2* __restrict__ cinfo,
     int* __restrict__ counts, int* __restrict__ maxcount, long long* __restrict__ keys, int cap,
     int* __restrict__ over) {
-    __shared__ int s_cstart[64];
-    __shared__ int s_cpref[65];
-    __shared__ int s_cimg[64];
+    __shared__ int s_cstart[NB_SC];
+    __shared__ int s_cpref[NB_SC + 1];
+    __shared__ int s_cimg[NB_SC];
     __shared__ int s_n;
     extern __shared__ long long s_key[];   // MODE 1: cap keys (dynamic LDS)
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
@@ -273,35 +321,46 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
     const int sx = 2 * S.st[0] + 1, sy = 2 * S.st[1] + 1, sz = 2 * S.st[2] + 1;
     const int nst = sx * sy * sz;
     if (tid == 0) s_n = 0;
-    for (int cb = 0; cb < nst; cb += 64) {
-        // stencil cells of this chunk: start, size and image shift, with an inclusive prefix (wave 0)
+    for (int cb = 0; cb < nst; cb += NB_SC) {
+        // stencil cells cb + 2 lane and cb + 2 lane + 1 of this pass: start, size and image shift, with the
+        // exclusive prefix of the candidate counts (wave 0)
         __syncthreads();
         if (wv == 0) {
-            const int q = cb + lane;
-            int cnt = 0, start = 0, shift = 0;
-            if (q < nst) {
-                const int d0 = q % sx - S.st[0], d1 = (q / sx) % sy - S.st[1], d2 = q / (sx * sy) - S.st[2];
-                const int a0 = ci[0] + d0, a1 = ci[1] + d1, a2 = ci[2] + d2;
-                const int n0 = floordiv(a0, S.nc[0]), n1 = floordiv(a1, S.nc[1]), n2 = floordiv(a2, S.nc[2]);
-                const int c0 = a0 - n0 * S.nc[0], c1 = a1 - n1 * S.nc[1], c2 = a2 - n2 * S.nc[2];
-                const int cell = cs0 + (c2 * S.nc[1] + c1) * S.nc[0] + c0;
-                start = cell_start[cell];
-                cnt = cell_start[cell + 1] - start;
-                shift = img_pack(n0, n1, n2);
+            int cnt[2], start[2], shift[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int q = cb + 2 * lane + h;
+                cnt[h] = 0;
+                start[h] = 0;
+                shift[h] = 0;
+                if (q < nst) {
+                    const int d0 = q % sx - S.st[0], d1 = (q / sx) % sy - S.st[1], d2 = q / (sx * sy) - S.st[2];
+                    const int a0 = ci[0] + d0, a1 = ci[1] + d1, a2 = ci[2] + d2;
+                    const int n0 = floordiv(a0, S.nc[0]), n1 = floordiv(a1, S.nc[1]), n2 = floordiv(a2, S.nc[2]);
+                    const int c0 = a0 - n0 * S.nc[0], c1 = a1 - n1 * S.nc[1], c2 = a2 - n2 * S.nc[2];
+                    const int cell = cs0 + (c2 * S.nc[1] + c1) * S.nc[0] + c0;
+                    start[h] = cell_start[cell];
+                    cnt[h] = cell_start[cell + 1] - start[h];
+                    shift[h] = img_pack(n0, n1, n2);
+                }
             }
-            int pre = cnt;
+            const int two = cnt[0] + cnt[1];
+            int pre = two;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
                 const int t = __shfl_up(pre, o, 64);
                 if (lane >= o) pre += t;
             }
-            s_cstart[lane] = start;
-            s_cimg[lane] = shift;
-            s_cpref[lane + 1] = pre;
-            if (lane == 0) s_cpref[0] = 0;
+            s_cstart[2 * lane] = start[0];
+            s_cstart[2 * lane + 1] = start[1];
+            s_cimg[2 * lane] = shift[0];
+            s_cimg[2 * lane + 1] = shift[1];
+            s_cpref[2 * lane] = pre - two;
+            s_cpref[2 * lane + 1] = pre - two + cnt[0];
+            if (lane == 63) s_cpref[NB_SC] = pre;
         }
         __syncthreads();
-        const int total = s_cpref[64];
+        const int total = s_cpref[NB_SC];
         // chunk q of 64 candidates goes to wave q mod NB_SEARCH_WAVES, so a short stencil keeps every wave busy
         for (int tb = 0; tb < total; tb += NB_SEARCH_WAVES * 64 * NB_U) {
             double4 pj[NB_U];
@@ -313,7 +372,7 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
                 // stencil slot of candidate x: largest c with s_cpref[c] <= x
                 int c = 0;
 #pragma unroll
-                for (int st = 32; st > 0; st >>= 1)
+                for (int st = NB_SC / 2; st > 0; st >>= 1)
                     if (s_cpref[c + st] <= x) c += st;
                 sl[u] = c;
                 if (x < total) {
@@ -373,9 +432,21 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
         if (tid == 0) atomicMax(over, 1);
         return;
     }
-    // rank sort into the reference order (the keys of one atom are distinct)
+    // rank sort into the reference order (the keys of one atom are distinct): with 32-bit keys when they
+    // hold j_local and the image offsets and the row fits four keys per thread, else with the full keys
     long long* row = keys + (long long)i * cap;
-    for (int e = tid; e < count; e += NB_SEARCH_WAVES * 64) {
+    constexpr int BT = NB_SEARCH_WAVES * 64;
+    const int m = max(S.cf[0], max(S.cf[1], S.cf[2]));
+    const int b = 32 - __clz(2 * m);                  // bits of 0 .. 2 m
+    const int nk = (count + BT - 1) / BT;
+    if (3 * b < 32 && (long long)S.natoms <= (1LL << (32 - 3 * b)) && nk <= 4) {
+        if (nk <= 1) nb_rank_sort32<1, BT>(s_key, count, tid, m, b, row);
+        else if (nk == 2) nb_rank_sort32<2, BT>(s_key, count, tid, m, b, row);
+        else if (nk == 3) nb_rank_sort32<3, BT>(s_key, count, tid, m, b, row);
+        else nb_rank_sort32<4, BT>(s_key, count, tid, m, b, row);
+        return;
+    }
+    for (int e = tid; e < count; e += BT) {
         const long long k = s_key[e];
         int rank = 0;
         for (int f = 0; f < count; ++f) rank += (s_key[f] < k);

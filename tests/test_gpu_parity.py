@@ -146,6 +146,39 @@ def test_large_triclinic_cells_binning_free_search():
     check_against_oracle(m, strucs, "triclinic")
 
 
+def test_neighbour_rows_of_every_length():
+    """Binned search (a cell shorter than 2 rc in the batch) over densities whose rows hold 22 to 362
+    neighbours: one to four sort keys per lane with the packed 32-bit keys, and the full-key rank sort
+    beyond 256; pairs, images and order bit for bit."""
+    rng = np.random.default_rng(41)
+    m = configs.c3_model(rng)
+    L_ = 14.0
+    strucs = [structure(np.eye(3) * 7.4, rng.uniform(0, 7.4, (30, 3)), rng.integers(0, 3, 30))]
+    for n in (121, 303, 485, 697, 970):
+        strucs.append(structure(np.eye(3) * L_, rng.uniform(0, L_, (n, 3)), rng.integers(0, 3, n)))
+    orc = O.Oracle(m)
+    dev = NNPDevice(m)
+    dev.upload(strucs)
+    dev.build_neighbors()
+    off, j, img = dev.neighbors()
+    dev.close()
+    a0 = p0 = 0
+    lengths = []
+    for k, s in enumerate(strucs):
+        nl = orc.neighbors(s)
+        n = s.natoms
+        assert np.array_equal(off[a0:a0 + n + 1] - off[a0], nl.off), f"rows[{k}] offsets"
+        assert np.array_equal(j[p0:p0 + len(nl.j)], nl.j), f"rows[{k}] indices"
+        assert np.array_equal(img[p0:p0 + len(nl.j)], nl.img), f"rows[{k}] images"
+        lengths.append(np.diff(nl.off))
+        a0 += n
+        p0 += len(nl.j)
+    lengths = np.concatenate(lengths)
+    log(f"row lengths {lengths.min()}..{lengths.max()}")
+    for lo, hi in ((1, 64), (65, 128), (129, 192), (193, 256), (257, 1024)):
+        assert ((lengths >= lo) & (lengths <= hi)).any(), (lo, hi)
+
+
 def test_isolated_and_single_atom():
     rng = np.random.default_rng(6)
     m = configs.c3_model(rng)

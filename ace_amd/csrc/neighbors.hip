@@ -288,6 +288,7 @@ __device__ inline void nb_rank_sort32(long long* s_key, int count, int tid, int 
 #define NB_LMAX 1024   // neighbours per atom held while sorting
 #define NB_U 2         // candidate chunks loaded together by one wave
 #define NB_SC 128      // stencil cells per pass (two per lane): the 5 x 5 x 5 stencil in one pass
+#define NB_MAP 1024    // candidates of a pass mapped to their stencil cell in LDS (longer passes search the prefix)
 
 // Workgroup of NB_SEARCH_WAVES waves per atom (one: the most atoms in flight per CU, which hides the dependent
 // loads of the stencil and the candidates best); the stencil is taken NB_SC cells at a time (each pass is two
@@ -306,6 +307,7 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
     __shared__ int s_cstart[NB_SC];
     __shared__ int s_cpref[NB_SC + 1];
     __shared__ int s_cimg[NB_SC];
+    __shared__ unsigned char s_slot[NB_MAP];
     __shared__ int s_n;
     extern __shared__ long long s_key[];   // MODE 1: cap keys (dynamic LDS)
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
@@ -358,9 +360,15 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
             s_cpref[2 * lane] = pre - two;
             s_cpref[2 * lane + 1] = pre - two + cnt[0];
             if (lane == 63) s_cpref[NB_SC] = pre;
+            // stencil slot of every candidate of the pass, when they fit the map
+            if (__shfl(pre, 63, 64) <= NB_MAP) {
+                for (int t = 0; t < cnt[0]; ++t) s_slot[pre - two + t] = (unsigned char)(2 * lane);
+                for (int t = 0; t < cnt[1]; ++t) s_slot[pre - cnt[1] + t] = (unsigned char)(2 * lane + 1);
+            }
         }
         __syncthreads();
         const int total = s_cpref[NB_SC];
+        const bool mapped = total <= NB_MAP;
         // chunk q of 64 candidates goes to wave q mod NB_SEARCH_WAVES, so a short stencil keeps every wave busy
         for (int tb = 0; tb < total; tb += NB_SEARCH_WAVES * 64 * NB_U) {
             double4 pj[NB_U];
@@ -371,9 +379,13 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
                 const int x = tb + 64 * (u * NB_SEARCH_WAVES + wv) + lane;
                 // stencil slot of candidate x: largest c with s_cpref[c] <= x
                 int c = 0;
+                if (mapped) {
+                    if (x < total) c = s_slot[x];
+                } else {
 #pragma unroll
-                for (int st = NB_SC / 2; st > 0; st >>= 1)
-                    if (s_cpref[c + st] <= x) c += st;
+                    for (int st = NB_SC / 2; st > 0; st >>= 1)
+                        if (s_cpref[c + st] <= x) c += st;
+                }
                 sl[u] = c;
                 if (x < total) {
                     const int e = s_cstart[c] + (x - s_cpref[c]);

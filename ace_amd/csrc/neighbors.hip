@@ -322,6 +322,11 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
     const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
     const int sx = 2 * S.st[0] + 1, sy = 2 * S.st[1] + 1, sz = 2 * S.st[2] + 1;
     const int nst = sx * sy * sz;
+    // stencils within one image of the cell grid (st <= nc): the cell of stencil slot q by multiply-high with
+    // m = floor((2^32 - 1) / d) + 1 (exact while q e < 2^32, e = m d - 2^32 < d) and one-image wrapping
+    const bool fast = S.st[0] <= S.nc[0] && S.st[1] <= S.nc[1] && S.st[2] <= S.nc[2] && nst < (1 << 20) &&
+                      sx * sy < 4096;
+    const unsigned mx = 0xffffffffu / (unsigned)sx + 1u, mxy = 0xffffffffu / (unsigned)(sx * sy) + 1u;
     if (tid == 0) s_n = 0;
     for (int cb = 0; cb < nst; cb += NB_SC) {
         // stencil cells cb + 2 lane and cb + 2 lane + 1 of this pass: start, size and image shift, with the
@@ -336,9 +341,23 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
                 start[h] = 0;
                 shift[h] = 0;
                 if (q < nst) {
-                    const int d0 = q % sx - S.st[0], d1 = (q / sx) % sy - S.st[1], d2 = q / (sx * sy) - S.st[2];
-                    const int a0 = ci[0] + d0, a1 = ci[1] + d1, a2 = ci[2] + d2;
-                    const int n0 = floordiv(a0, S.nc[0]), n1 = floordiv(a1, S.nc[1]), n2 = floordiv(a2, S.nc[2]);
+                    int a0, a1, a2, n0, n1, n2;
+                    if (fast) {
+                        const int qy = (int)__umulhi((unsigned)q, mx), qz = (int)__umulhi((unsigned)q, mxy);
+                        a0 = ci[0] + q - qy * sx - S.st[0];
+                        a1 = ci[1] + qy - qz * sy - S.st[1];
+                        a2 = ci[2] + qz - S.st[2];
+                        n0 = a0 < 0 ? -1 : (a0 >= S.nc[0] ? 1 : 0);
+                        n1 = a1 < 0 ? -1 : (a1 >= S.nc[1] ? 1 : 0);
+                        n2 = a2 < 0 ? -1 : (a2 >= S.nc[2] ? 1 : 0);
+                    } else {
+                        a0 = ci[0] + q % sx - S.st[0];
+                        a1 = ci[1] + (q / sx) % sy - S.st[1];
+                        a2 = ci[2] + q / (sx * sy) - S.st[2];
+                        n0 = floordiv(a0, S.nc[0]);
+                        n1 = floordiv(a1, S.nc[1]);
+                        n2 = floordiv(a2, S.nc[2]);
+                    }
                     const int c0 = a0 - n0 * S.nc[0], c1 = a1 - n1 * S.nc[1], c2 = a2 - n2 * S.nc[2];
                     const int cell = cs0 + (c2 * S.nc[1] + c1) * S.nc[0] + c0;
                     start[h] = cell_start[cell];

@@ -213,15 +213,24 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(int n, const int* __
 
 __device__ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
 
-// reference distance test (neighbor.cpp:72-82), FP64 without contraction
-__device__ inline double ref_ds2(const double* R, double dx, double dy, double dz, int n0, int n1, int n2) {
+// reference distance test (neighbor.cpp:72-82), FP64 without contraction: the lattice shift R n, then the
+// squared distance of (dx, dy, dz) - R n
+__device__ inline void ref_shift(const double* R, int n0, int n1, int n2, double& s0, double& s1, double& s2) {
 #pragma clang fp contract(off)
     const double f0 = 1.0 * n0, f1 = 1.0 * n1, f2 = 1.0 * n2;
-    const double s0 = R[0] * f0 + R[3] * f1 + R[6] * f2;
-    const double s1 = R[1] * f0 + R[4] * f1 + R[7] * f2;
-    const double s2 = R[2] * f0 + R[5] * f1 + R[8] * f2;
+    s0 = R[0] * f0 + R[3] * f1 + R[6] * f2;
+    s1 = R[1] * f0 + R[4] * f1 + R[7] * f2;
+    s2 = R[2] * f0 + R[5] * f1 + R[8] * f2;
+}
+__device__ inline double ref_ds2_shifted(double dx, double dy, double dz, double s0, double s1, double s2) {
+#pragma clang fp contract(off)
     const double v0 = dx - s0, v1 = dy - s1, v2 = dz - s2;
     return (v0 * v0 + v1 * v1) + v2 * v2;
+}
+__device__ inline double ref_ds2(const double* R, double dx, double dy, double dz, int n0, int n1, int n2) {
+    double s0, s1, s2;
+    ref_shift(R, n0, n1, n2, s0, s1, s2);
+    return ref_ds2_shifted(dx, dy, dz, s0, s1, s2);
 }
 
 // neighbour order key: j_local in the high bits, then the image (n0, n1, n2) offset by 512 in three
@@ -308,6 +317,7 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
     __shared__ int s_cpref[NB_SC + 1];
     __shared__ int s_cimg[NB_SC];
     __shared__ unsigned char s_slot[NB_MAP];
+    __shared__ double4 s_rn[27];           // R n for n in {-1, 0, 1}^3, index (n0 + 1) + 3 (n1 + 1) + 9 (n2 + 1)
     __shared__ int s_n;
     extern __shared__ long long s_key[];   // MODE 1: cap keys (dynamic LDS)
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
@@ -327,7 +337,13 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
     const bool fast = S.st[0] <= S.nc[0] && S.st[1] <= S.nc[1] && S.st[2] <= S.nc[2] && nst < (1 << 20) &&
                       sx * sy < 4096;
     const unsigned mx = 0xffffffffu / (unsigned)sx + 1u, mxy = 0xffffffffu / (unsigned)(sx * sy) + 1u;
+    const int cf0 = S.cf[0], cf1 = S.cf[1], cf2 = S.cf[2], aoff = S.atom_off;
     if (tid == 0) s_n = 0;
+    if (tid < 27) {
+        double s0, s1, s2;
+        ref_shift(S.R, tid % 3 - 1, (tid / 3) % 3 - 1, tid / 9 - 1, s0, s1, s2);
+        s_rn[tid] = make_double4(s0, s1, s2, 0.0);
+    }
     for (int cb = 0; cb < nst; cb += NB_SC) {
         // stencil cells cb + 2 lane and cb + 2 lane + 1 of this pass: start, size and image shift, with the
         // exclusive prefix of the candidate counts (wave 0)
@@ -423,8 +439,9 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
                     img_unpack(s_cimg[sl[u]], sh0, sh1, sh2);
                     img_unpack(inf[u].y, wj0, wj1, wj2);
                     const int n0 = sh0 + wi0 - wj0, n1 = sh1 + wi1 - wj1, n2 = sh2 + wi2 - wj2;
-                    if (n0 >= -S.cf[0] && n0 <= S.cf[0] && n1 >= -S.cf[1] && n1 <= S.cf[1] &&
-                        n2 >= -S.cf[2] && n2 <= S.cf[2]) {
+                    // |n_d| <= cf_d, one unsigned comparison per dimension
+                    if (((unsigned)(n0 + cf0) <= (unsigned)(2 * cf0)) & ((unsigned)(n1 + cf1) <= (unsigned)(2 * cf1)) &
+                        ((unsigned)(n2 + cf2) <= (unsigned)(2 * cf2))) {
                         double dx, dy, dz;
                         {
 #pragma clang fp contract(off)
@@ -432,10 +449,21 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
                             dy = piy - pj[u].y;
                             dz = piz - pj[u].z;
                         }
-                        const double ds2 = ref_ds2(S.R, dx, dy, dz, n0, n1, n2);
+                        // R n from the table for the one-image shifts (the same operations, so the same bits)
+                        double s0, s1, s2;
+                        const unsigned t0 = n0 + 1, t1 = n1 + 1, t2 = n2 + 1;
+                        if ((t0 <= 2u) & (t1 <= 2u) & (t2 <= 2u)) {
+                            const double4 r = s_rn[t0 + 3 * t1 + 9 * t2];
+                            s0 = r.x;
+                            s1 = r.y;
+                            s2 = r.z;
+                        } else {
+                            ref_shift(S.R, n0, n1, n2, s0, s1, s2);
+                        }
+                        const double ds2 = ref_ds2_shifted(dx, dy, dz, s0, s1, s2);
                         if ((j != i || ds2 > 1e-12) && ds2 < rc2) {
                             acc = true;
-                            key = nb_key(j - S.atom_off, n0, n1, n2);
+                            key = nb_key(j - aoff, n0, n1, n2);
                         }
                     }
                 }

@@ -51,6 +51,7 @@ SIGNATURES = {
     "ace_forward": (C.c_int, [P, dp, dp, dp]),
     "ace_tile_shape": (C.c_int, [P, C.c_int, dp]),
     "ace_set_level_split": (C.c_int, [P, C.c_int]),
+    "ace_set_force_gather": (C.c_int, [P, C.c_int]),
     "ace_set_tile_shape": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
     "ace_backward": (C.c_int, [P, dp, dp, dp, dp]),
     "ace_profile_enable": (C.c_int, [P, C.c_int]),

@@ -64,6 +64,7 @@ struct ace_ctx {
     bool all_canon = true;
     int lsplit = 0;                  // level split of the two eta groups' canonical levels (0: none)
     bool lsplit_on = true;           // ace_set_level_split
+    bool fgather = true;             // ace_set_force_gather: energy/force passes gather per-pair gradients
     DevModel hdm;
     DBuf<DevModel> d_model;
     DBuf<float> d_r_rs, d_r_eta, d_a_lamf, d_a_zeta, d_inpw, d_inpb, d_wpad;
@@ -125,6 +126,8 @@ struct ace_ctx {
     DBuf<double4> d_cpos;      // cell-ordered positions
     DBuf<int2> d_cinfo;        // cell-ordered (atom, wrap)
     DBuf<long long> d_nb_keys; // per-atom rows of sorted neighbour keys
+    DBuf<double> d_gin;        // gathered energy/force pass: per-entry gradients, rows like the keys
+    DBuf<double> d_fown;       // and each atom's own share
     long long npairs = 0;       // pairs of the current list (set when the CSR arrays are built)
     bool csr_valid = false;     // CSR arrays (export, unbatched port) match the key rows
     int slots_cap = 32;         // neighbour slots per atom held in LDS by the basis kernels
@@ -909,7 +912,7 @@ int split_waves(ace_ctx* ctx, int mode, int precision) {
 // mode 3: per-atom energies and per-pair gradients in one pass.  waves: 4 or 2 per atom, -1 for
 // split_waves of the precision (modes 0-2; -1: the context's precision) or four (mode 3)
 int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = nullptr, int waves = -1,
-              int precision = -1) {
+              int precision = -1, bool gather = false) {
     ++ctx->seq;
     BasisFn fns[8];
     const bool f64 = ctx->basis_f64;
@@ -952,6 +955,8 @@ int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = 
     const int b = ctx->fbuf;
     io.F_fix = ctx->d_Ffix[b].p; io.E_fix = ctx->d_Efix[b].p; io.W_fix = ctx->d_Wfix[b].p;
     io.F_nxt = ctx->d_Ffix[1 - b].p; io.E_nxt = ctx->d_Efix[1 - b].p; io.W_nxt = ctx->d_Wfix[1 - b].p;
+    io.gin = gather ? ctx->d_gin.p : nullptr;
+    io.fown = gather ? ctx->d_fown.p : nullptr;
     if (timed(ctx, names[f64][mode], [&] {
             f<<<ctx->N, 64 * waves, bytes, ctx->stream>>>(ctx->N, ctx->d_model.p, P, io, nbm, ctx->nt, ctx->kmax);
         }))
@@ -1003,15 +1008,28 @@ int run_mlp(ace_ctx* ctx, bool bf16, int precision = ACE_PREC_FP32) {
     });
 }
 
-// the energy/force launches (fused, or symm -> network -> adjoint); forces, energies and virials land in
-// the fixed-point accumulators.  The BF16 first layer runs on the matrix cores of the network kernel, so
-// that mode takes the split path.
+// forces of the last gradient launch (which wrote per-pair gradients) into its fixed-point buffer
+int run_force_gather(ace_ctx* ctx) {
+    ++ctx->seq;
+    const int b = 1 - ctx->fbuf;
+    return timed(ctx, "force_gather", [&] {
+        k_force_gather<<<cdiv(ctx->N, FG_WAVES), FG_WAVES * 64, 0, ctx->stream>>>(
+            ctx->N, ctx->slots_cap, ctx->d_nb_count.p, ctx->d_gin.p, ctx->d_fown.p, ctx->d_Ffix[b].p);
+    });
+}
+
+// the energy/force launches (fused, or symm -> network -> adjoint -> force gather); forces, energies and
+// virials land in the fixed-point accumulators.  The split adjoint writes each pair's gradient once and
+// the gather sums them by atom, instead of three fixed-point atomics per pair into memory.  The BF16 first
+// layer runs on the matrix cores of the network kernel, so that mode takes the split path.
 int run_ef(ace_ctx* ctx, int precision) {
     const int fw = fused_waves(ctx, precision);
     if (fw) return run_basis(ctx, 3, nullptr, nullptr, fw);
     if (run_basis(ctx, 0, nullptr, nullptr, -1, precision)) return -1;
     if (run_mlp(ctx, precision == ACE_PREC_BF16L1, precision)) return -1;
-    return run_basis(ctx, 1, nullptr, nullptr, -1, precision);
+    const bool gather = ctx->fgather && ctx->d_gin.p;
+    if (run_basis(ctx, 1, nullptr, nullptr, -1, precision, gather)) return -1;
+    return gather ? run_force_gather(ctx) : 0;
 }
 
 // one velocity-Verlet step: half kick with the forces of the last gradient launch, drift, energy/force
@@ -1138,6 +1156,7 @@ void ace_destroy(ace_ctx* ctx) {
     for (int b = 0; b < 2; ++b) { release(ctx->d_Ffix[b]); release(ctx->d_Efix[b]); release(ctx->d_Wfix[b]); }
     release(ctx->d_sinfo); release(ctx->d_cell_of); release(ctx->d_wrap_of); release(ctx->d_slot_of);
     release(ctx->d_cell_start); release(ctx->d_cpos); release(ctx->d_cinfo); release(ctx->d_nb_keys);
+    release(ctx->d_gin); release(ctx->d_fown);
     release(ctx->d_nb_count);
     release(ctx->d_nb_off); release(ctx->d_scan_blk); release(ctx->d_scal); release(ctx->d_nb_j);
     release(ctx->d_nb_img); release(ctx->d_G64);
@@ -1313,7 +1332,9 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
         double hgt[3];
         for (int d = 0; d < 3; ++d) {
             const double coln = std::sqrt(R[3 * d] * R[3 * d] + R[3 * d + 1] * R[3 * d + 1] + R[3 * d + 2] * R[3 * d + 2]);
-            I.cf[d] = (int)std::ceil(3.0 * rcl / coln);                       // neighbor.cpp:49-53
+            // image range of the reference's list radius (neighbor.cpp:49-53), the model cutoff: a skin
+            // widens the distance test only, so the pairs within the cutoff are the reference's
+            I.cf[d] = (int)std::ceil(3.0 * ctx->rc / coln);
             // row d of R^-1 (column-major storage: (d, c) at d + 3 c)
             const double* Ri = I.Rinv;
             const double rown = std::sqrt(Ri[d] * Ri[d] + Ri[d + 3] * Ri[d + 3] + Ri[d + 6] * Ri[d + 6]);
@@ -1393,6 +1414,8 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
         ctx->slots_cap = std::min(NB_LMAX, ((mx + 16 + 7) & ~7));
     }
     if (ensure(ctx, ctx->d_nb_keys, (size_t)N * ctx->slots_cap)) return -1;
+    if (ctx->fgather && (ensure(ctx, ctx->d_gin, 3 * (size_t)N * ctx->slots_cap) || ensure(ctx, ctx->d_fown, 3 * (size_t)N)))
+        return -1;
     if (timed(ctx, "nb_search", [&] {
             if (brute)
                 k_nb_brute<1><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
@@ -1860,6 +1883,17 @@ int ace_set_tile_shape(ace_ctx* ctx, int precision, int chunk_w, int net_waves) 
 int ace_set_level_split(ace_ctx* ctx, int enable) {
     if (!ctx) return -1;
     ctx->lsplit_on = enable != 0;
+    return 0;
+}
+
+int ace_set_force_gather(ace_ctx* ctx, int enable) {
+    if (!ctx) return -1;
+    ctx->fgather = enable != 0;
+    if (!ctx->fgather) {
+        release(ctx->d_gin);
+        release(ctx->d_fown);
+        ++ctx->gen;   // graphs captured with the gather are stale
+    }
     return 0;
 }
 

@@ -925,7 +925,33 @@ struct BasisIO {
     double* e_atom;
     unsigned long long *F_fix, *E_fix, *W_fix;
     unsigned long long *F_nxt, *E_nxt, *W_nxt;
+    // gradient launches: gin == nullptr adds each pair's g to F_j and -g to F_i with fixed-point atomics;
+    // else g of pair (i -> j, n) goes to the entry of its reverse (j -> i, -n) in j's row,
+    // gin[3 (j cap + p) ..] (zero for a pair past the cutoff), -sum g to fown[3 i ..], and k_force_gather
+    // sums each row
+    double* gin;
+    double* fown;
 };
+
+
+// Entry of the reverse pair (j -> i, -n) of key `key` (pair i -> j, n, j = j_local + a0) in j's sorted key
+// row: binary search; -1 if absent (the list is symmetric, so only a refused build gets there).
+__device__ inline long long reverse_entry(const long long* __restrict__ keys, int cap, const int* __restrict__ counts,
+                                         long long key, int il, int a0) {
+    const int j = (int)(key >> 30) + a0;
+    const int c = (int)(key & 0x3fffffff);
+    const long long target = ((long long)il << 30) | ((long long)(1024 - (c >> 20)) << 20) |
+                             ((1024 - ((c >> 10) & 1023)) << 10) | (1024 - (c & 1023));
+    const int n = min(counts[j], cap);
+    const long long* rj = keys + (long long)j * cap;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (rj[mid] < target) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < n && rj[lo] == target) ? (long long)j * cap + lo : -1;
+}
 
 // GEN = 0 compiles the kernel for models whose angular blocks are all canonical (no generic path).
 // NG: eta groups per species (1, 2 or 4, at least the model's largest count); NZ: canonical power levels
@@ -1042,7 +1068,21 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
                 if (J < nt) cnt[J] += __popcll(__ballot(Jl == J));
         };
         // slot q of species Jl: the next free one in ballot order, i.e. the row order
-        auto place = [&](int j, int Jl, double dx, double dy, double dz, double r) {
+        // the gathered gradient launches keep the row position e of slot q in p[q] instead of j, and store
+        // zero gradients at the reverse entries of the row's pairs past the cutoff
+        const bool pos_e = GACC && io.gin != nullptr;
+        auto place = [&](int e, int j, int Jl, double dx, double dy, double dz, double r) {
+            if (pos_e && Jl < 0 && e < nn) {
+                const long long t = reverse_entry(io.keys, io.cap, io.counts, krow[e], i - S.atom_off, S.atom_off);
+                if (t >= 0) {
+                    double* gp = io.gin + 3 * t;
+                    gp[0] = 0.0;
+                    gp[1] = 0.0;
+                    gp[2] = 0.0;
+                } else {
+                    atomicAdd(io.err, 1);
+                }
+            }
 #pragma unroll
             for (int J = 0; J < ACE_MAXT; ++J) {
                 if (J < nt) {
@@ -1051,7 +1091,7 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
                         // unit vector, |d| and 1/|d| from the FP64 displacement, each rounded once
                         const int q = fill[J] + __popcll(m & lane_mask_lt());
                         const double ir = 1.0 / r;
-                        L.p[q] = j;
+                        L.p[q] = pos_e ? e : j;
                         L.u[4 * q] = (T)(dx * ir);
                         L.u[4 * q + 1] = (T)(dy * ir);
                         L.u[4 * q + 2] = (T)(dz * ir);
@@ -1099,12 +1139,12 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
         }
         if (fits) {
 #pragma unroll
-            for (int c = 0; c < NCH; ++c) place(jr[c], Jr[c], dxr[c], dyr[c], dzr[c], rr[c]);
+            for (int c = 0; c < NCH; ++c) place(64 * c + lane, jr[c], Jr[c], dxr[c], dyr[c], dzr[c], rr[c]);
             for (int cb = 64 * NCH; cb < nn; cb += 64) {
                 int j, Jl;
                 double dx, dy, dz, r;
                 decode(cb + lane, j, Jl, dx, dy, dz, r);
-                place(j, Jl, dx, dy, dz, r);
+                place(cb + lane, j, Jl, dx, dy, dz, r);
             }
         }
         if (!fits && lane == 0) s_cnt[nt] = nbm + 1;
@@ -1237,6 +1277,7 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
         // g_q = dE_i/dd_q of slot q (pair i -> j): F_j -= -g_q, F_i -= g_q summed over the slots; the atom's
         // virial share -sum_q g_q (x) d_q; the atom's energy into its structure's total
         double fi[3] = {0.0, 0.0, 0.0}, w[9];
+        const int a0i = io.sinfo[st].atom_off;
 #pragma unroll
         for (int q9 = 0; q9 < 9; ++q9) w[q9] = 0.0;
         for (int q = tid; q < ntot; q += BT) {
@@ -1248,10 +1289,22 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
                 g3[1] += (double)L.g[ww * 3 * nbm + 3 * q + 1];
                 g3[2] += (double)L.g[ww * 3 * nbm + 3 * q + 2];
             }
-            const int j = L.p[q];
-            atomicAdd(&io.F_fix[3 * j], fix_of(g3[0]));
-            atomicAdd(&io.F_fix[3 * j + 1], fix_of(g3[1]));
-            atomicAdd(&io.F_fix[3 * j + 2], fix_of(g3[2]));
+            if (io.gin) {
+                const long long t = reverse_entry(io.keys, io.cap, io.counts, krow[L.p[q]], i - a0i, a0i);
+                if (t >= 0) {
+                    double* gp = io.gin + 3 * t;
+                    gp[0] = g3[0];
+                    gp[1] = g3[1];
+                    gp[2] = g3[2];
+                } else {
+                    atomicAdd(io.err, 1);
+                }
+            } else {
+                const int j = L.p[q];
+                atomicAdd(&io.F_fix[3 * j], fix_of(g3[0]));
+                atomicAdd(&io.F_fix[3 * j + 1], fix_of(g3[1]));
+                atomicAdd(&io.F_fix[3 * j + 2], fix_of(g3[2]));
+            }
             // |d| for the virial: the lean layout keeps 1/|d| only (its rounding is the unit vector's)
             const double r = LEAN ? 1.0 / (double)L.u[4 * q + 3] : slot_r64(L, q);
             const double d3[3] = {ux * r, uy * r, uz * r};
@@ -1279,8 +1332,10 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
             } else {
                 a = io.e_atom[i];
             }
-            if (tid < 3) atomicAdd(&io.F_fix[3 * i + tid], fix_of(a));
-            else if (tid < 12) atomicAdd(&io.W_fix[9 * st + tid - 3], fix_of(a));
+            if (tid < 3) {
+                if (io.gin) io.fown[3 * i + tid] = a;
+                else atomicAdd(&io.F_fix[3 * i + tid], fix_of(a));
+            } else if (tid < 12) atomicAdd(&io.W_fix[9 * st + tid - 3], fix_of(a));
             else atomicAdd(&io.E_fix[st], fix_of(a));
         }
     }

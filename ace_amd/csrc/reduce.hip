@@ -20,6 +20,34 @@ __global__ __launch_bounds__(256) void k_finalize(int n3, int ns, const unsigned
     if (t < 9 * ns) W[t] = (double)(long long)W_fix[t] * inv;
 }
 
+// Forces from per-pair gradients, the gathered form of the same scatter: the gradient launch wrote g of
+// every pair (i -> a, n) at the entry of its reverse (a -> i, -n) in a's row and -sum g of a's own pairs to
+// fown[a], so F_a = fown[a] + sum over a's row; wave per atom, each lane summing its entries in row order
+// and the wave in a fixed butterfly (lane 0's total), so the forces depend on neither the batch nor the
+// launch; stored once as fixed point.  A row longer than cap belongs to a refused build.
+#define FG_WAVES 4
+__global__ __launch_bounds__(FG_WAVES * 64) void k_force_gather(int natoms, int cap, const int* __restrict__ counts,
+                                                               const double* __restrict__ gin,
+                                                               const double* __restrict__ fown,
+                                                               unsigned long long* __restrict__ F_fix) {
+    const int lane = threadIdx.x & 63;
+    const int a = blockIdx.x * FG_WAVES + (threadIdx.x >> 6);
+    if (a >= natoms) return;
+    const int n = min(counts[a], cap);
+    const double* g = gin + 3 * (long long)a * cap;
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int e = lane; e < n; e += 64) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f[c] += g[3 * e + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) f[c] += __shfl_xor(f[c], o, 64);
+    }
+    if (lane < 3) F_fix[3 * a + lane] = fix_of(fown[3 * a + lane] + f[lane]);
+}
+
 // n 64-bit words set to zero (a kernel, not hipMemsetAsync, so that callers may capture it in a HIP graph)
 __global__ __launch_bounds__(256) void k_zero_u64(size_t n, unsigned long long* __restrict__ p) {
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) p[t] = 0ull;

@@ -211,6 +211,11 @@ class NNPDevice:
         """Level-split angular kernels (two eta groups over disjoint zeta levels) on or off."""
         self._check(self.lib.ace_set_level_split(self.ctx, 1 if enable else 0), "ace_set_level_split")
 
+    def set_force_gather(self, enable):
+        """Split energy/force passes: per-pair gradients summed by atom (True, default; from the next
+        neighbour build) or fixed-point atomics per pair (False)."""
+        self._check(self.lib.ace_set_force_gather(self.ctx, 1 if enable else 0), "ace_set_force_gather")
+
     def set_path(self, path, precision=PREC_FP32):
         """Fixes the energy/force path (a name of PATHS, or None for the size rule) for every batch."""
         k = -1 if path is None else self.PATHS.index(path)

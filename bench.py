@@ -97,7 +97,7 @@ def timed_steps(dev, prec, steps):
 
 
 KERNELS = ("nb_bin", "nb_search", "basis_fused", "basis_symm", "mlp_energy_grad", "mlp_energy_grad_bf16",
-           "basis_adjoint", "finalize")
+           "basis_adjoint", "force_gather", "finalize")
 
 
 def kernel_times(dev, prec, steps):

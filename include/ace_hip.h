@@ -35,6 +35,8 @@
  *   ace_tile_shape ................. (no counterpart) the shapes in use
  *   ace_set_tile_shape ............. (no counterpart) fixed tile shapes (chunk weight, network waves)
  *   ace_set_level_split ............ (no counterpart) level-split angular kernels on or off
+ *   ace_set_force_gather ........... (no counterpart) forces gathered from per-pair gradients, or
+ *                                    fixed-point atomics per pair
  *   ace_set_path ................... (no counterpart) fixed choice of the energy/force path
  *   ace_plain_energy_forces ........ same as ace_energy_forces, unbatched FP64 one-thread-per-atom
  *                                   GPU port (the baseline the batched path is measured against)
@@ -244,6 +246,12 @@ int  ace_set_tile_shape(ace_ctx* ctx, int precision, int chunk_w, int net_waves)
 /* (no counterpart) Enables (default) or disables the level-split kernels; both give the oracle's results
  * within the FP32 bounds, in a different rounding order. */
 int  ace_set_level_split(ace_ctx* ctx, int enable);
+/* (no counterpart) Forces of the split energy/force passes: 1 (default) stores each pair's gradient
+ * dE_i/dd at the key-row entry of its reverse pair (3 doubles per entry and 3 per atom, allocated by
+ * ace_build_neighbors) and sums each row in a fixed order; 0 adds them with 64-bit fixed-point atomics as
+ * the fused kernel and training do.  Both scatter as NNP::compute_force (nnp.cpp:442-471) and agree to
+ * ~1e-9 eV/A; each is deterministic.  Takes effect from the next ace_build_neighbors when turned on. */
+int  ace_set_force_gather(ace_ctx* ctx, int enable);
 /* Fixes the energy/force path of a precision (ACE_PATH_*) for every batch, the wave count of the
  * symmetry-function, adjoint and tangent kernels of training included (a fused path whose workgroup does
  * not fit in LDS runs split); -1 returns to the size rules.  The BF16 first-layer mode has the split

@@ -1022,14 +1022,20 @@ int run_force_gather(ace_ctx* ctx) {
 // virials land in the fixed-point accumulators.  The split adjoint writes each pair's gradient once and
 // the gather sums them by atom, instead of three fixed-point atomics per pair into memory.  The BF16 first
 // layer runs on the matrix cores of the network kernel, so that mode takes the split path.
+// the split adjoint and, with gathered forces, the gather: forces, energies and virials of the last dE/dG
+// rows in the fixed-point accumulators
+int run_adjoint(ace_ctx* ctx, int precision = -1) {
+    const bool gather = ctx->fgather && ctx->d_gin.p;
+    if (run_basis(ctx, 1, nullptr, nullptr, -1, precision, gather)) return -1;
+    return gather ? run_force_gather(ctx) : 0;
+}
+
 int run_ef(ace_ctx* ctx, int precision) {
     const int fw = fused_waves(ctx, precision);
     if (fw) return run_basis(ctx, 3, nullptr, nullptr, fw);
     if (run_basis(ctx, 0, nullptr, nullptr, -1, precision)) return -1;
     if (run_mlp(ctx, precision == ACE_PREC_BF16L1, precision)) return -1;
-    const bool gather = ctx->fgather && ctx->d_gin.p;
-    if (run_basis(ctx, 1, nullptr, nullptr, -1, precision, gather)) return -1;
-    return gather ? run_force_gather(ctx) : 0;
+    return run_adjoint(ctx, precision);
 }
 
 // one velocity-Verlet step: half kick with the forces of the last gradient launch, drift, energy/force

@@ -104,7 +104,7 @@ int train_launch(ace_ctx* ctx, const ace_loss_desc& L, bool wf, bool want_grad) 
     const int S = ctx->S;
     if (run_mlp(ctx, false)) return -1;
     if (wf) {
-        if (run_basis(ctx, 1)) return -1;
+        if (run_adjoint(ctx)) return -1;
         ++ctx->seq;
         const int b = 1 - ctx->fbuf;   // the accumulators the adjoint added into
         if (timed(ctx, "loss", [&] {
@@ -415,7 +415,7 @@ int ace_forward(ace_ctx* ctx, double* E_struc, double* F, double* virial) {
         if (run_basis(ctx, 0)) return -1;
         T.g_valid = true;
     }
-    if (run_mlp(ctx, false) || run_basis(ctx, 1) || run_finalize(ctx)) return -1;
+    if (run_mlp(ctx, false) || run_adjoint(ctx) || run_finalize(ctx)) return -1;
     if (E_struc) HC(hipMemcpyAsync(E_struc, ctx->d_E.p, ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (F) HC(hipMemcpyAsync(F, ctx->d_F.p, 3 * (size_t)ctx->N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (virial) HC(hipMemcpyAsync(virial, ctx->d_vir.p, 9 * ctx->S * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

@@ -1023,9 +1023,11 @@ int run_force_gather(ace_ctx* ctx) {
 // the gather sums them by atom, instead of three fixed-point atomics per pair into memory.  The BF16 first
 // layer runs on the matrix cores of the network kernel, so that mode takes the split path.
 // the split adjoint and, with gathered forces, the gather: forces, energies and virials of the last dE/dG
-// rows in the fixed-point accumulators
+// rows in the fixed-point accumulators.  The FP64-basis adjoint keeps the atomics: it holds fewer atoms
+// per CU (its LDS per atom is larger), too few to hide the reverse-entry searches of its epilogue (2.5 ->
+// 3.4 ms on the 108,000-atom cell with them).
 int run_adjoint(ace_ctx* ctx, int precision = -1) {
-    const bool gather = ctx->fgather && ctx->d_gin.p;
+    const bool gather = ctx->fgather && ctx->d_gin.p && !ctx->basis_f64;
     if (run_basis(ctx, 1, nullptr, nullptr, -1, precision, gather)) return -1;
     return gather ? run_force_gather(ctx) : 0;
 }
@@ -1345,19 +1347,36 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
             const double* Ri = I.Rinv;
             const double rown = std::sqrt(Ri[d] * Ri[d] + Ri[d + 3] * Ri[d + 3] + Ri[d + 6] * Ri[d + 6]);
             hgt[d] = 1.0 / rown;                                              // perpendicular height
-            // cells of about half the list radius: a 5-cell stencil per dimension (st = 2) scans (5/2)^3 rc^3 of
-            // candidates instead of 27 rc^3 or more with cells of one radius
-            I.nc[d] = std::max(1, std::min((int)std::floor(NB_CELL_DIV * hgt[d] / rsearch), 1024));
             if (I.cf[d] > 511) return fail(ctx, "ace_build_neighbors: image range too large for this cell");
         }
-        // the binning workgroup holds one counter per cell: coarsen the grid to NB_MAXC cells
-        while ((long long)I.nc[0] * I.nc[1] * I.nc[2] > NB_MAXC) {
-            int d = 0;
-            for (int e = 1; e < 3; ++e)
-                if (I.nc[e] > I.nc[d]) d = e;
-            I.nc[d] -= 1;
+        // Cells of about half the list radius: a 5-cell stencil per dimension (st = 2) scans (5/2)^3 rc^3 of
+        // candidates instead of 27 rc^3 with cells of one radius.  The binning workgroup holds one counter
+        // per cell, so a grid over NB_MAXC cells is coarsened; when that leaves the half-radius stencil more
+        // candidate volume, prod_d (2 st_d + 1) h_d / n_d, than cells of one radius, the latter are taken.
+        auto grid = [&](int div, int* nc, int* st) {
+            for (int d = 0; d < 3; ++d) nc[d] = std::max(1, std::min((int)std::floor(div * hgt[d] / rsearch), 1024));
+            while ((long long)nc[0] * nc[1] * nc[2] > NB_MAXC) {
+                int d = 0;
+                for (int e = 1; e < 3; ++e)
+                    if (nc[e] > nc[d]) d = e;
+                nc[d] -= 1;
+            }
+            double v = 1.0;
+            for (int d = 0; d < 3; ++d) {
+                st[d] = std::max(1, (int)std::ceil(rsearch * nc[d] / hgt[d]));
+                v *= (2 * st[d] + 1) * hgt[d] / nc[d];
+            }
+            return v;
+        };
+        {
+            int nc1[3], st1[3];
+            const double v2 = grid(NB_CELL_DIV, I.nc, I.st), v1 = grid(1, nc1, st1);
+            if (v1 < v2)
+                for (int d = 0; d < 3; ++d) {
+                    I.nc[d] = nc1[d];
+                    I.st[d] = st1[d];
+                }
         }
-        for (int d = 0; d < 3; ++d) I.st[d] = std::max(1, (int)std::ceil(rsearch * I.nc[d] / hgt[d]));
         // every structure small and with heights over twice the list radius: search without binning
         if (ctx->natoms[s] > NB_BRUTE_MAX || !(hgt[0] > 2 * rsearch && hgt[1] > 2 * rsearch && hgt[2] > 2 * rsearch))
             brute = false;

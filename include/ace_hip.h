@@ -250,7 +250,8 @@ int  ace_set_level_split(ace_ctx* ctx, int enable);
  * dE_i/dd at the key-row entry of its reverse pair (3 doubles per entry and 3 per atom, allocated by
  * ace_build_neighbors) and sums each row in a fixed order; 0 adds them with 64-bit fixed-point atomics as
  * the fused kernel and training do.  Both scatter as NNP::compute_force (nnp.cpp:442-471) and agree to
- * ~1e-9 eV/A; each is deterministic.  Takes effect from the next ace_build_neighbors when turned on. */
+ * ~1e-9 eV/A; each is deterministic.  Takes effect from the next ace_build_neighbors when turned on.  The
+ * FP64-basis mode always uses the atomics. */
 int  ace_set_force_gather(ace_ctx* ctx, int enable);
 /* Fixes the energy/force path of a precision (ACE_PATH_*) for every batch, the wave count of the
  * symmetry-function, adjoint and tangent kernels of training included (a fused path whose workgroup does

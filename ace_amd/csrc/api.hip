@@ -128,6 +128,9 @@ struct ace_ctx {
     DBuf<long long> d_nb_keys; // per-atom rows of sorted neighbour keys
     DBuf<double> d_gin;        // gathered energy/force pass: per-entry gradients, rows like the keys
     DBuf<double> d_fown;       // and each atom's own share
+    DBuf<int> d_rev;           // reverse entries of the key-row entries (written by the symmetry pass)
+    long long nb_builds = 0;   // neighbour builds so far
+    long long rev_build = -1;  // the build whose reverse entries d_rev holds
     long long npairs = 0;       // pairs of the current list (set when the CSR arrays are built)
     bool csr_valid = false;     // CSR arrays (export, unbatched port) match the key rows
     int slots_cap = 32;         // neighbour slots per atom held in LDS by the basis kernels
@@ -957,11 +960,16 @@ int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = 
     io.F_nxt = ctx->d_Ffix[1 - b].p; io.E_nxt = ctx->d_Efix[1 - b].p; io.W_nxt = ctx->d_Wfix[1 - b].p;
     io.gin = gather ? ctx->d_gin.p : nullptr;
     io.fown = gather ? ctx->d_fown.p : nullptr;
+    io.rev = nullptr;
+    const bool rev_ok = ctx->d_rev.p && ctx->d_rev.cap >= (size_t)ctx->N * ctx->slots_cap;
+    if (mode == 0 && ctx->fgather && rev_ok && !f64) io.rev = ctx->d_rev.p;
+    if (mode == 1 && gather && rev_ok && ctx->rev_build == ctx->nb_builds) io.rev = ctx->d_rev.p;
     if (timed(ctx, names[f64][mode], [&] {
             f<<<ctx->N, 64 * waves, bytes, ctx->stream>>>(ctx->N, ctx->d_model.p, P, io, nbm, ctx->nt, ctx->kmax);
         }))
         return -1;
     if (mode == 1 || mode == 3) ctx->fbuf = 1 - b;   // the next gradient launch adds into the zeroed buffer
+    if (mode == 0 && io.rev) ctx->rev_build = ctx->nb_builds;
     return 0;
 }
 
@@ -1164,7 +1172,7 @@ void ace_destroy(ace_ctx* ctx) {
     for (int b = 0; b < 2; ++b) { release(ctx->d_Ffix[b]); release(ctx->d_Efix[b]); release(ctx->d_Wfix[b]); }
     release(ctx->d_sinfo); release(ctx->d_cell_of); release(ctx->d_wrap_of); release(ctx->d_slot_of);
     release(ctx->d_cell_start); release(ctx->d_cpos); release(ctx->d_cinfo); release(ctx->d_nb_keys);
-    release(ctx->d_gin); release(ctx->d_fown);
+    release(ctx->d_gin); release(ctx->d_fown); release(ctx->d_rev);
     release(ctx->d_nb_count);
     release(ctx->d_nb_off); release(ctx->d_scan_blk); release(ctx->d_scal); release(ctx->d_nb_j);
     release(ctx->d_nb_img); release(ctx->d_G64);
@@ -1441,6 +1449,8 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
     if (ensure(ctx, ctx->d_nb_keys, (size_t)N * ctx->slots_cap)) return -1;
     if (ctx->fgather && (ensure(ctx, ctx->d_gin, 3 * (size_t)N * ctx->slots_cap) || ensure(ctx, ctx->d_fown, 3 * (size_t)N)))
         return -1;
+    if (ctx->fgather && (long long)N * ctx->slots_cap < (1LL << 31) && ensure(ctx, ctx->d_rev, (size_t)N * ctx->slots_cap))
+        return -1;
     if (timed(ctx, "nb_search", [&] {
             if (brute)
                 k_nb_brute<1><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
@@ -1457,6 +1467,7 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
     ctx->nb_over_idx = brute ? p_over : 5;
     if (brute) ctx->nb_parity ^= 1;
     ctx->nb_valid = true;
+    ++ctx->nb_builds;
     ctx->nb_pending = true;
     ctx->csr_valid = false;
     if (!async) {
@@ -1917,6 +1928,7 @@ int ace_set_force_gather(ace_ctx* ctx, int enable) {
     if (!ctx->fgather) {
         release(ctx->d_gin);
         release(ctx->d_fown);
+        release(ctx->d_rev);
         ++ctx->gen;   // graphs captured with the gather are stale
     }
     return 0;

@@ -931,6 +931,9 @@ struct BasisIO {
     // sums each row
     double* gin;
     double* fown;
+    // reverse entries (j cap + p) of the key-row entries (i, e): written by the symmetry-function pass, read
+    // by the gradient pass (nullptr: not written / searched in the gradient pass)
+    int* rev;
 };
 
 
@@ -1018,6 +1021,14 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
         if (ADJ)
             for (int k = lane; k < gsp.K; k += 64) L.row[k] = (double)io.dEdG[(long long)row * io.ldg + k];
     }
+    // the symmetry-function pass of a gathered energy/force pass finds, for each entry of the row, the entry
+    // of its reverse pair in the neighbour's row (where the gradient pass writes); wave 1 searches while
+    // wave 0 buckets
+    if (MODE == 0 && wv == 1 && io.rev && nn <= io.cap) {
+        const int a0s = io.sinfo[st].atom_off;
+        for (int e = lane; e < nn; e += 64)
+            io.rev[(long long)i * io.cap + e] = (int)reverse_entry(io.keys, io.cap, io.counts, krow[e], i - a0s, a0s);
+    }
     if (wv == W_ANG) {
         const int nA = gsp.K - gsp.nR, a0 = gsp.a_poff[0];
         for (int q = lane; q < nA; q += 64) L.ci[q] = P.a_cidx[a0 + q];
@@ -1073,7 +1084,8 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
         const bool pos_e = GACC && io.gin != nullptr;
         auto place = [&](int e, int j, int Jl, double dx, double dy, double dz, double r) {
             if (pos_e && Jl < 0 && e < nn) {
-                const long long t = reverse_entry(io.keys, io.cap, io.counts, krow[e], i - S.atom_off, S.atom_off);
+                const long long t = io.rev ? (long long)io.rev[(long long)i * io.cap + e]
+                                           : reverse_entry(io.keys, io.cap, io.counts, krow[e], i - S.atom_off, S.atom_off);
                 if (t >= 0) {
                     double* gp = io.gin + 3 * t;
                     gp[0] = 0.0;
@@ -1290,7 +1302,8 @@ __global__ __launch_bounds__(BW * 64) void k_basis(int natoms, const DevModel* _
                 g3[2] += (double)L.g[ww * 3 * nbm + 3 * q + 2];
             }
             if (io.gin) {
-                const long long t = reverse_entry(io.keys, io.cap, io.counts, krow[L.p[q]], i - a0i, a0i);
+                const long long t = io.rev ? (long long)io.rev[(long long)i * io.cap + L.p[q]]
+                                           : reverse_entry(io.keys, io.cap, io.counts, krow[L.p[q]], i - a0i, a0i);
                 if (t >= 0) {
                     double* gp = io.gin + 3 * t;
                     gp[0] = g3[0];

@@ -316,7 +316,7 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
     __shared__ int s_cstart[NB_SC];
     __shared__ int s_cpref[NB_SC + 1];
     __shared__ int s_cimg[NB_SC];
-    __shared__ unsigned char s_slot[NB_MAP];
+    __shared__ __align__(8) unsigned char s_slot[NB_MAP];   // after the search: the j_local bitmap
     __shared__ double4 s_rn[27];           // R n for n in {-1, 0, 1}^3, index (n0 + 1) + 3 (n1 + 1) + 9 (n2 + 1)
     __shared__ int s_n;
     extern __shared__ long long s_key[];   // MODE 1: cap keys (dynamic LDS)
@@ -491,10 +491,52 @@ __global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
         if (tid == 0) atomicMax(over, 1);
         return;
     }
-    // rank sort into the reference order (the keys of one atom are distinct): with 32-bit keys when they
-    // hold j_local and the image offsets and the row fits four keys per thread, else with the full keys
     long long* row = keys + (long long)i * cap;
     constexpr int BT = NB_SEARCH_WAVES * 64;
+    // Rows whose neighbours are distinct atoms (one image each: every cell height over twice the radius does
+    // it) of a structure of at most 8 NB_MAP atoms: the rank of key (j, n) is the number of neighbours with a
+    // smaller j, read from a bitmap of j_local (in the map's LDS, free after the search) and an exclusive
+    // prefix of its words' popcounts (in the prefix's LDS).  A repeated j sends the row to the rank sort.
+    if (S.natoms <= 8 * NB_MAP) {
+        unsigned long long* bits = reinterpret_cast<unsigned long long*>(s_slot);
+        int* wpre = s_cpref;
+        const int nw = (S.natoms + 63) >> 6;
+        __syncthreads();
+        for (int w = tid; w < nw; w += BT) bits[w] = 0ull;
+        if (tid == 0) s_n = 0;
+        __syncthreads();
+        for (int e = tid; e < count; e += BT) {
+            const int jl = (int)(s_key[e] >> 30);
+            const unsigned long long bit = 1ull << (jl & 63);
+            if (atomicOr(&bits[jl >> 6], bit) & bit) s_n = 1;   // benign race: every writer stores 1
+        }
+        __syncthreads();
+        if (s_n == 0) {
+            if (wv == 0) {
+                // exclusive prefix of the popcounts, two words per lane (nw <= 2 * 64)
+                const int c0 = 2 * lane < nw ? __popcll(bits[2 * lane]) : 0;
+                const int c1 = 2 * lane + 1 < nw ? __popcll(bits[2 * lane + 1]) : 0;
+                int pre = c0 + c1;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t = __shfl_up(pre, o, 64);
+                    if (lane >= o) pre += t;
+                }
+                wpre[2 * lane] = pre - c0 - c1;
+                wpre[2 * lane + 1] = pre - c1;
+            }
+            __syncthreads();
+            for (int e = tid; e < count; e += BT) {
+                const long long k = s_key[e];
+                const int jl = (int)(k >> 30), w = jl >> 6;
+                const int r = wpre[w] + __popcll(bits[w] & ((1ull << (jl & 63)) - 1ull));
+                __builtin_nontemporal_store(k, &row[r]);   // read once, by the next kernel
+            }
+            return;
+        }
+    }
+    // rank sort into the reference order (the keys of one atom are distinct): with 32-bit keys when they
+    // hold j_local and the image offsets and the row fits four keys per thread, else with the full keys
     const int m = max(S.cf[0], max(S.cf[1], S.cf[2]));
     const int b = 32 - __clz(2 * m);                  // bits of 0 .. 2 m
     const int nk = (count + BT - 1) / BT;

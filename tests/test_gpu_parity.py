@@ -148,7 +148,8 @@ def test_large_triclinic_cells_binning_free_search():
 
 def test_neighbour_rows_of_every_length():
     """Binned search (a cell shorter than 2 rc in the batch) over densities whose rows hold 22 to 362
-    neighbours: one to four sort keys per lane with the packed 32-bit keys, and the full-key rank sort
+    neighbours: the 14 A cells (one image per neighbour) ranked from the j bitmap, the 7.4 A cell (several
+    images of a neighbour) by the rank sort with one to four packed 32-bit keys per lane, or the full keys
     beyond 256; pairs, images and order bit for bit."""
     rng = np.random.default_rng(41)
     m = configs.c3_model(rng)

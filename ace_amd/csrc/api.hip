@@ -1453,7 +1453,7 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
         return -1;
     if (timed(ctx, "nb_search", [&] {
             if (brute)
-                k_nb_brute<1><<<N, NB_WAVES * 64, 0, ctx->stream>>>(
+                k_nb_brute<1><<<N, NB_WAVES * 64, ctx->slots_cap * sizeof(long long), ctx->stream>>>(
                     N, rc2, ctx->d_pos.p, ctx->d_struc_of.p, ctx->d_sinfo.p, ctx->d_nb_count.p, ctx->d_scal.p,
                     ctx->d_nb_keys.p, ctx->slots_cap, ctx->d_scal.p + p_over, ctx->d_scal.p + p_next,
                     ctx->d_scal.p + 3);

@@ -613,7 +613,7 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_brute(
         base = __shfl(base, 0, 64);
         if (MODE == 1 && acc) {
             const int r = base + __popcll(m & lane_mask_lt());
-            if (r < NB_LMAX) s_key[r] = key;
+            if (r < cap) s_key[r] = key;
         }
     }
     __syncthreads();
@@ -627,13 +627,36 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nb_brute(
         if (tid == 0) atomicMax(over, 1);
         return;
     }
-    // rank sort into the reference order (the keys of one atom are distinct)
+    // At most one image of each neighbour lies within the radius here, so the rank of key (j, n) in the
+    // reference order is the number of neighbours with a smaller j: set bits below j in a bitmap of j_local
+    // plus the exclusive prefix of the words' popcounts.
+    __shared__ unsigned long long s_bits[NB_BRUTE_MAX / 64];
+    __shared__ int s_wpre[NB_BRUTE_MAX / 64];
+    constexpr int BT = NB_WAVES * 64;
     long long* row = keys + (long long)i * cap;
-    for (int e = tid; e < count; e += NB_WAVES * 64) {
+    const int nw = (na + 63) >> 6;
+    for (int w = tid; w < nw; w += BT) s_bits[w] = 0ull;
+    __syncthreads();
+    for (int e = tid; e < count; e += BT) {
+        const int jl = (int)(s_key[e] >> 30);
+        atomicOr(&s_bits[jl >> 6], 1ull << (jl & 63));
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const int c = tid < nw ? __popcll(s_bits[tid]) : 0;
+        int pre = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(pre, o, 64);
+            if (lane >= o) pre += t;
+        }
+        s_wpre[tid] = pre - c;
+    }
+    __syncthreads();
+    for (int e = tid; e < count; e += BT) {
         const long long k = s_key[e];
-        int rank = 0;
-        for (int f = 0; f < count; ++f) rank += (s_key[f] < k);
-        __builtin_nontemporal_store(k, &row[rank]);
+        const int jl = (int)(k >> 30), w = jl >> 6;
+        __builtin_nontemporal_store(k, &row[s_wpre[w] + __popcll(s_bits[w] & ((1ull << (jl & 63)) - 1ull))]);
     }
 }
 

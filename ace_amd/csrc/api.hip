@@ -1442,9 +1442,11 @@ int ace_build_neighbors(ace_ctx* ctx, double skin) {
         HC(hipMemcpyAsync(&mx, ctx->d_scal.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HC(hipStreamSynchronize(ctx->stream));
         if (mx > NB_LMAX) return fail(ctx, "ace_build_neighbors: an atom has more than 1024 neighbours");
-        // the largest count plus a margin of 16 for later asynchronous rebuilds, rounded up to 8: the basis
-        // kernels' LDS is sized by it, and every slot less keeps more atoms in flight per CU
-        ctx->slots_cap = std::min(NB_LMAX, ((mx + 16 + 7) & ~7));
+        // the largest count plus a margin of 8 for later asynchronous rebuilds, rounded up to 8: the basis
+        // kernels' LDS is sized by it, and every slot less keeps more atoms in flight per CU (the 568k-atom
+        // batch's adjoint holds 11 atoms per CU at 136 slots, 10 at 144); a row that outgrows it costs a
+        // counting rebuild
+        ctx->slots_cap = std::min(NB_LMAX, ((mx + 8 + 7) & ~7));
     }
     if (ensure(ctx, ctx->d_nb_keys, (size_t)N * ctx->slots_cap)) return -1;
     if (ctx->fgather && (ensure(ctx, ctx->d_gin, 3 * (size_t)N * ctx->slots_cap) || ensure(ctx, ctx->d_fown, 3 * (size_t)N)))

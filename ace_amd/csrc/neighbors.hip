@@ -307,7 +307,8 @@ __device__ inline void nb_rank_sort32(long long* s_key, int count, int tid, int 
 // (j_local << 30 | image order code: the reference order) and write them to row i of keys (cap slots); an
 // atom with more than cap neighbours raises *over and writes none.
 template <int MODE>
-__global__ __launch_bounds__(NB_SEARCH_WAVES * 64) void k_nb_search(
+// 8 waves per SIMD (its SGPRs alone would allow 7; a few spill to VGPR lanes)
+__global__ __launch_bounds__(NB_SEARCH_WAVES * 64) __attribute__((amdgpu_waves_per_eu(8))) void k_nb_search(
     int natoms, double rc2, const double* __restrict__ pos, const int* __restrict__ struc_of,
     const StrucInfo* __restrict__ sinfo, const int* __restrict__ cell_of, const int* __restrict__ wrap_of,
     const int* __restrict__ cell_start, const double4* __restrict__ cpos, const int2* __restrict__ cinfo,

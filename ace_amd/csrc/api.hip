@@ -952,6 +952,7 @@ int run_basis(ace_ctx* ctx, int mode, float* out = nullptr, const double* eps = 
     io.G = rows;
     io.dEdG = ctx->d_dEdG.p;
     io.err = err;
+    io.lds_bytes = bytes;
     io.eps = eps;
     io.wpad = ctx->d_wpad.p;
     io.e_atom = ctx->d_e_atom.p;

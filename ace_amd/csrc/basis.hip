@@ -51,11 +51,11 @@ __host__ __device__ inline int basis_nred(int nt, int ng, int mode) {
 
 // byte offset of the T area: the FP64 arrays, rounded up to 16
 template <int BW>
-__host__ __device__ inline int basis_t_off(int nbm, int ng, int kmax, int nrp, int mode, int nt) {
+__host__ __device__ inline int basis_t_off(int nbm, int ng, int kmax, int nrp, int mode, int nt, bool rowf) {
     const int kp = (kmax + 3) & ~3;
     const bool gacc = (mode == 1 || mode == 3);
     const int gb = gacc ? 8 * BW * 3 * nbm : 0;
-    const int nd = kp + basis_nred<BW>(nt, ng, mode) + 2 * nrp + (mode == 3 ? 2 * kp : 0);
+    const int nd = (rowf ? kp / 2 : kp) + basis_nred<BW>(nt, ng, mode) + 2 * nrp + (mode == 3 ? 2 * kp : 0);
     return (gb + 8 * nd + 15) & ~15;
 }
 
@@ -64,18 +64,30 @@ __host__ __device__ inline int basis_t_off(int nbm, int ng, int kmax, int nrp, i
 // FP64 passes from the bucketing to that pass through wave 1's accumulators, which are cleared afterwards.
 __host__ __device__ inline bool basis_lean(int mode, bool gen) { return mode == 1 && !gen; }
 
+// The lean FP32 adjoint keeps its dE/dG row in FP32 (its source is FP32); the others in FP64.
+__host__ __device__ inline bool basis_rowf(int mode, bool gen, int tsize) { return basis_lean(mode, gen) && tsize == 4; }
+
+// Elements of the angular-factor table E: 2 ng per slot, and at least the epilogue's BW x 16 FP64 partial
+// sums, which reuse it once the sums are done.
+template <int BW>
+__host__ __device__ inline int basis_ne(int nbm, int ng, int tsize) {
+    const int n = 2 * ng * nbm, m = BW * 16 * 8 / tsize;
+    return n > m ? n : m;
+}
+
 template <class T, int BW>
 __host__ __device__ inline int basis_lds_bytes(int nbm, int ng, int kmax, int nrp, int mode, int netf, int nt, bool gen) {
     const int kp = (kmax + 3) & ~3;
     const bool gacc = (mode == 1 || mode == 3);
     const bool lean = basis_lean(mode, gen);
-    int n = 4 * nbm + 2 * ng * nbm + (lean ? 0 : 3 * nbm);
+    int n = 4 * nbm + basis_ne<BW>(nbm, ng, (int)sizeof(T)) + (lean ? 0 : 3 * nbm);
     if (mode == 2) n += 4 * nbm;
     if (gacc && !lean) n += nbm;
     if (gen) n += 2 * nbm;
     const int nf = (sizeof(T) == 4 && !lean) ? nbm : 0;
-    const int base = (basis_t_off<BW>(nbm, ng, kmax, nrp, mode, nt) + (int)sizeof(T) * n + 4 * (nbm + kp) + 4 * nf +
-                      15) & ~15;
+    // slot records p (the lean layout keeps a 16-bit row position) and the canonical slot table ci (bytes)
+    const int base = (basis_t_off<BW>(nbm, ng, kmax, nrp, mode, nt, basis_rowf(mode, gen, (int)sizeof(T))) +
+                      (int)sizeof(T) * n + (lean ? 2 : 4) * nbm + kp + 4 * nf + 15) & ~15;
     return base + (mode == 3 ? 4 * netf : 0);
 }
 
@@ -91,12 +103,15 @@ struct BasisLds {
     T* fA;     // kernels with the generic angular path: angular cutoff of each slot and its r derivative
     T* gA;
     double* g; // [BW][3 nbm] per-wave dE/dd accumulators (one array per wave: a fixed order of adds)
+    double* sred;  // [BW][16] the epilogue's partial sums (in E's area, free by then)
+    float* rowf;   // the lean FP32 adjoint's dE/dG row
     double* row;
     double* red;   // [BW][64] partial sums; adjoint: canonical dE/dG weights [block][14 ng] as FP32
     double* rp;    // radial eta [nR], rs [nR] of the centre species
     double* sc;    // mode 3: network input scaling inpw [K], inpb [K]
     int* p;
-    int* ci;   // canonical slot of each angular function of the species
+    unsigned short* pe;   // lean layout: row position of the slot's pair
+    unsigned char* ci;    // canonical slot of each angular function of the species
     float* rlo;    // T = float: |d| - (float)|d|, so the radial functions see |d| to FP64 precision
     float* nw; // mode 3: the species' network parameters (matrix-core layout), then activations
 };
@@ -106,15 +121,20 @@ __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, i
     BasisLds<T> L;
     const int kp = (kmax + 3) & ~3;
     const bool gacc = (mode == 1 || mode == 3);
+    const bool rowf = basis_rowf(mode, gen, (int)sizeof(T));
     L.g = (double*)base;
     double* d = (double*)((char*)base + (gacc ? 8 * BW * 3 * nbm : 0));
-    L.row = d; d += kp;
+    L.row = d;
+    L.rowf = (float*)d;
+    d += rowf ? kp / 2 : kp;
     L.red = d; d += basis_nred<BW>(nt, ng, mode);
     L.rp = d; d += 2 * nrp;
     L.sc = d;
-    T* c = (T*)((char*)base + basis_t_off<BW>(nbm, ng, kmax, nrp, mode, nt));
+    T* c = (T*)((char*)base + basis_t_off<BW>(nbm, ng, kmax, nrp, mode, nt, rowf));
     L.u = c; c += 4 * nbm;
-    L.E = c; c += 2 * ng * nbm;
+    L.E = c;
+    L.sred = (double*)c;
+    c += basis_ne<BW>(nbm, ng, (int)sizeof(T));
     L.te = c; if (mode == 2) c += 4 * nbm;
     const bool lean = basis_lean(mode, gen);
     L.fR = c; if (!lean) c += nbm;
@@ -124,7 +144,8 @@ __device__ inline BasisLds<T> basis_lds(void* base, int nbm, int ng, int kmax, i
     L.fA = c; if (gen) c += nbm;
     L.gA = c; if (gen) c += nbm;
     L.p = (int*)c;
-    L.ci = L.p + nbm;
+    L.pe = (unsigned short*)c;
+    L.ci = (unsigned char*)c + (lean ? 2 : 4) * nbm;
     L.rlo = (float*)(L.ci + kp);
     float* e = L.rlo + ((sizeof(T) == 4 && !lean) ? nbm : 0);
     const long long end = ((long long)((char*)e - (char*)base) + 15) & ~15ll;
@@ -273,7 +294,8 @@ struct Bands {
 // The adjoint (mode 1) adds each pair's derivative for v into the wave's FP64 accumulators.
 template <class T, int NG, int NZ, int MODE, int SPL>
 __device__ inline void angular_canon(const BasisLds<T>& L, int bJ, int bK, int nJ, int nK, bool same,
-                                     const int* __restrict__ cidx, const float* wtab, int nA, int goff, double* gw,
+                                     const unsigned char* __restrict__ cidx, const float* wtab, int nA, int goff,
+                                     double* gw,
                                      int wv, int lane, int rs_f, bool rs_writer, int chunk_w) {
     using B = Bands<NG, NZ, SPL>;
     constexpr int ADJ = (MODE == 1);
@@ -495,6 +517,7 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
     constexpr int TAN = (PM == 2);
     constexpr int MAXA = 16;   // functions per register chunk of the generic path
     static_assert(!LEAN || (ADJ && !GEN), "the lean layout is the split adjoint's without the generic path");
+    constexpr bool ROWF = LEAN && sizeof(T) == 4;   // basis_rowf: FP32 dE/dG row
     const int ng = sp.ng;
     if (LEAN && wv > 0) {
         // waves 1.. clear their own accumulators (wave 1's held |d| until the per-slot pass); the barriers
@@ -594,7 +617,8 @@ __device__ inline void basis_sums(const BasisLds<T>& L, const DevSpecies& sp, co
         for (int b = 0; b < nab; ++b)
             if (!GEN || sp.a_canon[b]) {
                 const int o = sp.a_poff[b] - sp.a_poff[0];
-                for (int m = tid; m < sp.a_n[b]; m += BT) wt[14 * NG * b + L.ci[o + m]] = (float)L.row[sp.a_goff[b] + m];
+                for (int m = tid; m < sp.a_n[b]; m += BT)
+                    wt[14 * NG * b + L.ci[o + m]] = ROWF ? L.rowf[sp.a_goff[b] + m] : (float)L.row[sp.a_goff[b] + m];
             }
         __syncthreads();
     }
@@ -920,6 +944,7 @@ struct BasisIO {
     float* G;
     const float* dEdG;
     int* err;
+    int lds_bytes;      // the launch's dynamic LDS: a layout that would not fit is refused
     const double* eps;
     const float* wpad;
     double* e_atom;
@@ -973,6 +998,7 @@ __global__ __launch_bounds__(BW * 64) __attribute__((amdgpu_waves_per_eu(
     constexpr int FUSED = (MODE == 3);        // symmetry functions, network and adjoint in one pass
     constexpr int GACC = (ADJ || FUSED);      // per-slot gradient accumulators
     constexpr bool LEAN = (MODE == 1 && !GEN);   // basis_lean: no per-slot radial arrays
+    constexpr bool ROWF = LEAN && sizeof(T) == 4;  // basis_rowf: FP32 dE/dG row
     constexpr bool F64 = sizeof(T) == 8;
     constexpr int BT = BW * 64;
     // the waves that stage the radial and the angular tables (wave 1 copies the species record)
@@ -980,7 +1006,6 @@ __global__ __launch_bounds__(BW * 64) __attribute__((amdgpu_waves_per_eu(
     static_assert(BW >= 2 && BW <= 4, "two to four waves per atom");
     extern __shared__ double4 basis_smem[];
     __shared__ int s_cnt[ACE_MAXT + 1];
-    __shared__ double s_red[BW][16];
     __shared__ DevSpecies s_sp;   // the fused kernel's copy of the species record (LDS does not limit it)
     const int tid = threadIdx.x, wv = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int i = blockIdx.x;
@@ -997,6 +1022,10 @@ __global__ __launch_bounds__(BW * 64) __attribute__((amdgpu_waves_per_eu(
     const int I = io.type[i];
     const DevSpecies& gsp = M->sp[I];
     const BasisLds<T> L = basis_lds<T, BW>(basis_smem, nbm, NG, kmax, io.nrp, MODE, nt, GEN != 0);
+    if ((long long)((char*)L.nw - (char*)basis_smem) > io.lds_bytes) {
+        if (tid == 0) atomicAdd(io.err, 1);
+        return;
+    }
     const int nn = io.counts[i];
     const long long* krow = io.keys + (long long)i * io.cap;
     const float rcut = gsp.rc_max;
@@ -1023,7 +1052,11 @@ __global__ __launch_bounds__(BW * 64) __attribute__((amdgpu_waves_per_eu(
             L.rp[nR + q] = P.r_rs64[r0 + q];
         }
         if (ADJ)
-            for (int k = lane; k < gsp.K; k += 64) L.row[k] = (double)io.dEdG[(long long)row * io.ldg + k];
+            for (int k = lane; k < gsp.K; k += 64) {
+                const float v = io.dEdG[(long long)row * io.ldg + k];
+                if (ROWF) L.rowf[k] = v;
+                else L.row[k] = (double)v;
+            }
     }
     // the symmetry-function pass of a gathered energy/force pass finds, for each entry of the row, the entry
     // of its reverse pair in the neighbour's row (where the gradient pass writes); wave 1 searches while
@@ -1035,7 +1068,7 @@ __global__ __launch_bounds__(BW * 64) __attribute__((amdgpu_waves_per_eu(
     }
     if (wv == W_ANG) {
         const int nA = gsp.K - gsp.nR, a0 = gsp.a_poff[0];
-        for (int q = lane; q < nA; q += 64) L.ci[q] = P.a_cidx[a0 + q];
+        for (int q = lane; q < nA; q += 64) L.ci[q] = (unsigned char)P.a_cidx[a0 + q];
         if (FUSED)
             for (int k = lane; k < gsp.K; k += 64) {
                 L.sc[k] = P.inpw64[gsp.in_off + k];
@@ -1107,7 +1140,8 @@ __global__ __launch_bounds__(BW * 64) __attribute__((amdgpu_waves_per_eu(
                         // unit vector, |d| and 1/|d| from the FP64 displacement, each rounded once
                         const int q = fill[J] + __popcll(m & lane_mask_lt());
                         const double ir = 1.0 / r;
-                        L.p[q] = pos_e ? e : j;
+                        if (LEAN) L.pe[q] = (unsigned short)e;   // e < cap <= 1024
+                        else L.p[q] = pos_e ? e : j;
                         L.u[4 * q] = (T)(dx * ir);
                         L.u[4 * q + 1] = (T)(dy * ir);
                         L.u[4 * q + 2] = (T)(dz * ir);
@@ -1192,13 +1226,13 @@ __global__ __launch_bounds__(BW * 64) __attribute__((amdgpu_waves_per_eu(
                 for (int n = 0; n < nR; ++n) {
                     T v, dv;
                     t_radial_r64<T>(ACE_RAD_GAUSSIAN, L.rp[rq0 + n], L.rp[sp.nR + rq0 + n], r, fc, gc, v, dv);
-                    acc += (T)L.row[goff + n] * dv;
+                    acc += (T)(ROWF ? L.rowf[goff + n] : L.row[goff + n]) * dv;
                 }
             else
                 for (int n = 0; n < nR; ++n) {
                     T v, dv;
                     t_radial_r64<T>(form, L.rp[rq0 + n], L.rp[sp.nR + rq0 + n], r, fc, gc, v, dv);
-                    acc += (T)L.row[goff + n] * dv;
+                    acc += (T)(ROWF ? L.rowf[goff + n] : L.row[goff + n]) * dv;
                 }
             if (sp.a_cut != sp.r_cut[J] || sp.a_rc64 != sp.r_rc64[J]) t_cutoff_r64<T>(sp.a_cut, sp.a_rc64, r, fc, gc);
             const T fa = fc, ga = gc, rt = (T)r;
@@ -1306,8 +1340,9 @@ __global__ __launch_bounds__(BW * 64) __attribute__((amdgpu_waves_per_eu(
                 g3[2] += (double)L.g[ww * 3 * nbm + 3 * q + 2];
             }
             if (io.gin) {
-                const long long t = io.rev ? (long long)io.rev[(long long)i * io.cap + L.p[q]]
-                                           : reverse_entry(io.keys, io.cap, io.counts, krow[L.p[q]], i - a0i, a0i);
+                const int e = LEAN ? (int)L.pe[q] : L.p[q];
+                const long long t = io.rev ? (long long)io.rev[(long long)i * io.cap + e]
+                                           : reverse_entry(io.keys, io.cap, io.counts, krow[e], i - a0i, a0i);
                 if (t >= 0) {
                     double* gp = io.gin + 3 * t;
                     gp[0] = g3[0];
@@ -1317,7 +1352,8 @@ __global__ __launch_bounds__(BW * 64) __attribute__((amdgpu_waves_per_eu(
                     atomicAdd(io.err, 1);
                 }
             } else {
-                const int j = L.p[q];
+                // the lean layout keeps the row position: j from the key
+                const int j = LEAN ? (int)(krow[L.pe[q]] >> 30) + a0i : L.p[q];
                 atomicAdd(&io.F_fix[3 * j], fix_of(g3[0]));
                 atomicAdd(&io.F_fix[3 * j + 1], fix_of(g3[1]));
                 atomicAdd(&io.F_fix[3 * j + 2], fix_of(g3[2]));
@@ -1340,12 +1376,12 @@ __global__ __launch_bounds__(BW * 64) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
         for (int q4 = 12; q4 < 16; ++q4) v[q4] = 0.0;
         const double tot = wave_reduce_scatter<4, double>(v, lane);
-        if ((lane & 3) == 0) s_red[wv][lane >> 2] = tot;
+        if ((lane & 3) == 0) L.sred[16 * wv + (lane >> 2)] = tot;
         __syncthreads();
         if (tid < 13) {
             double a = 0.0;
             if (tid < 12) {
-                for (int ww = 0; ww < BW; ++ww) a += s_red[ww][tid];
+                for (int ww = 0; ww < BW; ++ww) a += L.sred[16 * ww + tid];
             } else {
                 a = io.e_atom[i];
             }

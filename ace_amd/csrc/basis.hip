@@ -30,13 +30,15 @@
 // functions of a species):
 //   g[BW 3 nbm] (FP64, modes 1 and 3: per-wave dE/dd accumulators; LDS FP64 atomic adds run several
 //         times faster than FP32 ones on gfx950);
-//   FP64: row[kp] (G sums, then dE/dG), red[nred], rp[2 nrp] (radial eta, rs of the species),
-//         sc[2 kp] (mode 3: inpw, inpb);
+//   FP64: row[kp] (G sums, then dE/dG; FP32 in the lean FP32 adjoint), red[nred], rp[2 nrp] (radial eta,
+//         rs of the species), sc[2 kp] (mode 3: inpw, inpb);
 //   T (16-byte aligned): u[4 nbm] (unit vector, 1/|d|), E[nbm][2 NG] (the angular factors E and their
-//      r-derivatives Ed of a slot side by side), te[4 nbm] (mode 2), fR[nbm], gR[nbm], r[nbm],
-//      fr[nbm] (modes 1 and 3), fA[nbm], gA[nbm] (generic angular path: angular cutoff and derivative);
-//   int: p[nbm], ci[kp];  float: rlo[nbm] (T = float);  mode 3 appends the network area (netf floats:
-//   staged weights, activations, partial sums), 16-byte aligned.
+//      r-derivatives Ed of a slot side by side; at least BW x 16 doubles, which the gradient epilogue
+//      reuses for its partial sums), te[4 nbm] (mode 2), fR[nbm], gR[nbm], r[nbm], fr[nbm] (modes 1 and
+//      3), fA[nbm], gA[nbm] (generic angular path: angular cutoff and derivative);
+//   p[nbm] (int; 16-bit row positions in the lean layout), ci[kp] (bytes);  float: rlo[nbm] (T = float);
+//   mode 3 appends the network area (netf floats: staged weights, activations, partial sums), 16-byte
+//   aligned.
 // The triplet loops read a slot's geometry and its angular factors as whole 16-byte records.
 // red holds the per-wave partial sums (BW 64, FP64) and, in the adjoint, the dE/dG weights of every
 // canonical angular block in (sign, level, group) order (14 ng FP32 values per TensorTri block).
@@ -927,8 +929,9 @@ __device__ inline unsigned long long fix_of(double v) { return (unsigned long lo
 
 // Kernel inputs and outputs.  Neighbours of atom i: counts[i] keys in row keys[i * cap ...], sorted in
 // the reference order (key = j_local << 30 | image offset by 512 in three 10-bit fields).  The gradient
-// modes add -dE/dr into the fixed-point accumulators F_fix (per atom), E_fix and W_fix (per structure)
-// and zero the entries of the other buffer that this workgroup owns, which the next call uses.
+// modes add -dE/dr into the fixed-point accumulators F_fix (per atom; or, gathered, leave each pair's
+// gradient in gin for k_force_gather), E_fix and W_fix (per structure) and zero the entries of the other
+// buffer that this workgroup owns, which the next call uses.
 struct BasisIO {
     const int* type;
     const double* pos;

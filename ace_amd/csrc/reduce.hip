@@ -1,8 +1,10 @@
 // reduce.hip -- fixed-point accumulator readout and the per-structure energy sum of the energy-only loss.
 //
-// The basis kernels add, for each pair p = (i -> j, n) with g_p = dE_i/dd_p and d_p = r_i - r_j - R n,
-// +g_p to F_j and -g_p to F_i (the scatter of NNP::compute_force, src/nnp/nnp.cpp:442-471) as 64-bit
-// fixed-point integers, so the sums are exact and independent of the order of the additions.
+// For each pair p = (i -> j, n) with g_p = dE_i/dd_p and d_p = r_i - r_j - R n, F_j gets +g_p and F_i -g_p
+// (the scatter of NNP::compute_force, src/nnp/nnp.cpp:442-471): the fused and FP64-basis kernels add them
+// as 64-bit fixed-point integers, so the sums are exact and independent of the order of the additions;
+// the FP32 split pass leaves g_p at the reverse entry of j's key row and k_force_gather sums each row in a
+// fixed order and stores the fixed-point force.
 #include "ace_common.hpp"
 #include "basis_math.hpp"
 
